@@ -1,0 +1,137 @@
+/*
+ * ssc_mi355x.h -- C-ABI of the MI355X (gfx950) stream-particle integration library.
+ *
+ * Every entry point takes plain pointers and sizes.  Array arguments are DEVICE pointers
+ * (caller-owned, e.g. torch tensors' data_ptr()); descriptor structs are HOST pointers and are
+ * copied into the kernel arguments, so they may be freed after the call returns.  Work is
+ * enqueued on the caller's HIP stream (`stream`, a hipStream_t; NULL = default stream) and the
+ * functions do not synchronise unless stated.  Return value: 0 on success, a negative
+ * SSC_E_* code otherwise; ssc_last_error() then returns a thread-local message.
+ *
+ * Units follow the reference (kpc, Myr, Msun): every mass enters as G*m with
+ * G = 4.498502151469553e-12 kpc^3 Msun^-1 Myr^-2, and phase-space rows are AoS
+ * [x, y, z, vx, vy, vz] (reference streamsculptor/main.py:117-120).
+ *
+ * Integration grid: a fixed-step integration runs on the global grid t_k = t0 + k*dt,
+ * k = 0..nsteps.  Particle i starts at grid index start_step[i] (NULL = all 0) and ends at
+ * k = nsteps; this is how a spray stream integrates each particle from its release time ts[i]
+ * to the observation time ts[-1] (reference streamsculptor/main.py:343-368).
+ */
+#ifndef SSC_MI355X_H
+#define SSC_MI355X_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SSC_OK 0
+#define SSC_E_ARG (-1)      /* invalid argument (size, pointer, descriptor) */
+#define SSC_E_HIP (-2)      /* HIP runtime error */
+#define SSC_E_LAUNCH (-3)   /* kernel launch failed */
+#define SSC_E_LIMIT (-4)    /* a size exceeds a documented limit */
+
+#define SSC_MAX_COMPONENTS 8
+
+/* Analytic host-potential components.  p[] holds, per kind:
+ *   MIYAMOTO_NAGAI: {G*m, a, b, -}     reference streamsculptor/potential.py:66-72
+ *   NFW:            {G*m, r_s, -, -}   reference streamsculptor/potential.py:74-84
+ *   HERNQUIST:      {G*m, r_s, soft,-} reference streamsculptor/potential.py:132-138
+ *   PLUMMER:        {G*m, r_s, -, -}   reference streamsculptor/potential.py:124-130      */
+enum {
+  SSC_COMP_NONE = 0,
+  SSC_COMP_MIYAMOTO_NAGAI = 1,
+  SSC_COMP_NFW = 2,
+  SSC_COMP_HERNQUIST = 3,
+  SSC_COMP_PLUMMER = 4
+};
+
+typedef struct ssc_component {
+  int32_t kind;
+  int32_t reserved;
+  double p[4];
+} ssc_component;
+
+/* Sum of analytic components: the kernel-side form of Potential_Combine
+ * (reference streamsculptor/potential.py:1279-1296). */
+typedef struct ssc_host_potential {
+  int32_t n_components;
+  int32_t reserved;
+  ssc_component c[SSC_MAX_COMPONENTS];
+} ssc_host_potential;
+
+/* Plummer point sources summed directly (N particles x M sources).
+ *   STATIC: pos[n][3], fixed in time (clumps).
+ *   TRACK:  pos[n_track][n][3], source positions sampled on the integration grid: row k is the
+ *           position at t_k = t0 + k*dt (subhalos on their own orbits).  n_track >= nsteps + 1.
+ *   LINE:   straight-line subhalos, position pos[j] + vel[j]*(t - t0[j]), active only while
+ *           |t - t0[j]| < t_window[j] (reference SubhaloLinePotential,
+ *           streamsculptor/potential.py:802-830).                                            */
+enum { SSC_SRC_NONE = 0, SSC_SRC_STATIC = 1, SSC_SRC_TRACK = 2, SSC_SRC_LINE = 3 };
+
+#define SSC_MAX_SOURCES 1024
+
+typedef struct ssc_sources {
+  int32_t kind;
+  int32_t n;            /* number of sources, 0..SSC_MAX_SOURCES */
+  int32_t n_track;      /* TRACK: rows in pos */
+  int32_t reserved;
+  const double* gm;     /* [n] G*m (device) */
+  const double* rs;     /* [n] Plummer scale radius (device) */
+  const double* pos;    /* STATIC/LINE: [n][3]; TRACK: [n_track][n][3] (device) */
+  const double* vel;    /* LINE: [n][3] (device) */
+  const double* t0;     /* LINE: [n] (device) */
+  const double* t_window; /* LINE: [n] (device) */
+} ssc_sources;
+
+/* Thread-local message for the last failing call on this thread. */
+const char* ssc_last_error(void);
+
+/* Library version string. */
+const char* ssc_version(void);
+
+/* Potential, gradient and Hessian of host + sources at n points (any output may be NULL).
+ *   xyz[n][3], phi[n], grad[n][3], hess[n][9] (row-major, symmetric).
+ *   t: evaluation time (LINE sources); step: grid row used for TRACK sources.
+ * Replaces Potential.potential / gradient (jax.grad) / jacobian_force (jax.jacfwd of grad)
+ * of reference streamsculptor/main.py:37-40,60-65 and the Combine sum potential.py:1285-1296. */
+int ssc_eval(const ssc_host_potential* host, const ssc_sources* src,
+             const double* xyz, int64_t n, double t, int32_t step,
+             double* phi, double* grad, double* hess, void* stream);
+
+/* Fixed-step kick-drift-kick leapfrog for n independent orbits in host + sources.
+ *   w0[n][6]       initial rows at grid index start_step[i]
+ *   start_step[n]  (NULL = 0); particles with start_step >= nsteps are copied unchanged
+ *   w_out[n][6]    rows at grid index nsteps
+ *   save_idx[nsave] NULL, or ascending grid indices in [0, nsteps] at which to record the state
+ *                  into ys[nsave][n][6]; a particle not yet started records its initial row
+ * Particles should be ordered by start_step (a spray stream already is): a block iterates the
+ * grid from its earliest start, masking lanes that are not yet released.
+ * Fixed-step counterpart of integrate_orbit / integrate_orbit_batch_vmapped /
+ * gen_stream_vmapped (reference streamsculptor/main.py:125-202,343-368), which the reference
+ * only offers with adaptive diffrax solvers. */
+int ssc_leapfrog(const ssc_host_potential* host, const ssc_sources* src,
+                 const double* w0, const int32_t* start_step, int64_t n,
+                 double t0, double dt, int32_t nsteps, double* w_out,
+                 const int32_t* save_idx, int32_t nsave, double* ys, void* stream);
+
+/* Spray-stream release conditions (reference Potential.release_model + gen_stream_ics,
+ * streamsculptor/main.py:209-306).  For each row i < nrel, from the progenitor row
+ * prog_w[i][6] with satellite mass G*Msat = g_msat[i] and release index rel_idx[i]
+ * (NULL = i; the index seeds the per-release keys):
+ *   pos_close/pos_far/vel_close/vel_far [nrel][3]  (close = leading, far = trailing).
+ * kvals: NULL selects the reference defaults {2.0, 0.3, 0, 0, 0.4, 0.4, 0.5, 0.5}, else 8
+ * host doubles.  Offsets are drawn with the reference's key scheme (PRNGKey(seed) ->
+ * randint(5, 0, 1000) -> PRNGKey(i*r_k) -> normal) on a Threefry-2x32-20 counter PRNG.
+ * g_msat is G*Msat per release (device). */
+int ssc_release(const ssc_host_potential* host, const double* prog_w, const int32_t* rel_idx,
+                const double* g_msat, int32_t nrel, int64_t seed, const double* kvals,
+                double* pos_close, double* pos_far, double* vel_close, double* vel_far,
+                void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SSC_MI355X_H */

@@ -1,0 +1,245 @@
+"""ORACLE -- test infrastructure only.
+
+CPU FP64 restatement of the reference's stream-particle integration path (C++ in
+``oracle/ssc_oracle.cpp``, wrapped here with ctypes + numpy).  Only ``tests/``,
+``__graft_entry__.smoke()`` and ``bench.py``'s ``cpu_baseline`` leg may import this package, and
+only as the checker.  The product package (``streamsculptor_amd``) never imports it.
+
+What pins each function (independently of this file) -- see tests/test_oracle_*.py:
+
+* G and units: closed form from IAU/CODATA constants; the reference notebook value
+  ``custom_potential.ipynb:132`` to 15 digits.
+* Component potentials / gradients / Hessians (MN, NFW, Hernquist, Plummer): sympy
+  differentiation of the reference formulas (potential.py:66-84,124-138), central finite
+  differences, Laplacian = 4 pi G rho for the analytic densities, and the reference notebook
+  NFW acceleration ``tests.ipynb:1671``.
+* Leapfrog: time reversibility, the closed-form discrete harmonic oscillator (Plummer core),
+  second-order convergence, bounded energy error.
+* Threefry-2x32-20: the Random123 known-answer vectors.
+* erfinv: scipy.special.erfinv; normal / randint: bit-level restatement of the key scheme
+  (JAX bit compatibility of the draws themselves is parity-unpinned: jax is not installable).
+* Release model: closed-form geometry on hand-built cases; its Jacobian against central
+  finite differences of the release model itself.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "libssc_oracle.so")
+
+G = 4.498502151469553e-12  # kpc^3 / (Msun Myr^2)
+
+COMP_MN, COMP_NFW, COMP_HERNQUIST, COMP_PLUMMER = 1, 2, 3, 4
+SRC_NONE, SRC_STATIC, SRC_TRACK, SRC_LINE = 0, 1, 2, 3
+MAX_COMPONENTS = 8
+
+
+class _Component(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int32), ("reserved", ctypes.c_int32),
+                ("p", ctypes.c_double * 4)]
+
+
+class _Host(ctypes.Structure):
+    _fields_ = [("n_components", ctypes.c_int32), ("reserved", ctypes.c_int32),
+                ("c", _Component * MAX_COMPONENTS)]
+
+
+class _Sources(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int32), ("n", ctypes.c_int32), ("n_track", ctypes.c_int32),
+                ("reserved", ctypes.c_int32),
+                ("gm", ctypes.c_void_p), ("rs", ctypes.c_void_p), ("pos", ctypes.c_void_p),
+                ("vel", ctypes.c_void_p), ("t0", ctypes.c_void_p), ("t_window", ctypes.c_void_p)]
+
+
+def build(force: bool = False) -> str:
+    """Compile the oracle with its Makefile (g++ + OpenMP)."""
+    if force or not os.path.exists(_LIB_PATH) or (
+            os.path.getmtime(_LIB_PATH) < os.path.getmtime(os.path.join(_HERE, "ssc_oracle.cpp"))):
+        subprocess.run(["make", "-s", "-C", _HERE], check=True)
+    return _LIB_PATH
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        build()
+        L = ctypes.CDLL(_LIB_PATH)
+        vp, i64, i32, d = ctypes.c_void_p, ctypes.c_int64, ctypes.c_int32, ctypes.c_double
+        L.orc_threefry2x32.argtypes = [ctypes.c_uint32] * 4 + [vp]
+        L.orc_randint5_1000.argtypes = [i64, vp]
+        L.orc_prng_normal.argtypes = [i64]
+        L.orc_prng_normal.restype = d
+        L.orc_erfinv.argtypes = [d]
+        L.orc_erfinv.restype = d
+        L.orc_release_normals.argtypes = [i64, i64, vp]
+        L.orc_eval.argtypes = [vp, vp, vp, i64, d, i32, vp, vp, vp]
+        L.orc_leapfrog.argtypes = [vp, vp, vp, vp, i64, d, d, i32, vp, vp, ctypes.c_int]
+        L.orc_release.argtypes = [vp, vp, vp, vp, i32, i64, vp, vp, vp, vp, vp]
+        L.orc_release_jacobian.argtypes = [vp, vp, vp, i32, i64, vp, vp]
+        _lib = L
+    return _lib
+
+
+# ------------------------------------------------------------------------------------------
+# descriptors
+# ------------------------------------------------------------------------------------------
+def host(components):
+    """components: list of (kind, [p0, p1, p2, p3]) with p0 = G*m."""
+    h = _Host()
+    h.n_components = len(components)
+    for i, (kind, p) in enumerate(components):
+        h.c[i].kind = kind
+        pp = list(p) + [0.0] * (4 - len(p))
+        for k in range(4):
+            h.c[i].p[k] = float(pp[k])
+    return h
+
+
+def milky_way_host(m_halo=5.4e11, rs_halo=15.62):
+    """Config host: MN disk (6.8e10, 3, 0.28) + Hernquist bulge (5e9, 1) + NFW halo
+    (reference GalaMilkyWayPotential minus its nucleus, potential.py:390-414)."""
+    return host([(COMP_MN, [G * 6.8e10, 3.0, 0.28]),
+                 (COMP_HERNQUIST, [G * 5e9, 1.0, 0.0]),
+                 (COMP_NFW, [G * m_halo, rs_halo])])
+
+
+class Sources:
+    """Holds numpy arrays alive for a _Sources descriptor."""
+
+    def __init__(self, kind=SRC_NONE, gm=None, rs=None, pos=None, vel=None, t0=None,
+                 t_window=None):
+        self.kind = kind
+        self.arrays = {}
+        s = _Sources()
+        s.kind = kind
+        if kind != SRC_NONE:
+            gm = np.ascontiguousarray(gm, dtype=np.float64)
+            rs = np.ascontiguousarray(rs, dtype=np.float64)
+            pos = np.ascontiguousarray(pos, dtype=np.float64)
+            s.n = gm.shape[0]
+            s.n_track = pos.shape[0] if kind == SRC_TRACK else 0
+            self.arrays.update(gm=gm, rs=rs, pos=pos)
+            s.gm, s.rs, s.pos = gm.ctypes.data, rs.ctypes.data, pos.ctypes.data
+            if kind == SRC_LINE:
+                vel = np.ascontiguousarray(vel, dtype=np.float64)
+                t0 = np.ascontiguousarray(t0, dtype=np.float64)
+                t_window = np.ascontiguousarray(t_window, dtype=np.float64)
+                self.arrays.update(vel=vel, t0=t0, t_window=t_window)
+                s.vel, s.t0, s.t_window = vel.ctypes.data, t0.ctypes.data, t_window.ctypes.data
+        self.desc = s
+
+
+NO_SOURCES = None
+
+
+def _src_ptr(src):
+    if src is None:
+        src = Sources()
+    return src, ctypes.addressof(src.desc)
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data
+
+
+# ------------------------------------------------------------------------------------------
+# functions
+# ------------------------------------------------------------------------------------------
+def threefry2x32(k0, k1, x0, x1):
+    out = np.zeros(2, dtype=np.uint32)
+    lib().orc_threefry2x32(k0, k1, x0, x1, out.ctypes.data)
+    return int(out[0]), int(out[1])
+
+
+def randint5_1000(seed):
+    out = np.zeros(5, dtype=np.int64)
+    lib().orc_randint5_1000(seed, out.ctypes.data)
+    return out
+
+
+def prng_normal(key_seed):
+    return lib().orc_prng_normal(key_seed)
+
+
+def erfinv(u):
+    return lib().orc_erfinv(float(u))
+
+
+def release_normals(seed, i):
+    z = np.zeros(4)
+    lib().orc_release_normals(seed, i, z.ctypes.data)
+    return z
+
+
+def evaluate(h, xyz, t=0.0, step=0, src=None):
+    xyz = np.ascontiguousarray(np.atleast_2d(xyz), dtype=np.float64)
+    n = xyz.shape[0]
+    phi, grad, hess = np.zeros(n), np.zeros((n, 3)), np.zeros((n, 9))
+    src, sp = _src_ptr(src)
+    lib().orc_eval(ctypes.addressof(h), sp, xyz.ctypes.data, n, t, step,
+                   phi.ctypes.data, grad.ctypes.data, hess.ctypes.data)
+    return phi, grad, hess.reshape(n, 3, 3)
+
+
+def leapfrog(h, w0, t0, dt, nsteps, start=None, src=None, traj=False, nthreads=0):
+    w0 = np.ascontiguousarray(np.atleast_2d(w0), dtype=np.float64)
+    n = w0.shape[0]
+    out = np.zeros_like(w0)
+    st = None if start is None else np.ascontiguousarray(start, dtype=np.int32)
+    tr = np.zeros((nsteps + 1, n, 6)) if traj else None
+    src, sp = _src_ptr(src)
+    lib().orc_leapfrog(ctypes.addressof(h), sp, w0.ctypes.data, _p(st), n, t0, dt, nsteps,
+                       out.ctypes.data, _p(tr), nthreads)
+    return (out, tr) if traj else out
+
+
+def release(h, prog_w, ts, msat, seed, kvals=None):
+    prog_w = np.ascontiguousarray(prog_w, dtype=np.float64)
+    nrel = prog_w.shape[0]
+    gms = np.ascontiguousarray(G * np.broadcast_to(np.asarray(msat, dtype=np.float64), (nrel,)))
+    ts = np.ascontiguousarray(ts, dtype=np.float64)
+    kv = None if kvals is None else np.ascontiguousarray(kvals, dtype=np.float64)
+    outs = [np.zeros((nrel, 3)) for _ in range(4)]
+    lib().orc_release(ctypes.addressof(h), prog_w.ctypes.data, ts.ctypes.data, gms.ctypes.data,
+                      nrel, seed, _p(kv), *[o.ctypes.data for o in outs])
+    return tuple(outs)  # pos_close, pos_far, vel_close, vel_far
+
+
+def release_jacobian(h, prog_w, msat, seed, kvals=None):
+    prog_w = np.ascontiguousarray(prog_w, dtype=np.float64)
+    nrel = prog_w.shape[0]
+    gms = np.ascontiguousarray(G * np.broadcast_to(np.asarray(msat, dtype=np.float64), (nrel,)))
+    kv = None if kvals is None else np.ascontiguousarray(kvals, dtype=np.float64)
+    nin = 6 + 2 * h.n_components
+    jac = np.zeros((nrel, 12, nin))
+    lib().orc_release_jacobian(ctypes.addressof(h), prog_w.ctypes.data, gms.ctypes.data, nrel,
+                               seed, _p(kv), jac.ctypes.data)
+    return jac
+
+
+def stream_leapfrog(h, prog_w0, ts, msat, seed, dt, src=None, kvals=None, nthreads=0,
+                    release_host=None):
+    """Plain spray stream with fixed-step leapfrog (restates gen_stream_vmapped,
+    main.py:343-368): progenitor orbit saved at every ts, release model at every ts, then each
+    particle i < len(ts)-1 integrated from ts[i] to ts[-1].  ts must lie on the dt grid."""
+    ts = np.asarray(ts, dtype=np.float64)
+    T = len(ts) - 1
+    kidx = np.rint((ts - ts[0]) / dt).astype(np.int64)
+    nsteps = int(kidx[-1])
+    _, prog = leapfrog(h, prog_w0, ts[0], dt, nsteps, src=src, traj=True)
+    prog_at = prog[kidx, 0, :]
+    pc, pf, vc, vf = release(release_host or h, prog_at, ts, msat, seed, kvals)
+    w0 = np.empty((2 * T, 6))
+    w0[0::2, :3], w0[0::2, 3:] = pc[:T], vc[:T]
+    w0[1::2, :3], w0[1::2, 3:] = pf[:T], vf[:T]
+    start = np.repeat(kidx[:T], 2).astype(np.int32)
+    out = leapfrog(h, w0, ts[0], dt, nsteps, start=start, src=src, nthreads=nthreads)
+    return out[0::2], out[1::2]

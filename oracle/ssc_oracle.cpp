@@ -1,0 +1,500 @@
+// ORACLE -- test infrastructure only.  This is a CPU FP64 restatement of the reference
+// algorithms on the stream-particle integration path, used by tests/, by smoke() and by the
+// cpu_baseline leg of bench.py as the CHECKER.  No product code may link, import or call it.
+//
+// Each function cites the reference file:line it restates (paths relative to the reference
+// repository root).  Pins (what checks each part independently of this file) are listed in the
+// Python wrapper oracle/__init__.py and exercised by tests/test_oracle_*.py.
+//
+// Build: oracle/Makefile (g++ -O2 -fopenmp, no -ffast-math, no FMA contraction).
+
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+#include <omp.h>
+
+#include "../include/ssc_mi355x.h"
+
+namespace {
+
+// ---------------------------------------------------------------------------------------
+// Forward-mode dual numbers (restates jax.jacfwd, used by the reference for release-model
+// Jacobians, streamsculptor/perturbative.py:282-296, and for the tidal tensor, fields.py:193).
+// ---------------------------------------------------------------------------------------
+template <int N>
+struct Dual {
+  double v;
+  double d[N];
+  Dual() : v(0) { for (int i = 0; i < N; ++i) d[i] = 0; }
+  Dual(double x) : v(x) { for (int i = 0; i < N; ++i) d[i] = 0; }
+};
+template <int N> Dual<N> operator+(const Dual<N>& a, const Dual<N>& b) { Dual<N> r(a.v + b.v); for (int i = 0; i < N; ++i) r.d[i] = a.d[i] + b.d[i]; return r; }
+template <int N> Dual<N> operator-(const Dual<N>& a, const Dual<N>& b) { Dual<N> r(a.v - b.v); for (int i = 0; i < N; ++i) r.d[i] = a.d[i] - b.d[i]; return r; }
+template <int N> Dual<N> operator-(const Dual<N>& a) { Dual<N> r(-a.v); for (int i = 0; i < N; ++i) r.d[i] = -a.d[i]; return r; }
+template <int N> Dual<N> operator*(const Dual<N>& a, const Dual<N>& b) { Dual<N> r(a.v * b.v); for (int i = 0; i < N; ++i) r.d[i] = a.d[i] * b.v + a.v * b.d[i]; return r; }
+template <int N> Dual<N> operator/(const Dual<N>& a, const Dual<N>& b) { double q = a.v / b.v; Dual<N> r(q); for (int i = 0; i < N; ++i) r.d[i] = (a.d[i] - q * b.d[i]) / b.v; return r; }
+template <int N> Dual<N> operator+(const Dual<N>& a, double b) { Dual<N> r = a; r.v += b; return r; }
+template <int N> Dual<N> operator+(double b, const Dual<N>& a) { return a + b; }
+template <int N> Dual<N> operator-(const Dual<N>& a, double b) { Dual<N> r = a; r.v -= b; return r; }
+template <int N> Dual<N> operator-(double b, const Dual<N>& a) { return (-a) + b; }
+template <int N> Dual<N> operator*(const Dual<N>& a, double b) { Dual<N> r(a.v * b); for (int i = 0; i < N; ++i) r.d[i] = a.d[i] * b; return r; }
+template <int N> Dual<N> operator*(double b, const Dual<N>& a) { return a * b; }
+template <int N> Dual<N> operator/(const Dual<N>& a, double b) { Dual<N> r(a.v / b); for (int i = 0; i < N; ++i) r.d[i] = a.d[i] / b; return r; }
+template <int N> Dual<N> operator/(double a, const Dual<N>& b) { return Dual<N>(a) / b; }
+template <int N> Dual<N> sqrt(const Dual<N>& a) { double s = std::sqrt(a.v); Dual<N> r(s); for (int i = 0; i < N; ++i) r.d[i] = a.d[i] / (2.0 * s); return r; }
+template <int N> Dual<N> log1p(const Dual<N>& a) { Dual<N> r(std::log1p(a.v)); for (int i = 0; i < N; ++i) r.d[i] = a.d[i] / (1.0 + a.v); return r; }
+template <int N> Dual<N> cbrt(const Dual<N>& a) { double c = std::cbrt(a.v); Dual<N> r(c); for (int i = 0; i < N; ++i) r.d[i] = a.d[i] / (3.0 * c * c); return r; }
+
+inline double sqrt(double x) { return std::sqrt(x); }
+inline double log1p(double x) { return std::log1p(x); }
+inline double cbrt(double x) { return std::cbrt(x); }
+inline double val(double x) { return x; }
+template <int N> inline double val(const Dual<N>& x) { return x.v; }
+
+// ---------------------------------------------------------------------------------------
+// Host components: phi, gradient and Hessian, closed forms.
+// ---------------------------------------------------------------------------------------
+
+// Spherical helper: given r (with r^2 = |x|^2 + soft), dphi/dr and d2phi/dr2, accumulate
+// grad = phi' x / r and H = (phi'/r) I + (phi'' - phi'/r) x x^T / r^2.
+template <typename T>
+void spherical_accum(const T x[3], T r, T d1, T d2, T g[3], T H[9]) {
+  T d1r = d1 / r;
+  for (int i = 0; i < 3; ++i) g[i] = g[i] + d1r * x[i];
+  if (H) {
+    T c = (d2 - d1r) / (r * r);
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) {
+        T h = c * x[i] * x[j];
+        if (i == j) h = h + d1r;
+        H[3 * i + j] = H[3 * i + j] + h;
+      }
+  }
+}
+
+// Miyamoto-Nagai disk, reference streamsculptor/potential.py:66-72:
+//   phi = -G m / sqrt(R^2 + (a + sqrt(z^2 + b^2))^2)
+template <typename T>
+void mn_eval(const T p[4], const T x[3], T* phi, T g[3], T H[9]) {
+  T gm = p[0], a = p[1], b = p[2];
+  T zeta = sqrt(x[2] * x[2] + b * b);
+  T S = a + zeta;
+  T D2 = x[0] * x[0] + x[1] * x[1] + S * S;
+  T D = sqrt(D2);
+  T iD3 = 1.0 / (D2 * D);
+  if (phi) *phi = *phi - gm / D;
+  T dz = x[2] * S / zeta;  // d(D^2/2)/dz
+  g[0] = g[0] + gm * x[0] * iD3;
+  g[1] = g[1] + gm * x[1] * iD3;
+  g[2] = g[2] + gm * dz * iD3;
+  if (H) {
+    T iD5 = iD3 / D2;
+    T q[3] = {x[0], x[1], dz};
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) H[3 * i + j] = H[3 * i + j] - 3.0 * gm * q[i] * q[j] * iD5;
+    H[0] = H[0] + gm * iD3;
+    H[4] = H[4] + gm * iD3;
+    // d/dz (z S / zeta) = S b^2 / zeta^3 + z^2 / zeta^2
+    T z2 = zeta * zeta;
+    T ddz = S * b * b / (z2 * zeta) + x[2] * x[2] / z2;
+    H[8] = H[8] + gm * ddz * iD3;
+  }
+}
+
+// NFW, reference streamsculptor/potential.py:74-84: phi = -G m ln(1 + r/r_s) / r
+template <typename T>
+void nfw_eval(const T p[4], const T x[3], T* phi, T g[3], T H[9]) {
+  T gm = p[0], rs = p[1];
+  T r = sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2]);
+  T L = log1p(r / rs);
+  if (phi) *phi = *phi - gm * L / r;
+  T rpr = r + rs;
+  T d1 = gm * (L / (r * r) - 1.0 / (r * rpr));
+  T d2 = gm * (1.0 / (r * r * rpr) - 2.0 * L / (r * r * r) + (2.0 * r + rs) / (r * r * rpr * rpr));
+  spherical_accum(x, r, d1, d2, g, H);
+}
+
+// Hernquist, reference streamsculptor/potential.py:132-138: phi = -G m / (r + c),
+// r = sqrt(|x|^2 + soft)
+template <typename T>
+void hernquist_eval(const T p[4], const T x[3], T* phi, T g[3], T H[9]) {
+  T gm = p[0], c = p[1], soft = p[2];
+  T r = sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2] + soft);
+  T rc = r + c;
+  if (phi) *phi = *phi - gm / rc;
+  T d1 = gm / (rc * rc);
+  T d2 = -2.0 * gm / (rc * rc * rc);
+  spherical_accum(x, r, d1, d2, g, H);
+}
+
+// Plummer, reference streamsculptor/potential.py:124-130: phi = -G m / sqrt(r^2 + r_s^2)
+template <typename T>
+void plummer_eval(T gm, T rs, const T x[3], T* phi, T g[3], T H[9]) {
+  T s2 = x[0] * x[0] + x[1] * x[1] + x[2] * x[2] + rs * rs;
+  T s = sqrt(s2);
+  T i1 = 1.0 / s;
+  T i3 = i1 / s2;
+  if (phi) *phi = *phi - gm * i1;
+  for (int i = 0; i < 3; ++i) g[i] = g[i] + gm * i3 * x[i];
+  if (H) {
+    T i5 = i3 / s2;
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) {
+        T h = -3.0 * gm * i5 * x[i] * x[j];
+        if (i == j) h = h + gm * i3;
+        H[3 * i + j] = H[3 * i + j] + h;
+      }
+  }
+}
+
+template <typename T>
+void comp_eval(int kind, const T p[4], const T x[3], T* phi, T g[3], T H[9]) {
+  switch (kind) {
+    case SSC_COMP_MIYAMOTO_NAGAI: mn_eval(p, x, phi, g, H); break;
+    case SSC_COMP_NFW: nfw_eval(p, x, phi, g, H); break;
+    case SSC_COMP_HERNQUIST: hernquist_eval(p, x, phi, g, H); break;
+    case SSC_COMP_PLUMMER: plummer_eval(p[0], p[1], x, phi, g, H); break;
+    default: break;
+  }
+}
+
+// Sum over components (reference Potential_Combine, streamsculptor/potential.py:1279-1296).
+template <typename T>
+void host_eval_T(const ssc_host_potential* hp, const T x[3], T* phi, T g[3], T H[9]) {
+  if (phi) *phi = T(0.0);
+  for (int i = 0; i < 3; ++i) g[i] = T(0.0);
+  if (H) for (int i = 0; i < 9; ++i) H[i] = T(0.0);
+  for (int c = 0; c < hp->n_components; ++c) {
+    T p[4];
+    for (int k = 0; k < 4; ++k) p[k] = T(hp->c[c].p[k]);
+    comp_eval(hp->c[c].kind, p, x, phi, g, H);
+  }
+}
+
+// Position of source j at grid row `step` / time t; returns false when the source is inactive
+// (LINE window, reference streamsculptor/potential.py:826).
+bool source_pos(const ssc_sources* s, int j, int step, double t, double out[3]) {
+  if (s->kind == SSC_SRC_STATIC) {
+    for (int i = 0; i < 3; ++i) out[i] = s->pos[3 * j + i];
+    return true;
+  }
+  if (s->kind == SSC_SRC_TRACK) {
+    const double* row = s->pos + ((int64_t)step * s->n + j) * 3;
+    for (int i = 0; i < 3; ++i) out[i] = row[i];
+    return true;
+  }
+  if (s->kind == SSC_SRC_LINE) {
+    double dtt = t - s->t0[j];
+    if (!(std::fabs(dtt) < s->t_window[j])) return false;
+    for (int i = 0; i < 3; ++i) out[i] = s->pos[3 * j + i] + s->vel[3 * j + i] * dtt;
+    return true;
+  }
+  return false;
+}
+
+// Direct N x M Plummer sum (reference SubhaloLinePotential.potential,
+// streamsculptor/potential.py:813-830, which sums Plummer spheres over subhalos).
+void sources_eval(const ssc_sources* s, const double x[3], int step, double t,
+                  double* phi, double g[3], double H[9]) {
+  if (!s || s->kind == SSC_SRC_NONE) return;
+  for (int j = 0; j < s->n; ++j) {
+    double sp[3];
+    if (!source_pos(s, j, step, t, sp)) continue;
+    double d[3] = {x[0] - sp[0], x[1] - sp[1], x[2] - sp[2]};
+    plummer_eval<double>(s->gm[j], s->rs[j], d, phi, g, H);
+  }
+}
+
+void total_eval(const ssc_host_potential* hp, const ssc_sources* s, const double x[3], int step,
+                double t, double* phi, double g[3], double H[9]) {
+  host_eval_T<double>(hp, x, phi, g, H);
+  sources_eval(s, x, step, t, phi, g, H);
+}
+
+void accel(const ssc_host_potential* hp, const ssc_sources* s, const double x[3], int step,
+           double t, double a[3]) {
+  double g[3];
+  total_eval(hp, s, x, step, t, nullptr, g, nullptr);
+  for (int i = 0; i < 3; ++i) a[i] = -g[i];
+}
+
+// ---------------------------------------------------------------------------------------
+// Threefry-2x32 with 20 rounds (Salmon et al., SC'11; the cipher behind jax.random's default
+// PRNG used by release_model, reference streamsculptor/main.py:220-228,263-266).
+// ---------------------------------------------------------------------------------------
+inline uint32_t rotl32(uint32_t v, int r) { return (v << r) | (v >> (32 - r)); }
+
+void threefry2x32(uint32_t k0, uint32_t k1, uint32_t x0, uint32_t x1, uint32_t out[2]) {
+  static const int R[8] = {13, 15, 26, 6, 17, 29, 16, 24};
+  uint32_t ks[3] = {k0, k1, k0 ^ k1 ^ 0x1BD11BDAu};
+  uint32_t v0 = x0 + ks[0], v1 = x1 + ks[1];
+  for (int blk = 0; blk < 5; ++blk) {
+    for (int r = 0; r < 4; ++r) {
+      v0 += v1;
+      v1 = rotl32(v1, R[(blk % 2) * 4 + r]);
+      v1 ^= v0;
+    }
+    v0 += ks[(blk + 1) % 3];
+    v1 += ks[(blk + 2) % 3] + (uint32_t)(blk + 1);
+  }
+  out[0] = v0;
+  out[1] = v1;
+}
+
+// PRNGKey(seed) for an int64 seed: key = (seed >> 32, seed & 0xffffffff).
+void prng_key(int64_t seed, uint32_t key[2]) {
+  key[0] = (uint32_t)((uint64_t)seed >> 32);
+  key[1] = (uint32_t)((uint64_t)seed & 0xffffffffu);
+}
+
+// 64-bit random words for a flat shape of n elements: counters 0..2n-1 split into halves
+// (j, n + j); word j = (y0 << 32) | y1.
+uint64_t random_bits64(const uint32_t key[2], int n, int j) {
+  uint32_t y[2];
+  threefry2x32(key[0], key[1], (uint32_t)j, (uint32_t)(n + j), y);
+  return ((uint64_t)y[0] << 32) | (uint64_t)y[1];
+}
+
+// split(key) -> (k1, k2): counters 0..3 split into halves (0,2), (1,3).
+void prng_split(const uint32_t key[2], uint32_t k1[2], uint32_t k2[2]) {
+  uint32_t a[2], b[2];
+  threefry2x32(key[0], key[1], 0u, 2u, a);
+  threefry2x32(key[0], key[1], 1u, 3u, b);
+  k1[0] = a[0]; k1[1] = b[0];
+  k2[0] = a[1]; k2[1] = b[1];
+}
+
+// randint(key, (5,), 0, 1000) with int64 output (reference streamsculptor/main.py:221).
+void randint5_1000(int64_t seed, int64_t out[5]) {
+  uint32_t key[2], k1[2], k2[2];
+  prng_key(seed, key);
+  prng_split(key, k1, k2);
+  const uint64_t span = 1000;
+  uint64_t mult = ((uint64_t)1 << 32) % span;
+  mult = (mult * mult) % span;
+  for (int j = 0; j < 5; ++j) {
+    uint64_t hb = random_bits64(k1, 5, j), lb = random_bits64(k2, 5, j);
+    uint64_t off = ((hb % span) * mult + (lb % span)) % span;
+    out[j] = (int64_t)off;
+  }
+}
+
+// Inverse error function: Winitzki's closed-form start (rel. err ~2e-3) refined by three
+// Halley steps on erf (|u| < 0.5) or erfc (tails, where 1 - |u| is exact).
+double erfinv(double u) {
+  if (u == 0.0) return 0.0;
+  if (u >= 1.0) return INFINITY;
+  if (u <= -1.0) return -INFINITY;
+  double s = u < 0 ? -1.0 : 1.0;
+  double a = std::fabs(u);
+  const double A = 0.147;
+  double ln = std::log((1.0 - a) * (1.0 + a));
+  double t1 = 2.0 / (M_PI * A) + 0.5 * ln;
+  double x = std::sqrt(std::sqrt(t1 * t1 - ln / A) - t1);
+  const double c = 2.0 / std::sqrt(M_PI);
+  for (int it = 0; it < 3; ++it) {
+    double f = (a < 0.5) ? (std::erf(x) - a) : ((1.0 - a) - std::erfc(x));
+    double fp = c * std::exp(-x * x);
+    x = x - f / (fp + x * f);
+  }
+  return s * x;
+}
+
+// normal(key, (1,)) in float64: uniform on [nextafter(-1, 0), 1) from the top 52 bits of one
+// 64-bit word, then sqrt(2) * erfinv.
+double prng_normal(int64_t key_seed) {
+  uint32_t key[2];
+  prng_key(key_seed, key);
+  uint64_t bits = random_bits64(key, 1, 0);
+  uint64_t fb = (bits >> 12) | 0x3FF0000000000000ull;
+  double f;
+  std::memcpy(&f, &fb, 8);
+  f -= 1.0;
+  const double lo = std::nextafter(-1.0, 0.0);
+  double uval = f * 2.0 + lo;  // (hi - lo) rounds to 2.0 in float64
+  if (uval < lo) uval = lo;
+  return std::sqrt(2.0) * erfinv(uval);
+}
+
+// ---------------------------------------------------------------------------------------
+// Release model (reference Potential.release_model, streamsculptor/main.py:209-280, with
+// lagrange_pts/tidalr/omega/d2phidr2 from :77-112).  Templated so the Jacobian can be
+// taken with duals (reference release_func_jacobian, perturbative.py:282-296).
+// in[6] = progenitor (x, v); hostp = host component parameters as scalars.
+// ---------------------------------------------------------------------------------------
+template <typename T>
+void release_T(const ssc_host_potential* hp, const T* hostp /*[n_comp*4]*/, const T in[6],
+               T g_msat, const double z[4], const double kv[8], T out[12]) {
+  T x[3] = {in[0], in[1], in[2]}, v[3] = {in[3], in[4], in[5]};
+  // Hessian of the host at x.
+  T g[3], H[9];
+  for (int i = 0; i < 3; ++i) g[i] = T(0.0);
+  for (int i = 0; i < 9; ++i) H[i] = T(0.0);
+  for (int c = 0; c < hp->n_components; ++c) comp_eval(hp->c[c].kind, hostp + 4 * c, x, (T*)nullptr, g, H);
+  T rad = sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2]);
+  T rhat[3] = {x[0] / rad, x[1] / rad, x[2] / rad};
+  // d2phidr2 = rhat^T H rhat (rhat held fixed), main.py:77-85
+  T d2 = T(0.0);
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) d2 = d2 + rhat[i] * H[3 * i + j] * rhat[j];
+  // omega = |x cross v| / r^2, main.py:89-96
+  T L[3] = {x[1] * v[2] - x[2] * v[1], x[2] * v[0] - x[0] * v[2], x[0] * v[1] - x[1] * v[0]};
+  T r2 = x[0] * x[0] + x[1] * x[1] + x[2] * x[2];
+  T ov[3] = {L[0] / r2, L[1] / r2, L[2] / r2};
+  T om = sqrt(ov[0] * ov[0] + ov[1] * ov[1] + ov[2] * ov[2]);
+  // tidal radius, main.py:99-104
+  T rt = cbrt(g_msat / (om * om - d2));
+  T relv = om * rt;
+  T Ln = sqrt(L[0] * L[0] + L[1] * L[1] + L[2] * L[2]);
+  T zh[3] = {L[0] / Ln, L[1] / Ln, L[2] / Ln};
+  T vr = v[0] * rhat[0] + v[1] * rhat[1] + v[2] * rhat[2];
+  T pv[3] = {v[0] - vr * rhat[0], v[1] - vr * rhat[1], v[2] - vr * rhat[2]};
+  T pn = sqrt(pv[0] * pv[0] + pv[1] * pv[1] + pv[2] * pv[2]);
+  T ph[3] = {pv[0] / pn, pv[1] / pn, pv[2] / pn};
+  // samples, main.py:263-266
+  double kr = kv[0] + z[0] * kv[4];
+  double kvphi = kr * (kv[1] + z[1] * kv[5]);
+  double kz = kv[2] + z[2] * kv[6];
+  double kvz = kv[3] + z[3] * kv[7];
+  // main.py:269-278; out = [pos_lead(3), pos_trail(3), v_lead(3), v_trail(3)]
+  for (int i = 0; i < 3; ++i) {
+    T ptr = x[i] + kr * rhat[i] * rt;
+    ptr = ptr + zh[i] * kz * rt;
+    T vtr = v[i] + (kvphi * relv) * ph[i];
+    vtr = vtr + (kvz * relv) * zh[i];
+    T pld = x[i] + kr * rhat[i] * (-rt);
+    pld = pld + zh[i] * kz * (-rt);
+    T vld = v[i] + (kvphi * relv * -1.0) * ph[i];
+    vld = vld + (kvz * relv * -1.0) * zh[i];
+    out[i] = pld;
+    out[3 + i] = ptr;
+    out[6 + i] = vld;
+    out[9 + i] = vtr;
+  }
+}
+
+const double kDefaultK[8] = {2.0, 0.3, 0.0, 0.0, 0.4, 0.4, 0.5, 0.5};
+
+void release_normals(int64_t seed, int64_t i, double z[4]) {
+  int64_t ri[5];
+  randint5_1000(seed, ri);
+  for (int k = 0; k < 4; ++k) z[k] = prng_normal(i * ri[k]);
+}
+
+}  // namespace
+
+// =========================================================================================
+// extern "C" surface of the oracle
+// =========================================================================================
+extern "C" {
+
+void orc_threefry2x32(uint32_t k0, uint32_t k1, uint32_t x0, uint32_t x1, uint32_t* out) {
+  threefry2x32(k0, k1, x0, x1, out);
+}
+void orc_randint5_1000(int64_t seed, int64_t* out) { randint5_1000(seed, out); }
+double orc_prng_normal(int64_t key_seed) { return prng_normal(key_seed); }
+double orc_erfinv(double u) { return erfinv(u); }
+void orc_release_normals(int64_t seed, int64_t i, double* z) { release_normals(seed, i, z); }
+
+// Potential / gradient / Hessian at n points.
+int orc_eval(const ssc_host_potential* hp, const ssc_sources* src, const double* xyz, int64_t n,
+             double t, int32_t step, double* phi, double* grad, double* hess) {
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < n; ++i) {
+    double p = 0, g[3], H[9];
+    total_eval(hp, src, xyz + 3 * i, step, t, &p, g, H);
+    if (phi) phi[i] = p;
+    if (grad) for (int k = 0; k < 3; ++k) grad[3 * i + k] = g[k];
+    if (hess) for (int k = 0; k < 9; ++k) hess[9 * i + k] = H[k];
+  }
+  return 0;
+}
+
+// Kick-drift-kick leapfrog on the grid t_k = t0 + k dt (Verlet 1967; Hairer, Lubich & Wanner,
+// Geometric Numerical Integration, sec. I.1.4).  The reference has no fixed-step solver: this
+// restates the north-star fixed-step replacement of integrate_orbit (main.py:125-163).
+int orc_leapfrog(const ssc_host_potential* hp, const ssc_sources* src, const double* w0,
+                 const int32_t* start, int64_t n, double t0, double dt, int32_t nsteps,
+                 double* w_out, double* traj, int nthreads) {
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#pragma omp parallel for schedule(dynamic, 16)
+  for (int64_t i = 0; i < n; ++i) {
+    double x[3], v[3], a[3];
+    for (int k = 0; k < 3; ++k) { x[k] = w0[6 * i + k]; v[k] = w0[6 * i + 3 + k]; }
+    int s = start ? start[i] : 0;
+    if (s < 0) s = 0;
+    if (traj)
+      for (int k = 0; k <= (s < nsteps ? s : nsteps); ++k)
+        for (int c = 0; c < 6; ++c) traj[((int64_t)k * n + i) * 6 + c] = w0[6 * i + c];
+    if (s < nsteps) accel(hp, src, x, s, t0 + s * dt, a);
+    for (int k = s; k < nsteps; ++k) {
+      double h = 0.5 * dt;
+      for (int c = 0; c < 3; ++c) v[c] = v[c] + h * a[c];
+      for (int c = 0; c < 3; ++c) x[c] = x[c] + dt * v[c];
+      accel(hp, src, x, k + 1, t0 + (k + 1) * dt, a);
+      for (int c = 0; c < 3; ++c) v[c] = v[c] + h * a[c];
+      if (traj)
+        for (int c = 0; c < 3; ++c) {
+          traj[((int64_t)(k + 1) * n + i) * 6 + c] = x[c];
+          traj[((int64_t)(k + 1) * n + i) * 6 + 3 + c] = v[c];
+        }
+    }
+    for (int c = 0; c < 3; ++c) { w_out[6 * i + c] = x[c]; w_out[6 * i + 3 + c] = v[c]; }
+  }
+  return 0;
+}
+
+// Spray release conditions (reference gen_stream_ics, main.py:287-306).
+int orc_release(const ssc_host_potential* hp, const double* prog_w, const double* ts,
+                const double* g_msat, int32_t nrel, int64_t seed, const double* kvals,
+                double* pos_close, double* pos_far, double* vel_close, double* vel_far) {
+  const double* kv = kvals ? kvals : kDefaultK;
+  std::vector<double> hostp(4 * hp->n_components);
+  for (int c = 0; c < hp->n_components; ++c)
+    for (int k = 0; k < 4; ++k) hostp[4 * c + k] = hp->c[c].p[k];
+  (void)ts;
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < nrel; ++i) {
+    double z[4], out[12];
+    release_normals(seed, i, z);
+    release_T<double>(hp, hostp.data(), prog_w + 6 * i, g_msat[i], z, kv, out);
+    for (int k = 0; k < 3; ++k) {
+      pos_close[3 * i + k] = out[k];
+      pos_far[3 * i + k] = out[3 + k];
+      vel_close[3 * i + k] = out[6 + k];
+      vel_far[3 * i + k] = out[9 + k];
+    }
+  }
+  return 0;
+}
+
+// Jacobian of the release model w.r.t. (x, v) of the progenitor and the Gm / scale parameter
+// (p[0], p[1]) of every host component: jac[nrel][12][6 + 2*n_comp] (reference
+// release_func_jacobian, perturbative.py:282-296, extended with host parameters).
+int orc_release_jacobian(const ssc_host_potential* hp, const double* prog_w, const double* g_msat,
+                         int32_t nrel, int64_t seed, const double* kvals, double* jac) {
+  const double* kv = kvals ? kvals : kDefaultK;
+  const int nc = hp->n_components;
+  constexpr int ND = 6 + 2 * SSC_MAX_COMPONENTS;
+  const int nin = 6 + 2 * nc;
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < nrel; ++i) {
+    using D = Dual<ND>;
+    double z[4];
+    release_normals(seed, i, z);
+    D in[6], hostp[4 * SSC_MAX_COMPONENTS], out[12];
+    for (int k = 0; k < 6; ++k) { in[k] = D(prog_w[6 * i + k]); in[k].d[k] = 1.0; }
+    for (int c = 0; c < nc; ++c)
+      for (int k = 0; k < 4; ++k) {
+        hostp[4 * c + k] = D(hp->c[c].p[k]);
+        if (k < 2) hostp[4 * c + k].d[6 + 2 * c + k] = 1.0;
+      }
+    release_T<D>(hp, hostp, in, D(g_msat[i]), z, kv, out);
+    for (int o = 0; o < 12; ++o)
+      for (int k = 0; k < nin; ++k) jac[((int64_t)i * 12 + o) * nin + k] = out[o].d[k];
+  }
+  return 0;
+}
+
+}  // extern "C"

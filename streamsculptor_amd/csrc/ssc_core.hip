@@ -1,0 +1,374 @@
+// libssc_mi355x core: error plumbing, potential evaluation and the fixed-step leapfrog
+// stream kernel (one particle per lane, FP64 state in VGPRs, Plummer source tables in LDS).
+#include <hip/hip_runtime.h>
+
+#include <climits>
+#include <cstdio>
+#include <cstring>
+#include <string>
+
+#include "ssc_device.h"
+
+namespace ssc {
+
+static thread_local std::string g_last_error;
+
+void set_error(const std::string& msg) { g_last_error = msg; }
+
+int fail(int code, const std::string& msg) {
+  g_last_error = msg;
+  return code;
+}
+
+int check_launch(const char* what) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess)
+    return fail(SSC_E_LAUNCH, std::string(what) + ": kernel launch failed: " + hipGetErrorString(e));
+  return SSC_OK;
+}
+
+int validate_host(const ssc_host_potential* host) {
+  if (!host) return fail(SSC_E_ARG, "host potential descriptor is NULL");
+  if (host->n_components < 0 || host->n_components > SSC_MAX_COMPONENTS)
+    return fail(SSC_E_ARG, "host potential: n_components out of range [0, 8]");
+  for (int c = 0; c < host->n_components; ++c) {
+    const int k = host->c[c].kind;
+    if (k < SSC_COMP_MIYAMOTO_NAGAI || k > SSC_COMP_PLUMMER)
+      return fail(SSC_E_ARG, "host potential: unknown component kind " + std::to_string(k));
+  }
+  return SSC_OK;
+}
+
+int validate_sources(const ssc_sources* src, int32_t nsteps_needed) {
+  if (!src || src->kind == SSC_SRC_NONE || src->n == 0) return SSC_OK;
+  if (src->n < 0 || src->n > SSC_MAX_SOURCES)
+    return fail(SSC_E_LIMIT, "sources: n out of range [0, " + std::to_string(SSC_MAX_SOURCES) + "]");
+  if (!src->gm || !src->rs || !src->pos) return fail(SSC_E_ARG, "sources: NULL gm/rs/pos");
+  if (src->kind == SSC_SRC_TRACK && src->n_track < nsteps_needed + 1)
+    return fail(SSC_E_ARG, "sources: TRACK table has " + std::to_string(src->n_track) +
+                               " rows, the integration needs " + std::to_string(nsteps_needed + 1));
+  if (src->kind == SSC_SRC_LINE && (!src->vel || !src->t0 || !src->t_window))
+    return fail(SSC_E_ARG, "sources: LINE needs vel, t0 and t_window");
+  if (src->kind < SSC_SRC_NONE || src->kind > SSC_SRC_LINE)
+    return fail(SSC_E_ARG, "sources: unknown kind " + std::to_string(src->kind));
+  return SSC_OK;
+}
+
+static ssc_sources empty_sources() {
+  ssc_sources s;
+  memset(&s, 0, sizeof(s));
+  return s;
+}
+
+// ------------------------------------------------------------------------------------------
+// ssc_eval: one point per lane
+// ------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void eval_kernel(ssc_host_potential hp, ssc_sources src,
+                                                   const double* __restrict__ xyz, int64_t n,
+                                                   double t, int step, double* __restrict__ phi_out,
+                                                   double* __restrict__ grad_out,
+                                                   double* __restrict__ hess_out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double x = xyz[3 * i], y = xyz[3 * i + 1], z = xyz[3 * i + 2];
+  double phi, g[3], H[9];
+  host_full(hp, x, y, z, phi, g, H);
+  if (src.kind != SSC_SRC_NONE) {
+    for (int j = 0; j < src.n; ++j) {
+      double sx, sy, sz;
+      if (!source_position(src, j, step, t, sx, sy, sz)) continue;
+      const double rs = src.rs[j];
+      plummer_full(x - sx, y - sy, z - sz, src.gm[j], rs * rs, phi, g, H);
+    }
+  }
+  if (phi_out) phi_out[i] = phi;
+  if (grad_out) {
+    grad_out[3 * i] = g[0];
+    grad_out[3 * i + 1] = g[1];
+    grad_out[3 * i + 2] = g[2];
+  }
+  if (hess_out) {
+#pragma unroll
+    for (int k = 0; k < 9; ++k) hess_out[9 * i + k] = H[k];
+  }
+}
+
+// ------------------------------------------------------------------------------------------
+// Leapfrog: all lanes of a block walk the global grid together (k = block's earliest start ..
+// nsteps-1); a lane kicks/drifts only once k >= its start.  Source tables live in LDS:
+//   STATIC: rec[j] = (x, y, z, G*m), rs2[j]           (loaded once)
+//   TRACK:  rows kb..kb+CH of the trajectory table, rec[r][j] = (x, y, z, G*m)
+//   LINE:   (x0, v, t0, t_window) per source, position formed per step
+// ------------------------------------------------------------------------------------------
+constexpr int LF_BLOCK = 256;
+
+struct alignas(16) SrcRec {
+  double x, y, z, gm;
+};
+
+template <int SK>
+struct SourceLDS;
+
+// No sources.
+template <>
+struct SourceLDS<SSC_SRC_NONE> {
+  __device__ void init(const ssc_sources&, char*) {}
+  __device__ void stage(const ssc_sources&, int, int) {}
+  __device__ void add_grad(const ssc_sources&, int, double, double, double, double, double&,
+                           double&, double&) const {}
+  static size_t bytes(const ssc_sources&, int) { return 0; }
+};
+
+template <>
+struct SourceLDS<SSC_SRC_STATIC> {
+  SrcRec* rec;
+  double* rs2;
+  int n;
+  static size_t bytes(const ssc_sources& s, int) { return (size_t)s.n * (sizeof(SrcRec) + 8); }
+  __device__ void init(const ssc_sources& s, char* lds) {
+    n = s.n;
+    rec = reinterpret_cast<SrcRec*>(lds);
+    rs2 = reinterpret_cast<double*>(lds + (size_t)n * sizeof(SrcRec));
+    for (int j = threadIdx.x; j < n; j += blockDim.x) {
+      rec[j] = SrcRec{s.pos[3 * j], s.pos[3 * j + 1], s.pos[3 * j + 2], s.gm[j]};
+      rs2[j] = s.rs[j] * s.rs[j];
+    }
+    __syncthreads();
+  }
+  __device__ void stage(const ssc_sources&, int, int) {}
+  __device__ void add_grad(const ssc_sources&, int, double, double x, double y, double z,
+                           double& gx, double& gy, double& gz) const {
+    for (int j = 0; j < n; ++j) {
+      const SrcRec r = rec[j];
+      plummer_grad(x - r.x, y - r.y, z - r.z, r.gm, rs2[j], gx, gy, gz);
+    }
+  }
+};
+
+// Grid steps staged per LDS fill for TRACK sources: (chunk + 1) rows of n records stay under
+// 64 KiB (16 steps for the 64-subhalo configuration, 1 step at 1024 sources).
+__host__ __device__ inline int track_chunk(int n) {
+  int c = 2048 / (n > 0 ? n : 1) - 1;
+  return c < 1 ? 1 : (c > 16 ? 16 : c);
+}
+
+template <>
+struct SourceLDS<SSC_SRC_TRACK> {
+  SrcRec* rec;  // [chunk + 1][n]
+  double* rs2;  // [n]
+  int n;
+  int chunk;
+  int kb;       // grid row held in rec[0]
+  static size_t bytes(const ssc_sources& s, int) {
+    return (size_t)(track_chunk(s.n) + 1) * s.n * sizeof(SrcRec) + (size_t)s.n * 8;
+  }
+  __device__ void init(const ssc_sources& s, char* lds) {
+    n = s.n;
+    chunk = track_chunk(n);
+    rec = reinterpret_cast<SrcRec*>(lds);
+    rs2 = reinterpret_cast<double*>(lds + (size_t)(chunk + 1) * n * sizeof(SrcRec));
+    for (int j = threadIdx.x; j < n; j += blockDim.x) rs2[j] = s.rs[j] * s.rs[j];
+    kb = INT_MIN;
+  }
+  // Make rows k and k+1 resident (block-uniform k; contains barriers).
+  __device__ void stage(const ssc_sources& s, int k, int nrows) {
+    if (kb != INT_MIN && k >= kb && k + 1 <= kb + chunk) return;
+    __syncthreads();
+    kb = k;
+    const int rows = min(chunk + 1, nrows - k);
+    const int tot = rows * n;
+    for (int e = threadIdx.x; e < tot; e += blockDim.x) {
+      const int r = e / n, j = e - r * n;
+      const double* p = s.pos + ((int64_t)(k + r) * n + j) * 3;
+      rec[r * n + j] = SrcRec{p[0], p[1], p[2], s.gm[j]};
+    }
+    __syncthreads();
+  }
+  __device__ void add_grad(const ssc_sources&, int k, double, double x, double y, double z,
+                           double& gx, double& gy, double& gz) const {
+    const SrcRec* row = rec + (k - kb) * n;
+    for (int j = 0; j < n; ++j) {
+      const SrcRec r = row[j];
+      plummer_grad(x - r.x, y - r.y, z - r.z, r.gm, rs2[j], gx, gy, gz);
+    }
+  }
+};
+
+struct alignas(16) LineRec {
+  double x0, y0, z0, gm, vx, vy, vz, rs2, t0, tw;
+};
+
+template <>
+struct SourceLDS<SSC_SRC_LINE> {
+  LineRec* rec;
+  int n;
+  static size_t bytes(const ssc_sources& s, int) { return (size_t)s.n * sizeof(LineRec); }
+  __device__ void init(const ssc_sources& s, char* lds) {
+    n = s.n;
+    rec = reinterpret_cast<LineRec*>(lds);
+    for (int j = threadIdx.x; j < n; j += blockDim.x)
+      rec[j] = LineRec{s.pos[3 * j], s.pos[3 * j + 1], s.pos[3 * j + 2], s.gm[j],
+                       s.vel[3 * j], s.vel[3 * j + 1], s.vel[3 * j + 2], s.rs[j] * s.rs[j],
+                       s.t0[j], s.t_window[j]};
+    __syncthreads();
+  }
+  __device__ void stage(const ssc_sources&, int, int) {}
+  __device__ void add_grad(const ssc_sources&, int, double t, double x, double y, double z,
+                           double& gx, double& gy, double& gz) const {
+    for (int j = 0; j < n; ++j) {
+      const LineRec r = rec[j];
+      const double dt = t - r.t0;
+      if (!(fabs(dt) < r.tw)) continue;
+      plummer_grad(x - (r.x0 + r.vx * dt), y - (r.y0 + r.vy * dt), z - (r.z0 + r.vz * dt), r.gm,
+                   r.rs2, gx, gy, gz);
+    }
+  }
+};
+
+template <int SK>
+__device__ __forceinline__ void accel_at(const ssc_host_potential& hp, const ssc_sources& src,
+                                         const SourceLDS<SK>& lds, int k, double t, double x,
+                                         double y, double z, double& ax, double& ay, double& az) {
+  double gx, gy, gz;
+  host_grad(hp, x, y, z, gx, gy, gz);
+  lds.add_grad(src, k, t, x, y, z, gx, gy, gz);
+  ax = -gx;
+  ay = -gy;
+  az = -gz;
+}
+
+template <int SK>
+__global__ __launch_bounds__(LF_BLOCK) void leapfrog_kernel(
+    ssc_host_potential hp, ssc_sources src, const double* __restrict__ w0,
+    const int32_t* __restrict__ start, int64_t n, double t0, double dt, int nsteps,
+    double* __restrict__ w_out, const int32_t* __restrict__ save_idx, int nsave,
+    double* __restrict__ ys) {
+  extern __shared__ __align__(16) char lds_raw[];
+  __shared__ int kmin_sh;
+  const int64_t i = (int64_t)blockIdx.x * LF_BLOCK + threadIdx.x;
+  const bool valid = i < n;
+  int s = nsteps;
+  double x = 0, y = 0, z = 0, vx = 0, vy = 0, vz = 0;
+  if (valid) {
+    s = start ? start[i] : 0;
+    s = s < 0 ? 0 : (s > nsteps ? nsteps : s);
+    const double* w = w0 + 6 * i;
+    x = w[0]; y = w[1]; z = w[2]; vx = w[3]; vy = w[4]; vz = w[5];
+  }
+  if (threadIdx.x == 0) kmin_sh = INT_MAX;
+  __syncthreads();
+  atomicMin(&kmin_sh, s);
+  SourceLDS<SK> lds;
+  lds.init(src, lds_raw);
+  __syncthreads();
+  const int kmin = kmin_sh;
+  // Save rows at grid indices <= kmin hold the initial state of every lane.
+  int next = 0;
+  while (next < nsave && save_idx[next] <= kmin) {
+    if (valid) {
+      double* o = ys + ((int64_t)next * n + i) * 6;
+      o[0] = x; o[1] = y; o[2] = z; o[3] = vx; o[4] = vy; o[5] = vz;
+    }
+    ++next;
+  }
+  const double h = 0.5 * dt;
+  double ax = 0, ay = 0, az = 0;
+  for (int k = kmin; k < nsteps; ++k) {
+    lds.stage(src, k, nsteps + 1);
+    if (valid && k == s) accel_at<SK>(hp, src, lds, k, t0 + k * dt, x, y, z, ax, ay, az);
+    if (valid && k >= s) {
+      vx = vx + h * ax; vy = vy + h * ay; vz = vz + h * az;
+      x = x + dt * vx; y = y + dt * vy; z = z + dt * vz;
+      accel_at<SK>(hp, src, lds, k + 1, t0 + (k + 1) * dt, x, y, z, ax, ay, az);
+      vx = vx + h * ax; vy = vy + h * ay; vz = vz + h * az;
+    }
+    // block-uniform: every lane is now at grid index k + 1 (or still holds its initial row)
+    while (next < nsave && save_idx[next] == k + 1) {
+      if (valid) {
+        double* o = ys + ((int64_t)next * n + i) * 6;
+        o[0] = x; o[1] = y; o[2] = z; o[3] = vx; o[4] = vy; o[5] = vz;
+      }
+      ++next;
+    }
+  }
+  if (valid) {
+    double* o = w_out + 6 * i;
+    o[0] = x; o[1] = y; o[2] = z; o[3] = vx; o[4] = vy; o[5] = vz;
+  }
+}
+
+template <int SK>
+static int launch_leapfrog(const ssc_host_potential& hp, const ssc_sources& src, const double* w0,
+                           const int32_t* start, int64_t n, double t0, double dt, int nsteps,
+                           double* w_out, const int32_t* save_idx, int nsave, double* ys,
+                           hipStream_t st) {
+  const size_t lds = SourceLDS<SK>::bytes(src, nsteps);
+  if (lds > 160 * 1024) return fail(SSC_E_LIMIT, "ssc_leapfrog: source table exceeds LDS");
+  if (lds > 60 * 1024) {
+    hipError_t e = hipFuncSetAttribute((const void*)leapfrog_kernel<SK>,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return fail(SSC_E_HIP, std::string("hipFuncSetAttribute: ") + hipGetErrorString(e));
+  }
+  const int64_t blocks = (n + LF_BLOCK - 1) / LF_BLOCK;
+  hipLaunchKernelGGL(leapfrog_kernel<SK>, dim3((unsigned)blocks), dim3(LF_BLOCK), lds, st, hp,
+                     src, w0, start, n, t0, dt, nsteps, w_out, save_idx, nsave, ys);
+  return check_launch("ssc_leapfrog");
+}
+
+}  // namespace ssc
+
+using namespace ssc;
+
+extern "C" {
+
+const char* ssc_last_error(void) { return g_last_error.c_str(); }
+
+const char* ssc_version(void) { return "libssc_mi355x 0.1 (gfx950)"; }
+
+int ssc_eval(const ssc_host_potential* host, const ssc_sources* src, const double* xyz, int64_t n,
+             double t, int32_t step, double* phi, double* grad, double* hess, void* stream) {
+  int rc = validate_host(host);
+  if (rc) return rc;
+  if (n < 0) return fail(SSC_E_ARG, "ssc_eval: n < 0");
+  if (n == 0) return SSC_OK;
+  if (!xyz) return fail(SSC_E_ARG, "ssc_eval: xyz is NULL");
+  rc = validate_sources(src, step < 0 ? 0 : step);
+  if (rc) return rc;
+  ssc_sources s = src ? *src : empty_sources();
+  if (s.n == 0) s.kind = SSC_SRC_NONE;
+  const int64_t blocks = (n + 255) / 256;
+  hipLaunchKernelGGL(eval_kernel, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream, *host,
+                     s, xyz, n, t, step, phi, grad, hess);
+  return check_launch("ssc_eval");
+}
+
+int ssc_leapfrog(const ssc_host_potential* host, const ssc_sources* src, const double* w0,
+                 const int32_t* start_step, int64_t n, double t0, double dt, int32_t nsteps,
+                 double* w_out, const int32_t* save_idx, int32_t nsave, double* ys,
+                 void* stream) {
+  int rc = validate_host(host);
+  if (rc) return rc;
+  if (n < 0 || nsteps < 0 || nsave < 0) return fail(SSC_E_ARG, "ssc_leapfrog: negative size");
+  if (n == 0) return SSC_OK;
+  if (!w0 || !w_out) return fail(SSC_E_ARG, "ssc_leapfrog: w0 or w_out is NULL");
+  if (nsave > 0 && (!save_idx || !ys)) return fail(SSC_E_ARG, "ssc_leapfrog: nsave > 0 needs save_idx and ys");
+  if (nsave == 0) { save_idx = nullptr; ys = nullptr; }
+  if ((n + LF_BLOCK - 1) / LF_BLOCK > (int64_t)INT_MAX)
+    return fail(SSC_E_LIMIT, "ssc_leapfrog: too many particles");
+  rc = validate_sources(src, nsteps);
+  if (rc) return rc;
+  ssc_sources s = src ? *src : empty_sources();
+  if (s.n == 0) s.kind = SSC_SRC_NONE;
+  hipStream_t st = (hipStream_t)stream;
+#define SSC_LF(K) launch_leapfrog<K>(*host, s, w0, start_step, n, t0, dt, nsteps, w_out, save_idx, nsave, ys, st)
+  switch (s.kind) {
+    case SSC_SRC_NONE: return SSC_LF(SSC_SRC_NONE);
+    case SSC_SRC_STATIC: return SSC_LF(SSC_SRC_STATIC);
+    case SSC_SRC_TRACK: return SSC_LF(SSC_SRC_TRACK);
+    case SSC_SRC_LINE: return SSC_LF(SSC_SRC_LINE);
+    default: return fail(SSC_E_ARG, "ssc_leapfrog: bad source kind");
+  }
+#undef SSC_LF
+}
+
+}  // extern "C"

@@ -1,0 +1,266 @@
+// Device-side building blocks for the gfx950 stream-particle kernels: analytic host
+// components (gradient and Hessian), Plummer source terms, and the launch/error plumbing
+// shared by every translation unit of libssc_mi355x.so.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string>
+
+#include "../../include/ssc_mi355x.h"
+
+namespace ssc {
+
+// ------------------------------------------------------------------------------------------
+// error plumbing (host)
+// ------------------------------------------------------------------------------------------
+void set_error(const std::string& msg);
+int fail(int code, const std::string& msg);
+int check_launch(const char* what);
+int validate_host(const ssc_host_potential* host);
+int validate_sources(const ssc_sources* src, int32_t nsteps_needed);
+
+// ------------------------------------------------------------------------------------------
+// small vector helpers
+// ------------------------------------------------------------------------------------------
+struct d3 {
+  double x, y, z;
+};
+
+__device__ __forceinline__ double rsq(double v) { return rsqrt(v); }
+
+// ------------------------------------------------------------------------------------------
+// Host components, gradient only (the per-step force).  Formulas: reference
+// streamsculptor/potential.py:66-84 (MN, NFW), :124-138 (Plummer, Hernquist), differentiated
+// in closed form.
+// ------------------------------------------------------------------------------------------
+__device__ __forceinline__ void comp_grad(const ssc_component& c, double x, double y, double z,
+                                          double& gx, double& gy, double& gz) {
+  const double gm = c.p[0];
+  switch (c.kind) {
+    case SSC_COMP_MIYAMOTO_NAGAI: {
+      const double a = c.p[1], b = c.p[2];
+      const double zeta = sqrt(z * z + b * b);
+      const double S = a + zeta;
+      const double D2 = x * x + y * y + S * S;
+      const double iD = rsq(D2);
+      const double f = gm * iD * iD * iD;
+      gx += f * x;
+      gy += f * y;
+      gz += f * z * S / zeta;
+      break;
+    }
+    case SSC_COMP_NFW: {
+      const double rs = c.p[1];
+      const double r2 = x * x + y * y + z * z;
+      const double ir = rsq(r2);
+      const double r = r2 * ir;
+      const double L = log1p(r / rs);
+      const double f = gm * ir * ir * (L * ir - 1.0 / (r + rs));
+      gx += f * x;
+      gy += f * y;
+      gz += f * z;
+      break;
+    }
+    case SSC_COMP_HERNQUIST: {
+      const double cc = c.p[1], soft = c.p[2];
+      const double r2 = x * x + y * y + z * z + soft;
+      const double ir = rsq(r2);
+      const double r = r2 * ir;
+      const double rc = r + cc;
+      const double f = gm * ir / (rc * rc);
+      gx += f * x;
+      gy += f * y;
+      gz += f * z;
+      break;
+    }
+    case SSC_COMP_PLUMMER: {
+      const double rs = c.p[1];
+      const double s2 = x * x + y * y + z * z + rs * rs;
+      const double is = rsq(s2);
+      const double f = gm * is * is * is;
+      gx += f * x;
+      gy += f * y;
+      gz += f * z;
+      break;
+    }
+    default:
+      break;
+  }
+}
+
+__device__ __forceinline__ void host_grad(const ssc_host_potential& hp, double x, double y,
+                                          double z, double& gx, double& gy, double& gz) {
+  gx = 0.0;
+  gy = 0.0;
+  gz = 0.0;
+  for (int c = 0; c < hp.n_components; ++c) comp_grad(hp.c[c], x, y, z, gx, gy, gz);
+}
+
+// Spherical Hessian accumulation: H += (phi'/r) I + (phi'' - phi'/r) x x^T / r^2.
+__device__ __forceinline__ void sph_hess(double x, double y, double z, double d1r, double c,
+                                         double* H) {
+  H[0] += d1r + c * x * x;
+  H[1] += c * x * y;
+  H[2] += c * x * z;
+  H[4] += d1r + c * y * y;
+  H[5] += c * y * z;
+  H[8] += d1r + c * z * z;
+}
+
+// Potential, gradient and Hessian of one component (H upper triangle in H[0,1,2,4,5,8]).
+__device__ __forceinline__ void comp_full(const ssc_component& c, double x, double y, double z,
+                                          double& phi, double* g, double* H) {
+  const double gm = c.p[0];
+  switch (c.kind) {
+    case SSC_COMP_MIYAMOTO_NAGAI: {
+      const double a = c.p[1], b = c.p[2];
+      const double zeta = sqrt(z * z + b * b);
+      const double S = a + zeta;
+      const double D2 = x * x + y * y + S * S;
+      const double D = sqrt(D2);
+      const double iD3 = 1.0 / (D2 * D);
+      const double iD5 = iD3 / D2;
+      const double dz = z * S / zeta;
+      phi -= gm / D;
+      g[0] += gm * x * iD3;
+      g[1] += gm * y * iD3;
+      g[2] += gm * dz * iD3;
+      const double k = -3.0 * gm * iD5;
+      const double ddz = S * b * b / (zeta * zeta * zeta) + z * z / (zeta * zeta);
+      H[0] += k * x * x + gm * iD3;
+      H[1] += k * x * y;
+      H[2] += k * x * dz;
+      H[4] += k * y * y + gm * iD3;
+      H[5] += k * y * dz;
+      H[8] += k * dz * dz + gm * ddz * iD3;
+      break;
+    }
+    case SSC_COMP_NFW: {
+      const double rs = c.p[1];
+      const double r = sqrt(x * x + y * y + z * z);
+      const double L = log1p(r / rs);
+      const double rp = r + rs;
+      phi -= gm * L / r;
+      const double d1 = gm * (L / (r * r) - 1.0 / (r * rp));
+      const double d2 = gm * (1.0 / (r * r * rp) - 2.0 * L / (r * r * r) +
+                              (2.0 * r + rs) / (r * r * rp * rp));
+      const double d1r = d1 / r;
+      g[0] += d1r * x;
+      g[1] += d1r * y;
+      g[2] += d1r * z;
+      sph_hess(x, y, z, d1r, (d2 - d1r) / (r * r), H);
+      break;
+    }
+    case SSC_COMP_HERNQUIST: {
+      const double cc = c.p[1], soft = c.p[2];
+      const double r = sqrt(x * x + y * y + z * z + soft);
+      const double rc = r + cc;
+      phi -= gm / rc;
+      const double d1 = gm / (rc * rc);
+      const double d2 = -2.0 * gm / (rc * rc * rc);
+      const double d1r = d1 / r;
+      g[0] += d1r * x;
+      g[1] += d1r * y;
+      g[2] += d1r * z;
+      sph_hess(x, y, z, d1r, (d2 - d1r) / (r * r), H);
+      break;
+    }
+    case SSC_COMP_PLUMMER: {
+      const double rs = c.p[1];
+      const double s2 = x * x + y * y + z * z + rs * rs;
+      const double s = sqrt(s2);
+      const double i3 = 1.0 / (s2 * s);
+      phi -= gm / s;
+      const double f = gm * i3;
+      g[0] += f * x;
+      g[1] += f * y;
+      g[2] += f * z;
+      const double k = -3.0 * f / s2;
+      H[0] += f + k * x * x;
+      H[1] += k * x * y;
+      H[2] += k * x * z;
+      H[4] += f + k * y * y;
+      H[5] += k * y * z;
+      H[8] += f + k * z * z;
+      break;
+    }
+    default:
+      break;
+  }
+}
+
+__device__ __forceinline__ void host_full(const ssc_host_potential& hp, double x, double y,
+                                          double z, double& phi, double* g, double* H) {
+  phi = 0.0;
+  g[0] = g[1] = g[2] = 0.0;
+#pragma unroll
+  for (int i = 0; i < 9; ++i) H[i] = 0.0;
+  for (int c = 0; c < hp.n_components; ++c) comp_full(hp.c[c], x, y, z, phi, g, H);
+  H[3] = H[1];
+  H[6] = H[2];
+  H[7] = H[5];
+}
+
+// Plummer source (x - s): gradient contribution, the N x M hot loop body.
+__device__ __forceinline__ void plummer_grad(double dx, double dy, double dz, double gm,
+                                             double rs2, double& gx, double& gy, double& gz) {
+  const double s2 = dx * dx + dy * dy + dz * dz + rs2;
+  const double is = rsq(s2);
+  const double f = gm * is * is * is;
+  gx += f * dx;
+  gy += f * dy;
+  gz += f * dz;
+}
+
+// Plummer source: potential, gradient and full Hessian contribution.
+__device__ __forceinline__ void plummer_full(double dx, double dy, double dz, double gm,
+                                             double rs2, double& phi, double* g, double* H) {
+  const double s2 = dx * dx + dy * dy + dz * dz + rs2;
+  const double is = rsq(s2);
+  const double f = gm * is * is * is;
+  phi -= gm * is;
+  g[0] += f * dx;
+  g[1] += f * dy;
+  g[2] += f * dz;
+  const double k = -3.0 * f * is * is;
+  H[0] += f + k * dx * dx;
+  H[1] += k * dx * dy;
+  H[2] += k * dx * dz;
+  H[3] += k * dx * dy;
+  H[4] += f + k * dy * dy;
+  H[5] += k * dy * dz;
+  H[6] += k * dx * dz;
+  H[7] += k * dy * dz;
+  H[8] += f + k * dz * dz;
+}
+
+// Position of source j at grid row `step` / time t; false when inactive (LINE window,
+// reference streamsculptor/potential.py:826).
+__device__ __forceinline__ bool source_position(const ssc_sources& s, int j, int step, double t,
+                                                double& sx, double& sy, double& sz) {
+  if (s.kind == SSC_SRC_STATIC) {
+    sx = s.pos[3 * j];
+    sy = s.pos[3 * j + 1];
+    sz = s.pos[3 * j + 2];
+    return true;
+  }
+  if (s.kind == SSC_SRC_TRACK) {
+    const double* row = s.pos + ((int64_t)step * s.n + j) * 3;
+    sx = row[0];
+    sy = row[1];
+    sz = row[2];
+    return true;
+  }
+  if (s.kind == SSC_SRC_LINE) {
+    const double dt = t - s.t0[j];
+    if (!(fabs(dt) < s.t_window[j])) return false;
+    sx = s.pos[3 * j] + s.vel[3 * j] * dt;
+    sy = s.pos[3 * j + 1] + s.vel[3 * j + 1] * dt;
+    sz = s.pos[3 * j + 2] + s.vel[3 * j + 2] * dt;
+    return true;
+  }
+  return false;
+}
+
+}  // namespace ssc

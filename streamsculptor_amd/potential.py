@@ -1,0 +1,219 @@
+"""Potential zoo on the stream-particle path (reference streamsculptor/potential.py).
+
+Analytic classes lower to kernel components; point-source classes lower to a Plummer
+``SourceSet``.  Parameters follow the reference constructors; masses are in Msun and enter the
+kernels as G*m.
+"""
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+from . import _lib
+from .main import Potential, SourceSet, usys, _as_float
+
+
+def _arr(x):
+    return np.asarray(_as_float(x), dtype=np.float64)
+
+
+class MiyamotoNagaiDisk(Potential):
+    """Reference streamsculptor/potential.py:66-72."""
+
+    def __init__(self, m, a, b, units=None):
+        super().__init__(units, {"m": m, "a": a, "b": b})
+
+    def _kernel_components(self):
+        return [(_lib.COMP_MIYAMOTO_NAGAI, [self._G * float(self.m), float(self.a), float(self.b)])]
+
+
+class NFWPotential(Potential):
+    """Reference streamsculptor/potential.py:74-84 (no softening: singular at r = 0)."""
+
+    def __init__(self, m, r_s, units=None):
+        super().__init__(units, {"m": m, "r_s": r_s})
+
+    def _kernel_components(self):
+        return [(_lib.COMP_NFW, [self._G * float(self.m), float(self.r_s)])]
+
+
+class PlummerPotential(Potential):
+    """Reference streamsculptor/potential.py:124-130."""
+
+    def __init__(self, m, r_s, units=None):
+        super().__init__(units, {"m": m, "r_s": r_s})
+
+    def _kernel_components(self):
+        return [(_lib.COMP_PLUMMER, [self._G * float(self.m), float(self.r_s)])]
+
+
+class HernquistPotential(Potential):
+    """Reference streamsculptor/potential.py:132-138; soft is added to r^2."""
+
+    def __init__(self, m, r_s, soft=0.0, units=None):
+        super().__init__(units, {"m": m, "r_s": r_s, "soft": soft})
+
+    def _kernel_components(self):
+        return [(_lib.COMP_HERNQUIST, [self._G * float(self.m), float(self.r_s), float(self.soft)])]
+
+
+class Potential_Combine(Potential):
+    """Sum of potentials (reference streamsculptor/potential.py:1279-1296)."""
+
+    def __init__(self, potential_list, units=None):
+        super().__init__(units, {"potential_list": potential_list})
+
+    def _kernel_components(self):
+        comps = []
+        for p in self.potential_list:
+            comps += p._kernel_components()
+        return comps
+
+    def _kernel_sources(self):
+        from .main import _merge_sources
+        return _merge_sources([p._kernel_sources() for p in self.potential_list])
+
+
+class GalaMilkyWayPotential(Potential):
+    """Disk + bulge + nucleus + halo (reference streamsculptor/potential.py:390-414)."""
+
+    def __init__(self, units=None):
+        super().__init__(units, {"params": None})
+        self.m_disk, self.a_disk, self.b_disk = 6.80e10, 3.0, 0.28
+        self.m_bulge, self.c_bulge = 5e9, 1.0
+        self.m_nucleus, self.c_nucleus = 1.71e9, 0.07
+        self.m_halo, self.r_s_halo = 5.4e11, 15.62
+        self.pot = Potential_Combine([
+            MiyamotoNagaiDisk(self.m_disk, self.a_disk, self.b_disk, units=self.units),
+            HernquistPotential(self.m_bulge, self.c_bulge, units=self.units),
+            HernquistPotential(self.m_nucleus, self.c_nucleus, units=self.units),
+            NFWPotential(self.m_halo, self.r_s_halo, units=self.units)], units=self.units)
+
+    def _kernel_components(self):
+        return self.pot._kernel_components()
+
+
+class CustomPotential(Potential):
+    """Arbitrary Python potential (reference streamsculptor/potential.py:787-799).  It cannot
+    be lowered to the gfx950 kernels: every kernel-backed method raises NotImplementedError."""
+
+    def __init__(self, potential_func=None, units=None):
+        super().__init__(units, {"potential_func": potential_func})
+
+
+class SubhaloLinePotential(Potential):
+    """Plummer subhalos on straight lines x0 + v (t - t0), active while |t - t0| < t_window
+    (reference streamsculptor/potential.py:802-830)."""
+
+    def __init__(self, m, a, subhalo_x0, subhalo_v, subhalo_t0, t_window, units=None):
+        super().__init__(units, {"m": m, "a": a, "subhalo_x0": subhalo_x0, "subhalo_v": subhalo_v,
+                                 "subhalo_t0": subhalo_t0, "t_window": t_window})
+
+    def _kernel_components(self):
+        return []
+
+    def _kernel_sources(self):
+        m = np.atleast_1d(_arr(self.m))
+        n = m.shape[0]
+        return SourceSet(_lib.SRC_LINE, gm=self._G * m,
+                         rs=np.broadcast_to(np.atleast_1d(_arr(self.a)), (n,)).copy(),
+                         pos=_arr(self.subhalo_x0).reshape(n, 3),
+                         vel=_arr(self.subhalo_v).reshape(n, 3),
+                         t0=np.broadcast_to(np.atleast_1d(_arr(self.subhalo_t0)), (n,)).copy(),
+                         t_window=np.broadcast_to(np.atleast_1d(_arr(self.t_window)), (n,)).copy())
+
+    def potential_per_SH(self, xyz, t):
+        """Per-subhalo potential at xyz (reference streamsculptor/potential.py:833-850)."""
+        return _per_source(self, xyz, t, "phi")
+
+
+class PlummerClumps(Potential):
+    """Static Plummer clumps summed directly (the clumpy-halo substructure of the surrogate
+    workload).  Equivalent to a SubhaloLinePotential with zero velocity and an infinite
+    window."""
+
+    def __init__(self, m, r_s, positions, units=None):
+        super().__init__(units, {"m": m, "r_s": r_s, "positions": positions})
+
+    def _kernel_components(self):
+        return []
+
+    def _kernel_sources(self):
+        m = np.atleast_1d(_arr(self.m))
+        n = m.shape[0]
+        return SourceSet(_lib.SRC_STATIC, gm=self._G * m,
+                         rs=np.broadcast_to(np.atleast_1d(_arr(self.r_s)), (n,)).copy(),
+                         pos=_arr(self.positions).reshape(n, 3))
+
+
+class LiveSubhaloPotential(Potential):
+    """Plummer subhalos moving on their own orbits, given as a track sampled on the integration
+    grid: ``track[k, j]`` is the position of subhalo j at t0 + k*dt.  The nearest reference
+    construct is a potential translating along an interpolated centre
+    (TimeDepTranslatingPotential, streamsculptor/potential.py:448-462)."""
+
+    def __init__(self, m, r_s, track, t0, dt, units=None):
+        super().__init__(units, {"m": m, "r_s": r_s, "track": track, "t0": t0, "dt": dt})
+
+    @classmethod
+    def from_orbits(cls, host, w0, m, r_s, t0, dt, nsteps, units=None):
+        """Integrate subhalo initial conditions w0 (M, 6) in ``host`` with the leapfrog kernel and
+        keep every grid row as the track."""
+        w0 = _lib.f64(w0).reshape(-1, 6)
+        _, ys = host._leapfrog(w0, t0, dt, nsteps, save_idx=np.arange(nsteps + 1, dtype=np.int32))
+        obj = cls(m, r_s, ys[:, :, :3].contiguous(), t0, dt, units=units or host.units)
+        obj.orbits = ys
+        obj.w0 = w0
+        return obj
+
+    def _kernel_components(self):
+        return []
+
+    def _kernel_sources(self):
+        m = self.m
+        gm = (self._G * m.to(torch.float64)) if isinstance(m, torch.Tensor) else self._G * np.atleast_1d(_arr(m))
+        n = gm.shape[0]
+        rs = self.r_s if isinstance(self.r_s, torch.Tensor) else np.broadcast_to(np.atleast_1d(_arr(self.r_s)), (n,)).copy()
+        return SourceSet(_lib.SRC_TRACK, gm=gm, rs=rs, pos=self.track,
+                         grid=(float(self.t0), float(self.dt)))
+
+    def potential_per_SH(self, xyz, t):
+        return _per_source(self, xyz, t, "phi")
+
+
+def _per_source(pot, xyz, t, what):
+    """Per-source potential (M,) or gradient (M, 3) at one point, one kernel call per source
+    set row (used by the perturbation helpers)."""
+    src = pot._kernel_sources()
+    tens = src.tensors()
+    n = tens["gm"].numel()
+    outs = []
+    for j in range(n):
+        sub = SourceSet(src.kind, gm=tens["gm"][j:j + 1],
+                        rs=tens["rs"][j:j + 1],
+                        pos=(tens["pos"][:, j:j + 1, :].contiguous() if src.kind == _lib.SRC_TRACK
+                             else tens["pos"][j:j + 1]),
+                        vel=None if tens["vel"] is None else tens["vel"][j:j + 1],
+                        t0=None if tens["t0"] is None else tens["t0"][j:j + 1],
+                        t_window=None if tens["t_window"] is None else tens["t_window"][j:j + 1],
+                        grid=src.grid)
+        single = _SourceOnly(sub, pot.units)
+        outs.append(single.potential(xyz, t) if what == "phi" else single.gradient(xyz, t))
+    return torch.stack(outs, dim=-1 if what == "phi" else -2)
+
+
+class _SourceOnly(Potential):
+    def __init__(self, src, units):
+        super().__init__(units, {})
+        self._src = src
+
+    def _kernel_components(self):
+        return []
+
+    def _kernel_sources(self):
+        return self._src
+
+
+__all__ = ["MiyamotoNagaiDisk", "NFWPotential", "PlummerPotential", "HernquistPotential",
+           "Potential_Combine", "GalaMilkyWayPotential", "CustomPotential",
+           "SubhaloLinePotential", "PlummerClumps", "LiveSubhaloPotential", "usys"]

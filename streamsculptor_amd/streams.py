@@ -1,0 +1,97 @@
+"""Spray-stream driver shared by ``Potential.gen_stream_vmapped`` (reference
+streamsculptor/main.py:343-368) and the perturbed-stream helpers (reference
+streamsculptor/streamhelpers.py:57-146).
+
+Three potentials take part, as in the reference: the progenitor orbit is integrated in
+``prog_pot``, release conditions come from ``release_pot`` (the smooth base potential in the
+perturbed variants, streamhelpers.py:58-66) and particles are integrated in ``particle_pot``.
+"""
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+from . import _lib
+from . import solvers as _solvers
+from ._grid import release_grid
+
+
+def stream_ics(prog_pot, release_pot, ts, prog_w0, Msat, seed_num, solver=None, kval_arr=1.0,
+               dt0=None, **adaptive_kw):
+    """(pos_close, pos_far, vel_close, vel_far), each (len(ts), 3), plus the grid used."""
+    method = _solvers.resolve(solver)
+    ts_np = np.asarray(ts.detach().cpu().numpy() if isinstance(ts, torch.Tensor) else ts,
+                       dtype=np.float64).reshape(-1)
+    if method == "leapfrog":
+        t0, dt, nsteps, kidx = release_grid(ts_np, dt0)
+        _, ys = prog_pot._leapfrog(_lib.f64(prog_w0).reshape(1, 6), t0, dt, nsteps, save_idx=kidx)
+        ws = ys[:, 0, :].contiguous()
+        grid = (t0, dt, nsteps, kidx)
+    else:
+        ws = prog_pot.integrate_orbit(w0=prog_w0, ts=ts_np, solver=solver, dt0=dt0,
+                                      **adaptive_kw).ys
+        grid = None
+    ics = release_pot._release_along(ws, len(ts_np), Msat, seed_num, kval_arr)
+    return ics, grid, ts_np
+
+
+def gen_stream(prog_pot, release_pot, ts, prog_w0, Msat, seed_num, solver=None, kval_arr=1.0,
+               rtol=1e-7, atol=1e-7, dtmin=0.3, dtmax=None, max_steps=10_000, throw=False,
+               dt0=None, particle_pot=None):
+    """Returns (lead (T, 6), trail (T, 6)) with T = len(ts) - 1."""
+    particle_pot = particle_pot or prog_pot
+    method = _solvers.resolve(solver)
+    adaptive_kw = dict(rtol=rtol, atol=atol, dtmin=dtmin, dtmax=dtmax, max_steps=max_steps)
+    (pc, pf, vc, vf), grid, ts_np = stream_ics(prog_pot, release_pot, ts, prog_w0, Msat, seed_num,
+                                               solver=solver, kval_arr=kval_arr, dt0=dt0,
+                                               **adaptive_kw)
+    T = len(ts_np) - 1
+    w0 = torch.empty(2 * T, 6, dtype=torch.float64, device=pc.device)
+    w0[0::2, :3], w0[0::2, 3:] = pc[:T], vc[:T]
+    w0[1::2, :3], w0[1::2, 3:] = pf[:T], vf[:T]
+    if method == "leapfrog":
+        t0, dt, nsteps, kidx = grid
+        start = np.repeat(kidx[:T], 2).astype(np.int32)
+        out, _ = particle_pot._leapfrog(w0, t0, dt, nsteps, start=start)
+    else:
+        from . import adaptive
+        t_start = np.repeat(ts_np[:T], 2)
+        out = adaptive.integrate_to(particle_pot, w0, t_start, float(ts_np[-1]), throw=throw,
+                                    **adaptive_kw)
+    return out[0::2], out[1::2]
+
+
+def spray_particles(prog_pot, release_pot, ts, prog_w0, Msat, seed_num, per_step, dt0,
+                    kval_arr=1.0):
+    """Initial rows of a many-particle spray stream with ``per_step`` particles released at
+    every ts[i], i < len(ts)-1: ceil(per_step/2) release-model draws (seeds seed_num,
+    seed_num+1, ...) per step, lead/trail alternating (workloads.spray_layout).  Returns
+    (w0 (N, 6), start_step (N,), arm (N,), grid) ordered by release step."""
+    from .workloads import spray_layout
+    ts_np = np.asarray(ts, dtype=np.float64).reshape(-1)
+    t0, dt, nsteps, kidx = release_grid(ts_np, dt0)
+    _, ys = prog_pot._leapfrog(_lib.f64(prog_w0).reshape(1, 6), t0, dt, nsteps, save_idx=kidx)
+    ws = ys[:, 0, :].contiguous()
+    T = len(ts_np) - 1
+    even, odd = spray_layout(per_step)
+    nseed = (per_step + 1) // 2
+    dev = ws.device
+    # table[s, arm, i, 6]
+    table = torch.empty(nseed, 2, T + 1, 6, dtype=torch.float64, device=dev)
+    for s in range(nseed):
+        pc, pf, vc, vf = release_pot._release_along(ws, T + 1, Msat, seed_num + s, kval_arr)
+        table[s, 0, :, :3], table[s, 0, :, 3:] = pc, vc
+        table[s, 1, :, :3], table[s, 1, :, 3:] = pf, vf
+    sel_s = np.empty((T, per_step), dtype=np.int64)
+    sel_a = np.empty((T, per_step), dtype=np.int64)
+    for i in range(T):
+        lay = even if i % 2 == 0 else odd
+        sel_s[i] = [p[0] for p in lay]
+        sel_a[i] = [p[1] for p in lay]
+    rel = np.repeat(np.arange(T), per_step)
+    s_idx = torch.as_tensor(sel_s.reshape(-1), device=dev)
+    a_idx = torch.as_tensor(sel_a.reshape(-1), device=dev)
+    r_idx = torch.as_tensor(rel, device=dev)
+    w0 = table[s_idx, a_idx, r_idx].contiguous()
+    start = kidx[rel].astype(np.int32)
+    return w0, start, sel_a.reshape(-1), (t0, dt, nsteps, kidx)

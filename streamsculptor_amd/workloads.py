@@ -1,0 +1,102 @@
+"""Synthetic inputs of the stream workloads (seeded numpy; no GPU needed to build them).
+
+* ``plain``       -- 3-component host (MN 6.8e10/3/0.28 + Hernquist 5e9/1 + NFW 5.4e11/15.62),
+                     progenitor today at (12, 0, 0) kpc, (0, 0.153, 0.051) kpc/Myr, Msat = 1e4,
+                     integrated back 4000 x 1 Myr; 2 spray particles per step (seed 0): 8000
+                     particles, 16,004,000 particle-steps.
+* ``subhalos``    -- 64 Plummer subhalos on their own host orbits: positions uniform on the
+                     r = 30 kpc sphere, isotropic Gaussian velocities (sigma = 0.15 kpc/Myr),
+                     masses log-uniform in [1e6, 1e8] Msun, r_s = 1.62 (M/1e8)^0.5 kpc (seed 2);
+                     1e5 spray particles = 25 per release step.
+* ``clumps``      -- 512 static Plummer clumps, radii drawn from the NFW(r_s = 15.62) enclosed
+                     mass within 60 kpc, isotropic (seed 1), masses log-uniform in [1e5, 1e7],
+                     r_s = 1.62 (M/1e8)^0.5; 1e6 spray particles = 250 per release step.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+G = 4.498502151469553e-12
+
+HOST = dict(m_disk=6.8e10, a_disk=3.0, b_disk=0.28, m_bulge=5e9, c_bulge=1.0,
+            m_halo=5.4e11, rs_halo=15.62)
+PROG_TODAY = np.array([12.0, 0.0, 0.0, 0.0, 0.153, 0.051])
+MSAT = 1e4
+PROG_RS = 0.01
+T_STEPS = 4000
+DT = 1.0
+
+
+def host_components(m_halo=HOST["m_halo"], rs_halo=HOST["rs_halo"]):
+    """(kind, params) list of the host with G*m first (kinds: 1 MN, 3 Hernquist, 2 NFW)."""
+    return [(1, [G * HOST["m_disk"], HOST["a_disk"], HOST["b_disk"]]),
+            (3, [G * HOST["m_bulge"], HOST["c_bulge"], 0.0]),
+            (2, [G * m_halo, rs_halo])]
+
+
+def host_potential(m_halo=HOST["m_halo"], rs_halo=HOST["rs_halo"]):
+    from .potential import (HernquistPotential, MiyamotoNagaiDisk, NFWPotential,
+                            Potential_Combine)
+    from .main import usys
+    return Potential_Combine([
+        MiyamotoNagaiDisk(HOST["m_disk"], HOST["a_disk"], HOST["b_disk"], units=usys),
+        HernquistPotential(HOST["m_bulge"], HOST["c_bulge"], units=usys),
+        NFWPotential(m_halo, rs_halo, units=usys)], units=usys)
+
+
+def release_times(nsteps=T_STEPS, dt=DT):
+    """ts = -nsteps*dt .. 0; the last entry is the observation time."""
+    return -nsteps * dt + dt * np.arange(nsteps + 1)
+
+
+def subhalo_ics(n=64, seed=2, r_shell=30.0, sigma_v=0.15, mmin=1e6, mmax=1e8):
+    rng = np.random.default_rng(seed)
+    u = rng.normal(size=(n, 3))
+    pos = r_shell * u / np.linalg.norm(u, axis=1, keepdims=True)
+    vel = rng.normal(scale=sigma_v, size=(n, 3))
+    m = 10 ** rng.uniform(np.log10(mmin), np.log10(mmax), n)
+    rs = 1.62 * np.sqrt(m / 1e8)
+    return np.hstack([pos, vel]), m, rs
+
+
+def nfw_mass_profile(r, rs):
+    x = r / rs
+    return np.log1p(x) - x / (1 + x)
+
+
+def clump_table(n=512, seed=1, rmax=60.0, rs_halo=HOST["rs_halo"], mmin=1e5, mmax=1e7):
+    """Clump positions drawn from the NFW density inside rmax (inverse CDF of the enclosed
+    mass, tabulated), isotropic directions, log-uniform masses."""
+    rng = np.random.default_rng(seed)
+    grid = np.linspace(0.0, rmax, 200001)
+    cdf = nfw_mass_profile(grid, rs_halo) / nfw_mass_profile(rmax, rs_halo)
+    r = np.interp(rng.uniform(size=n), cdf, grid)
+    u = rng.normal(size=(n, 3))
+    pos = r[:, None] * u / np.linalg.norm(u, axis=1, keepdims=True)
+    m = 10 ** rng.uniform(np.log10(mmin), np.log10(mmax), n)
+    rs = 1.62 * np.sqrt(m / 1e8)
+    return pos, m, rs
+
+
+def particles_per_step(total, nsteps=T_STEPS):
+    per = total // nsteps
+    if per * nsteps != total:
+        raise ValueError("particle count must be a multiple of the number of release steps")
+    return per
+
+
+def spray_layout(per_step):
+    """Arms and seeds of the particles released at one step: ceil(per/2) seeds, lead and trail
+    alternating; at odd steps an odd count drops a lead instead of a trail so the two arms
+    stay balanced.  Returns a list (per step parity) of [(seed_offset, arm)] with arm 0 = lead
+    (close), 1 = trail (far)."""
+    nseed = (per_step + 1) // 2
+    even, odd = [], []
+    for s in range(nseed):
+        even += [(s, 0), (s, 1)]
+        odd += [(s, 1), (s, 0)]
+    return even[:per_step], odd[:per_step]
+
+
+def particle_steps(per_step, nsteps=T_STEPS):
+    return per_step * nsteps * (nsteps + 1) // 2
