@@ -31,6 +31,7 @@ import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_PATH = os.path.join(_HERE, "libssc_oracle.so")
+_FAST_PATH = os.path.join(_HERE, "libssc_oracle_fast.so")
 
 G = 4.498502151469553e-12  # kpc^3 / (Msun Myr^2)
 
@@ -58,34 +59,47 @@ class _Sources(ctypes.Structure):
 
 def build(force: bool = False) -> str:
     """Compile the oracle with its Makefile (g++ + OpenMP)."""
-    if force or not os.path.exists(_LIB_PATH) or (
-            os.path.getmtime(_LIB_PATH) < os.path.getmtime(os.path.join(_HERE, "ssc_oracle.cpp"))):
-        subprocess.run(["make", "-s", "-C", _HERE], check=True)
+    src = os.path.join(_HERE, "ssc_oracle.cpp")
+    stale = [p for p in (_LIB_PATH, _FAST_PATH)
+             if not os.path.exists(p) or os.path.getmtime(p) < os.path.getmtime(src)]
+    if force or stale:
+        subprocess.run(["make", "-s", "-C", _HERE] + (["-B"] if force else []), check=True)
     return _LIB_PATH
 
 
 _lib = None
+_fast = None
 
 
-def lib():
-    global _lib
+def lib(fast: bool = False):
+    """The checker build, or (fast=True) the -O3 x86-64-v3 build used to time the CPU
+    baseline."""
+    global _lib, _fast
+    if fast:
+        if _fast is None:
+            build()
+            _fast = _bind(ctypes.CDLL(_FAST_PATH))
+        return _fast
     if _lib is None:
         build()
-        L = ctypes.CDLL(_LIB_PATH)
-        vp, i64, i32, d = ctypes.c_void_p, ctypes.c_int64, ctypes.c_int32, ctypes.c_double
-        L.orc_threefry2x32.argtypes = [ctypes.c_uint32] * 4 + [vp]
-        L.orc_randint5_1000.argtypes = [i64, vp]
-        L.orc_prng_normal.argtypes = [i64]
-        L.orc_prng_normal.restype = d
-        L.orc_erfinv.argtypes = [d]
-        L.orc_erfinv.restype = d
-        L.orc_release_normals.argtypes = [i64, i64, vp]
-        L.orc_eval.argtypes = [vp, vp, vp, i64, d, i32, vp, vp, vp]
-        L.orc_leapfrog.argtypes = [vp, vp, vp, vp, i64, d, d, i32, vp, vp, ctypes.c_int]
-        L.orc_release.argtypes = [vp, vp, vp, vp, i32, i64, vp, vp, vp, vp, vp]
-        L.orc_release_jacobian.argtypes = [vp, vp, vp, i32, i64, vp, vp]
-        _lib = L
+        _lib = _bind(ctypes.CDLL(_LIB_PATH))
     return _lib
+
+
+def _bind(L):
+    vp, i64, i32, d = ctypes.c_void_p, ctypes.c_int64, ctypes.c_int32, ctypes.c_double
+    L.orc_threefry2x32.argtypes = [ctypes.c_uint32] * 4 + [vp]
+    L.orc_randint5_1000.argtypes = [i64, vp]
+    L.orc_prng_normal.argtypes = [i64]
+    L.orc_prng_normal.restype = d
+    L.orc_erfinv.argtypes = [d]
+    L.orc_erfinv.restype = d
+    L.orc_release_normals.argtypes = [i64, i64, vp]
+    L.orc_eval.argtypes = [vp, vp, vp, i64, d, i32, vp, vp, vp]
+    L.orc_leapfrog.argtypes = [vp, vp, vp, vp, i64, d, d, i32, vp, vp, ctypes.c_int]
+    L.orc_release.argtypes = [vp, vp, vp, vp, i32, i64, vp, vp, vp, vp, vp]
+    L.orc_release_jacobian.argtypes = [vp, vp, vp, i32, i64, vp, vp]
+    return L
 
 
 # ------------------------------------------------------------------------------------------
@@ -189,14 +203,14 @@ def evaluate(h, xyz, t=0.0, step=0, src=None):
     return phi, grad, hess.reshape(n, 3, 3)
 
 
-def leapfrog(h, w0, t0, dt, nsteps, start=None, src=None, traj=False, nthreads=0):
+def leapfrog(h, w0, t0, dt, nsteps, start=None, src=None, traj=False, nthreads=0, fast=False):
     w0 = np.ascontiguousarray(np.atleast_2d(w0), dtype=np.float64)
     n = w0.shape[0]
     out = np.zeros_like(w0)
     st = None if start is None else np.ascontiguousarray(start, dtype=np.int32)
     tr = np.zeros((nsteps + 1, n, 6)) if traj else None
     src, sp = _src_ptr(src)
-    lib().orc_leapfrog(ctypes.addressof(h), sp, w0.ctypes.data, _p(st), n, t0, dt, nsteps,
+    lib(fast).orc_leapfrog(ctypes.addressof(h), sp, w0.ctypes.data, _p(st), n, t0, dt, nsteps,
                        out.ctypes.data, _p(tr), nthreads)
     return (out, tr) if traj else out
 

@@ -95,10 +95,12 @@ __global__ __launch_bounds__(256) void eval_kernel(ssc_host_potential hp, ssc_so
 
 // ------------------------------------------------------------------------------------------
 // Leapfrog: all lanes of a block walk the global grid together (k = block's earliest start ..
-// nsteps-1); a lane kicks/drifts only once k >= its start.  Source tables live in LDS:
+// nsteps-1); a particle kicks/drifts only once k >= its start.  Each lane carries P particles
+// so that one LDS read of a source record serves P force terms.  Source tables live in LDS:
 //   STATIC: rec[j] = (x, y, z, G*m), rs2[j]           (loaded once)
-//   TRACK:  rows kb..kb+CH of the trajectory table, rec[r][j] = (x, y, z, G*m)
-//   LINE:   (x0, v, t0, t_window) per source, position formed per step
+//   TRACK:  rows kb..kb+chunk of the trajectory table, rec[r][j] = (x, y, z, G*m)
+//   LINE:   (x0, v, G*m, r_s^2, t0, t_window) per source, position formed per step
+// add_grad(j0, js) sums sources j0, j0 + js, ... so a wave can also split one orbit's sum.
 // ------------------------------------------------------------------------------------------
 constexpr int LF_BLOCK = 256;
 
@@ -114,9 +116,10 @@ template <>
 struct SourceLDS<SSC_SRC_NONE> {
   __device__ void init(const ssc_sources&, char*) {}
   __device__ void stage(const ssc_sources&, int, int) {}
-  __device__ void add_grad(const ssc_sources&, int, double, double, double, double, double&,
-                           double&, double&) const {}
-  static size_t bytes(const ssc_sources&, int) { return 0; }
+  template <int P>
+  __device__ void add_grad(int, double, const double*, const double*, const double*, double*,
+                           double*, double*, int, int) const {}
+  static size_t bytes(const ssc_sources&) { return 0; }
 };
 
 template <>
@@ -124,7 +127,7 @@ struct SourceLDS<SSC_SRC_STATIC> {
   SrcRec* rec;
   double* rs2;
   int n;
-  static size_t bytes(const ssc_sources& s, int) { return (size_t)s.n * (sizeof(SrcRec) + 8); }
+  static size_t bytes(const ssc_sources& s) { return (size_t)s.n * (sizeof(SrcRec) + 8); }
   __device__ void init(const ssc_sources& s, char* lds) {
     n = s.n;
     rec = reinterpret_cast<SrcRec*>(lds);
@@ -136,17 +139,20 @@ struct SourceLDS<SSC_SRC_STATIC> {
     __syncthreads();
   }
   __device__ void stage(const ssc_sources&, int, int) {}
-  __device__ void add_grad(const ssc_sources&, int, double, double x, double y, double z,
-                           double& gx, double& gy, double& gz) const {
-    for (int j = 0; j < n; ++j) {
+  template <int P>
+  __device__ void add_grad(int, double, const double* x, const double* y, const double* z,
+                           double* gx, double* gy, double* gz, int j0, int js) const {
+    for (int j = j0; j < n; j += js) {
       const SrcRec r = rec[j];
-      plummer_grad(x - r.x, y - r.y, z - r.z, r.gm, rs2[j], gx, gy, gz);
+      const double q = rs2[j];
+#pragma unroll
+      for (int p = 0; p < P; ++p) plummer_grad(x[p] - r.x, y[p] - r.y, z[p] - r.z, r.gm, q, gx[p], gy[p], gz[p]);
     }
   }
 };
 
 // Grid steps staged per LDS fill for TRACK sources: (chunk + 1) rows of n records stay under
-// 64 KiB (16 steps for the 64-subhalo configuration, 1 step at 1024 sources).
+// 64 KiB (16 steps for 64 subhalos, 1 step at 1024 sources).
 __host__ __device__ inline int track_chunk(int n) {
   int c = 2048 / (n > 0 ? n : 1) - 1;
   return c < 1 ? 1 : (c > 16 ? 16 : c);
@@ -159,7 +165,7 @@ struct SourceLDS<SSC_SRC_TRACK> {
   int n;
   int chunk;
   int kb;       // grid row held in rec[0]
-  static size_t bytes(const ssc_sources& s, int) {
+  static size_t bytes(const ssc_sources& s) {
     return (size_t)(track_chunk(s.n) + 1) * s.n * sizeof(SrcRec) + (size_t)s.n * 8;
   }
   __device__ void init(const ssc_sources& s, char* lds) {
@@ -184,12 +190,15 @@ struct SourceLDS<SSC_SRC_TRACK> {
     }
     __syncthreads();
   }
-  __device__ void add_grad(const ssc_sources&, int k, double, double x, double y, double z,
-                           double& gx, double& gy, double& gz) const {
+  template <int P>
+  __device__ void add_grad(int k, double, const double* x, const double* y, const double* z,
+                           double* gx, double* gy, double* gz, int j0, int js) const {
     const SrcRec* row = rec + (k - kb) * n;
-    for (int j = 0; j < n; ++j) {
+    for (int j = j0; j < n; j += js) {
       const SrcRec r = row[j];
-      plummer_grad(x - r.x, y - r.y, z - r.z, r.gm, rs2[j], gx, gy, gz);
+      const double q = rs2[j];
+#pragma unroll
+      for (int p = 0; p < P; ++p) plummer_grad(x[p] - r.x, y[p] - r.y, z[p] - r.z, r.gm, q, gx[p], gy[p], gz[p]);
     }
   }
 };
@@ -202,7 +211,7 @@ template <>
 struct SourceLDS<SSC_SRC_LINE> {
   LineRec* rec;
   int n;
-  static size_t bytes(const ssc_sources& s, int) { return (size_t)s.n * sizeof(LineRec); }
+  static size_t bytes(const ssc_sources& s) { return (size_t)s.n * sizeof(LineRec); }
   __device__ void init(const ssc_sources& s, char* lds) {
     n = s.n;
     rec = reinterpret_cast<LineRec*>(lds);
@@ -213,31 +222,38 @@ struct SourceLDS<SSC_SRC_LINE> {
     __syncthreads();
   }
   __device__ void stage(const ssc_sources&, int, int) {}
-  __device__ void add_grad(const ssc_sources&, int, double t, double x, double y, double z,
-                           double& gx, double& gy, double& gz) const {
-    for (int j = 0; j < n; ++j) {
+  template <int P>
+  __device__ void add_grad(int, double t, const double* x, const double* y, const double* z,
+                           double* gx, double* gy, double* gz, int j0, int js) const {
+    for (int j = j0; j < n; j += js) {
       const LineRec r = rec[j];
       const double dt = t - r.t0;
       if (!(fabs(dt) < r.tw)) continue;
-      plummer_grad(x - (r.x0 + r.vx * dt), y - (r.y0 + r.vy * dt), z - (r.z0 + r.vz * dt), r.gm,
-                   r.rs2, gx, gy, gz);
+      const double sx = r.x0 + r.vx * dt, sy = r.y0 + r.vy * dt, sz = r.z0 + r.vz * dt;
+#pragma unroll
+      for (int p = 0; p < P; ++p) plummer_grad(x[p] - sx, y[p] - sy, z[p] - sz, r.gm, r.rs2, gx[p], gy[p], gz[p]);
     }
   }
 };
 
-template <int SK>
-__device__ __forceinline__ void accel_at(const ssc_host_potential& hp, const ssc_sources& src,
-                                         const SourceLDS<SK>& lds, int k, double t, double x,
-                                         double y, double z, double& ax, double& ay, double& az) {
-  double gx, gy, gz;
-  host_grad(hp, x, y, z, gx, gy, gz);
-  lds.add_grad(src, k, t, x, y, z, gx, gy, gz);
-  ax = -gx;
-  ay = -gy;
-  az = -gz;
+// Acceleration of P particles at grid row k / time t.
+template <int SK, int P>
+__device__ __forceinline__ void accel_P(const ssc_host_potential& hp, const SourceLDS<SK>& lds,
+                                        int k, double t, const double* x, const double* y,
+                                        const double* z, double* ax, double* ay, double* az) {
+  double gx[P], gy[P], gz[P];
+#pragma unroll
+  for (int p = 0; p < P; ++p) host_grad(hp, x[p], y[p], z[p], gx[p], gy[p], gz[p]);
+  lds.template add_grad<P>(k, t, x, y, z, gx, gy, gz, 0, 1);
+#pragma unroll
+  for (int p = 0; p < P; ++p) {
+    ax[p] = -gx[p];
+    ay[p] = -gy[p];
+    az[p] = -gz[p];
+  }
 }
 
-template <int SK>
+template <int SK, int P>
 __global__ __launch_bounds__(LF_BLOCK) void leapfrog_kernel(
     ssc_host_potential hp, ssc_sources src, const double* __restrict__ w0,
     const int32_t* __restrict__ start, int64_t n, double t0, double dt, int nsteps,
@@ -245,27 +261,129 @@ __global__ __launch_bounds__(LF_BLOCK) void leapfrog_kernel(
     double* __restrict__ ys) {
   extern __shared__ __align__(16) char lds_raw[];
   __shared__ int kmin_sh;
-  const int64_t i = (int64_t)blockIdx.x * LF_BLOCK + threadIdx.x;
-  const bool valid = i < n;
-  int s = nsteps;
-  double x = 0, y = 0, z = 0, vx = 0, vy = 0, vz = 0;
-  if (valid) {
-    s = start ? start[i] : 0;
-    s = s < 0 ? 0 : (s > nsteps ? nsteps : s);
-    const double* w = w0 + 6 * i;
-    x = w[0]; y = w[1]; z = w[2]; vx = w[3]; vy = w[4]; vz = w[5];
+  double x[P], y[P], z[P], vx[P], vy[P], vz[P], ax[P], ay[P], az[P];
+  int s[P];
+  int64_t idx[P];
+  bool valid[P];
+  int smin = nsteps;
+#pragma unroll
+  for (int p = 0; p < P; ++p) {
+    idx[p] = (int64_t)blockIdx.x * (LF_BLOCK * P) + p * LF_BLOCK + threadIdx.x;
+    valid[p] = idx[p] < n;
+    s[p] = nsteps;
+    x[p] = 1.0; y[p] = 0.0; z[p] = 0.0; vx[p] = vy[p] = vz[p] = 0.0;
+    if (valid[p]) {
+      int sp = start ? start[idx[p]] : 0;
+      s[p] = sp < 0 ? 0 : (sp > nsteps ? nsteps : sp);
+      const double* w = w0 + 6 * idx[p];
+      x[p] = w[0]; y[p] = w[1]; z[p] = w[2]; vx[p] = w[3]; vy[p] = w[4]; vz[p] = w[5];
+    }
+    smin = min(smin, s[p]);
   }
   if (threadIdx.x == 0) kmin_sh = INT_MAX;
   __syncthreads();
-  atomicMin(&kmin_sh, s);
+  atomicMin(&kmin_sh, smin);
   SourceLDS<SK> lds;
   lds.init(src, lds_raw);
   __syncthreads();
   const int kmin = kmin_sh;
-  // Save rows at grid indices <= kmin hold the initial state of every lane.
+  // Save rows at grid indices <= kmin hold the initial state of every particle.
   int next = 0;
   while (next < nsave && save_idx[next] <= kmin) {
-    if (valid) {
+#pragma unroll
+    for (int p = 0; p < P; ++p)
+      if (valid[p]) {
+        double* o = ys + ((int64_t)next * n + idx[p]) * 6;
+        o[0] = x[p]; o[1] = y[p]; o[2] = z[p]; o[3] = vx[p]; o[4] = vy[p]; o[5] = vz[p];
+      }
+    ++next;
+  }
+  const double h = 0.5 * dt;
+  if (kmin < nsteps) {
+    lds.stage(src, kmin, nsteps + 1);
+    accel_P<SK, P>(hp, lds, kmin, t0 + kmin * dt, x, y, z, ax, ay, az);
+  }
+  // A particle that has not started keeps its initial row; the force evaluated there at
+  // t_{k+1} is exactly the initial force it needs when it starts at k + 1.
+  for (int k = kmin; k < nsteps; ++k) {
+    lds.stage(src, k, nsteps + 1);
+#pragma unroll
+    for (int p = 0; p < P; ++p)
+      if (k >= s[p]) {
+        vx[p] = vx[p] + h * ax[p]; vy[p] = vy[p] + h * ay[p]; vz[p] = vz[p] + h * az[p];
+        x[p] = x[p] + dt * vx[p]; y[p] = y[p] + dt * vy[p]; z[p] = z[p] + dt * vz[p];
+      }
+    accel_P<SK, P>(hp, lds, k + 1, t0 + (k + 1) * dt, x, y, z, ax, ay, az);
+#pragma unroll
+    for (int p = 0; p < P; ++p)
+      if (k >= s[p]) {
+        vx[p] = vx[p] + h * ax[p]; vy[p] = vy[p] + h * ay[p]; vz[p] = vz[p] + h * az[p];
+      }
+    // block-uniform: every particle is now at grid index k + 1 (or holds its initial row)
+    while (next < nsave && save_idx[next] == k + 1) {
+#pragma unroll
+      for (int p = 0; p < P; ++p)
+        if (valid[p]) {
+          double* o = ys + ((int64_t)next * n + idx[p]) * 6;
+          o[0] = x[p]; o[1] = y[p]; o[2] = z[p]; o[3] = vx[p]; o[4] = vy[p]; o[5] = vz[p];
+        }
+      ++next;
+    }
+  }
+#pragma unroll
+  for (int p = 0; p < P; ++p)
+    if (valid[p]) {
+      double* o = w_out + 6 * idx[p];
+      o[0] = x[p]; o[1] = y[p]; o[2] = z[p]; o[3] = vx[p]; o[4] = vy[p]; o[5] = vz[p];
+    }
+}
+
+// One wave per orbit: for a handful of orbits in a large source table (the progenitor in a
+// clumpy halo) the 64 lanes split the source sum and combine it with a butterfly reduction;
+// the orbit state is replicated in every lane.
+constexpr int WAVE = 64;
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int m = WAVE / 2; m >= 1; m >>= 1) v += __shfl_xor(v, m, WAVE);
+  return v;
+}
+
+template <int SK>
+__device__ __forceinline__ void accel_wave(const ssc_host_potential& hp, const SourceLDS<SK>& lds,
+                                           int k, double t, double x, double y, double z,
+                                           double& ax, double& ay, double& az) {
+  double gx = 0.0, gy = 0.0, gz = 0.0;
+  lds.template add_grad<1>(k, t, &x, &y, &z, &gx, &gy, &gz, threadIdx.x, WAVE);
+  gx = wave_sum(gx);
+  gy = wave_sum(gy);
+  gz = wave_sum(gz);
+  double hx, hy, hz;
+  host_grad(hp, x, y, z, hx, hy, hz);
+  ax = -(hx + gx);
+  ay = -(hy + gy);
+  az = -(hz + gz);
+}
+
+template <int SK>
+__global__ __launch_bounds__(WAVE) void leapfrog_wave_kernel(
+    ssc_host_potential hp, ssc_sources src, const double* __restrict__ w0,
+    const int32_t* __restrict__ start, int64_t n, double t0, double dt, int nsteps,
+    double* __restrict__ w_out, const int32_t* __restrict__ save_idx, int nsave,
+    double* __restrict__ ys) {
+  extern __shared__ __align__(16) char lds_raw[];
+  const int64_t i = blockIdx.x;
+  const bool lead = threadIdx.x == 0;
+  int s = start ? start[i] : 0;
+  s = s < 0 ? 0 : (s > nsteps ? nsteps : s);
+  const double* w = w0 + 6 * i;
+  double x = w[0], y = w[1], z = w[2], vx = w[3], vy = w[4], vz = w[5];
+  SourceLDS<SK> lds;
+  lds.init(src, lds_raw);
+  __syncthreads();
+  int next = 0;
+  while (next < nsave && save_idx[next] <= s) {
+    if (lead) {
       double* o = ys + ((int64_t)next * n + i) * 6;
       o[0] = x; o[1] = y; o[2] = z; o[3] = vx; o[4] = vy; o[5] = vz;
     }
@@ -273,44 +391,66 @@ __global__ __launch_bounds__(LF_BLOCK) void leapfrog_kernel(
   }
   const double h = 0.5 * dt;
   double ax = 0, ay = 0, az = 0;
-  for (int k = kmin; k < nsteps; ++k) {
+  if (s < nsteps) {
+    lds.stage(src, s, nsteps + 1);
+    accel_wave<SK>(hp, lds, s, t0 + s * dt, x, y, z, ax, ay, az);
+  }
+  for (int k = s; k < nsteps; ++k) {
     lds.stage(src, k, nsteps + 1);
-    if (valid && k == s) accel_at<SK>(hp, src, lds, k, t0 + k * dt, x, y, z, ax, ay, az);
-    if (valid && k >= s) {
-      vx = vx + h * ax; vy = vy + h * ay; vz = vz + h * az;
-      x = x + dt * vx; y = y + dt * vy; z = z + dt * vz;
-      accel_at<SK>(hp, src, lds, k + 1, t0 + (k + 1) * dt, x, y, z, ax, ay, az);
-      vx = vx + h * ax; vy = vy + h * ay; vz = vz + h * az;
-    }
-    // block-uniform: every lane is now at grid index k + 1 (or still holds its initial row)
+    vx = vx + h * ax; vy = vy + h * ay; vz = vz + h * az;
+    x = x + dt * vx; y = y + dt * vy; z = z + dt * vz;
+    accel_wave<SK>(hp, lds, k + 1, t0 + (k + 1) * dt, x, y, z, ax, ay, az);
+    vx = vx + h * ax; vy = vy + h * ay; vz = vz + h * az;
     while (next < nsave && save_idx[next] == k + 1) {
-      if (valid) {
+      if (lead) {
         double* o = ys + ((int64_t)next * n + i) * 6;
         o[0] = x; o[1] = y; o[2] = z; o[3] = vx; o[4] = vy; o[5] = vz;
       }
       ++next;
     }
   }
-  if (valid) {
+  if (lead) {
     double* o = w_out + 6 * i;
     o[0] = x; o[1] = y; o[2] = z; o[3] = vx; o[4] = vy; o[5] = vz;
   }
 }
 
+static int set_lds(const void* fn, size_t lds) {
+  if (lds > 160 * 1024) return fail(SSC_E_LIMIT, "ssc_leapfrog: source table exceeds LDS");
+  if (lds > 60 * 1024) {
+    hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return fail(SSC_E_HIP, std::string("hipFuncSetAttribute: ") + hipGetErrorString(e));
+  }
+  return SSC_OK;
+}
+
+// Orbits per lane of the block kernel: two once there are enough orbits and sources for the
+// halved LDS traffic to matter, one otherwise.
 template <int SK>
 static int launch_leapfrog(const ssc_host_potential& hp, const ssc_sources& src, const double* w0,
                            const int32_t* start, int64_t n, double t0, double dt, int nsteps,
                            double* w_out, const int32_t* save_idx, int nsave, double* ys,
                            hipStream_t st) {
-  const size_t lds = SourceLDS<SK>::bytes(src, nsteps);
-  if (lds > 160 * 1024) return fail(SSC_E_LIMIT, "ssc_leapfrog: source table exceeds LDS");
-  if (lds > 60 * 1024) {
-    hipError_t e = hipFuncSetAttribute((const void*)leapfrog_kernel<SK>,
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    if (e != hipSuccess) return fail(SSC_E_HIP, std::string("hipFuncSetAttribute: ") + hipGetErrorString(e));
+  const size_t lds = SourceLDS<SK>::bytes(src);
+  if (SK != SSC_SRC_NONE && src.n >= 64 && n <= 8192) {
+    int rc = set_lds((const void*)leapfrog_wave_kernel<SK>, lds);
+    if (rc) return rc;
+    hipLaunchKernelGGL(leapfrog_wave_kernel<SK>, dim3((unsigned)n), dim3(WAVE), lds, st, hp, src,
+                       w0, start, n, t0, dt, nsteps, w_out, save_idx, nsave, ys);
+    return check_launch("ssc_leapfrog");
   }
+  if (SK != SSC_SRC_NONE && src.n >= 16 && n >= 65536) {
+    int rc = set_lds((const void*)leapfrog_kernel<SK, 2>, lds);
+    if (rc) return rc;
+    const int64_t blocks = (n + 2 * LF_BLOCK - 1) / (2 * LF_BLOCK);
+    hipLaunchKernelGGL((leapfrog_kernel<SK, 2>), dim3((unsigned)blocks), dim3(LF_BLOCK), lds, st,
+                       hp, src, w0, start, n, t0, dt, nsteps, w_out, save_idx, nsave, ys);
+    return check_launch("ssc_leapfrog");
+  }
+  int rc = set_lds((const void*)leapfrog_kernel<SK, 1>, lds);
+  if (rc) return rc;
   const int64_t blocks = (n + LF_BLOCK - 1) / LF_BLOCK;
-  hipLaunchKernelGGL(leapfrog_kernel<SK>, dim3((unsigned)blocks), dim3(LF_BLOCK), lds, st, hp,
+  hipLaunchKernelGGL((leapfrog_kernel<SK, 1>), dim3((unsigned)blocks), dim3(LF_BLOCK), lds, st, hp,
                      src, w0, start, n, t0, dt, nsteps, w_out, save_idx, nsave, ys);
   return check_launch("ssc_leapfrog");
 }

@@ -1,0 +1,40 @@
+"""Algorithmic FP64 operation counts per particle-step, frozen before the first measurement.
+
+Convention: +, -, x, /, sqrt and log1p count 1 FLOP each and a fused multiply-add counts 2;
+counts are per formula as written (no sharing between components).  These are the figures the
+roofline fractions are computed from; hardware instruction counts (rocprof) exceed them because
+FP64 sqrt, division, rsqrt and log1p expand to multi-instruction sequences on CDNA.
+
+  host gradient   MN 20 + Hernquist 16 + NFW 21                      = 57
+  KDK update      2 half kicks (2 x 6) + drift (6)                    = 18
+  Plummer source  difference 3 + squared radius 6 + rsqrt 2 + cube 3 + accumulate 6 - 2
+                  (the squared-radius add of r_s^2 fuses)             = 18 per source
+  host Hessian    (response and adjoint forms)                        = 96
+  response vector per subhalo: unit-mass Plummer 16 + H dx 15 + source add 3 + KDK 18 = 52
+"""
+HOST_GRAD = 57
+KDK = 18
+PLUMMER_SOURCE = 18
+HOST_HESSIAN = 96
+RESPONSE_PER_SUBHALO = 52
+
+
+def plain(n_sources: int = 0) -> int:
+    """Leapfrog particle-step in host + n Plummer sources (plain workload: 75; 512 clumps:
+    9,291; 64 subhalos: 1,227)."""
+    return KDK + HOST_GRAD + PLUMMER_SOURCE * n_sources
+
+
+def responses(n_subhalos: int) -> int:
+    """Base orbit + host Hessian + first-order response vectors (64 subhalos: 3,499)."""
+    return plain(0) + HOST_HESSIAN + RESPONSE_PER_SUBHALO * n_subhalos
+
+
+def mlp(widths=(3, 256, 256, 256, 4)) -> int:
+    """Dense MLP forward, 2 FLOP per multiply-accumulate (3-256-256-256-4: 265,728)."""
+    return sum(2 * a * b for a, b in zip(widths[:-1], widths[1:]))
+
+
+def surrogate() -> int:
+    """Surrogate particle-step: MLP in BF16 MFMA plus the FP64 host force and KDK (75)."""
+    return plain(0)

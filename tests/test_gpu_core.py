@@ -117,6 +117,30 @@ def test_leapfrog_parity(oracle, ssc, kind):
     assert err <= 1e-9
 
 
+@pytest.mark.parametrize("path", ["wave_per_orbit", "two_per_lane", "one_per_lane"])
+def test_leapfrog_launch_paths(oracle, ssc, path):
+    """The three launch shapes of ssc_leapfrog (one wave per orbit for few orbits in a large
+    source table; two particles per lane for large batches; one otherwise) against the oracle."""
+    from streamsculptor_amd.potential import Potential_Combine, PlummerClumps
+    from streamsculptor_amd import workloads as W
+    rng = np.random.default_rng(21)
+    h, host = _host_pair(oracle, ssc)
+    pos, m, rs = W.clump_table(n=512 if path == "wave_per_orbit" else 48, seed=1)
+    n, nsteps = {"wave_per_orbit": (7, 800), "two_per_lane": (70001, 120),
+                 "one_per_lane": (3001, 300)}[path]
+    w0 = bound_orbits(rng, n)
+    start = np.sort(rng.integers(0, nsteps, n)).astype(np.int32)
+    src_o = oracle.Sources(oracle.SRC_STATIC, gm=G * m, rs=rs, pos=pos)
+    pot = Potential_Combine([host, PlummerClumps(m, rs, pos, units=ssc.usys)], units=ssc.usys)
+    save = np.array([0, nsteps // 3, nsteps], dtype=np.int32)
+    out, ys = pot._leapfrog(w0, -float(nsteps), 1.0, nsteps, start=start, save_idx=save)
+    ref, tr = oracle.leapfrog(h, w0, -float(nsteps), 1.0, nsteps, start=start, src=src_o, traj=True)
+    err = rel_err_rows(out, ref)
+    print(f"path {path}: n={n} max row rel err {err:.2e}")
+    assert err <= 1e-9
+    assert rel_err_rows(ys.reshape(-1, 6), tr[save].reshape(-1, 6)) <= 1e-9
+
+
 def test_integrate_orbit_saves_and_backward(oracle, ssc):
     from streamsculptor_amd import workloads as W
     h, host = _host_pair(oracle, ssc)
