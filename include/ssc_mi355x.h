@@ -130,6 +130,21 @@ int ssc_release(const ssc_host_potential* host, const double* prog_w, const int3
                 double* pos_close, double* pos_far, double* vel_close, double* vel_far,
                 void* stream);
 
+/* Base orbits in the host potential plus first-order responses to each perturbing Plummer
+ * source (reference fields.integrate_field with MassRadiusPerturbation_OTF,
+ * streamsculptor/fields.py:34-99,159-206, driven per particle as in
+ * GenerateMassRadiusPerturbation.compute_perturbation_OTF, perturbative.py:102-135), integrated
+ * as the exact derivative of the fixed-step leapfrog map.
+ *   pert            the m perturbing sources (STATIC, TRACK or LINE)
+ *   w0[n][6]        base rows at grid index start_step[i]
+ *   dw0[n][m][D]    initial responses (NULL = zeros), D = 6 (mass block: d(x,v)/d eps_j) or
+ *                   12 with with_radius (adds d^2(x,v)/(d a_j d eps_j), a_j = Plummer radius)
+ *   w_out[n][6], dw_out[n][m][D] at grid index nsteps. */
+int ssc_perturb_leapfrog(const ssc_host_potential* host, const ssc_sources* pert,
+                         const double* w0, const double* dw0, const int32_t* start_step,
+                         int64_t n, double t0, double dt, int32_t nsteps, int32_t with_radius,
+                         double* w_out, double* dw_out, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -99,6 +99,8 @@ def _bind(L):
     L.orc_leapfrog.argtypes = [vp, vp, vp, vp, i64, d, d, i32, vp, vp, ctypes.c_int]
     L.orc_release.argtypes = [vp, vp, vp, vp, i32, i64, vp, vp, vp, vp, vp]
     L.orc_release_jacobian.argtypes = [vp, vp, vp, i32, i64, vp, vp]
+    L.orc_perturb_leapfrog.argtypes = [vp, vp, vp, vp, vp, vp, i64, d, d, i32, ctypes.c_int,
+                                       vp, vp, ctypes.c_int]
     return L
 
 
@@ -257,3 +259,22 @@ def stream_leapfrog(h, prog_w0, ts, msat, seed, dt, src=None, kvals=None, nthrea
     start = np.repeat(kidx[:T], 2).astype(np.int32)
     out = leapfrog(h, w0, ts[0], dt, nsteps, start=start, src=src, nthreads=nthreads)
     return out[0::2], out[1::2]
+
+
+def perturb_leapfrog(h, pert, w0, t0, dt, nsteps, dw0=None, start=None, base_src=None,
+                     with_radius=False, nthreads=0):
+    """Base orbits + first-order responses to each source of ``pert`` (a Sources).
+    Returns (w (n, 6), dw (n, m, 6 or 12))."""
+    w0 = np.ascontiguousarray(np.atleast_2d(w0), dtype=np.float64)
+    n = w0.shape[0]
+    m = pert.desc.n
+    dd = 12 if with_radius else 6
+    out = np.zeros_like(w0)
+    dw = np.zeros((n, m, dd))
+    d0 = None if dw0 is None else np.ascontiguousarray(dw0, dtype=np.float64).reshape(n, m, dd)
+    st = None if start is None else np.ascontiguousarray(start, dtype=np.int32)
+    base_src, bp = _src_ptr(base_src)
+    lib().orc_perturb_leapfrog(ctypes.addressof(h), bp, ctypes.addressof(pert.desc),
+                               w0.ctypes.data, _p(d0), _p(st), n, t0, dt, nsteps,
+                               int(with_radius), out.ctypes.data, dw.ctypes.data, nthreads)
+    return out, dw

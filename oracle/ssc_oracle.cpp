@@ -497,4 +497,87 @@ int orc_release_jacobian(const ssc_host_potential* hp, const double* prog_w, con
   return 0;
 }
 
+// First-order responses to the perturbing sources, integrated as the exact derivative of the
+// KDK map of the perturbed system Phi_base + sum_j eps_j Phi_j at eps = 0 (restates
+// MassRadiusPerturbation_OTF.term, reference streamsculptor/fields.py:175-206: the base flow
+// [v0, a0], the mass block d(dx)/dt = dv, d(dv)/dt = -grad Phi_j(x0) - H(x0) dx, and, when
+// with_radius, the radius block driven by -grad(dPhi_j/da_j)).  State per particle: w (6) and
+// dw[j] = (dx, dv[, d2x/da, d2v/da]) for j < m.  H is the Hessian of the base potential.
+int orc_perturb_leapfrog(const ssc_host_potential* hp, const ssc_sources* base_src,
+                         const ssc_sources* pert, const double* w0, const double* dw0,
+                         const int32_t* start, int64_t n, double t0, double dt, int32_t nsteps,
+                         int with_radius, double* w_out, double* dw_out, int nthreads) {
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+  const int m = pert->n;
+  const int dd = with_radius ? 12 : 6;
+#pragma omp parallel for schedule(dynamic, 4)
+  for (int64_t i = 0; i < n; ++i) {
+    double x[3], v[3], a[3], H[9], g[3];
+    std::vector<double> d((size_t)m * dd), ad((size_t)m * dd / 2);
+    for (int c = 0; c < 3; ++c) { x[c] = w0[6 * i + c]; v[c] = w0[6 * i + 3 + c]; }
+    for (int j = 0; j < m * dd; ++j) d[j] = dw0 ? dw0[(size_t)i * m * dd + j] : 0.0;
+    int s = start ? start[i] : 0;
+    if (s < 0) s = 0;
+    // accelerations of base and responses at grid row k
+    auto forces = [&](int k) {
+      double t = t0 + k * dt;
+      total_eval(hp, base_src, x, k, t, nullptr, g, H);
+      for (int c = 0; c < 3; ++c) a[c] = -g[c];
+      for (int j = 0; j < m; ++j) {
+        double sp[3];
+        double gj[3] = {0, 0, 0}, rj[3] = {0, 0, 0};
+        if (source_pos(pert, j, k, t, sp)) {
+          double dv3[3] = {x[0] - sp[0], x[1] - sp[1], x[2] - sp[2]};
+          plummer_eval<double>(pert->gm[j], pert->rs[j], dv3, nullptr, gj, nullptr);
+          double A = pert->rs[j];
+          double s2 = dv3[0] * dv3[0] + dv3[1] * dv3[1] + dv3[2] * dv3[2] + A * A;
+          double c5 = -3.0 * pert->gm[j] * A / (s2 * s2 * std::sqrt(s2));  // grad dPhi/da
+          for (int c = 0; c < 3; ++c) rj[c] = c5 * dv3[c];
+        }
+        const double* dx = &d[(size_t)j * dd];
+        double* o = &ad[(size_t)j * dd / 2];
+        for (int r = 0; r < 3; ++r) {
+          double hd = H[3 * r] * dx[0] + H[3 * r + 1] * dx[1] + H[3 * r + 2] * dx[2];
+          o[r] = -gj[r] - hd;
+        }
+        if (with_radius) {
+          const double* rx = dx + 6;
+          for (int r = 0; r < 3; ++r) {
+            double hd = H[3 * r] * rx[0] + H[3 * r + 1] * rx[1] + H[3 * r + 2] * rx[2];
+            o[3 + r] = -rj[r] - hd;
+          }
+        }
+      }
+    };
+    if (s < nsteps) forces(s);
+    const double h = 0.5 * dt;
+    for (int k = s; k < nsteps; ++k) {
+      for (int c = 0; c < 3; ++c) v[c] = v[c] + h * a[c];
+      for (int j = 0; j < m; ++j) {
+        double* q = &d[(size_t)j * dd];
+        const double* o = &ad[(size_t)j * dd / 2];
+        for (int c = 0; c < 3; ++c) q[3 + c] = q[3 + c] + h * o[c];
+        if (with_radius) for (int c = 0; c < 3; ++c) q[9 + c] = q[9 + c] + h * o[3 + c];
+      }
+      for (int c = 0; c < 3; ++c) x[c] = x[c] + dt * v[c];
+      for (int j = 0; j < m; ++j) {
+        double* q = &d[(size_t)j * dd];
+        for (int c = 0; c < 3; ++c) q[c] = q[c] + dt * q[3 + c];
+        if (with_radius) for (int c = 0; c < 3; ++c) q[6 + c] = q[6 + c] + dt * q[9 + c];
+      }
+      forces(k + 1);
+      for (int c = 0; c < 3; ++c) v[c] = v[c] + h * a[c];
+      for (int j = 0; j < m; ++j) {
+        double* q = &d[(size_t)j * dd];
+        const double* o = &ad[(size_t)j * dd / 2];
+        for (int c = 0; c < 3; ++c) q[3 + c] = q[3 + c] + h * o[c];
+        if (with_radius) for (int c = 0; c < 3; ++c) q[9 + c] = q[9 + c] + h * o[3 + c];
+      }
+    }
+    for (int c = 0; c < 3; ++c) { w_out[6 * i + c] = x[c]; w_out[6 * i + 3 + c] = v[c]; }
+    for (int j = 0; j < m * dd; ++j) dw_out[(size_t)i * m * dd + j] = d[j];
+  }
+  return 0;
+}
+
 }  // extern "C"

@@ -44,6 +44,7 @@ _vp, _i64, _i32, _d = ctypes.c_void_p, ctypes.c_int64, ctypes.c_int32, ctypes.c_
 SIGNATURES = {
     "ssc_eval": [_vp, _vp, _vp, _i64, _d, _i32, _vp, _vp, _vp, _vp],
     "ssc_leapfrog": [_vp, _vp, _vp, _vp, _i64, _d, _d, _i32, _vp, _vp, _i32, _vp, _vp],
+    "ssc_perturb_leapfrog": [_vp, _vp, _vp, _vp, _vp, _i64, _d, _d, _i32, _i32, _vp, _vp],
     "ssc_release": [_vp, _vp, _vp, _vp, _i32, ctypes.c_int64, _vp, _vp, _vp, _vp, _vp, _vp],
 }
 

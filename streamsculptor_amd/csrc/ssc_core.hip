@@ -8,6 +8,7 @@
 #include <string>
 
 #include "ssc_device.h"
+#include "ssc_sources_lds.h"
 
 namespace ssc {
 
@@ -103,138 +104,6 @@ __global__ __launch_bounds__(256) void eval_kernel(ssc_host_potential hp, ssc_so
 // add_grad(j0, js) sums sources j0, j0 + js, ... so a wave can also split one orbit's sum.
 // ------------------------------------------------------------------------------------------
 constexpr int LF_BLOCK = 256;
-
-struct alignas(16) SrcRec {
-  double x, y, z, gm;
-};
-
-template <int SK>
-struct SourceLDS;
-
-// No sources.
-template <>
-struct SourceLDS<SSC_SRC_NONE> {
-  __device__ void init(const ssc_sources&, char*) {}
-  __device__ void stage(const ssc_sources&, int, int) {}
-  template <int P>
-  __device__ void add_grad(int, double, const double*, const double*, const double*, double*,
-                           double*, double*, int, int) const {}
-  static size_t bytes(const ssc_sources&) { return 0; }
-};
-
-template <>
-struct SourceLDS<SSC_SRC_STATIC> {
-  SrcRec* rec;
-  double* rs2;
-  int n;
-  static size_t bytes(const ssc_sources& s) { return (size_t)s.n * (sizeof(SrcRec) + 8); }
-  __device__ void init(const ssc_sources& s, char* lds) {
-    n = s.n;
-    rec = reinterpret_cast<SrcRec*>(lds);
-    rs2 = reinterpret_cast<double*>(lds + (size_t)n * sizeof(SrcRec));
-    for (int j = threadIdx.x; j < n; j += blockDim.x) {
-      rec[j] = SrcRec{s.pos[3 * j], s.pos[3 * j + 1], s.pos[3 * j + 2], s.gm[j]};
-      rs2[j] = s.rs[j] * s.rs[j];
-    }
-    __syncthreads();
-  }
-  __device__ void stage(const ssc_sources&, int, int) {}
-  template <int P>
-  __device__ void add_grad(int, double, const double* x, const double* y, const double* z,
-                           double* gx, double* gy, double* gz, int j0, int js) const {
-    for (int j = j0; j < n; j += js) {
-      const SrcRec r = rec[j];
-      const double q = rs2[j];
-#pragma unroll
-      for (int p = 0; p < P; ++p) plummer_grad(x[p] - r.x, y[p] - r.y, z[p] - r.z, r.gm, q, gx[p], gy[p], gz[p]);
-    }
-  }
-};
-
-// Grid steps staged per LDS fill for TRACK sources: (chunk + 1) rows of n records stay under
-// 64 KiB (16 steps for 64 subhalos, 1 step at 1024 sources).
-__host__ __device__ inline int track_chunk(int n) {
-  int c = 2048 / (n > 0 ? n : 1) - 1;
-  return c < 1 ? 1 : (c > 16 ? 16 : c);
-}
-
-template <>
-struct SourceLDS<SSC_SRC_TRACK> {
-  SrcRec* rec;  // [chunk + 1][n]
-  double* rs2;  // [n]
-  int n;
-  int chunk;
-  int kb;       // grid row held in rec[0]
-  static size_t bytes(const ssc_sources& s) {
-    return (size_t)(track_chunk(s.n) + 1) * s.n * sizeof(SrcRec) + (size_t)s.n * 8;
-  }
-  __device__ void init(const ssc_sources& s, char* lds) {
-    n = s.n;
-    chunk = track_chunk(n);
-    rec = reinterpret_cast<SrcRec*>(lds);
-    rs2 = reinterpret_cast<double*>(lds + (size_t)(chunk + 1) * n * sizeof(SrcRec));
-    for (int j = threadIdx.x; j < n; j += blockDim.x) rs2[j] = s.rs[j] * s.rs[j];
-    kb = INT_MIN;
-  }
-  // Make rows k and k+1 resident (block-uniform k; contains barriers).
-  __device__ void stage(const ssc_sources& s, int k, int nrows) {
-    if (kb != INT_MIN && k >= kb && k + 1 <= kb + chunk) return;
-    __syncthreads();
-    kb = k;
-    const int rows = min(chunk + 1, nrows - k);
-    const int tot = rows * n;
-    for (int e = threadIdx.x; e < tot; e += blockDim.x) {
-      const int r = e / n, j = e - r * n;
-      const double* p = s.pos + ((int64_t)(k + r) * n + j) * 3;
-      rec[r * n + j] = SrcRec{p[0], p[1], p[2], s.gm[j]};
-    }
-    __syncthreads();
-  }
-  template <int P>
-  __device__ void add_grad(int k, double, const double* x, const double* y, const double* z,
-                           double* gx, double* gy, double* gz, int j0, int js) const {
-    const SrcRec* row = rec + (k - kb) * n;
-    for (int j = j0; j < n; j += js) {
-      const SrcRec r = row[j];
-      const double q = rs2[j];
-#pragma unroll
-      for (int p = 0; p < P; ++p) plummer_grad(x[p] - r.x, y[p] - r.y, z[p] - r.z, r.gm, q, gx[p], gy[p], gz[p]);
-    }
-  }
-};
-
-struct alignas(16) LineRec {
-  double x0, y0, z0, gm, vx, vy, vz, rs2, t0, tw;
-};
-
-template <>
-struct SourceLDS<SSC_SRC_LINE> {
-  LineRec* rec;
-  int n;
-  static size_t bytes(const ssc_sources& s) { return (size_t)s.n * sizeof(LineRec); }
-  __device__ void init(const ssc_sources& s, char* lds) {
-    n = s.n;
-    rec = reinterpret_cast<LineRec*>(lds);
-    for (int j = threadIdx.x; j < n; j += blockDim.x)
-      rec[j] = LineRec{s.pos[3 * j], s.pos[3 * j + 1], s.pos[3 * j + 2], s.gm[j],
-                       s.vel[3 * j], s.vel[3 * j + 1], s.vel[3 * j + 2], s.rs[j] * s.rs[j],
-                       s.t0[j], s.t_window[j]};
-    __syncthreads();
-  }
-  __device__ void stage(const ssc_sources&, int, int) {}
-  template <int P>
-  __device__ void add_grad(int, double t, const double* x, const double* y, const double* z,
-                           double* gx, double* gy, double* gz, int j0, int js) const {
-    for (int j = j0; j < n; j += js) {
-      const LineRec r = rec[j];
-      const double dt = t - r.t0;
-      if (!(fabs(dt) < r.tw)) continue;
-      const double sx = r.x0 + r.vx * dt, sy = r.y0 + r.vy * dt, sz = r.z0 + r.vz * dt;
-#pragma unroll
-      for (int p = 0; p < P; ++p) plummer_grad(x[p] - sx, y[p] - sy, z[p] - sz, r.gm, r.rs2, gx[p], gy[p], gz[p]);
-    }
-  }
-};
 
 // Acceleration of P particles at grid row k / time t.
 template <int SK, int P>

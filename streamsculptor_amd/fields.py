@@ -1,0 +1,164 @@
+"""Auxiliary-ODE ("fields") interface (reference streamsculptor/fields.py).
+
+A field is a class with ``term(t, coords, args)``.  ``integrate_field`` lowers the field types
+whose right-hand side is known to the gfx950 kernels:
+
+* ``hamiltonian_field(pot)``            -> the orbit integrator (``[v, -grad Phi]``);
+* ``MassRadiusPerturbation_OTF(pertgen)`` -> the response kernel ``ssc_perturb_leapfrog``:
+  base orbit in ``pertgen.potential_base`` plus first-order responses to each perturbing
+  subhalo of ``pertgen.potential_perturbation`` (mass block) and, when the generator carries a
+  structural potential, the radius block (coords[1] of shape (Nsh, 12));
+* ``CustomField(term)`` holds arbitrary Python and cannot be lowered: NotImplementedError.
+
+Fixed-step fields need ``solver="leapfrog"`` and ``dt0=``.
+"""
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+from . import _lib
+from . import solvers as _solvers
+from .main import Solution, _as_float
+
+
+class hamiltonian_field:
+    """Standard Hamiltonian field (reference fields.py:101-113)."""
+
+    def __init__(self, pot):
+        self.pot = pot
+
+    def term(self, t, xv, args=None):
+        return self.pot.velocity_acceleration(t, xv, args)
+
+
+class MassRadiusPerturbation_OTF:
+    """First-order perturbations in subhalo mass (and radius) computed on the fly (reference
+    fields.py:159-206).  ``perturbation_generator`` exposes ``potential_base``,
+    ``potential_perturbation`` and optionally ``potential_structural`` (any non-None value
+    enables the radius block)."""
+
+    def __init__(self, perturbation_generator):
+        self.pertgen = perturbation_generator
+
+    def term(self, t, coords, args=None):
+        raise NotImplementedError("MassRadiusPerturbation_OTF is integrated by the fused response "
+                                  "kernel; evaluate the pieces through the potentials instead")
+
+
+class CustomField:
+    """User field with arbitrary Python ``term`` (reference fields.py:362-378): not lowerable."""
+
+    def __init__(self, term=None):
+        self._term = term
+
+    def term(self, t, coords, args=None):
+        return self._term(t, coords, args)
+
+
+def _times(ts, t0, t1, backwards_int):
+    """t0/t1 resolution of reference fields.py:58-82."""
+    tsn = np.asarray(_as_float(ts), dtype=np.float64).reshape(-1)
+    t0v, t1v = float(_as_float(t0)), float(_as_float(t1))
+    if t0v != t1v:
+        return tsn, t0v, t1v
+    if backwards_int:
+        return tsn, float(tsn.max()), float(tsn.min())
+    return tsn, float(tsn.min()), float(tsn.max())
+
+
+def integrate_field(w0=None, ts=None, dense=False, solver=None, field=None, args=None, rtol=1e-7,
+                    atol=1e-7, dtmin=0.05, dtmax=None, max_steps=1_000, jump_ts=None,
+                    backwards_int=False, t0=0.0, t1=0.0, step_ts=None, dt0=None):
+    """Integrate ``w0`` along ``field`` and save at ``ts`` (reference fields.py:34-99)."""
+    if dense:
+        raise NotImplementedError("dense=True is not available")
+    tsn, t0v, t1v = _times(ts, t0, t1, backwards_int)
+    if isinstance(field, hamiltonian_field):
+        return field.pot.integrate_orbit(w0=w0, ts=tsn, solver=solver, rtol=rtol, atol=atol,
+                                         dtmin=dtmin, dtmax=dtmax, max_steps=max_steps, t0=t0v,
+                                         t1=t1v, jump_ts=jump_ts, dt0=dt0)
+    if isinstance(field, MassRadiusPerturbation_OTF):
+        return _integrate_perturbation(field.pertgen, w0, tsn, t0v, t1v, solver, dt0)
+    if isinstance(field, CustomField):
+        raise NotImplementedError("CustomField holds arbitrary Python and cannot be lowered to "
+                                  "the gfx950 kernels")
+    raise NotImplementedError(f"field {type(field).__name__} has no kernel form")
+
+
+def _pert_parts(pertgen):
+    base = pertgen.potential_base
+    pert = pertgen.potential_perturbation
+    if base._kernel_sources() is not None:
+        raise NotImplementedError("the base potential of a perturbation must be analytic")
+    if pert._kernel_components():
+        raise NotImplementedError("the perturbing potential must consist of Plummer sources")
+    src = pert._kernel_sources()
+    radius = getattr(pertgen, "potential_structural", None) is not None
+    return base, src, radius
+
+
+def perturb_leapfrog(base, src, w0, dw0, start, t0, dt, nsteps, with_radius):
+    """Run ssc_perturb_leapfrog: base rows w0 (n, 6), responses dw0 (n, m, 6|12) or None."""
+    h = base._host_only()
+    w0 = _lib.f64(w0).reshape(-1, 6).contiguous()
+    n = w0.shape[0]
+    m = src.tensors()["gm"].numel()
+    dd = 12 if with_radius else 6
+    d0 = None if dw0 is None else _lib.f64(dw0).reshape(n, m, dd).contiguous()
+    st = None if start is None else _lib.i32(start)
+    out = torch.empty_like(w0)
+    dout = torch.empty(n, m, dd, dtype=torch.float64, device=w0.device)
+    if src.kind == _lib.SRC_TRACK:
+        g0, gdt = src.grid
+        if abs(g0 - t0) > 1e-9 * max(1.0, abs(t0)) or abs(gdt - dt) > 1e-12 * max(1.0, abs(dt)):
+            raise ValueError("the subhalo track must be sampled on the integration grid")
+    sd = src.desc()
+    _lib.call("ssc_perturb_leapfrog", _lib.ctypes.byref(h), _lib.ctypes.byref(sd), _lib.ptr(w0),
+              _lib.ptr(d0), _lib.ptr(st), n, float(t0), float(dt), int(nsteps), int(with_radius),
+              _lib.ptr(out), _lib.ptr(dout))
+    return out, dout
+
+
+def _integrate_perturbation(pertgen, w0, tsn, t0v, t1v, solver, dt0):
+    if _solvers.resolve(solver, default="dopri5") != "leapfrog":
+        raise NotImplementedError("perturbation fields run with solver='leapfrog' and dt0=")
+    from ._grid import leapfrog_grid
+    base, src, radius = _pert_parts(pertgen)
+    w, d = w0
+    w = _lib.f64(w).reshape(1, 6)
+    m = src.tensors()["gm"].numel()
+    d = _lib.f64(d).reshape(1, m, -1)
+    dd = d.shape[-1]
+    with_radius = radius and dd == 12
+    if not with_radius:
+        d = d[..., :6].contiguous()
+    dt, nsteps, kidx = leapfrog_grid(tsn, t0v, t1v, dt0)
+    if src.kind == _lib.SRC_TRACK:
+        g0, gdt = src.grid
+        off = (t0v - g0) / gdt
+        if abs(off - round(off)) > 1e-9 or abs(gdt - dt) > 1e-12 * abs(dt):
+            raise ValueError("the subhalo track must be sampled on the integration grid")
+        src = _shift_track(src, int(round(off)), nsteps)
+    ys_w, ys_d = [], []
+    cur_w, cur_d, kprev = w, d, 0
+    for k in kidx:
+        if k > kprev:
+            seg = _shift_track(src, int(kprev), int(k - kprev)) if src.kind == _lib.SRC_TRACK else src
+            cur_w, cur_d = perturb_leapfrog(base, seg, cur_w, cur_d, None, t0v + kprev * dt, dt,
+                                            int(k - kprev), with_radius)
+            kprev = int(k)
+        ys_w.append(cur_w[0])
+        full = torch.zeros(m, 12, dtype=torch.float64, device=cur_d.device)
+        full[:, :cur_d.shape[-1]] = cur_d[0]
+        ys_d.append(full if dd == 12 else full[:, :dd])
+    return Solution(_lib.f64(tsn), [torch.stack(ys_w), torch.stack(ys_d)])
+
+
+def _shift_track(src, k0, nsteps):
+    """A TRACK source set whose row 0 is row k0 of ``src``."""
+    from .main import SourceSet
+    t = src.tensors()
+    g0, gdt = src.grid
+    return SourceSet(src.kind, gm=t["gm"], rs=t["rs"], pos=t["pos"][k0:k0 + nsteps + 1],
+                     grid=(g0 + k0 * gdt, gdt))
