@@ -145,6 +145,42 @@ int ssc_perturb_leapfrog(const ssc_host_potential* host, const ssc_sources* pert
                          int64_t n, double t0, double dt, int32_t nsteps, int32_t with_radius,
                          double* w_out, double* dw_out, void* stream);
 
+/* ---------------------------------------------------------------------------------------
+ * Clump-force MLP (no counterpart in the reference, which has no network): 3 -> 256 -> 256 ->
+ * 256 -> 4 with SiLU, fp32 master parameters, BF16 MFMA products with fp32 accumulation.
+ * Parameter vector (floats): W1[256][3] b1[256] W2[256][256] b2[256] W3[256][256] b3[256]
+ * W4[4][256] b4[4].  x is fp32 [batch][3] (scaled coordinates), y fp32 [batch][4].
+ * --------------------------------------------------------------------------------------- */
+int64_t ssc_mlp_num_params(void);
+int64_t ssc_mlp_packed_bytes(void);
+int64_t ssc_mlp_workspace_bytes(int64_t batch);
+
+/* Training-form forward (activations kept in the caller's workspace). */
+int ssc_mlp_forward(const float* params, const float* x, int64_t batch, float* y,
+                    void* workspace, int64_t workspace_bytes, void* stream);
+
+/* Forward, weighted squared-error loss sum_b w_b |y_b - t_b|^2 / weight_sum (weight NULL =
+ * ones) into loss[0], and the gradient of that loss into grads[num_params]. */
+int ssc_mlp_loss_backward(const float* params, const float* x, const float* target,
+                          const float* weight, float weight_sum, int64_t batch, float* y,
+                          float* grads, float* loss, void* workspace, int64_t workspace_bytes,
+                          void* stream);
+
+/* Gradient of sum_b dy_b . y_b with respect to the parameters (vector-Jacobian product). */
+int ssc_mlp_backward(const float* params, const float* x, const float* dy, int64_t batch,
+                     float* grads, void* workspace, int64_t workspace_bytes, void* stream);
+
+/* Adam step with bias correction (step >= 1) and decoupled weight decay. */
+int ssc_mlp_adam(float* params, const float* grads, float* m, float* v, int64_t n, float lr,
+                 float beta1, float beta2, float eps, int32_t step, float weight_decay,
+                 void* stream);
+
+/* Pack parameters into the inference form (ssc_mlp_packed_bytes() bytes, device). */
+int ssc_mlp_pack(const float* params, void* packed, void* stream);
+
+/* Inference with the packed form: one wave evaluates 32 points, activations stay in VGPRs. */
+int ssc_mlp_infer(const void* packed, const float* x, int64_t batch, float* y, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

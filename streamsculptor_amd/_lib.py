@@ -45,6 +45,13 @@ SIGNATURES = {
     "ssc_eval": [_vp, _vp, _vp, _i64, _d, _i32, _vp, _vp, _vp, _vp],
     "ssc_leapfrog": [_vp, _vp, _vp, _vp, _i64, _d, _d, _i32, _vp, _vp, _i32, _vp, _vp],
     "ssc_perturb_leapfrog": [_vp, _vp, _vp, _vp, _vp, _i64, _d, _d, _i32, _i32, _vp, _vp],
+    "ssc_mlp_forward": [_vp, _vp, _i64, _vp, _vp, _i64],
+    "ssc_mlp_loss_backward": [_vp, _vp, _vp, _vp, ctypes.c_float, _i64, _vp, _vp, _vp, _vp, _i64],
+    "ssc_mlp_backward": [_vp, _vp, _vp, _i64, _vp, _vp, _i64],
+    "ssc_mlp_adam": [_vp, _vp, _vp, _vp, _i64, ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                     ctypes.c_float, _i32, ctypes.c_float],
+    "ssc_mlp_pack": [_vp, _vp],
+    "ssc_mlp_infer": [_vp, _vp, _i64, _vp],
     "ssc_release": [_vp, _vp, _vp, _vp, _i32, ctypes.c_int64, _vp, _vp, _vp, _vp, _vp, _vp],
 }
 
@@ -66,6 +73,11 @@ def load():
         L = ctypes.CDLL(LIB_PATH)
         L.ssc_last_error.restype = ctypes.c_char_p
         L.ssc_version.restype = ctypes.c_char_p
+        for name in ("ssc_mlp_num_params", "ssc_mlp_packed_bytes"):
+            getattr(L, name).restype = ctypes.c_int64
+            getattr(L, name).argtypes = []
+        L.ssc_mlp_workspace_bytes.restype = ctypes.c_int64
+        L.ssc_mlp_workspace_bytes.argtypes = [ctypes.c_int64]
         for name, args in SIGNATURES.items():
             fn = getattr(L, name)
             fn.argtypes = args

@@ -1,0 +1,552 @@
+// The clump-force MLP on gfx950: BF16 MFMA training (forward, backward, Adam) and the packed,
+// register-chained inference form.
+//
+// Network: 3 -> 256 -> 256 -> 256 -> 4, SiLU after the three hidden layers.  Master parameters
+// are fp32; every matrix product runs on v_mfma_f32_32x32x16_bf16 with BF16 operands (weights
+// and activations rounded to nearest-even) and fp32 accumulation.  The 3-wide first layer runs
+// on the fp32 VALU (BF16-rounded weights, fp32 inputs) so the input coordinates keep fp32
+// resolution.
+//
+// Training path: batch-major activations, generic strided GEMM C = A . B (64x64 block tile,
+// four 32x32 wave tiles, split-K for the batch reductions of the weight gradients).
+// Inference path: weights pre-packed in MFMA fragment order; a wave evaluates 32 points with
+// the activations kept in registers between layers (particles on the MFMA N axis).
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <string>
+
+#include "ssc_device.h"
+#include "ssc_mlp.h"
+
+namespace ssc {
+namespace mlp {
+
+// ------------------------------------------------------------------------------------------
+// generic GEMM: Cz[m][n] = sum_{k in chunk z} A(m,k) B(k,n) (+ bias[n] when bias != NULL)
+// A(m,k) = A[m*sAm + k*sAk], B(k,n) = B[k*sBk + n*sBn]  (BF16 bits)
+// ------------------------------------------------------------------------------------------
+constexpr int GT = 64;   // block tile
+constexpr int GK = 32;   // k per LDS stage
+constexpr int GPAD = 40; // LDS row pitch (elements): 80 B keeps 16-byte fragment reads aligned
+
+__global__ __launch_bounds__(256) void gemm_kernel(const uint16_t* __restrict__ A, int64_t sAm,
+                                                   int64_t sAk, const uint16_t* __restrict__ B,
+                                                   int64_t sBk, int64_t sBn, float* __restrict__ C,
+                                                   int64_t ldc, int64_t cz, int M, int N, int K,
+                                                   int kchunk, const float* __restrict__ bias) {
+  __shared__ __align__(16) uint16_t As[GT][GPAD];
+  __shared__ __align__(16) uint16_t Bs[GT][GPAD];
+  const int m0 = blockIdx.x * GT, n0 = blockIdx.y * GT;
+  const int kb = blockIdx.z * kchunk, ke = min(K, kb + kchunk);
+  const int t = threadIdx.x, l = t & 63, w = t >> 6;
+  const int wm = w & 1, wn = w >> 1;
+  f32x16 acc = {};
+  const int lr = t >> 2, lk = (t & 3) * 8;  // loader: row/col of the tile, 8 k's each
+  for (int k0 = kb; k0 < ke; k0 += GK) {
+    {
+      const int m = m0 + lr;
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        const int k = k0 + lk + e;
+        As[lr][lk + e] = (m < M && k < ke) ? A[(int64_t)m * sAm + (int64_t)k * sAk] : (uint16_t)0;
+      }
+      const int n = n0 + lr;
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        const int k = k0 + lk + e;
+        Bs[lr][lk + e] = (n < N && k < ke) ? B[(int64_t)k * sBk + (int64_t)n * sBn] : (uint16_t)0;
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int ks = 0; ks < GK; ks += 16) {
+      const bf16x8 a = *reinterpret_cast<const bf16x8*>(&As[32 * wm + (l & 31)][ks + 8 * (l >> 5)]);
+      const bf16x8 b = *reinterpret_cast<const bf16x8*>(&Bs[32 * wn + (l & 31)][ks + 8 * (l >> 5)]);
+      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+    }
+    __syncthreads();
+  }
+  const int col = n0 + 32 * wn + (l & 31);
+  if (col >= N) return;
+  const float bv = bias ? bias[col] : 0.0f;
+#pragma unroll
+  for (int r = 0; r < 16; ++r) {
+    const int row = m0 + 32 * wm + 8 * (r >> 2) + 4 * (l >> 5) + (r & 3);
+    if (row < M) C[(int64_t)blockIdx.z * cz + (int64_t)row * ldc + col] = acc[r] + bv;
+  }
+}
+
+__global__ void splitk_reduce_kernel(const float* __restrict__ P, int nz, int64_t mn,
+                                     float* __restrict__ C) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= mn) return;
+  float s = 0.0f;
+  for (int z = 0; z < nz; ++z) s += P[(int64_t)z * mn + i];
+  C[i] = s;
+}
+
+// ------------------------------------------------------------------------------------------
+// elementwise / reduction kernels
+// ------------------------------------------------------------------------------------------
+__global__ void round_kernel(const float* __restrict__ src, uint16_t* __restrict__ dst, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = f2bf(src[i]);
+}
+
+// Z1[b][o] = sum_i x[b][i] bf16(W1[o][i]) + b1[o];  A1 = bf16(silu(Z1))
+__global__ void layer1_fwd_kernel(const float* __restrict__ p, const float* __restrict__ x,
+                                  int64_t nb, float* __restrict__ Z, uint16_t* __restrict__ Aout) {
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= nb * H) return;
+  const int64_t b = idx / H;
+  const int o = (int)(idx - b * H);
+  float z = p[OB1 + o];
+#pragma unroll
+  for (int i = 0; i < IN; ++i) z = fmaf(x[b * IN + i], bf2f(f2bf(p[OW1 + o * IN + i])), z);
+  Z[idx] = z;
+  Aout[idx] = f2bf(silu(z));
+}
+
+__global__ void act_fwd_kernel(const float* __restrict__ Z, uint16_t* __restrict__ Aout, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) Aout[i] = f2bf(silu(Z[i]));
+}
+
+// dZ = dA * silu'(Z): fp32 in place in dA, BF16 copy in dZb
+__global__ void act_bwd_kernel(float* __restrict__ dA, const float* __restrict__ Z,
+                               uint16_t* __restrict__ dZb, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float d = dA[i] * silu_grad(Z[i]);
+  dA[i] = d;
+  if (dZb) dZb[i] = f2bf(d);
+}
+
+// Weighted squared error of the 4 outputs: loss = sum_b w_b |Y_b - T_b|^2 / sum_b w_b;
+// dY = 2 w_b (Y_b - T_b) / sum_b w_b (fp32 and BF16); per-block partial losses.
+__global__ __launch_bounds__(256) void mse_kernel(const float* __restrict__ Y,
+                                                  const float* __restrict__ T,
+                                                  const float* __restrict__ wt, float inv_wsum,
+                                                  int64_t nb, float* __restrict__ dY,
+                                                  uint16_t* __restrict__ dYb,
+                                                  float* __restrict__ partial) {
+  __shared__ float red[256];
+  const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  float acc = 0.0f;
+  if (b < nb) {
+    const float w = wt ? wt[b] : 1.0f;
+#pragma unroll
+    for (int j = 0; j < OUT; ++j) {
+      const float d = Y[b * OUT + j] - T[b * OUT + j];
+      acc += w * d * d;
+      const float g = 2.0f * w * d * inv_wsum;
+      dY[b * OUT + j] = g;
+      dYb[b * OUT + j] = f2bf(g);
+    }
+  }
+  red[threadIdx.x] = acc * inv_wsum;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
+__global__ __launch_bounds__(256) void sum_kernel(const float* __restrict__ partial, int n,
+                                                  float* __restrict__ out) {
+  __shared__ float red[256];
+  float s = 0.0f;
+  for (int i = threadIdx.x; i < n; i += 256) s += partial[i];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int k = 128; k > 0; k >>= 1) {
+    if (threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[0] = red[0];
+}
+
+// out[o] = sum_b D[b*ld + o] for o < ncol (block: 64 columns x 4 row phases)
+__global__ __launch_bounds__(256) void colsum_kernel(const float* __restrict__ D, int64_t ld,
+                                                     int64_t nb, int ncol, float* __restrict__ out) {
+  __shared__ float red[4][64];
+  const int c = blockIdx.x * 64 + (threadIdx.x & 63), ph = threadIdx.x >> 6;
+  float s = 0.0f;
+  if (c < ncol)
+    for (int64_t b = ph; b < nb; b += 4) s += D[b * ld + c];
+  red[ph][threadIdx.x & 63] = s;
+  __syncthreads();
+  if (ph == 0 && c < ncol)
+    out[c] = red[0][threadIdx.x] + red[1][threadIdx.x] + red[2][threadIdx.x] + red[3][threadIdx.x];
+}
+
+// dW1[o][i] = sum_b dZ1[b][o] x[b][i]
+__global__ __launch_bounds__(256) void layer1_wgrad_kernel(const float* __restrict__ dZ,
+                                                           const float* __restrict__ x, int64_t nb,
+                                                           float* __restrict__ dW1) {
+  __shared__ float red[4][64][IN];
+  const int o = blockIdx.x * 64 + (threadIdx.x & 63), ph = threadIdx.x >> 6;
+  float s[IN] = {0.0f, 0.0f, 0.0f};
+  for (int64_t b = ph; b < nb; b += 4) {
+    const float d = dZ[b * H + o];
+#pragma unroll
+    for (int i = 0; i < IN; ++i) s[i] = fmaf(d, x[b * IN + i], s[i]);
+  }
+#pragma unroll
+  for (int i = 0; i < IN; ++i) red[ph][threadIdx.x & 63][i] = s[i];
+  __syncthreads();
+  if (ph == 0)
+#pragma unroll
+    for (int i = 0; i < IN; ++i)
+      dW1[o * IN + i] = red[0][threadIdx.x][i] + red[1][threadIdx.x][i] + red[2][threadIdx.x][i] +
+                        red[3][threadIdx.x][i];
+}
+
+__global__ void adam_kernel(float* __restrict__ p, const float* __restrict__ g,
+                            float* __restrict__ m, float* __restrict__ v, int64_t n, float lr,
+                            float b1, float b2, float eps, float c1, float c2, float wd) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float gi = g[i];
+  const float mi = b1 * m[i] + (1.0f - b1) * gi;
+  const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
+  m[i] = mi;
+  v[i] = vi;
+  p[i] -= lr * ((mi / c1) / (sqrtf(vi / c2) + eps) + wd * p[i]);
+}
+
+// Packed inference form (layout in ssc_mlp.h).
+__global__ void pack_kernel(const float* __restrict__ p, uint16_t* __restrict__ pk16,
+                            float* __restrict__ pk32) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < 2 * FRAG_L + FRAG_4) {
+    int64_t r = i;
+    const float* W;
+    int rows;
+    if (r < FRAG_L) { W = p + OW2; rows = H; }
+    else if (r < 2 * FRAG_L) { r -= FRAG_L; W = p + OW3; rows = H; }
+    else { r -= 2 * FRAG_L; W = p + OW4; rows = OUT; }
+    const int e = (int)(r & 7), l = (int)((r >> 3) & 63);
+    const int64_t tq = r >> 9;
+    const int q = (int)(tq % (H / 16)), tt = (int)(tq / (H / 16));
+    const int row = 32 * tt + (l & 31);
+    const int col = kperm(16 * q + 8 * (l >> 5) + e);
+    pk16[i] = row < rows ? f2bf(W[(int64_t)row * H + col]) : (uint16_t)0;
+  }
+  if (i < PK_F32_END) {
+    float v;
+    if (i < PK_B1) v = bf2f(f2bf(p[OW1 + i]));
+    else if (i < PK_B2) v = p[OB1 + (i - PK_B1)];
+    else if (i < PK_B3) v = p[OB2 + (i - PK_B2)];
+    else if (i < PK_B4) v = p[OB3 + (i - PK_B3)];
+    else v = (i - PK_B4) < OUT ? p[OB4 + (i - PK_B4)] : 0.0f;
+    pk32[i] = v;
+  }
+}
+
+}  // namespace mlp
+
+// ------------------------------------------------------------------------------------------
+// register-chained forward of 32 points per wave (used by ssc_mlp_infer and the surrogate
+// integrator).  Lane l evaluates point l % 32; lanes of the upper half (l >= 32) carry the same
+// point.  Returns the 4 outputs in y on lanes l < 32.
+// ------------------------------------------------------------------------------------------
+__device__ __forceinline__ mlp::bf16x8 pack8(const float* v) {
+  mlp::i16x8 r;
+#pragma unroll
+  for (int e = 0; e < 8; ++e) r[e] = (short)mlp::f2bf(v[e]);
+  return __builtin_bit_cast(mlp::bf16x8, r);
+}
+
+__device__ void mlp_tile_forward(const uint16_t* __restrict__ pk16, const float* __restrict__ pk32,
+                                 float x0, float x1, float x2, float* y) {
+  using namespace mlp;
+  const int l = threadIdx.x & 63, hh = l >> 5;
+  const bf16x8* W2 = reinterpret_cast<const bf16x8*>(pk16 + PK_W2);
+  const bf16x8* W3 = reinterpret_cast<const bf16x8*>(pk16 + PK_W3);
+  const bf16x8* W4 = reinterpret_cast<const bf16x8*>(pk16 + PK_W4);
+  const float* w1 = pk32 + PK_W1;
+  bf16x8 bin[H / 16];
+  // layer 1 (VALU): k-slot 16q + 8hh + e of the layer-2 B operand is neuron kperm(.)
+#pragma unroll
+  for (int q = 0; q < H / 16; ++q) {
+    float v[8];
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      const int n = kperm(16 * q + 8 * hh + e);
+      float z = pk32[PK_B1 + n];
+      z = fmaf(x0, w1[3 * n], z);
+      z = fmaf(x1, w1[3 * n + 1], z);
+      z = fmaf(x2, w1[3 * n + 2], z);
+      v[e] = silu(z);
+    }
+    bin[q] = pack8(v);
+  }
+  // two chained 256x256 layers
+#pragma unroll
+  for (int layer = 0; layer < 2; ++layer) {
+    const bf16x8* W = layer == 0 ? W2 : W3;
+    const float* bias = pk32 + (layer == 0 ? PK_B2 : PK_B3);
+    f32x16 acc[H / 32];
+#pragma unroll
+    for (int t = 0; t < H / 32; ++t)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) acc[t][r] = bias[32 * t + 8 * (r >> 2) + 4 * hh + (r & 3)];
+#pragma unroll
+    for (int q = 0; q < H / 16; ++q)
+#pragma unroll
+      for (int t = 0; t < H / 32; ++t)
+        acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(W[(t * (H / 16) + q) * 64 + l], bin[q],
+                                                        acc[t], 0, 0, 0);
+#pragma unroll
+    for (int t = 0; t < H / 32; ++t)
+#pragma unroll
+      for (int u = 0; u < 2; ++u) {
+        float v[8];
+#pragma unroll
+        for (int e = 0; e < 8; ++e) v[e] = silu(acc[t][8 * u + e]);
+        bin[2 * t + u] = pack8(v);
+      }
+  }
+  // output layer (4 rows of one padded 32-row tile)
+  f32x16 out;
+#pragma unroll
+  for (int r = 0; r < 16; ++r) {
+    const int row = 8 * (r >> 2) + 4 * hh + (r & 3);
+    out[r] = row < OUT ? pk32[PK_B4 + row] : 0.0f;
+  }
+#pragma unroll
+  for (int q = 0; q < H / 16; ++q)
+    out = __builtin_amdgcn_mfma_f32_32x32x16_bf16(W4[q * 64 + l], bin[q], out, 0, 0, 0);
+#pragma unroll
+  for (int j = 0; j < OUT; ++j) y[j] = out[j];
+}
+
+__global__ __launch_bounds__(256) void mlp_infer_kernel(const uint16_t* __restrict__ pk16,
+                                                        const float* __restrict__ pk32,
+                                                        const float* __restrict__ x, int64_t nb,
+                                                        float* __restrict__ y) {
+  const int64_t p = ((int64_t)blockIdx.x * 256 + threadIdx.x) / 64 * 32 + (threadIdx.x & 31);
+  const bool ok = p < nb;
+  const float x0 = ok ? x[3 * p] : 0.0f, x1 = ok ? x[3 * p + 1] : 0.0f, x2 = ok ? x[3 * p + 2] : 0.0f;
+  float o[4];
+  mlp_tile_forward(pk16, pk32, x0, x1, x2, o);
+  if (ok && (threadIdx.x & 63) < 32)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) y[4 * p + j] = o[j];
+}
+
+// ------------------------------------------------------------------------------------------
+// host orchestration
+// ------------------------------------------------------------------------------------------
+namespace {
+
+using namespace mlp;
+
+struct Workspace {
+  float *Z1, *Z2, *Z3, *dA, *dY, *part, *red;
+  uint16_t *A1, *A2, *A3, *dZb, *dYb, *W2b, *W3b, *W4b;
+};
+
+int64_t splitk_for(int64_t nb) {
+  int64_t z = nb / 512;
+  return z < 1 ? 1 : (z > 64 ? 64 : z);
+}
+
+int64_t ws_bytes(int64_t nb) {
+  const int64_t nz = splitk_for(nb);
+  int64_t b = 0;
+  b += 3 * nb * H * 4;            // Z1..Z3
+  b += nb * H * 4;                // dA / dZ (fp32)
+  b += 2 * nb * OUT * 4;          // dY, (spare)
+  b += nz * H * H * 4;            // split-K partials
+  b += ((nb + 255) / 256 + 64) * 4;  // loss partials
+  b += 3 * nb * H * 2;            // A1..A3
+  b += nb * H * 2;                // dZ bf16
+  b += nb * OUT * 2;              // dY bf16
+  b += (2 * H * H + OUT * H) * 2; // BF16 weights
+  return b + 16 * 64;             // alignment slack
+}
+
+Workspace carve(void* base, int64_t nb) {
+  char* p = (char*)base;
+  auto take = [&](int64_t bytes) {
+    char* r = p;
+    p += (bytes + 63) / 64 * 64;
+    return r;
+  };
+  const int64_t nz = splitk_for(nb);
+  Workspace w;
+  w.Z1 = (float*)take(nb * H * 4);
+  w.Z2 = (float*)take(nb * H * 4);
+  w.Z3 = (float*)take(nb * H * 4);
+  w.dA = (float*)take(nb * H * 4);
+  w.dY = (float*)take(2 * nb * OUT * 4);
+  w.part = (float*)take(nz * H * H * 4);
+  w.red = (float*)take(((nb + 255) / 256 + 64) * 4);
+  w.A1 = (uint16_t*)take(nb * H * 2);
+  w.A2 = (uint16_t*)take(nb * H * 2);
+  w.A3 = (uint16_t*)take(nb * H * 2);
+  w.dZb = (uint16_t*)take(nb * H * 2);
+  w.dYb = (uint16_t*)take(nb * OUT * 2);
+  w.W2b = (uint16_t*)take(H * H * 2);
+  w.W3b = (uint16_t*)take(H * H * 2);
+  w.W4b = (uint16_t*)take(OUT * H * 2);
+  return w;
+}
+
+inline unsigned nblk(int64_t n, int b = 256) { return (unsigned)((n + b - 1) / b); }
+
+void gemm(const uint16_t* A, int64_t sAm, int64_t sAk, const uint16_t* B, int64_t sBk,
+          int64_t sBn, float* C, int M, int N, int K, const float* bias, hipStream_t st,
+          float* part = nullptr, int nz = 1) {
+  if (nz <= 1 || !part) {
+    hipLaunchKernelGGL(gemm_kernel, dim3(nblk(M, GT), nblk(N, GT), 1), dim3(256), 0, st, A, sAm,
+                       sAk, B, sBk, sBn, C, (int64_t)N, (int64_t)0, M, N, K, K, bias);
+    return;
+  }
+  int kchunk = (K + nz - 1) / nz;
+  kchunk = (kchunk + GK - 1) / GK * GK;
+  nz = (K + kchunk - 1) / kchunk;
+  hipLaunchKernelGGL(gemm_kernel, dim3(nblk(M, GT), nblk(N, GT), nz), dim3(256), 0, st, A, sAm, sAk,
+                     B, sBk, sBn, part, (int64_t)N, (int64_t)M * N, M, N, K, kchunk, nullptr);
+  hipLaunchKernelGGL(splitk_reduce_kernel, dim3(nblk((int64_t)M * N)), dim3(256), 0, st, part, nz,
+                     (int64_t)M * N, C);
+}
+
+void round_weights(const float* p, Workspace& w, hipStream_t st) {
+  hipLaunchKernelGGL(round_kernel, dim3(nblk(H * H)), dim3(256), 0, st, p + OW2, w.W2b, (int64_t)H * H);
+  hipLaunchKernelGGL(round_kernel, dim3(nblk(H * H)), dim3(256), 0, st, p + OW3, w.W3b, (int64_t)H * H);
+  hipLaunchKernelGGL(round_kernel, dim3(nblk(OUT * H)), dim3(256), 0, st, p + OW4, w.W4b, (int64_t)OUT * H);
+}
+
+void forward(const float* p, const float* x, int64_t nb, float* y, Workspace& w, hipStream_t st) {
+  round_weights(p, w, st);
+  hipLaunchKernelGGL(layer1_fwd_kernel, dim3(nblk(nb * H)), dim3(256), 0, st, p, x, nb, w.Z1, w.A1);
+  gemm(w.A1, H, 1, w.W2b, 1, H, w.Z2, (int)nb, H, H, p + OB2, st);
+  hipLaunchKernelGGL(act_fwd_kernel, dim3(nblk(nb * H)), dim3(256), 0, st, w.Z2, w.A2, nb * H);
+  gemm(w.A2, H, 1, w.W3b, 1, H, w.Z3, (int)nb, H, H, p + OB3, st);
+  hipLaunchKernelGGL(act_fwd_kernel, dim3(nblk(nb * H)), dim3(256), 0, st, w.Z3, w.A3, nb * H);
+  gemm(w.A3, H, 1, w.W4b, 1, H, y, (int)nb, OUT, H, p + OB4, st);
+}
+
+// Backward from dY (fp32 in w.dY, BF16 in w.dYb) given a forward with the same workspace.
+void backward(const float* x, int64_t nb, float* g, Workspace& w, hipStream_t st) {
+  const int nz = (int)splitk_for(nb);
+  // output layer
+  gemm(w.dYb, 1, OUT, w.A3, H, 1, g + OW4, OUT, H, (int)nb, nullptr, st, w.part, nz);
+  hipLaunchKernelGGL(colsum_kernel, dim3(1), dim3(256), 0, st, w.dY, (int64_t)OUT, nb, OUT, g + OB4);
+  gemm(w.dYb, OUT, 1, w.W4b, H, 1, w.dA, (int)nb, H, OUT, nullptr, st);
+  // hidden layers 3 and 2
+  const float* Zs[2] = {w.Z3, w.Z2};
+  const uint16_t* Ain[2] = {w.A2, w.A1};
+  const uint16_t* Wb[2] = {w.W3b, w.W2b};
+  const int64_t oW[2] = {OW3, OW2}, oB[2] = {OB3, OB2};
+  for (int s = 0; s < 2; ++s) {
+    hipLaunchKernelGGL(act_bwd_kernel, dim3(nblk(nb * H)), dim3(256), 0, st, w.dA, Zs[s], w.dZb, nb * H);
+    gemm(w.dZb, 1, H, Ain[s], H, 1, g + oW[s], H, H, (int)nb, nullptr, st, w.part, nz);
+    hipLaunchKernelGGL(colsum_kernel, dim3(H / 64), dim3(256), 0, st, w.dA, (int64_t)H, nb, H, g + oB[s]);
+    gemm(w.dZb, H, 1, Wb[s], H, 1, w.dA, (int)nb, H, H, nullptr, st);
+  }
+  // first layer (fp32)
+  hipLaunchKernelGGL(act_bwd_kernel, dim3(nblk(nb * H)), dim3(256), 0, st, w.dA, w.Z1,
+                     (uint16_t*)nullptr, nb * H);
+  hipLaunchKernelGGL(layer1_wgrad_kernel, dim3(H / 64), dim3(256), 0, st, w.dA, x, nb, g + OW1);
+  hipLaunchKernelGGL(colsum_kernel, dim3(H / 64), dim3(256), 0, st, w.dA, (int64_t)H, nb, H, g + OB1);
+}
+
+}  // namespace
+}  // namespace ssc
+
+using namespace ssc;
+using namespace ssc::mlp;
+
+extern "C" {
+
+int64_t ssc_mlp_num_params(void) { return NPARAM; }
+
+int64_t ssc_mlp_packed_bytes(void) { return PACKED_BYTES; }
+
+int64_t ssc_mlp_workspace_bytes(int64_t batch) { return batch > 0 ? ws_bytes(batch) : 0; }
+
+int ssc_mlp_forward(const float* params, const float* x, int64_t batch, float* y, void* workspace,
+                    int64_t workspace_bytes, void* stream) {
+  if (!params || !x || !y || !workspace || batch <= 0)
+    return fail(SSC_E_ARG, "ssc_mlp_forward: NULL pointer or empty batch");
+  if (batch > (1 << 24)) return fail(SSC_E_LIMIT, "ssc_mlp_forward: batch above 16M");
+  if (workspace_bytes < ws_bytes(batch)) return fail(SSC_E_ARG, "ssc_mlp_forward: workspace too small");
+  Workspace w = carve(workspace, batch);
+  forward(params, x, batch, y, w, (hipStream_t)stream);
+  return check_launch("ssc_mlp_forward");
+}
+
+int ssc_mlp_loss_backward(const float* params, const float* x, const float* target,
+                          const float* weight, float weight_sum, int64_t batch, float* y,
+                          float* grads, float* loss, void* workspace, int64_t workspace_bytes,
+                          void* stream) {
+  if (!params || !x || !target || !y || !grads || !loss || !workspace || batch <= 0)
+    return fail(SSC_E_ARG, "ssc_mlp_loss_backward: NULL pointer or empty batch");
+  if (batch > (1 << 24)) return fail(SSC_E_LIMIT, "ssc_mlp_loss_backward: batch above 16M");
+  if (workspace_bytes < ws_bytes(batch)) return fail(SSC_E_ARG, "ssc_mlp_loss_backward: workspace too small");
+  if (!(weight_sum > 0.0f)) return fail(SSC_E_ARG, "ssc_mlp_loss_backward: weight_sum must be > 0");
+  hipStream_t st = (hipStream_t)stream;
+  Workspace w = carve(workspace, batch);
+  forward(params, x, batch, y, w, st);
+  const unsigned nbk = nblk(batch);
+  hipLaunchKernelGGL(mse_kernel, dim3(nbk), dim3(256), 0, st, y, target, weight, 1.0f / weight_sum,
+                     batch, w.dY, w.dYb, w.red);
+  hipLaunchKernelGGL(sum_kernel, dim3(1), dim3(256), 0, st, w.red, (int)nbk, loss);
+  backward(x, batch, grads, w, st);
+  return check_launch("ssc_mlp_loss_backward");
+}
+
+int ssc_mlp_backward(const float* params, const float* x, const float* dy, int64_t batch,
+                     float* grads, void* workspace, int64_t workspace_bytes, void* stream) {
+  if (!params || !x || !dy || !grads || !workspace || batch <= 0)
+    return fail(SSC_E_ARG, "ssc_mlp_backward: NULL pointer or empty batch");
+  if (workspace_bytes < ws_bytes(batch)) return fail(SSC_E_ARG, "ssc_mlp_backward: workspace too small");
+  hipStream_t st = (hipStream_t)stream;
+  Workspace w = carve(workspace, batch);
+  // recompute the forward into the workspace, then copy dy into the fp32/BF16 buffers
+  float* ytmp = w.dY + batch * OUT;
+  forward(params, x, batch, ytmp, w, st);
+  hipMemcpyAsync(w.dY, dy, batch * OUT * 4, hipMemcpyDeviceToDevice, st);
+  hipLaunchKernelGGL(round_kernel, dim3(nblk(batch * OUT)), dim3(256), 0, st, dy, w.dYb, batch * OUT);
+  backward(x, batch, grads, w, st);
+  return check_launch("ssc_mlp_backward");
+}
+
+int ssc_mlp_adam(float* params, const float* grads, float* m, float* v, int64_t n, float lr,
+                 float beta1, float beta2, float eps, int32_t step, float weight_decay,
+                 void* stream) {
+  if (!params || !grads || !m || !v || n <= 0 || step <= 0)
+    return fail(SSC_E_ARG, "ssc_mlp_adam: bad argument");
+  const float c1 = 1.0f - powf(beta1, (float)step), c2 = 1.0f - powf(beta2, (float)step);
+  hipLaunchKernelGGL(adam_kernel, dim3(nblk(n)), dim3(256), 0, (hipStream_t)stream, params, grads,
+                     m, v, n, lr, beta1, beta2, eps, c1, c2, weight_decay);
+  return check_launch("ssc_mlp_adam");
+}
+
+int ssc_mlp_pack(const float* params, void* packed, void* stream) {
+  if (!params || !packed) return fail(SSC_E_ARG, "ssc_mlp_pack: NULL pointer");
+  uint16_t* pk16 = (uint16_t*)packed;
+  float* pk32 = (float*)((char*)packed + PK_F32_BASE_BYTES);
+  hipLaunchKernelGGL(pack_kernel, dim3(nblk(2 * FRAG_L + FRAG_4)), dim3(256), 0, (hipStream_t)stream,
+                     params, pk16, pk32);
+  return check_launch("ssc_mlp_pack");
+}
+
+int ssc_mlp_infer(const void* packed, const float* x, int64_t batch, float* y, void* stream) {
+  if (!packed || !x || !y || batch < 0) return fail(SSC_E_ARG, "ssc_mlp_infer: bad argument");
+  if (batch == 0) return SSC_OK;
+  const uint16_t* pk16 = (const uint16_t*)packed;
+  const float* pk32 = (const float*)((const char*)packed + PK_F32_BASE_BYTES);
+  const int64_t blocks = (batch + 127) / 128;  // 4 waves x 32 points
+  hipLaunchKernelGGL(mlp_infer_kernel, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream,
+                     pk16, pk32, x, batch, y);
+  return check_launch("ssc_mlp_infer");
+}
+
+}  // extern "C"
