@@ -181,6 +181,19 @@ int ssc_mlp_pack(const float* params, void* packed, void* stream);
 /* Inference with the packed form: one wave evaluates 32 points, activations stay in VGPRs. */
 int ssc_mlp_infer(const void* packed, const float* x, int64_t batch, float* y, void* stream);
 
+/* Leapfrog of n particles in host + clumps with the clump force from the packed MLP
+ * (no counterpart in the reference).  scaling[17] (host doubles): input centre[3], input
+ * half-width[3], phi scale, grad-phi scale, residual-grid origin[3], cell size[3], cells per
+ * axis[3].  flags[nx][ny][nz] (device, 1 = exact sum): positions in flagged cells or outside the
+ * grid use the exact sum over `clumps` (STATIC).  fallback_count (device, may be NULL)
+ * accumulates the number of particle-steps that fell back.  Same grid/start/output semantics
+ * as ssc_leapfrog. */
+int ssc_surrogate_leapfrog(const ssc_host_potential* host, const ssc_sources* clumps,
+                           const double* scaling, const void* packed, const uint8_t* flags,
+                           const double* w0, const int32_t* start_step, int64_t n, double t0,
+                           double dt, int32_t nsteps, double* w_out,
+                           unsigned long long* fallback_count, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -52,6 +52,7 @@ SIGNATURES = {
                      ctypes.c_float, _i32, ctypes.c_float],
     "ssc_mlp_pack": [_vp, _vp],
     "ssc_mlp_infer": [_vp, _vp, _i64, _vp],
+    "ssc_surrogate_leapfrog": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _i64, _d, _d, _i32, _vp, _vp],
     "ssc_release": [_vp, _vp, _vp, _vp, _i32, ctypes.c_int64, _vp, _vp, _vp, _vp, _vp, _vp],
 }
 

@@ -18,6 +18,7 @@
 
 #include "ssc_device.h"
 #include "ssc_mlp.h"
+#include "ssc_mlp_tile.h"
 
 namespace ssc {
 namespace mlp {
@@ -248,82 +249,6 @@ __global__ void pack_kernel(const float* __restrict__ p, uint16_t* __restrict__ 
 
 }  // namespace mlp
 
-// ------------------------------------------------------------------------------------------
-// register-chained forward of 32 points per wave (used by ssc_mlp_infer and the surrogate
-// integrator).  Lane l evaluates point l % 32; lanes of the upper half (l >= 32) carry the same
-// point.  Returns the 4 outputs in y on lanes l < 32.
-// ------------------------------------------------------------------------------------------
-__device__ __forceinline__ mlp::bf16x8 pack8(const float* v) {
-  mlp::i16x8 r;
-#pragma unroll
-  for (int e = 0; e < 8; ++e) r[e] = (short)mlp::f2bf(v[e]);
-  return __builtin_bit_cast(mlp::bf16x8, r);
-}
-
-__device__ void mlp_tile_forward(const uint16_t* __restrict__ pk16, const float* __restrict__ pk32,
-                                 float x0, float x1, float x2, float* y) {
-  using namespace mlp;
-  const int l = threadIdx.x & 63, hh = l >> 5;
-  const bf16x8* W2 = reinterpret_cast<const bf16x8*>(pk16 + PK_W2);
-  const bf16x8* W3 = reinterpret_cast<const bf16x8*>(pk16 + PK_W3);
-  const bf16x8* W4 = reinterpret_cast<const bf16x8*>(pk16 + PK_W4);
-  const float* w1 = pk32 + PK_W1;
-  bf16x8 bin[H / 16];
-  // layer 1 (VALU): k-slot 16q + 8hh + e of the layer-2 B operand is neuron kperm(.)
-#pragma unroll
-  for (int q = 0; q < H / 16; ++q) {
-    float v[8];
-#pragma unroll
-    for (int e = 0; e < 8; ++e) {
-      const int n = kperm(16 * q + 8 * hh + e);
-      float z = pk32[PK_B1 + n];
-      z = fmaf(x0, w1[3 * n], z);
-      z = fmaf(x1, w1[3 * n + 1], z);
-      z = fmaf(x2, w1[3 * n + 2], z);
-      v[e] = silu(z);
-    }
-    bin[q] = pack8(v);
-  }
-  // two chained 256x256 layers
-#pragma unroll
-  for (int layer = 0; layer < 2; ++layer) {
-    const bf16x8* W = layer == 0 ? W2 : W3;
-    const float* bias = pk32 + (layer == 0 ? PK_B2 : PK_B3);
-    f32x16 acc[H / 32];
-#pragma unroll
-    for (int t = 0; t < H / 32; ++t)
-#pragma unroll
-      for (int r = 0; r < 16; ++r) acc[t][r] = bias[32 * t + 8 * (r >> 2) + 4 * hh + (r & 3)];
-#pragma unroll
-    for (int q = 0; q < H / 16; ++q)
-#pragma unroll
-      for (int t = 0; t < H / 32; ++t)
-        acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(W[(t * (H / 16) + q) * 64 + l], bin[q],
-                                                        acc[t], 0, 0, 0);
-#pragma unroll
-    for (int t = 0; t < H / 32; ++t)
-#pragma unroll
-      for (int u = 0; u < 2; ++u) {
-        float v[8];
-#pragma unroll
-        for (int e = 0; e < 8; ++e) v[e] = silu(acc[t][8 * u + e]);
-        bin[2 * t + u] = pack8(v);
-      }
-  }
-  // output layer (4 rows of one padded 32-row tile)
-  f32x16 out;
-#pragma unroll
-  for (int r = 0; r < 16; ++r) {
-    const int row = 8 * (r >> 2) + 4 * hh + (r & 3);
-    out[r] = row < OUT ? pk32[PK_B4 + row] : 0.0f;
-  }
-#pragma unroll
-  for (int q = 0; q < H / 16; ++q)
-    out = __builtin_amdgcn_mfma_f32_32x32x16_bf16(W4[q * 64 + l], bin[q], out, 0, 0, 0);
-#pragma unroll
-  for (int j = 0; j < OUT; ++j) y[j] = out[j];
-}
-
 __global__ __launch_bounds__(256) void mlp_infer_kernel(const uint16_t* __restrict__ pk16,
                                                         const float* __restrict__ pk32,
                                                         const float* __restrict__ x, int64_t nb,
@@ -332,7 +257,8 @@ __global__ __launch_bounds__(256) void mlp_infer_kernel(const uint16_t* __restri
   const bool ok = p < nb;
   const float x0 = ok ? x[3 * p] : 0.0f, x1 = ok ? x[3 * p + 1] : 0.0f, x2 = ok ? x[3 * p + 2] : 0.0f;
   float o[4];
-  mlp_tile_forward(pk16, pk32, x0, x1, x2, o);
+  GlobalWeights gw{pk16};
+  mlp_tile_forward(gw, pk32, x0, x1, x2, o);
   if (ok && (threadIdx.x & 63) < 32)
 #pragma unroll
     for (int j = 0; j < 4; ++j) y[4 * p + j] = o[j];

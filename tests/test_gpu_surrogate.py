@@ -1,0 +1,99 @@
+"""The surrogate integrator: (1) with every cell flagged it is the exact clump sum and must
+reproduce the plain leapfrog kernel; (2) without fallback it must follow an oracle
+integrator whose clump force is the BF16-emulating numpy MLP; (3) a trained network must beat
+the method without it (host force only) on the stream-position error against the plain
+method."""
+import numpy as np
+import pytest
+import torch
+
+import oracle.mlp as OM
+from test_gpu_core import bound_orbits
+
+pytestmark = pytest.mark.gpu
+
+G = 4.498502151469553e-12
+
+
+def _setup(ssc, iters=300, nclump=96):
+    from streamsculptor_amd import workloads as W
+    from streamsculptor_amd.potential import PlummerClumps, Potential_Combine
+    from streamsculptor_amd.surrogate import ClumpSurrogate
+    host = W.host_potential()
+    pos, m, rs = W.clump_table(n=nclump, seed=1)
+    clumps = PlummerClumps(m, rs, pos, units=ssc.usys)
+    pot = Potential_Combine([host, clumps], units=ssc.usys)
+    rng = np.random.default_rng(4)
+    w0 = bound_orbits(rng, 512, rmin=6.0, rmax=14.0)
+    kidx = np.arange(0, 601, 3, dtype=np.int32)
+    _, ys = pot._leapfrog(w0, 0.0, 1.0, 600, save_idx=kidx)
+    P = ys[..., :3].reshape(-1, 3)
+    sur = ClumpSurrogate.train(host, clumps, P, n_train=P.shape[0] - 4096, n_val=4096,
+                               batch=2048, iters=iters, grid_cells=16)
+    return host, clumps, pot, sur, pos, m, rs
+
+
+def test_all_flagged_equals_plain(ssc):
+    host, clumps, pot, sur, *_ = _setup(ssc, iters=50)
+    rng = np.random.default_rng(6)
+    w0 = bound_orbits(rng, 333, rmin=6.0, rmax=14.0)
+    start = np.sort(rng.integers(0, 250, 333)).astype(np.int32)
+    ones = torch.ones_like(sur.flags)
+    out, cnt = sur.leapfrog(w0, -250.0, 1.0, 250, start=start, flags=ones)
+    ref, _ = pot._leapfrog(w0, -250.0, 1.0, 250, start=start)
+    err = (torch.linalg.norm(out - ref, dim=1) / torch.linalg.norm(ref, dim=1)).max().item()
+    print(f"all-flagged surrogate vs plain: {err:.2e}, fallback steps {int(cnt.item())}")
+    assert err < 1e-11
+    # every particle-step after its start (plus the initial force of each block) fell back
+    assert int(cnt.item()) >= int((250 - start).sum())
+
+
+def test_no_fallback_matches_oracle_integrator(oracle, ssc):
+    host, clumps, pot, sur, pos, m, rs = _setup(ssc, iters=200)
+    from streamsculptor_amd import workloads as W
+    h = oracle.host(W.host_components())
+    p = sur.mlp.params.cpu().numpy()
+    sc = sur.scaling
+    c, hw = np.array(sc.center), np.array(sc.half)
+    rng = np.random.default_rng(7)
+    n, nsteps, dt = 64, 120, 1.0
+    w0 = bound_orbits(rng, n, rmin=7.0, rmax=12.0)
+    zeros = torch.zeros_like(sur.flags)
+    out, cnt = sur.leapfrog(w0, 0.0, dt, nsteps, flags=zeros)
+
+    def accel(x):
+        u = ((x - c) / hw).astype(np.float32)
+        y, _ = OM.forward(p, u, emulate=True)
+        _, gh, _ = oracle.evaluate(h, x)
+        return -(gh + sc.g_scale * y[:, 1:])
+
+    x, v = w0[:, :3].copy(), w0[:, 3:].copy()
+    a = accel(x)
+    for k in range(nsteps):
+        v = v + 0.5 * dt * a
+        x = x + dt * v
+        a = accel(x)
+        v = v + 0.5 * dt * a
+    ref = np.hstack([x, v])
+    o = out.cpu().numpy()
+    err = np.linalg.norm(o - ref, axis=1) / np.linalg.norm(ref, axis=1)
+    print(f"surrogate vs oracle integrator: median {np.median(err):.2e} max {err.max():.2e}; "
+          f"fallback {int(cnt.item())}")
+    assert int(cnt.item()) == 0
+    assert np.median(err) < 1e-7 and err.max() < 1e-5
+
+
+def test_trained_network_beats_host_only(ssc):
+    """Median final-position error against the plain method: surrogate << host force only."""
+    host, clumps, pot, sur, *_ = _setup(ssc, iters=1500)
+    rng = np.random.default_rng(8)
+    w0 = bound_orbits(rng, 1024, rmin=6.0, rmax=14.0)
+    nsteps = 600
+    ref, _ = pot._leapfrog(w0, 0.0, 1.0, nsteps)
+    s_out, cnt = sur.leapfrog(w0, 0.0, 1.0, nsteps)
+    h_out, _ = host._leapfrog(w0, 0.0, 1.0, nsteps)
+    es = torch.linalg.norm(s_out[:, :3] - ref[:, :3], dim=1).median().item()
+    eh = torch.linalg.norm(h_out[:, :3] - ref[:, :3], dim=1).median().item()
+    print(f"median position error vs plain: surrogate {es * 1e3:.3f} pc, host-only {eh * 1e3:.3f} pc, "
+          f"fallback fraction {int(cnt.item()) / (1024 * nsteps):.3f}")
+    assert es < 0.25 * eh
