@@ -1,13 +1,18 @@
 #!/usr/bin/env python3
 """Throughput of the stream-particle integration path on one MI355X.
 
-Metric: particle-steps/s (one spray particle advanced one 1-Myr leapfrog step).  The default
-workload is the clumpy halo: the 3-component host plus 512 static Plummer clumps, 1e6 spray
+Metric: particle-steps/s (one spray particle advanced one 1-Myr leapfrog step).  Default
+workload (clumpy halo): the 3-component host plus 512 static Plummer clumps; 1e6 spray
 particles (250 released per step) integrated to the present over 4000 x 1 Myr, i.e.
-2,000,500,000 particle-steps per bench step.  A step is one full stream: progenitor orbit,
-release conditions and the particle integration, with inputs resident in HBM.
+2,000,500,000 particle-steps per bench step.  A step is one full stream -- progenitor orbit,
+release conditions, particle integration -- with inputs resident in HBM.  The clump force is
+computed two ways: the plain FP64 direct sum and the learned BF16-MFMA surrogate (trained on
+the GPU before the timed region); ``value`` is the surrogate rate when its median final
+stream-position error against the plain run is <= 10 pc, else the plain rate.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload clumps|subhalos|plain]
+Other workloads: --workload plain (8000 particles, host only), subhalos (1e5 particles, 64
+live subhalos, direct sums), responses (1e5 particles + 64 first-order response vectors each).
+
 Prints ONE JSON line on stdout; progress goes to stderr.
 """
 import argparse
@@ -21,7 +26,7 @@ sys.path.insert(0, ROOT)
 
 PEAK_FP64 = 78.6e12      # MI355X FP64 vector, FLOP/s
 PEAK_BF16 = 2.5e15       # MI355X dense BF16 MFMA
-PEAK_HBM = 8.0e12        # B/s
+ERR_LIMIT_PC = 10.0
 
 
 def log(msg):
@@ -29,7 +34,6 @@ def log(msg):
 
 
 def build_workload(name, particles=None):
-    import numpy as np
     import torch
     from streamsculptor_amd import workloads as W
     from streamsculptor_amd.potential import (LiveSubhaloPotential, PlummerClumps,
@@ -37,41 +41,90 @@ def build_workload(name, particles=None):
     from streamsculptor_amd.main import usys
     host = W.host_potential()
     ts = W.release_times()
+    extra = {}
     if name == "clumps":
         pos, m, rs = W.clump_table()
-        pot = Potential_Combine([host, PlummerClumps(m, rs, pos, units=usys)], units=usys)
+        clumps = PlummerClumps(m, rs, pos, units=usys)
+        pot = Potential_Combine([host, clumps], units=usys)
         total, nsrc = 1_000_000, 512
-    elif name == "subhalos":
+        extra["clumps"] = clumps
+    elif name in ("subhalos", "responses"):
         sw0, m, rs = W.subhalo_ics()
         live = LiveSubhaloPotential.from_orbits(host, sw0, m, rs, float(ts[0]), W.DT, W.T_STEPS)
-        pot = Potential_Combine([host, live], units=usys)
+        pot = Potential_Combine([host, live], units=usys) if name == "subhalos" else host
+        extra["live"] = live
         total, nsrc = 100_000, 64
     else:
         pot, total, nsrc = host, 8000, 0
     if particles:
         total = particles
     per_step = W.particles_per_step(total)
-    # the progenitor's present-day point integrated back 4 Gyr in the full potential
     back = pot.integrate_orbit(w0=W.PROG_TODAY, ts=[0.0, ts[0]], t0=0.0, t1=float(ts[0]),
                                solver="leapfrog", dt0=W.DT).ys[-1]
     torch.cuda.synchronize()
     return dict(name=name, pot=pot, host=host, ts=ts, prog_w0=back, per_step=per_step,
-                total=total, nsrc=nsrc, psteps=W.particle_steps(per_step))
+                total=total, nsrc=nsrc, psteps=W.particle_steps(per_step), **extra)
 
 
-def run_stream(wl, timer=None):
-    import torch
+def spray(wl):
     from streamsculptor_amd import workloads as W
     from streamsculptor_amd.streams import spray_particles
-    pot = wl["pot"]
-    w0, start, arm, (t0, dt, nsteps, _) = spray_particles(
-        pot, pot, wl["ts"], wl["prog_w0"], W.MSAT, 0, wl["per_step"], W.DT)
-    if timer is not None:
-        timer[0].record()
-    out, _ = pot._leapfrog(w0, t0, dt, nsteps, start=start)
-    if timer is not None:
-        timer[1].record()
+    return spray_particles(wl["pot"], wl["pot"], wl["ts"], wl["prog_w0"], W.MSAT, 0,
+                           wl["per_step"], W.DT)
+
+
+def run_plain(wl, ev=None):
+    w0, start, arm, (t0, dt, nsteps, _) = spray(wl)
+    if ev:
+        ev[0].record()
+    if wl["name"] == "responses":
+        from streamsculptor_amd.fields import perturb_leapfrog
+        out, _ = perturb_leapfrog(wl["host"], wl["live"]._kernel_sources(), w0, None, start, t0,
+                                  dt, nsteps, False)
+    else:
+        out, _ = wl["pot"]._leapfrog(w0, t0, dt, nsteps, start=start)
+    if ev:
+        ev[1].record()
     return w0, start, out
+
+
+def run_surrogate(wl, sur, ev=None):
+    w0, start, arm, (t0, dt, nsteps, _) = spray(wl)
+    if ev:
+        ev[0].record()
+    out, cnt = sur.leapfrog(w0, t0, dt, nsteps, start=start)
+    if ev:
+        ev[1].record()
+    return out, cnt
+
+
+def timed(fn, steps, warmup, label):
+    import torch
+    for i in range(warmup):
+        fn(None)
+        torch.cuda.synchronize()
+        log(f"{label}: warmup {i + 1}/{warmup} done")
+    evs, res = [], None
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ev = (torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+        res = fn(ev)
+        evs.append(ev)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    kern = sum(a.elapsed_time(b) for a, b in evs) / len(evs) / 1e3
+    log(f"{label}: {steps} steps in {wall:.3f} s; dominant kernel {kern * 1e3:.1f} ms avg")
+    return wall, kern, res
+
+
+def train_surrogate(wl, iters):
+    import numpy as np
+    from streamsculptor_amd.surrogate import ClumpSurrogate, orbit_positions
+    w0, start, arm, (t0, dt, nsteps, _) = spray(wl)
+    sel = np.linspace(0, w0.shape[0] - 1, 2100).astype(np.int64)
+    P = orbit_positions(wl["pot"], w0[sel], start[sel], t0, dt, nsteps)
+    return ClumpSurrogate.train(wl["host"], wl["clumps"], P, iters=iters, log=log)
 
 
 def cpu_baseline(wl, w0, start, out, sample_every):
@@ -86,23 +139,27 @@ def cpu_baseline(wl, w0, start, out, sample_every):
     if wl["name"] == "clumps":
         pos, m, rs = W.clump_table()
         src = O.Sources(O.SRC_STATIC, gm=W.G * m, rs=rs, pos=pos)
-    elif wl["name"] == "subhalos":
-        tens = wl["pot"]._kernel_sources().tensors()
+    elif wl["name"] in ("subhalos", "responses"):
+        tens = wl["live"]._kernel_sources().tensors()
         src = O.Sources(O.SRC_TRACK, gm=tens["gm"].cpu().numpy(), rs=tens["rs"].cpu().numpy(),
                         pos=tens["pos"].cpu().numpy())
     w0s = w0[::sample_every].cpu().numpy()
     sts = start[::sample_every]
     nsteps = len(wl["ts"]) - 1
     t0 = time.perf_counter()
-    ref = O.leapfrog(h, w0s, float(wl["ts"][0]), W.DT, nsteps, start=sts, src=src,
-                     nthreads=cores, fast=True)
+    if wl["name"] == "responses":
+        ref, _ = O.perturb_leapfrog(h, src, w0s, float(wl["ts"][0]), W.DT, nsteps, start=sts,
+                                    nthreads=cores)
+    else:
+        ref = O.leapfrog(h, w0s, float(wl["ts"][0]), W.DT, nsteps, start=sts, src=src,
+                         nthreads=cores, fast=True)
     el = time.perf_counter() - t0
     psteps = int((nsteps - sts).sum())
     g = out[::sample_every].cpu().numpy()
     err = float((np.linalg.norm(g - ref, axis=1) / np.linalg.norm(ref, axis=1)).max())
     return dict(value=psteps / el, unit="particle-steps/s", cores=cores, kind="port",
                 sample=f"every {sample_every}th particle of the same stream ({len(w0s)} particles, "
-                       f"{psteps} particle-steps, {el:.1f} s, oracle -O3 x86-64-v3 OpenMP)"), err
+                       f"{psteps} particle-steps, {el:.1f} s, oracle OpenMP)"), err
 
 
 def main():
@@ -110,14 +167,18 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="clumps", choices=["clumps", "subhalos", "plain"])
+    ap.add_argument("--workload", default="clumps",
+                    choices=["clumps", "subhalos", "responses", "plain"])
     ap.add_argument("--particles", type=int, default=None)
+    ap.add_argument("--train-iters", type=int, default=8000)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-surrogate", action="store_true")
     ap.add_argument("--cpu-sample-every", type=int, default=100)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: particles are independent, so a multi-GPU run is "
-              "N copies of this one (see the sharding notes); run with --gpus 1.", file=sys.stderr)
+              "N copies of this one on disjoint particle ranges; run with --gpus 1.",
+              file=sys.stderr)
         sys.exit(2)
 
     import torch
@@ -126,54 +187,75 @@ def main():
     _lib.require_gpu()
     log(f"device {torch.cuda.get_device_name(0)}; building workload {args.workload}")
     wl = build_workload(args.workload, args.particles)
-    log(f"{wl['total']} particles, {wl['psteps']} particle-steps per step, {wl['nsrc']} sources")
+    ps = wl["psteps"]
+    log(f"{wl['total']} particles, {ps} particle-steps per step, {wl['nsrc']} sources")
 
-    for i in range(args.warmup):
-        run_stream(wl)
-        torch.cuda.synchronize()
-        log(f"warmup {i + 1}/{args.warmup} done")
-
-    kern_ms = []
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for i in range(args.steps):
-        ev = (torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
-        w0, start, out = run_stream(wl, ev)
-        kern_ms.append(ev)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    kern = [a.elapsed_time(b) for a, b in kern_ms]
-    log(f"timed {args.steps} steps in {wall:.3f} s; particle kernel {sum(kern) / len(kern):.1f} ms avg")
-
-    ms_per_step = 1e3 * wall / args.steps
-    value = wl["psteps"] * args.steps / wall
-    fl = flops.plain(wl["nsrc"])
-    kavg = sum(kern) / len(kern) / 1e3
-    achieved = fl * wl["psteps"] / kavg
+    wall_p, kern_p, (w0, start, out_p) = timed(lambda ev: run_plain(wl, ev), args.steps,
+                                               args.warmup, "plain")
+    if args.workload == "responses":
+        fl_p = flops.responses(wl["nsrc"])
+    else:
+        fl_p = flops.plain(wl["nsrc"])
+    plain = {"value": ps * args.steps / wall_p, "ms_per_step": 1e3 * wall_p / args.steps,
+             "kernel_ms": kern_p * 1e3,
+             "roofline": {"bound": "fp64-valu", "achieved": fl_p * ps / kern_p / 1e12,
+                          "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s",
+                          "frac": fl_p * ps / kern_p / PEAK_FP64, "traffic": None,
+                          "kernel": "leapfrog_kernel" if args.workload != "responses" else "perturb_kernel",
+                          "flop_per_particle_step": fl_p}}
     result = {
         "metric": "particle-steps/s, surrogate vs plain FP64 force, at <=10 pc median stream error",
-        "value": value,
-        "unit": "particle-steps/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": ms_per_step,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f64",
-        "data": "synthetic",
-        "config": {"workload": f"{args.workload} stream, plain FP64 force",
-                   "particles": wl["total"], "sources": wl["nsrc"], "steps_per_orbit": 4000,
-                   "dt_myr": 1.0, "particle_steps": wl["psteps"], "form": "plain"},
-        "roofline": {"bound": "fp64-valu", "achieved": achieved / 1e12, "peak": PEAK_FP64 / 1e12,
-                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP64, "traffic": None,
-                     "kernel": "leapfrog_kernel", "kernel_ms": kavg * 1e3,
-                     "flop_per_particle_step": fl},
+        "value": plain["value"], "unit": "particle-steps/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": plain["ms_per_step"], "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"{args.workload} stream", "particles": wl["total"],
+                   "sources": wl["nsrc"], "steps_per_orbit": 4000, "dt_myr": 1.0,
+                   "particle_steps": ps, "form": "plain"},
+        "roofline": plain["roofline"],
+        "plain": plain,
     }
+
+    if args.workload == "clumps" and not args.no_surrogate:
+        t0 = time.perf_counter()
+        sur = train_surrogate(wl, args.train_iters)
+        torch.cuda.synchronize()
+        t_train = time.perf_counter() - t0
+        rep = {k: v for k, v in sur.report.items() if k != "losses"}
+        log(f"surrogate trained in {t_train:.1f} s: {rep}")
+        wall_s, kern_s, (out_s, cnt) = timed(lambda ev: run_surrogate(wl, sur, ev), args.steps,
+                                             args.warmup, "surrogate")
+        dpos = torch.linalg.norm(out_s[:, :3] - out_p[:, :3], dim=1)
+        med_pc = float(dpos.median().item()) * 1e3
+        from streamsculptor_amd.workloads import MSAT  # noqa: F401
+        host_only, _ = wl["host"]._leapfrog(w0, float(wl["ts"][0]), 1.0, len(wl["ts"]) - 1,
+                                            start=start)
+        med_host_pc = float(torch.linalg.norm(host_only[:, :3] - out_p[:, :3], dim=1).median().item()) * 1e3
+        fl_mlp = flops.mlp()
+        fb_frac = int(cnt.item()) / ps
+        surrogate = {"value": ps * args.steps / wall_s, "ms_per_step": 1e3 * wall_s / args.steps,
+                     "kernel_ms": kern_s * 1e3, "median_err_pc": med_pc,
+                     "median_err_pc_host_only": med_host_pc, "fallback_fraction": fb_frac,
+                     "train_seconds": t_train, "train": rep,
+                     "roofline": {"bound": "mfma", "achieved": fl_mlp * ps / kern_s / 1e12,
+                                  "peak": PEAK_BF16 / 1e12, "unit": "TFLOP/s",
+                                  "frac": fl_mlp * ps / kern_s / PEAK_BF16, "traffic": None,
+                                  "kernel": "surrogate_leapfrog_kernel",
+                                  "flop_per_particle_step": fl_mlp}}
+        result["surrogate"] = surrogate
+        result["surrogate_vs_plain"] = surrogate["value"] / plain["value"]
+        log(f"surrogate {surrogate['value']:.3e} vs plain {plain['value']:.3e} particle-steps/s; "
+            f"median error {med_pc:.2f} pc (host only {med_host_pc:.1f} pc); "
+            f"fallback {fb_frac:.3f}")
+        if med_pc <= ERR_LIMIT_PC:
+            result.update(value=surrogate["value"], ms_per_step=surrogate["ms_per_step"],
+                          roofline=surrogate["roofline"], dtype="bf16-mfma/f64")
+            result["config"]["form"] = "surrogate"
+        else:
+            result["config"]["form"] = "plain (surrogate error above 10 pc)"
+
     if not args.no_cpu:
         log("timing the CPU baseline (oracle on a sample)")
-        cb, err = cpu_baseline(wl, w0, start, out, args.cpu_sample_every)
+        cb, err = cpu_baseline(wl, w0, start, out_p, args.cpu_sample_every)
         result["cpu_baseline"] = cb
         result["parity_sample_max_rel_err"] = err
         log(f"cpu baseline {cb['value']:.3e} particle-steps/s on {cb['cores']} cores; "

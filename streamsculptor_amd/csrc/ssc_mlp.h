@@ -54,10 +54,16 @@ __host__ __device__ inline int kperm(int s) {
   return blk + 8 * (e >> 2) + 4 * h + (e & 3);
 }
 
-__device__ __forceinline__ float silu(float z) { return z / (1.0f + __expf(-z)); }
+// SiLU z * sigmoid(z) with the hardware exp2 and reciprocal (about 1 ulp each; the result is
+// rounded to BF16 by every consumer).  Training and inference use this same formula.
+__device__ __forceinline__ float sigmoid_fast(float z) {
+  return __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(-1.4426950408889634f * z));
+}
+
+__device__ __forceinline__ float silu(float z) { return z * sigmoid_fast(z); }
 
 __device__ __forceinline__ float silu_grad(float z) {
-  const float s = 1.0f / (1.0f + __expf(-z));
+  const float s = sigmoid_fast(z);
   return s * (1.0f + z * (1.0f - s));
 }
 

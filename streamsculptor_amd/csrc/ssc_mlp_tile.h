@@ -10,11 +10,12 @@
 
 namespace ssc {
 
+// Eight fp32 values to one BF16 fragment (hardware round-to-nearest-even conversion).
 __device__ __forceinline__ mlp::bf16x8 pack8(const float* v) {
-  mlp::i16x8 r;
+  mlp::bf16x8 r;
 #pragma unroll
-  for (int e = 0; e < 8; ++e) r[e] = (short)mlp::f2bf(v[e]);
-  return __builtin_bit_cast(mlp::bf16x8, r);
+  for (int e = 0; e < 8; ++e) r[e] = (__bf16)v[e];
+  return r;
 }
 
 struct GlobalWeights {
@@ -48,32 +49,46 @@ __device__ __forceinline__ void mlp_tile_forward(Weights& wsrc, const float* __r
       v[e] = silu(z);
     }
     bin[q] = pack8(v);
+    __builtin_amdgcn_sched_barrier(0);
   }
-  // two chained 256x256 layers
+  // two chained 256x256 layers; output tiles are produced two at a time so that only two
+  // accumulators, the input fragments and the output fragments are live
 #pragma unroll
   for (int layer = 0; layer < 2; ++layer) {
     const bf16x8* W = wsrc.layer(layer);
     const float* bias = pk32 + (layer == 0 ? PK_B2 : PK_B3);
-    f32x16 acc[H / 32];
+    bf16x8 bout[H / 16];
 #pragma unroll
-    for (int t = 0; t < H / 32; ++t)
+    for (int t = 0; t < H / 32; t += 2) {
+      f32x16 a0, a1;
 #pragma unroll
-      for (int r = 0; r < 16; ++r) acc[t][r] = bias[32 * t + 8 * (r >> 2) + 4 * hh + (r & 3)];
+      for (int r = 0; r < 16; ++r) {
+        const int row = 8 * (r >> 2) + 4 * hh + (r & 3);
+        a0[r] = bias[32 * t + row];
+        a1[r] = bias[32 * (t + 1) + row];
+      }
 #pragma unroll
-    for (int q = 0; q < H / 16; ++q)
-#pragma unroll
-      for (int t = 0; t < H / 32; ++t)
-        acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(W[(t * (H / 16) + q) * 64 + l], bin[q],
-                                                        acc[t], 0, 0, 0);
-#pragma unroll
-    for (int t = 0; t < H / 32; ++t)
+      for (int q = 0; q < H / 16; ++q) {
+        a0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(W[(t * (H / 16) + q) * 64 + l], bin[q], a0, 0, 0, 0);
+        a1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(W[((t + 1) * (H / 16) + q) * 64 + l], bin[q], a1,
+                                                    0, 0, 0);
+        // bound how far the scheduler hoists fragment loads (register pressure)
+        if ((q & 3) == 3) __builtin_amdgcn_sched_barrier(0);
+      }
 #pragma unroll
       for (int u = 0; u < 2; ++u) {
-        float v[8];
+        float v0[8], v1[8];
 #pragma unroll
-        for (int e = 0; e < 8; ++e) v[e] = silu(acc[t][8 * u + e]);
-        bin[2 * t + u] = pack8(v);
+        for (int e = 0; e < 8; ++e) {
+          v0[e] = silu(a0[8 * u + e]);
+          v1[e] = silu(a1[8 * u + e]);
+        }
+        bout[2 * t + u] = pack8(v0);
+        bout[2 * (t + 1) + u] = pack8(v1);
       }
+    }
+#pragma unroll
+    for (int q = 0; q < H / 16; ++q) bin[q] = bout[q];
   }
   // output layer (4 rows of one padded 32-row tile)
   const bf16x8* W4 = wsrc.layer(2);

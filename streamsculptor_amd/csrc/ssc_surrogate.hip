@@ -1,6 +1,6 @@
 // Leapfrog with the learned clump-force surrogate (gfx950).
 //
-// A block of 4 waves integrates 128 particles for the whole grid.  The FP64 state of particle
+// A block of 8 waves integrates 256 particles for the whole grid.  The FP64 state of particle
 // p lives in lanes p and p + 32 of its wave (both halves compute the same update, so the MLP's
 // 64-lane MFMA operands need no exchange).  Every step:
 //   kick + drift (FP64) -> MLP(x) on the matrix cores: the three 256-wide layers run as BF16
@@ -12,6 +12,7 @@
 #include <hip/hip_runtime.h>
 
 #include <climits>
+#include <cstdlib>
 
 #include "ssc_device.h"
 #include "ssc_mlp.h"
@@ -30,9 +31,10 @@ struct SurrogateDesc {
   int32_t reserved;
 };
 
-constexpr int SG_BLOCK = 256;  // 4 waves
-constexpr int SG_PARTS = 128;  // particles per block
+// Block shapes: WAVES = 4 (one wave per SIMD, 512-register budget) or 8 (two per SIMD,
+// 256-register budget); SG_PARTS = 32 particles per wave.
 
+template <int SG_BLOCK>
 struct LdsWeights {
   mlp::bf16x8* buf;        // one 256x256 layer in fragment order
   const mlp::bf16x8* w4;   // output-layer fragments (resident)
@@ -55,17 +57,18 @@ __device__ __forceinline__ double wave_sum64(double v) {
   return v;
 }
 
+template <int SG_BLOCK>
 __global__ __launch_bounds__(SG_BLOCK) void surrogate_leapfrog_kernel(
     ssc_host_potential hp, ssc_sources clumps, SurrogateDesc sd, const uint16_t* __restrict__ pk16,
     const float* __restrict__ pk32g, const uint8_t* __restrict__ flags,
     const double* __restrict__ w0, const int32_t* __restrict__ start, int64_t n, double t0,
     double dt, int nsteps, double* __restrict__ w_out, unsigned long long* __restrict__ counters) {
   using namespace mlp;
+  constexpr int SG_PARTS = SG_BLOCK / 2;
   extern __shared__ __align__(16) char lds_raw[];
   __shared__ int kmin_sh;
   __shared__ int nfb;
-  __shared__ double fb_pos[SG_PARTS][3];
-  __shared__ double fb_res[SG_PARTS][3];
+  __shared__ double fb_buf[SG_PARTS][3];  // fallback positions, overwritten by their forces
 
   bf16x8* wbuf = reinterpret_cast<bf16x8*>(lds_raw);
   bf16x8* w4 = reinterpret_cast<bf16x8*>(lds_raw + FRAG_L * 2);
@@ -100,7 +103,7 @@ __global__ __launch_bounds__(SG_BLOCK) void surrogate_leapfrog_kernel(
   __syncthreads();
   const int kmin = kmin_sh;
 
-  LdsWeights lw{wbuf, w4, reinterpret_cast<const bf16x8*>(pk16 + PK_W2),
+  LdsWeights<SG_BLOCK> lw{wbuf, w4, reinterpret_cast<const bf16x8*>(pk16 + PK_W2),
                 reinterpret_cast<const bf16x8*>(pk16 + PK_W3)};
   unsigned long long nfall = 0;
 
@@ -130,16 +133,16 @@ __global__ __launch_bounds__(SG_BLOCK) void surrogate_leapfrog_kernel(
       int slot = -1;
       if (fb && l < 32) {
         slot = atomicAdd(&nfb, 1);
-        fb_pos[slot][0] = x;
-        fb_pos[slot][1] = y;
-        fb_pos[slot][2] = z;
+        fb_buf[slot][0] = x;
+        fb_buf[slot][1] = y;
+        fb_buf[slot][2] = z;
         ++nfall;
       }
       slot = __shfl(slot, l & 31, 64);
       __syncthreads();
       const int cnt = nfb;
       for (int f = wv; f < cnt; f += SG_BLOCK / 64) {
-        const double px = fb_pos[f][0], py = fb_pos[f][1], pz = fb_pos[f][2];
+        const double px = fb_buf[f][0], py = fb_buf[f][1], pz = fb_buf[f][2];
         double sx = 0.0, sy = 0.0, sz = 0.0;
         for (int j = l; j < clumps.n; j += 64) {
           const double rs = clumps.rs[j];
@@ -150,16 +153,16 @@ __global__ __launch_bounds__(SG_BLOCK) void surrogate_leapfrog_kernel(
         sy = wave_sum64(sy);
         sz = wave_sum64(sz);
         if (l == 0) {
-          fb_res[f][0] = sx;
-          fb_res[f][1] = sy;
-          fb_res[f][2] = sz;
+          fb_buf[f][0] = sx;
+          fb_buf[f][1] = sy;
+          fb_buf[f][2] = sz;
         }
       }
       __syncthreads();
       if (fb) {
-        gx = fb_res[slot][0];
-        gy = fb_res[slot][1];
-        gz = fb_res[slot][2];
+        gx = fb_buf[slot][0];
+        gy = fb_buf[slot][1];
+        gz = fb_buf[slot][2];
       }
       __syncthreads();
       if (threadIdx.x == 0) nfb = 0;
@@ -226,14 +229,22 @@ extern "C" int ssc_surrogate_leapfrog(const ssc_host_potential* host, const ssc_
   sd.g_scale = scaling[7];
   sd.reserved = 0;
   const size_t lds = (size_t)(mlp::FRAG_L + mlp::FRAG_4) * 2 + (size_t)mlp::PK_F32_END * 4;
-  hipError_t e = hipFuncSetAttribute((const void*)surrogate_leapfrog_kernel,
-                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-  if (e != hipSuccess) return fail(SSC_E_HIP, std::string("hipFuncSetAttribute: ") + hipGetErrorString(e));
   const uint16_t* pk16 = (const uint16_t*)packed;
   const float* pk32 = (const float*)((const char*)packed + mlp::PK_F32_BASE_BYTES);
-  const int64_t blocks = (n + SG_PARTS - 1) / SG_PARTS;
-  hipLaunchKernelGGL(surrogate_leapfrog_kernel, dim3((unsigned)blocks), dim3(SG_BLOCK), lds,
-                     (hipStream_t)stream, *host, *clumps, sd, pk16, pk32, flags, w0, start_step, n,
-                     t0, dt, nsteps, w_out, fallback_count);
+  const char* env = getenv("SSC_SURROGATE_WAVES");
+  const int waves = (env && atoi(env) == 8) ? 8 : 4;
+#define SSC_SG(B)                                                                                 \
+  {                                                                                               \
+    hipError_t e = hipFuncSetAttribute((const void*)surrogate_leapfrog_kernel<B>,                 \
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);     \
+    if (e != hipSuccess)                                                                          \
+      return fail(SSC_E_HIP, std::string("hipFuncSetAttribute: ") + hipGetErrorString(e));       \
+    const int64_t blocks = (n + B / 2 - 1) / (B / 2);                                             \
+    hipLaunchKernelGGL(surrogate_leapfrog_kernel<B>, dim3((unsigned)blocks), dim3(B), lds,        \
+                       (hipStream_t)stream, *host, *clumps, sd, pk16, pk32, flags, w0, start_step, \
+                       n, t0, dt, nsteps, w_out, fallback_count);                                 \
+  }
+  if (waves == 8) SSC_SG(512) else SSC_SG(256)
+#undef SSC_SG
   return check_launch("ssc_surrogate_leapfrog");
 }
