@@ -145,6 +145,47 @@ int ssc_perturb_leapfrog(const ssc_host_potential* host, const ssc_sources* pert
                          int64_t n, double t0, double dt, int32_t nsteps, int32_t with_radius,
                          double* w_out, double* dw_out, void* stream);
 
+/* Vector-Jacobian product of the release model: given dL/d(pos_close, pos_far, vel_close,
+ * vel_far) [nrel][3] each, writes dL/d(progenitor row) g_prog[nrel][6] and dL/d(G m_c, p1_c) of
+ * the host components into g_host[n_components][2] (summed over releases).  scratch:
+ * [nrel][2 * n_components] doubles.  Extends the reference's release-model Jacobian
+ * (perturbative.py:282-296) with the host parameters. */
+int ssc_release_vjp(const ssc_host_potential* host, const double* prog_w, const int32_t* rel_idx,
+                    const double* g_msat, int32_t nrel, int64_t seed, const double* kvals,
+                    const double* g_pos_close, const double* g_pos_far,
+                    const double* g_vel_close, const double* g_vel_far, double* g_prog,
+                    double* g_host, double* scratch, void* stream);
+
+/* ---------------------------------------------------------------------------------------
+ * Discrete adjoint of ssc_leapfrog (no reference counterpart: the reference differentiates
+ * forward-mode only, streamsculptor/main.py:160).  Checkpointed: the forward stores the state
+ * every checkpoint_every grid steps in the caller's workspace; the backward re-runs each
+ * segment and sweeps it in reverse.  Sources: NONE, STATIC or TRACK.
+ * --------------------------------------------------------------------------------------- */
+int64_t ssc_adjoint_workspace_bytes(int64_t n, int32_t nsteps, int32_t checkpoint_every,
+                                    int32_t n_src);
+
+/* Forward pass (same result as ssc_leapfrog) keeping the checkpoints in workspace. */
+int ssc_adjoint_forward(const ssc_host_potential* host, const ssc_sources* src, const double* w0,
+                        const int32_t* start_step, int64_t n, double t0, double dt, int32_t nsteps,
+                        int32_t checkpoint_every, double* w_out, void* workspace,
+                        int64_t workspace_bytes, void* stream);
+
+/* Backward pass after ssc_adjoint_forward with the same arguments and workspace.
+ *   lam_T[n][6]    dL/d(final row)
+ *   forcing        NULL or [nsteps+1][n][6]: dL/d(row k) of every grid row a particle reaches
+ *                  (rows before its start are ignored)
+ *   lam_0[n][6]    dL/d(initial row)
+ *   g_host[n_components][2]  dL/d(G m_c), dL/d(p1_c) (p1: MN a, NFW r_s, Hernquist c, Plummer r_s)
+ *   g_gm[n_src]    NULL or dL/d(G m_j) of the sources
+ *   g_pos          NULL or dL/d(source positions): [nsteps+1][n_src][3] (TRACK), [n_src][3] (STATIC)
+ * Requires nsteps >= 1. */
+int ssc_adjoint_backward(const ssc_host_potential* host, const ssc_sources* src,
+                         const int32_t* start_step, int64_t n, double t0, double dt,
+                         int32_t nsteps, int32_t checkpoint_every, const double* lam_T,
+                         const double* forcing, double* lam_0, double* g_host, double* g_gm,
+                         double* g_pos, void* workspace, int64_t workspace_bytes, void* stream);
+
 /* ---------------------------------------------------------------------------------------
  * Clump-force MLP (no counterpart in the reference, which has no network): 3 -> 256 -> 256 ->
  * 256 -> 4 with SiLU, fp32 master parameters, BF16 MFMA products with fp32 accumulation.

@@ -99,6 +99,8 @@ def _bind(L):
     L.orc_leapfrog.argtypes = [vp, vp, vp, vp, i64, d, d, i32, vp, vp, ctypes.c_int]
     L.orc_release.argtypes = [vp, vp, vp, vp, i32, i64, vp, vp, vp, vp, vp]
     L.orc_release_jacobian.argtypes = [vp, vp, vp, i32, i64, vp, vp]
+    L.orc_adjoint_leapfrog.argtypes = [vp, vp, vp, vp, i64, d, d, i32, vp, vp, vp, vp, vp, vp,
+                                        ctypes.c_int]
     L.orc_perturb_leapfrog.argtypes = [vp, vp, vp, vp, vp, vp, i64, d, d, i32, ctypes.c_int,
                                        vp, vp, ctypes.c_int]
     return L
@@ -278,3 +280,25 @@ def perturb_leapfrog(h, pert, w0, t0, dt, nsteps, dw0=None, start=None, base_src
                                w0.ctypes.data, _p(d0), _p(st), n, t0, dt, nsteps,
                                int(with_radius), out.ctypes.data, dw.ctypes.data, nthreads)
     return out, dw
+
+
+def adjoint_leapfrog(h, w0, t0, dt, nsteps, lam_T, start=None, src=None, forcing=None,
+                     nthreads=0):
+    """Discrete adjoint of the leapfrog.  Returns dict(lam0 (n, 6), g_host (n_comp, 2):
+    dL/d(G m_c), dL/d(p1_c), g_gm (m,), g_pos (rows, m, 3))."""
+    w0 = np.ascontiguousarray(np.atleast_2d(w0), dtype=np.float64)
+    n = w0.shape[0]
+    lam_T = np.ascontiguousarray(lam_T, dtype=np.float64).reshape(n, 6)
+    st = None if start is None else np.ascontiguousarray(start, dtype=np.int32)
+    fo = None if forcing is None else np.ascontiguousarray(forcing, dtype=np.float64)
+    src, sp = _src_ptr(src)
+    m = src.desc.n if src.kind != SRC_NONE else 0
+    rows = nsteps + 1 if src.kind == SRC_TRACK else 1
+    lam0 = np.zeros((n, 6))
+    gh = np.zeros((max(h.n_components, 1), 2))
+    gg = np.zeros(max(m, 1))
+    gp = np.zeros((rows, max(m, 1), 3))
+    lib().orc_adjoint_leapfrog(ctypes.addressof(h), sp, w0.ctypes.data, _p(st), n, t0, dt, nsteps,
+                               lam_T.ctypes.data, _p(fo), lam0.ctypes.data, gh.ctypes.data,
+                               gg.ctypes.data, gp.ctypes.data, nthreads)
+    return dict(lam0=lam0, g_host=gh[:h.n_components], g_gm=gg[:m], g_pos=gp[:, :m])

@@ -172,6 +172,37 @@ void host_eval_T(const ssc_host_potential* hp, const T x[3], T* phi, T g[3], T H
   }
 }
 
+// d(grad Phi_c)/d(p1_c) for one component (p1 = MN a, NFW r_s, Hernquist c, Plummer r_s);
+// grad Phi_c is linear in G m_c, so d/d(G m_c) = grad Phi_c / (G m_c).
+void comp_grad_dp1(int kind, const double p[4], const double x[3], double out[3]) {
+  const double gm = p[0];
+  out[0] = out[1] = out[2] = 0.0;
+  if (kind == SSC_COMP_MIYAMOTO_NAGAI) {
+    double a = p[1], b = p[2];
+    double zeta = std::sqrt(x[2] * x[2] + b * b), S = a + zeta;
+    double D2 = x[0] * x[0] + x[1] * x[1] + S * S, D = std::sqrt(D2);
+    double iD3 = 1.0 / (D2 * D), iD5 = iD3 / D2;
+    out[0] = -3.0 * gm * x[0] * S * iD5;
+    out[1] = -3.0 * gm * x[1] * S * iD5;
+    out[2] = gm * (x[2] / zeta) * iD3 - 3.0 * gm * (x[2] * S / zeta) * S * iD5;
+  } else if (kind == SSC_COMP_NFW) {
+    double rs = p[1];
+    double r = std::sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2]);
+    double rp = r + rs;
+    double dd1 = gm * (-1.0 / (r * rs * rp) + 1.0 / (r * rp * rp));
+    for (int i = 0; i < 3; ++i) out[i] = dd1 / r * x[i];
+  } else if (kind == SSC_COMP_HERNQUIST) {
+    double c = p[1], soft = p[2];
+    double r = std::sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2] + soft);
+    double rc = r + c;
+    for (int i = 0; i < 3; ++i) out[i] = -2.0 * gm * x[i] / (r * rc * rc * rc);
+  } else if (kind == SSC_COMP_PLUMMER) {
+    double rs = p[1];
+    double s2 = x[0] * x[0] + x[1] * x[1] + x[2] * x[2] + rs * rs, s = std::sqrt(s2);
+    for (int i = 0; i < 3; ++i) out[i] = -3.0 * gm * rs * x[i] / (s2 * s2 * s);
+  }
+}
+
 // Position of source j at grid row `step` / time t; returns false when the source is inactive
 // (LINE window, reference streamsculptor/potential.py:826).
 bool source_pos(const ssc_sources* s, int j, int step, double t, double out[3]) {
@@ -576,6 +607,110 @@ int orc_perturb_leapfrog(const ssc_host_potential* hp, const ssc_sources* base_s
     }
     for (int c = 0; c < 3; ++c) { w_out[6 * i + c] = x[c]; w_out[6 * i + 3 + c] = v[c]; }
     for (int j = 0; j < m * dd; ++j) dw_out[(size_t)i * m * dd + j] = d[j];
+  }
+  return 0;
+}
+
+// Discrete adjoint of the kick-drift-kick leapfrog (no reference counterpart: the reference
+// differentiates forward-mode only, streamsculptor/main.py:160).  For every orbit, with the
+// forward trajectory stored, the reverse sweep is
+//   terminal: lx, lv = lam_T[i] (+ forcing at the last row); W = h lv; apply(x_T, W)
+//   k = T-1 .. s: lv_half = lv + dt lx; lv = lv_half + F_v(k); lx += F_x(k);
+//                 W = h lv_half + (k > s ? h lv : 0); apply(x_k, W)
+//   apply(x, W): lx -= H(x) W (H = Hessian of host + sources); dL/dtheta += (da/dtheta)^T W;
+//                per source j: dL/dS_jk += H_j(x - S_jk) W, dL/d(Gm_j) += -(x - S_jk).W / s^3
+// Outputs: lam_0[i] = adjoint of the starting row, g_host[2c] = dL/d(G m_c),
+// g_host[2c+1] = dL/d(p1_c), g_gm[j], g_pos[k][j][3] (TRACK rows; row 0 for STATIC).
+int orc_adjoint_leapfrog(const ssc_host_potential* hp, const ssc_sources* src, const double* w0,
+                         const int32_t* start, int64_t n, double t0, double dt, int32_t nsteps,
+                         const double* lam_T, const double* forcing, double* lam_0,
+                         double* g_host, double* g_gm, double* g_pos, int nthreads) {
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+  const int nc = hp->n_components;
+  const int m = (src && src->kind != SSC_SRC_NONE) ? src->n : 0;
+  const int64_t npos = (src && src->kind == SSC_SRC_TRACK) ? (int64_t)(nsteps + 1) * m * 3 : (int64_t)m * 3;
+  for (int k = 0; k < 2 * nc; ++k) g_host[k] = 0.0;
+  if (g_gm) for (int j = 0; j < m; ++j) g_gm[j] = 0.0;
+  if (g_pos) for (int64_t k = 0; k < npos; ++k) g_pos[k] = 0.0;
+  const double h = 0.5 * dt;
+#pragma omp parallel
+  {
+    std::vector<double> gh(2 * nc, 0.0), gg(m, 0.0), gp(g_pos ? npos : 0, 0.0);
+    std::vector<double> X;
+#pragma omp for schedule(dynamic, 8)
+    for (int64_t i = 0; i < n; ++i) {
+      int s = start ? start[i] : 0;
+      if (s < 0) s = 0;
+      if (s > nsteps) s = nsteps;
+      // forward, storing positions
+      X.assign((size_t)(nsteps - s + 1) * 3, 0.0);
+      double x[3], v[3], a[3];
+      for (int c = 0; c < 3; ++c) { x[c] = w0[6 * i + c]; v[c] = w0[6 * i + 3 + c]; }
+      for (int c = 0; c < 3; ++c) X[c] = x[c];
+      if (s < nsteps) accel(hp, src, x, s, t0 + s * dt, a);
+      for (int k = s; k < nsteps; ++k) {
+        for (int c = 0; c < 3; ++c) v[c] += h * a[c];
+        for (int c = 0; c < 3; ++c) x[c] += dt * v[c];
+        accel(hp, src, x, k + 1, t0 + (k + 1) * dt, a);
+        for (int c = 0; c < 3; ++c) v[c] += h * a[c];
+        for (int c = 0; c < 3; ++c) X[(size_t)(k + 1 - s) * 3 + c] = x[c];
+      }
+      auto F = [&](int k, int c) -> double {
+        return forcing ? forcing[((int64_t)k * n + i) * 6 + c] : 0.0;
+      };
+      auto apply = [&](int k, const double W[3], double lx[3]) {
+        const double* xk = &X[(size_t)(k - s) * 3];
+        const double t = t0 + k * dt;
+        double g[3], H[9];
+        total_eval(hp, src, xk, k, t, nullptr, g, H);
+        for (int r = 0; r < 3; ++r) lx[r] -= H[3 * r] * W[0] + H[3 * r + 1] * W[1] + H[3 * r + 2] * W[2];
+        for (int c = 0; c < nc; ++c) {
+          double gc[3] = {0, 0, 0};
+          comp_eval<double>(hp->c[c].kind, hp->c[c].p, xk, nullptr, gc, nullptr);
+          double dp[3];
+          comp_grad_dp1(hp->c[c].kind, hp->c[c].p, xk, dp);
+          // a = -grad Phi: da/dGm = -grad Phi_c / Gm, da/dp1 = -d(grad Phi_c)/dp1
+          gh[2 * c] += -(gc[0] * W[0] + gc[1] * W[1] + gc[2] * W[2]) / hp->c[c].p[0];
+          gh[2 * c + 1] += -(dp[0] * W[0] + dp[1] * W[1] + dp[2] * W[2]);
+        }
+        for (int j = 0; j < m; ++j) {
+          double sp[3];
+          if (!source_pos(src, j, k, t, sp)) continue;
+          double d[3] = {xk[0] - sp[0], xk[1] - sp[1], xk[2] - sp[2]};
+          double rs = src->rs[j];
+          double s2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2] + rs * rs, ss = std::sqrt(s2);
+          double i3 = 1.0 / (s2 * ss), i5 = i3 / s2;
+          double dW = d[0] * W[0] + d[1] * W[1] + d[2] * W[2];
+          gg[j] += -dW * i3;
+          if (g_pos) {
+            double* o = &gp[(src->kind == SSC_SRC_TRACK ? (int64_t)k * m + j : (int64_t)j) * 3];
+            double gmj = src->gm[j];
+            for (int r = 0; r < 3; ++r) o[r] += gmj * (i3 * W[r] - 3.0 * i5 * d[r] * dW);
+          }
+        }
+      };
+      double lx[3], lv[3];
+      for (int c = 0; c < 3; ++c) { lx[c] = lam_T[6 * i + c] + F(nsteps, c); lv[c] = lam_T[6 * i + 3 + c] + F(nsteps, 3 + c); }
+      if (s < nsteps) {
+        double W[3] = {h * lv[0], h * lv[1], h * lv[2]};
+        apply(nsteps, W, lx);
+      }
+      for (int k = nsteps - 1; k >= s; --k) {
+        double lh[3];
+        for (int c = 0; c < 3; ++c) lh[c] = lv[c] + dt * lx[c];
+        for (int c = 0; c < 3; ++c) { lv[c] = lh[c] + F(k, 3 + c); lx[c] += F(k, c); }
+        double W[3];
+        for (int c = 0; c < 3; ++c) W[c] = h * lh[c] + (k > s ? h * lv[c] : 0.0);
+        apply(k, W, lx);
+      }
+      for (int c = 0; c < 3; ++c) { lam_0[6 * i + c] = lx[c]; lam_0[6 * i + 3 + c] = lv[c]; }
+    }
+#pragma omp critical
+    {
+      for (int k = 0; k < 2 * nc; ++k) g_host[k] += gh[k];
+      if (g_gm) for (int j = 0; j < m; ++j) g_gm[j] += gg[j];
+      if (g_pos) for (int64_t k = 0; k < npos; ++k) g_pos[k] += gp[k];
+    }
   }
   return 0;
 }

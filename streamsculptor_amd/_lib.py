@@ -53,6 +53,11 @@ SIGNATURES = {
     "ssc_mlp_pack": [_vp, _vp],
     "ssc_mlp_infer": [_vp, _vp, _i64, _vp],
     "ssc_surrogate_leapfrog": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _i64, _d, _d, _i32, _vp, _vp],
+    "ssc_release_vjp": [_vp, _vp, _vp, _vp, _i32, ctypes.c_int64, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
+                        _vp],
+    "ssc_adjoint_forward": [_vp, _vp, _vp, _vp, _i64, _d, _d, _i32, _i32, _vp, _vp, _i64],
+    "ssc_adjoint_backward": [_vp, _vp, _vp, _i64, _d, _d, _i32, _i32, _vp, _vp, _vp, _vp, _vp, _vp,
+                             _vp, _i64],
     "ssc_release": [_vp, _vp, _vp, _vp, _i32, ctypes.c_int64, _vp, _vp, _vp, _vp, _vp, _vp],
 }
 
@@ -79,6 +84,9 @@ def load():
             getattr(L, name).argtypes = []
         L.ssc_mlp_workspace_bytes.restype = ctypes.c_int64
         L.ssc_mlp_workspace_bytes.argtypes = [ctypes.c_int64]
+        L.ssc_adjoint_workspace_bytes.restype = ctypes.c_int64
+        L.ssc_adjoint_workspace_bytes.argtypes = [ctypes.c_int64, ctypes.c_int32, ctypes.c_int32,
+                                                  ctypes.c_int32]
         for name, args in SIGNATURES.items():
             fn = getattr(L, name)
             fn.argtypes = args
@@ -111,11 +119,21 @@ def call(name, *args):
         raise RuntimeError(f"{name} failed ({rc}): {L.ssc_last_error().decode()}")
 
 
+class _Ptr(ctypes.c_void_p):
+    """A device pointer that keeps its tensor alive for as long as the pointer object lives,
+    i.e. until the ctypes call it is passed to returns.  Without this, a temporary built inside
+    the argument list is freed as soon as its address is taken and the caching allocator can
+    hand the same block to the next temporary.  (Reuse after the call is safe: the allocator
+    orders it after the work already enqueued on the stream.)"""
+
+
 def ptr(t):
     """Device pointer of a tensor (None -> NULL)."""
     if t is None:
         return None
-    return ctypes.c_void_p(t.data_ptr())
+    p = _Ptr(t.data_ptr())
+    p._keep = t
+    return p
 
 
 def f64(x, dev=None):

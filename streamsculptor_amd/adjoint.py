@@ -1,0 +1,265 @@
+"""Reverse-mode gradients of fixed-step spray streams (discrete leapfrog adjoint).
+
+The reference differentiates forward-mode only (``jax.jacfwd`` through diffrax ForwardMode,
+reference streamsculptor/main.py:160); this module provides the discrete adjoint of the
+leapfrog map with checkpoints every ``checkpoint_every`` steps, FP64 throughout.
+
+``stream_loss`` differentiates the squared distance of a perturbed spray stream's final
+positions to a target stream with respect to the halo mass and scale radius, the masses of
+live subhalos and the progenitor's initial phase-space row.  All dependencies are
+differentiated: particle forces, the release conditions (through the tidal radius, i.e. third
+derivatives of the host potential), the progenitor orbit (which feels the subhalos) and the
+subhalo orbits (which feel the host).  Subhalo scale radii are held fixed.
+"""
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+from . import _lib
+from .main import SourceSet
+
+G = _lib.G_USYS
+
+
+class LeapfrogAdjoint:
+    """Checkpointed forward/backward of ssc_leapfrog for a set of orbits in ``pot``."""
+
+    def __init__(self, pot, w0, t0, dt, nsteps, start=None, checkpoint_every=100, src=None):
+        self.h, psrc = pot._descriptor()
+        self.src = src if src is not None else psrc
+        self.sd = self.src.desc() if self.src is not None else None
+        self.w0 = _lib.f64(w0).reshape(-1, 6).contiguous()
+        self.n = self.w0.shape[0]
+        self.start = None if start is None else _lib.i32(start)
+        self.t0, self.dt, self.nsteps, self.E = float(t0), float(dt), int(nsteps), int(checkpoint_every)
+        m = 0 if self.src is None else self.src.tensors()["gm"].numel()
+        self.m = m
+        nbytes = _lib.load().ssc_adjoint_workspace_bytes(self.n, self.nsteps, self.E, m)
+        self.ws = torch.empty(max(nbytes, 1), dtype=torch.uint8, device=self.w0.device)
+        self.w_out = None
+
+    def _srcp(self):
+        return None if self.sd is None else _lib.ctypes.byref(self.sd)
+
+    def forward(self):
+        self.w_out = torch.empty_like(self.w0)
+        _lib.call("ssc_adjoint_forward", _lib.ctypes.byref(self.h), self._srcp(), _lib.ptr(self.w0),
+                  _lib.ptr(self.start), self.n, self.t0, self.dt, self.nsteps, self.E,
+                  _lib.ptr(self.w_out), _lib.ptr(self.ws), self.ws.numel())
+        return self.w_out
+
+    def backward(self, lam_T, forcing=None, want_sources=True):
+        lam_T = _lib.f64(lam_T).reshape(self.n, 6).contiguous()
+        F = None if forcing is None else _lib.f64(forcing).contiguous()
+        dev = lam_T.device
+        lam0 = torch.empty_like(lam_T)
+        nc = self.h.n_components
+        gh = torch.zeros(max(nc, 1), 2, dtype=torch.float64, device=dev)
+        gg = gp = None
+        if want_sources and self.m:
+            gg = torch.zeros(self.m, dtype=torch.float64, device=dev)
+            rows = self.nsteps + 1 if self.src.kind == _lib.SRC_TRACK else 1
+            gp = torch.zeros(rows, self.m, 3, dtype=torch.float64, device=dev)
+        _lib.call("ssc_adjoint_backward", _lib.ctypes.byref(self.h), self._srcp(),
+                  _lib.ptr(self.start), self.n, self.t0, self.dt, self.nsteps, self.E,
+                  _lib.ptr(lam_T), _lib.ptr(F), _lib.ptr(lam0), _lib.ptr(gh), _lib.ptr(gg),
+                  _lib.ptr(gp), _lib.ptr(self.ws), self.ws.numel())
+        return dict(lam0=lam0, g_host=gh[:nc], g_gm=gg, g_pos=gp)
+
+
+def release_vjp(pot, prog_w, rel_idx, g_msat, seed, g_pc, g_pf, g_vc, g_vf, kval_arr=1.0):
+    """dL/d(progenitor rows) (nrel, 6) and dL/d(host G m, p1) (n_comp, 2) of the release model."""
+    h = pot._host_only()
+    prog_w = _lib.f64(prog_w).contiguous()
+    nrel = prog_w.shape[0]
+    dev = prog_w.device
+    kv = np.asarray(kval_arr, dtype=np.float64)
+    kvp = None
+    if kv.ndim > 0:
+        kv = np.ascontiguousarray(np.ones(8) * kv)
+        kvp = kv.ctypes.data_as(_lib.ctypes.c_void_p)
+    g_prog = torch.empty(nrel, 6, dtype=torch.float64, device=dev)
+    gh = torch.zeros(max(h.n_components, 1), 2, dtype=torch.float64, device=dev)
+    scratch = torch.empty(nrel, 2 * max(h.n_components, 1), dtype=torch.float64, device=dev)
+    _lib.call("ssc_release_vjp", _lib.ctypes.byref(h), _lib.ptr(prog_w), _lib.ptr(_lib.i32(rel_idx)),
+              _lib.ptr(_lib.f64(g_msat)), nrel, int(seed), kvp,
+              *[_lib.ptr(_lib.f64(a).contiguous()) for a in (g_pc, g_pf, g_vc, g_vf)],
+              _lib.ptr(g_prog), _lib.ptr(gh), _lib.ptr(scratch))
+    return g_prog, gh[:h.n_components]
+
+
+# ------------------------------------------------------------------------------------------
+# the perturbed-stream loss (live subhalos, direct N x M sums)
+# ------------------------------------------------------------------------------------------
+class StreamModel:
+    """Fixed inputs of a perturbed spray stream: host components other than the halo, the
+    subhalo initial rows and radii, the release grid and the spray layout."""
+
+    def __init__(self, ts, sub_w0, sub_rs, msat, per_step, seed=0, dt=1.0, halo_index=2,
+                 base_components=None, checkpoint_every=100, kval_arr=1.0):
+        from . import workloads as W
+        self.ts = np.asarray(ts, dtype=np.float64)
+        self.t0 = float(self.ts[0])
+        self.dt = float(dt)
+        self.nsteps = int(round((self.ts[-1] - self.ts[0]) / dt))
+        self.sub_w0 = _lib.f64(sub_w0).reshape(-1, 6)
+        self.sub_rs = _lib.f64(sub_rs).reshape(-1)
+        self.msat = float(msat)
+        self.per_step = int(per_step)
+        self.seed = int(seed)
+        self.E = int(checkpoint_every)
+        self.kval_arr = kval_arr
+        self.base = base_components or W.host_components()
+        self.halo_index = halo_index
+
+    def host(self, m_halo, rs_halo):
+        from .main import Potential
+        comps = [list(c) for c in self.base]
+        comps[self.halo_index] = (self.base[self.halo_index][0], [G * float(m_halo), float(rs_halo)])
+
+        class _Host(Potential):
+            def __init__(self, cs):
+                super().__init__(None, {})
+                self._G = G
+                self._cs = cs
+
+            def _kernel_components(self):
+                return self._cs
+
+        return _Host(comps)
+
+    def track_sources(self, host, sub_masses):
+        _, ys = host._leapfrog(self.sub_w0, self.t0, self.dt, self.nsteps,
+                               save_idx=np.arange(self.nsteps + 1, dtype=np.int32))
+        src = SourceSet(_lib.SRC_TRACK, gm=G * _lib.f64(sub_masses), rs=self.sub_rs,
+                        pos=ys[:, :, :3].contiguous(), grid=(self.t0, self.dt))
+        return src, ys
+
+    def layout(self):
+        from .workloads import spray_layout
+        even, odd = spray_layout(self.per_step)
+        T = self.nsteps
+        sel_s = np.empty((T, self.per_step), dtype=np.int64)
+        sel_a = np.empty((T, self.per_step), dtype=np.int64)
+        for i in range(T):
+            lay = even if i % 2 == 0 else odd
+            sel_s[i] = [p[0] for p in lay]
+            sel_a[i] = [p[1] for p in lay]
+        rel = np.repeat(np.arange(T), self.per_step)
+        return sel_s.reshape(-1), sel_a.reshape(-1), rel
+
+    def forward(self, m_halo, rs_halo, sub_masses, prog_w0):
+        """Final particle rows and everything the backward pass needs."""
+        host = self.host(m_halo, rs_halo)
+        src, sub_traj = self.track_sources(host, sub_masses)
+        total = _with_sources(host, src)
+        T = self.nsteps
+        _, pys = total._leapfrog(_lib.f64(prog_w0).reshape(1, 6), self.t0, self.dt, T,
+                                 save_idx=np.arange(T + 1, dtype=np.int32))
+        prog_rows = pys[:, 0, :].contiguous()
+        nseed = (self.per_step + 1) // 2
+        dev = prog_rows.device
+        table = torch.empty(nseed, 2, T + 1, 6, dtype=torch.float64, device=dev)
+        for s in range(nseed):
+            pc, pf, vc, vf = host._release_along(prog_rows, T + 1, self.msat, self.seed + s,
+                                                 self.kval_arr)
+            table[s, 0, :, :3], table[s, 0, :, 3:] = pc, vc
+            table[s, 1, :, :3], table[s, 1, :, 3:] = pf, vf
+        sel_s, sel_a, rel = self.layout()
+        si, ai, ri = (torch.as_tensor(a, device=dev) for a in (sel_s, sel_a, rel))
+        w0 = table[si, ai, ri].contiguous()
+        start = rel.astype(np.int32)
+        padj = LeapfrogAdjoint(total, w0, self.t0, self.dt, T, start=start,
+                               checkpoint_every=self.E, src=src)
+        wT = padj.forward()
+        return dict(host=host, src=src, total=total, prog_rows=prog_rows, wT=wT, padj=padj,
+                    idx=(si, ai, ri), nseed=nseed, prog_w0=_lib.f64(prog_w0).reshape(1, 6))
+
+    def backward(self, st, lam_T):
+        """Gradients of L with dL/d(final rows) = lam_T: (dL/dm_halo, dL/drs_halo, dL/dm_j,
+        dL/dprog_w0)."""
+        T = self.nsteps
+        host, src, total = st["host"], st["src"], st["total"]
+        hi = self.halo_index
+        # particles
+        A = st["padj"].backward(lam_T)
+        g_host = A["g_host"].clone()
+        g_gm = A["g_gm"].clone()
+        g_pos = A["g_pos"].clone()
+        # release model: scatter the particle IC adjoints back to (seed, arm, release)
+        si, ai, ri = st["idx"]
+        dev = lam_T.device
+        g_table = torch.zeros(st["nseed"], 2, T + 1, 6, dtype=torch.float64, device=dev)
+        flat = (si * 2 + ai) * (T + 1) + ri
+        g_table.view(-1, 6).index_add_(0, flat, A["lam0"])
+        g_prog_rows = torch.zeros(T + 1, 6, dtype=torch.float64, device=dev)
+        rel_idx = np.arange(T + 1, dtype=np.int32)
+        gms = torch.full((T + 1,), G * self.msat, dtype=torch.float64, device=dev)
+        for s in range(st["nseed"]):
+            gp, gh = release_vjp(host, st["prog_rows"], rel_idx, gms, self.seed + s,
+                                 g_table[s, 0, :, :3], g_table[s, 1, :, :3],
+                                 g_table[s, 0, :, 3:], g_table[s, 1, :, 3:], self.kval_arr)
+            g_prog_rows += gp
+            g_host += gh
+        # progenitor orbit (in host + subhalos), forced by the release adjoints
+        pa = LeapfrogAdjoint(total, st["prog_w0"], self.t0, self.dt, T, checkpoint_every=self.E,
+                             src=src)
+        pa.forward()
+        C = pa.backward(torch.zeros(1, 6, dtype=torch.float64, device=dev),
+                        forcing=g_prog_rows.reshape(T + 1, 1, 6))
+        g_host += C["g_host"]
+        g_gm += C["g_gm"]
+        g_pos += C["g_pos"]
+        # subhalo orbits (in the host), forced by the source-position adjoints
+        M = self.sub_w0.shape[0]
+        F = torch.zeros(T + 1, M, 6, dtype=torch.float64, device=dev)
+        F[:, :, :3] = g_pos
+        sa = LeapfrogAdjoint(host, self.sub_w0, self.t0, self.dt, T, checkpoint_every=self.E)
+        sa.forward()
+        D = sa.backward(torch.zeros(M, 6, dtype=torch.float64, device=dev), forcing=F)
+        g_host += D["g_host"]
+        return (G * g_host[hi, 0], g_host[hi, 1], G * g_gm, C["lam0"].reshape(6))
+
+
+def _with_sources(host, src):
+    from .main import Potential
+
+    class _Total(Potential):
+        def __init__(self):
+            super().__init__(None, {})
+            self._G = G
+
+        def _kernel_components(self):
+            return host._kernel_components()
+
+        def _kernel_sources(self):
+            return src
+
+    return _Total()
+
+
+class _StreamLoss(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, halo, sub_masses, prog_w0, model, target):
+        st = model.forward(halo[0], halo[1], sub_masses, prog_w0)
+        d = st["wT"][:, :3] - target
+        loss = (d * d).sum()
+        ctx.model, ctx.st, ctx.d = model, st, d
+        return loss
+
+    @staticmethod
+    def backward(ctx, gout):
+        lam_T = torch.zeros_like(ctx.st["wT"])
+        lam_T[:, :3] = 2.0 * ctx.d
+        gm_h, grs_h, g_m, g_w0 = ctx.model.backward(ctx.st, lam_T)
+        go = gout.to(torch.float64)
+        ghalo = torch.stack([gm_h, grs_h]) * go
+        return ghalo, g_m * go, g_w0 * go, None, None
+
+
+def stream_loss(halo, sub_masses, prog_w0, model, target):
+    """L = sum_i |x_i(T) - target_i|^2 of the perturbed spray stream, differentiable with respect
+    to halo = (m_halo, r_s_halo), the subhalo masses and the progenitor row (torch tensors on the
+    GPU; use .backward() or torch.autograd.grad)."""
+    return _StreamLoss.apply(halo, sub_masses, prog_w0, model, _lib.f64(target))

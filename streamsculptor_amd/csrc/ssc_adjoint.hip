@@ -1,0 +1,486 @@
+// Discrete adjoint of the fixed-step leapfrog on gfx950 (checkpointed reverse mode).
+//
+// Forward: ssc_leapfrog with checkpoint rows every E grid steps (FP64, stored in the caller's
+// workspace).  Backward, segment by segment from the last: each lane re-runs its orbit from the
+// segment's checkpoint storing the positions in HBM, then sweeps back through the segment:
+//   lv_half = lv + dt lx;  lv = lv_half + F_v(k);  lx += F_x(k)
+//   W_k = h lv_half + h lv (interior; only h lv_half at the start row, h lv at the last row)
+//   lx -= H(x_k) W_k   (H: Hessian of host + sources),  dL/dtheta_host += (da/dtheta)^T W_k
+// and records W_k.  A second kernel turns the recorded (x_k, W_k) of all particles into the
+// source adjoints dL/dS_jk = sum_i H_j(x_ik - S_jk) W_ik and dL/d(Gm_j) = -sum (d.W)/s^3 with the
+// sources on the lanes, so no cross-lane reduction sits in the sweep.
+#include <hip/hip_runtime.h>
+
+#include <climits>
+#include <algorithm>
+#include <cstring>
+
+#include "ssc_device.h"
+#include "ssc_sources_lds.h"
+
+namespace ssc {
+
+constexpr int AJ_BLOCK = 256;
+constexpr int AJ_MAXC = 2 * SSC_MAX_COMPONENTS;
+
+// d(grad Phi_c)/d(p1) for one component (MN a, NFW r_s, Hernquist c, Plummer r_s).
+__device__ __forceinline__ void comp_grad_dp1(const ssc_component& c, double x, double y, double z,
+                                              double* out) {
+  const double gm = c.p[0];
+  out[0] = out[1] = out[2] = 0.0;
+  switch (c.kind) {
+    case SSC_COMP_MIYAMOTO_NAGAI: {
+      const double a = c.p[1], b = c.p[2];
+      const double zeta = sqrt(z * z + b * b), S = a + zeta;
+      const double D2 = x * x + y * y + S * S, D = sqrt(D2);
+      const double iD3 = 1.0 / (D2 * D), iD5 = iD3 / D2;
+      out[0] = -3.0 * gm * x * S * iD5;
+      out[1] = -3.0 * gm * y * S * iD5;
+      out[2] = gm * (z / zeta) * iD3 - 3.0 * gm * (z * S / zeta) * S * iD5;
+      break;
+    }
+    case SSC_COMP_NFW: {
+      const double rs = c.p[1];
+      const double r = sqrt(x * x + y * y + z * z), rp = r + rs;
+      const double f = gm * (-1.0 / (r * rs * rp) + 1.0 / (r * rp * rp)) / r;
+      out[0] = f * x; out[1] = f * y; out[2] = f * z;
+      break;
+    }
+    case SSC_COMP_HERNQUIST: {
+      const double cc = c.p[1], soft = c.p[2];
+      const double r = sqrt(x * x + y * y + z * z + soft), rc = r + cc;
+      const double f = -2.0 * gm / (r * rc * rc * rc);
+      out[0] = f * x; out[1] = f * y; out[2] = f * z;
+      break;
+    }
+    case SSC_COMP_PLUMMER: {
+      const double rs = c.p[1];
+      const double s2 = x * x + y * y + z * z + rs * rs, s = sqrt(s2);
+      const double f = -3.0 * gm * rs / (s2 * s2 * s);
+      out[0] = f * x; out[1] = f * y; out[2] = f * z;
+      break;
+    }
+    default:
+      break;
+  }
+}
+
+// Adjoint contribution at one grid point: lx -= H W, host-parameter gradients += (da/dp)^T W.
+template <int SK>
+__device__ __forceinline__ void adj_apply(const ssc_host_potential& hp, const SourceLDS<SK>& lds,
+                                          int k, double t, double x, double y, double z,
+                                          const double* W, double* lx, double* gh) {
+  double phi, g[3], H[9];
+  host_full(hp, x, y, z, phi, g, H);
+  for (int c = 0; c < hp.n_components; ++c) {
+    double gc[3] = {0.0, 0.0, 0.0};
+    comp_grad(hp.c[c], x, y, z, gc[0], gc[1], gc[2]);
+    double dp[3];
+    comp_grad_dp1(hp.c[c], x, y, z, dp);
+    gh[2 * c] -= (gc[0] * W[0] + gc[1] * W[1] + gc[2] * W[2]) / hp.c[c].p[0];
+    gh[2 * c + 1] -= dp[0] * W[0] + dp[1] * W[1] + dp[2] * W[2];
+  }
+  double hw[3];
+#pragma unroll
+  for (int r = 0; r < 3; ++r) hw[r] = H[3 * r] * W[0] + H[3 * r + 1] * W[1] + H[3 * r + 2] * W[2];
+  if (SK != SSC_SRC_NONE) {
+    for (int j = 0; j < lds.n; ++j) {
+      double sx, sy, sz, gm, q2;
+      if (!lds.get(k, t, j, sx, sy, sz, gm, q2)) continue;
+      const double dx = x - sx, dy = y - sy, dz = z - sz;
+      const double s2 = fma(dx, dx, fma(dy, dy, fma(dz, dz, q2)));
+      const double is = rsq(s2);
+      const double i3 = gm * is * is * is;
+      const double c5 = 3.0 * i3 * is * is * (dx * W[0] + dy * W[1] + dz * W[2]);
+      hw[0] += i3 * W[0] - c5 * dx;
+      hw[1] += i3 * W[1] - c5 * dy;
+      hw[2] += i3 * W[2] - c5 * dz;
+    }
+  }
+  lx[0] -= hw[0];
+  lx[1] -= hw[1];
+  lx[2] -= hw[2];
+}
+
+// One checkpoint segment [k0, k1] for all particles (block-uniform grid walk).
+//   ckpt: state at row k0 (or the initial row for particles starting later)
+//   lam:  in/out adjoint of each particle's state (at k1 on entry, at max(k0, start) on exit)
+//   X, Wb: [k1 - k0 + 1][n][3] recorded positions and weights (zero where inactive)
+//   gpart: [gridDim.x][2*n_comp] per-block host-parameter partials (accumulated)
+template <int SK>
+__global__ __launch_bounds__(AJ_BLOCK) void adjoint_segment_kernel(
+    ssc_host_potential hp, ssc_sources src, const double* __restrict__ ckpt,
+    const int32_t* __restrict__ start, int64_t n, double t0, double dt, int nsteps, int k0, int k1,
+    const double* __restrict__ lam_T, const double* __restrict__ forcing, double* __restrict__ lam,
+    double* __restrict__ X, double* __restrict__ Wb, double* __restrict__ gpart) {
+  extern __shared__ __align__(16) char lds_raw[];
+  __shared__ int kmin_sh;
+  __shared__ double red[AJ_BLOCK / 64][AJ_MAXC];
+  const int64_t i = (int64_t)blockIdx.x * AJ_BLOCK + threadIdx.x;
+  const bool valid = i < n;
+  int s = nsteps;
+  double x = 1.0, y = 0.0, z = 0.0, vx = 0.0, vy = 0.0, vz = 0.0;
+  if (valid) {
+    s = start ? start[i] : 0;
+    s = s < 0 ? 0 : (s > nsteps ? nsteps : s);
+    const double* w = ckpt + 6 * i;
+    x = w[0]; y = w[1]; z = w[2]; vx = w[3]; vy = w[4]; vz = w[5];
+  }
+  const int kb = max(k0, s);                  // this particle's first row in the segment
+  const bool act = valid && kb < k1;          // moves inside the segment
+  if (threadIdx.x == 0) kmin_sh = INT_MAX;
+  __syncthreads();
+  atomicMin(&kmin_sh, act ? kb : INT_MAX);
+  SourceLDS<SK> lds;
+  lds.init(src, lds_raw);
+  __syncthreads();
+  const int kmin = kmin_sh;
+  const int rows = k1 - k0 + 1;
+  const double h = 0.5 * dt;
+  // zero the records of this particle (rows it does not reach stay zero)
+  if (valid)
+    for (int r = 0; r < rows; ++r) {
+      double* wb = Wb + ((int64_t)r * n + i) * 3;
+      wb[0] = wb[1] = wb[2] = 0.0;
+    }
+  double gh[AJ_MAXC];
+#pragma unroll
+  for (int c = 0; c < AJ_MAXC; ++c) gh[c] = 0.0;
+  if (kmin < k1) {
+    // ---- recompute the segment, recording positions
+    double ax = 0.0, ay = 0.0, az = 0.0;
+    if (act) {
+      double* o = X + ((int64_t)(kb - k0) * n + i) * 3;
+      o[0] = x; o[1] = y; o[2] = z;
+    }
+    lds.stage(src, kmin, nsteps + 1);
+    {
+      double gx, gy, gz;
+      host_grad(hp, x, y, z, gx, gy, gz);
+      lds.template add_grad<1>(kmin, t0 + kmin * dt, &x, &y, &z, &gx, &gy, &gz, 0, 1);
+      ax = -gx; ay = -gy; az = -gz;
+    }
+    for (int k = kmin; k < k1; ++k) {
+      lds.stage(src, k, nsteps + 1);
+      const bool mv = act && k >= kb;
+      if (mv) {
+        vx += h * ax; vy += h * ay; vz += h * az;
+        x += dt * vx; y += dt * vy; z += dt * vz;
+      }
+      double gx, gy, gz;
+      host_grad(hp, x, y, z, gx, gy, gz);
+      lds.template add_grad<1>(k + 1, t0 + (k + 1) * dt, &x, &y, &z, &gx, &gy, &gz, 0, 1);
+      ax = -gx; ay = -gy; az = -gz;
+      if (mv) {
+        vx += h * ax; vy += h * ay; vz += h * az;
+        double* o = X + ((int64_t)(k + 1 - k0) * n + i) * 3;
+        o[0] = x; o[1] = y; o[2] = z;
+      }
+    }
+    // ---- reverse sweep
+    double lx[3] = {0, 0, 0}, lv[3] = {0, 0, 0};
+    if (valid) {
+      const double* L = (k1 == nsteps ? lam_T : lam) + 6 * i;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) { lx[c] = L[c]; lv[c] = L[3 + c]; }
+    }
+    auto F = [&](int k, int c) -> double {
+      return forcing ? forcing[((int64_t)k * n + i) * 6 + c] : 0.0;
+    };
+    if (k1 == nsteps) {  // terminal row: forcing there and the closing half kick
+      if (valid) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) { lx[c] += F(nsteps, c); lv[c] += F(nsteps, 3 + c); }
+      }
+      lds.stage_row(src, k1, nsteps + 1);
+      if (act) {
+        const double* xp = X + ((int64_t)(k1 - k0) * n + i) * 3;
+        const double W[3] = {h * lv[0], h * lv[1], h * lv[2]};
+        adj_apply<SK>(hp, lds, k1, t0 + k1 * dt, xp[0], xp[1], xp[2], W, lx, gh);
+        double* wb = Wb + ((int64_t)(k1 - k0) * n + i) * 3;
+        wb[0] = W[0]; wb[1] = W[1]; wb[2] = W[2];
+      }
+    }
+    for (int k = k1 - 1; k >= kmin; --k) {
+      lds.stage_row(src, k, nsteps + 1);
+      if (act && k >= kb) {
+        double lh[3], W[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) lh[c] = lv[c] + dt * lx[c];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) { lv[c] = lh[c] + F(k, 3 + c); lx[c] += F(k, c); }
+#pragma unroll
+        for (int c = 0; c < 3; ++c) W[c] = h * lh[c] + (k > s ? h * lv[c] : 0.0);
+        const double* xp = X + ((int64_t)(k - k0) * n + i) * 3;
+        adj_apply<SK>(hp, lds, k, t0 + k * dt, xp[0], xp[1], xp[2], W, lx, gh);
+        double* wb = Wb + ((int64_t)(k - k0) * n + i) * 3;
+        wb[0] = W[0]; wb[1] = W[1]; wb[2] = W[2];
+      }
+    }
+    if (valid) {
+      double* L = lam + 6 * i;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) { L[c] = lx[c]; L[3 + c] = lv[c]; }
+    }
+  } else if (valid && k1 == nsteps) {
+    // nothing moves in the last segment: the adjoint of the (unmoved) row is lam_T + forcing
+#pragma unroll
+    for (int c = 0; c < 6; ++c) {
+      const double f = forcing ? forcing[((int64_t)nsteps * n + i) * 6 + c] : 0.0;
+      lam[6 * i + c] = lam_T[6 * i + c] + f;
+    }
+  }
+  // block reduction of the host-parameter gradients (wave shuffles, then LDS)
+  const int nc2 = 2 * hp.n_components;
+  for (int c = 0; c < nc2; ++c) {
+    double v = gh[c];
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6][c] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < nc2) {
+    double v = 0.0;
+    for (int wv = 0; wv < AJ_BLOCK / 64; ++wv) v += red[wv][threadIdx.x];
+    gpart[(int64_t)blockIdx.x * AJ_MAXC + threadIdx.x] += v;
+  }
+}
+
+// Source adjoints of one segment: lanes = sources, one (row, particle chunk) per block.
+//   part[chunk][r][j][4] = (dL/dS_jk (3), dL/d(Gm_j)) summed over the chunk's particles
+constexpr int SR_CHUNK = 2048;
+
+template <int SK>
+__global__ __launch_bounds__(64) void source_adjoint_kernel(
+    ssc_sources src, const double* __restrict__ X, const double* __restrict__ Wb, int64_t n,
+    int k0, int rows, double t0, double dt, double* __restrict__ part) {
+  const int r = blockIdx.y, chunk = blockIdx.x;
+  const int k = k0 + r;
+  const double t = t0 + k * dt;
+  const int64_t ib = (int64_t)chunk * SR_CHUNK, ie = min(n, ib + SR_CHUNK);
+  for (int j = threadIdx.x; j < src.n; j += 64) {
+    double sx, sy, sz;
+    if (!source_position(src, j, k, t, sx, sy, sz)) {
+      double* o = part + (((int64_t)chunk * rows + r) * src.n + j) * 4;
+      o[0] = o[1] = o[2] = o[3] = 0.0;
+      continue;
+    }
+    const double gm = src.gm[j], q2 = src.rs[j] * src.rs[j];
+    double gx = 0.0, gy = 0.0, gz = 0.0, gg = 0.0;
+    for (int64_t i = ib; i < ie; ++i) {
+      const double* w = Wb + ((int64_t)r * n + i) * 3;
+      const double W0 = w[0], W1 = w[1], W2 = w[2];
+      if (W0 == 0.0 && W1 == 0.0 && W2 == 0.0) continue;  // uniform across the wave
+      const double* xp = X + ((int64_t)r * n + i) * 3;
+      const double dx = xp[0] - sx, dy = xp[1] - sy, dz = xp[2] - sz;
+      const double s2 = fma(dx, dx, fma(dy, dy, fma(dz, dz, q2)));
+      const double is = rsq(s2);
+      const double i3 = is * is * is;
+      const double dW = dx * W0 + dy * W1 + dz * W2;
+      const double c5 = 3.0 * i3 * is * is * dW;
+      gx += gm * (i3 * W0 - c5 * dx);
+      gy += gm * (i3 * W1 - c5 * dy);
+      gz += gm * (i3 * W2 - c5 * dz);
+      gg -= i3 * dW;
+    }
+    double* o = part + (((int64_t)chunk * rows + r) * src.n + j) * 4;
+    o[0] = gx; o[1] = gy; o[2] = gz; o[3] = gg;
+  }
+}
+
+// Sum the chunk partials: g_pos[row][j][3] += ..., g_gm[j] += ... (one thread per (row, j)).
+__global__ void source_adjoint_reduce(const double* __restrict__ part, int nchunk, int rows, int m,
+                                      int k0, int track, double* __restrict__ g_pos,
+                                      double* __restrict__ gm_rows) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (int64_t)rows * m) return;
+  const int r = (int)(e / m), j = (int)(e - (int64_t)r * m);
+  double a[4] = {0, 0, 0, 0};
+  for (int c = 0; c < nchunk; ++c) {
+    const double* p = part + (((int64_t)c * rows + r) * m + j) * 4;
+    a[0] += p[0]; a[1] += p[1]; a[2] += p[2]; a[3] += p[3];
+  }
+  if (g_pos) {
+    double* o = g_pos + ((track ? (int64_t)(k0 + r) * m : 0) + j) * 3;
+    o[0] += a[0]; o[1] += a[1]; o[2] += a[2];
+  }
+  gm_rows[(int64_t)r * m + j] = a[3];
+}
+
+__global__ void gm_rows_reduce(const double* __restrict__ gm_rows, int rows, int m,
+                               double* __restrict__ g_gm) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= m) return;
+  double a = 0.0;
+  for (int r = 0; r < rows; ++r) a += gm_rows[(int64_t)r * m + j];
+  g_gm[j] += a;
+}
+
+__global__ void host_partial_reduce(const double* __restrict__ gpart, int nblk, int nc2,
+                                    double* __restrict__ g_host) {
+  const int c = threadIdx.x;
+  if (c >= nc2) return;
+  double a = 0.0;
+  for (int b = 0; b < nblk; ++b) a += gpart[(int64_t)b * AJ_MAXC + c];
+  g_host[c] = a;
+}
+
+__global__ void ckpt_index_kernel(int32_t* __restrict__ idx, int64_t nseg, int E, int nsteps) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c <= nseg) idx[c] = (int32_t)min((int64_t)nsteps, c * E);
+}
+
+__global__ void zero_kernel(double* __restrict__ p, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0.0;
+}
+
+// ------------------------------------------------------------------------------------------
+// workspace
+// ------------------------------------------------------------------------------------------
+struct AdjWs {
+  double *ckpt, *lam, *X, *Wb, *gpart, *part, *gm_rows;
+  int32_t* save_idx;
+};
+
+static int64_t nseg_of(int nsteps, int E) { return nsteps == 0 ? 0 : (nsteps + E - 1) / E; }
+
+static int64_t adj_bytes(int64_t n, int nsteps, int E, int m, AdjWs* w, char* base) {
+  const int64_t nseg = nseg_of(nsteps, E);
+  const int64_t nblk = (n + AJ_BLOCK - 1) / AJ_BLOCK;
+  const int64_t nchunk = (n + SR_CHUNK - 1) / SR_CHUNK;
+  const int64_t rows = (int64_t)E + 1;
+  int64_t off = 0;
+  auto take = [&](int64_t bytes) {
+    char* r = base ? base + off : nullptr;
+    off += (bytes + 255) / 256 * 256;
+    return r;
+  };
+  char* c1 = take((nseg + 1) * n * 6 * 8);
+  char* c2 = take(n * 6 * 8);
+  char* c3 = take(rows * n * 3 * 8);
+  char* c4 = take(rows * n * 3 * 8);
+  char* c5 = take(nblk * AJ_MAXC * 8);
+  char* c6 = take(nchunk * rows * (m > 0 ? m : 1) * 4 * 8);
+  char* c7 = take(rows * (m > 0 ? m : 1) * 8);
+  char* c8 = take((nseg + 1) * 4);
+  if (w) {
+    w->ckpt = (double*)c1; w->lam = (double*)c2; w->X = (double*)c3; w->Wb = (double*)c4;
+    w->gpart = (double*)c5; w->part = (double*)c6; w->gm_rows = (double*)c7;
+    w->save_idx = (int32_t*)c8;
+  }
+  return off;
+}
+
+template <int SK>
+static int adjoint_backward(const ssc_host_potential& hp, const ssc_sources& src,
+                            const int32_t* start, int64_t n, double t0, double dt, int nsteps,
+                            int E, const double* lam_T, const double* forcing, double* lam_0,
+                            double* g_host, double* g_gm, double* g_pos, AdjWs& w,
+                            hipStream_t st) {
+  const int m = SK == SSC_SRC_NONE ? 0 : src.n;
+  const int64_t nseg = nseg_of(nsteps, E);
+  const unsigned nblk = (unsigned)((n + AJ_BLOCK - 1) / AJ_BLOCK);
+  const int64_t nchunk = (n + SR_CHUNK - 1) / SR_CHUNK;
+  const size_t lds = SourceLDS<SK>::bytes(src);
+  if (lds > 60 * 1024) {
+    hipError_t e = hipFuncSetAttribute((const void*)adjoint_segment_kernel<SK>,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return fail(SSC_E_HIP, std::string("hipFuncSetAttribute: ") + hipGetErrorString(e));
+  }
+  hipLaunchKernelGGL(zero_kernel, dim3((nblk * AJ_MAXC + 255) / 256), dim3(256), 0, st, w.gpart,
+                     (int64_t)nblk * AJ_MAXC);
+  if (g_gm && m) hipLaunchKernelGGL(zero_kernel, dim3((m + 255) / 256), dim3(256), 0, st, g_gm, (int64_t)m);
+  if (g_pos && m) {
+    const int64_t np = (SK == SSC_SRC_TRACK ? (int64_t)(nsteps + 1) : 1) * m * 3;
+    hipLaunchKernelGGL(zero_kernel, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, st, g_pos, np);
+  }
+  for (int64_t c = nseg - 1; c >= 0; --c) {
+    const int k0 = (int)(c * E), k1 = (int)std::min<int64_t>((c + 1) * E, nsteps);
+    const int rows = k1 - k0 + 1;
+    hipLaunchKernelGGL(adjoint_segment_kernel<SK>, dim3(nblk), dim3(AJ_BLOCK), lds, st, hp, src,
+                       w.ckpt + c * n * 6, start, n, t0, dt, nsteps, k0, k1, lam_T, forcing, w.lam,
+                       w.X, w.Wb, w.gpart);
+    if (m > 0 && (g_gm || g_pos)) {
+      hipLaunchKernelGGL(source_adjoint_kernel<SK>, dim3((unsigned)nchunk, (unsigned)rows), dim3(64),
+                         0, st, src, w.X, w.Wb, n, k0, rows, t0, dt, w.part);
+      hipLaunchKernelGGL(source_adjoint_reduce, dim3((unsigned)((rows * m + 255) / 256)), dim3(256),
+                         0, st, w.part, (int)nchunk, rows, m, k0, SK == SSC_SRC_TRACK ? 1 : 0,
+                         g_pos, w.gm_rows);
+      if (g_gm)
+        hipLaunchKernelGGL(gm_rows_reduce, dim3((m + 255) / 256), dim3(256), 0, st, w.gm_rows, rows,
+                           m, g_gm);
+    }
+  }
+  hipMemcpyAsync(lam_0, w.lam, n * 6 * 8, hipMemcpyDeviceToDevice, st);
+  hipLaunchKernelGGL(host_partial_reduce, dim3(1), dim3(64), 0, st, w.gpart, (int)nblk,
+                     2 * hp.n_components, g_host);
+  return check_launch("ssc_adjoint_backward");
+}
+
+}  // namespace ssc
+
+using namespace ssc;
+
+extern "C" {
+
+int64_t ssc_adjoint_workspace_bytes(int64_t n, int32_t nsteps, int32_t checkpoint_every,
+                                    int32_t n_src) {
+  if (n <= 0 || nsteps < 0 || checkpoint_every <= 0) return 0;
+  return adj_bytes(n, nsteps, checkpoint_every, n_src, nullptr, nullptr);
+}
+
+int ssc_adjoint_forward(const ssc_host_potential* host, const ssc_sources* src, const double* w0,
+                        const int32_t* start_step, int64_t n, double t0, double dt, int32_t nsteps,
+                        int32_t checkpoint_every, double* w_out, void* workspace,
+                        int64_t workspace_bytes, void* stream) {
+  if (n <= 0 || nsteps < 0 || checkpoint_every <= 0 || !w0 || !w_out || !workspace)
+    return fail(SSC_E_ARG, "ssc_adjoint_forward: bad argument");
+  const int m = (src && src->kind != SSC_SRC_NONE) ? src->n : 0;
+  if (workspace_bytes < adj_bytes(n, nsteps, checkpoint_every, m, nullptr, nullptr))
+    return fail(SSC_E_ARG, "ssc_adjoint_forward: workspace too small");
+  AdjWs w;
+  adj_bytes(n, nsteps, checkpoint_every, m, &w, (char*)workspace);
+  const int64_t nseg = nseg_of(nsteps, checkpoint_every);
+  hipLaunchKernelGGL(ckpt_index_kernel, dim3((unsigned)((nseg + 256) / 256)), dim3(256), 0,
+                     (hipStream_t)stream, w.save_idx, nseg, (int)checkpoint_every, (int)nsteps);
+  return ssc_leapfrog(host, src, w0, start_step, n, t0, dt, nsteps, w_out, w.save_idx,
+                      (int32_t)(nseg + 1), w.ckpt, stream);
+}
+
+int ssc_adjoint_backward(const ssc_host_potential* host, const ssc_sources* src,
+                         const int32_t* start_step, int64_t n, double t0, double dt,
+                         int32_t nsteps, int32_t checkpoint_every, const double* lam_T,
+                         const double* forcing, double* lam_0, double* g_host, double* g_gm,
+                         double* g_pos, void* workspace, int64_t workspace_bytes, void* stream) {
+  int rc = validate_host(host);
+  if (rc) return rc;
+  rc = validate_sources(src, nsteps);
+  if (rc) return rc;
+  if (n <= 0 || nsteps < 1 || checkpoint_every <= 0 || !lam_T || !lam_0 || !g_host || !workspace)
+    return fail(SSC_E_ARG, "ssc_adjoint_backward: bad argument (needs n >= 1, nsteps >= 1)");
+  ssc_sources s;
+  if (src) s = *src; else memset(&s, 0, sizeof(s));
+  if (s.n == 0) s.kind = SSC_SRC_NONE;
+  const int m = s.kind == SSC_SRC_NONE ? 0 : s.n;
+  if (workspace_bytes < adj_bytes(n, nsteps, checkpoint_every, m, nullptr, nullptr))
+    return fail(SSC_E_ARG, "ssc_adjoint_backward: workspace too small");
+  AdjWs w;
+  adj_bytes(n, nsteps, checkpoint_every, m, &w, (char*)workspace);
+  hipStream_t st = (hipStream_t)stream;
+  switch (s.kind) {
+    case SSC_SRC_NONE:
+      return adjoint_backward<SSC_SRC_NONE>(*host, s, start_step, n, t0, dt, nsteps, checkpoint_every,
+                                            lam_T, forcing, lam_0, g_host, g_gm, g_pos, w, st);
+    case SSC_SRC_STATIC:
+      return adjoint_backward<SSC_SRC_STATIC>(*host, s, start_step, n, t0, dt, nsteps, checkpoint_every,
+                                              lam_T, forcing, lam_0, g_host, g_gm, g_pos, w, st);
+    case SSC_SRC_TRACK:
+      return adjoint_backward<SSC_SRC_TRACK>(*host, s, start_step, n, t0, dt, nsteps, checkpoint_every,
+                                             lam_T, forcing, lam_0, g_host, g_gm, g_pos, w, st);
+    default:
+      return fail(SSC_E_ARG, "ssc_adjoint_backward: LINE sources are not differentiated");
+  }
+}
+
+}  // extern "C"

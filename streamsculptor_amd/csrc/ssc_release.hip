@@ -148,6 +148,177 @@ __global__ __launch_bounds__(256) void release_kernel(ssc_host_potential hp,
   }
 }
 
+
+// ------------------------------------------------------------------------------------------
+// Vector-Jacobian product of the release model.  The Jacobian is formed column by column in
+// forward mode (one dual tangent per input: the progenitor row and (G m, p1) of every host
+// component); the tidal radius depends on the host Hessian, so its derivative carries third
+// derivatives of the potential.  Restates release_func_jacobian (reference
+// streamsculptor/perturbative.py:282-296) with the host parameters as extra inputs.
+// ------------------------------------------------------------------------------------------
+struct D1 {
+  double v, d;
+};
+__device__ __forceinline__ D1 mk(double v) { return D1{v, 0.0}; }
+__device__ __forceinline__ D1 operator+(D1 a, D1 b) { return D1{a.v + b.v, a.d + b.d}; }
+__device__ __forceinline__ D1 operator-(D1 a, D1 b) { return D1{a.v - b.v, a.d - b.d}; }
+__device__ __forceinline__ D1 operator-(D1 a) { return D1{-a.v, -a.d}; }
+__device__ __forceinline__ D1 operator*(D1 a, D1 b) { return D1{a.v * b.v, a.d * b.v + a.v * b.d}; }
+__device__ __forceinline__ D1 operator/(D1 a, D1 b) {
+  const double q = a.v / b.v;
+  return D1{q, (a.d - q * b.d) / b.v};
+}
+__device__ __forceinline__ D1 operator*(double a, D1 b) { return D1{a * b.v, a * b.d}; }
+__device__ __forceinline__ D1 operator+(D1 a, double b) { return D1{a.v + b, a.d}; }
+__device__ __forceinline__ D1 dsqrt(D1 a) {
+  const double r = sqrt(a.v);
+  return D1{r, a.d / (2.0 * r)};
+}
+__device__ __forceinline__ D1 dlog1p(D1 a) { return D1{log1p(a.v), a.d / (1.0 + a.v)}; }
+__device__ __forceinline__ D1 dcbrt(D1 a) {
+  const double c = cbrt(a.v);
+  return D1{c, a.d / (3.0 * c * c)};
+}
+
+// Hessian of one component (full 3x3) for dual arguments.
+__device__ void comp_hess_d(int kind, const D1* p, const D1* x, D1* H) {
+  const D1 gm = p[0];
+  if (kind == SSC_COMP_MIYAMOTO_NAGAI) {
+    const D1 a = p[1], b = p[2];
+    const D1 zeta = dsqrt(x[2] * x[2] + b * b);
+    const D1 S = a + zeta;
+    const D1 D2 = x[0] * x[0] + x[1] * x[1] + S * S;
+    const D1 D = dsqrt(D2);
+    const D1 iD3 = mk(1.0) / (D2 * D);
+    const D1 iD5 = iD3 / D2;
+    const D1 q[3] = {x[0], x[1], x[2] * S / zeta};
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) H[3 * i + j] = H[3 * i + j] - 3.0 * (gm * q[i] * q[j] * iD5);
+    H[0] = H[0] + gm * iD3;
+    H[4] = H[4] + gm * iD3;
+    const D1 z2 = zeta * zeta;
+    const D1 ddz = S * b * b / (z2 * zeta) + x[2] * x[2] / z2;
+    H[8] = H[8] + gm * ddz * iD3;
+    return;
+  }
+  D1 r, d1, d2;
+  if (kind == SSC_COMP_NFW) {
+    const D1 rs = p[1];
+    r = dsqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2]);
+    const D1 L = dlog1p(r / rs);
+    const D1 rp = r + rs;
+    d1 = gm * (L / (r * r) - mk(1.0) / (r * rp));
+    d2 = gm * (mk(1.0) / (r * r * rp) - 2.0 * L / (r * r * r) + (2.0 * r + rs) / (r * r * rp * rp));
+  } else if (kind == SSC_COMP_HERNQUIST) {
+    const D1 c = p[1], soft = p[2];
+    r = dsqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2] + soft);
+    const D1 rc = r + c;
+    d1 = gm / (rc * rc);
+    d2 = -2.0 * gm / (rc * rc * rc);
+  } else if (kind == SSC_COMP_PLUMMER) {
+    const D1 rs = p[1];
+    const D1 s2 = x[0] * x[0] + x[1] * x[1] + x[2] * x[2] + rs * rs;
+    const D1 sq = dsqrt(s2);
+    const D1 f = gm / (s2 * sq);
+    const D1 k = -3.0 * f / s2;
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) {
+        D1 hh = k * x[i] * x[j];
+        if (i == j) hh = hh + f;
+        H[3 * i + j] = H[3 * i + j] + hh;
+      }
+    return;
+  } else {
+    return;
+  }
+  const D1 d1r = d1 / r;
+  const D1 c = (d2 - d1r) / (r * r);
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) {
+      D1 hh = c * x[i] * x[j];
+      if (i == j) hh = hh + d1r;
+      H[3 * i + j] = H[3 * i + j] + hh;
+    }
+}
+
+// Release model with dual inputs: out = [pos_close(3), pos_far(3), vel_close(3), vel_far(3)].
+__device__ void release_d(const ssc_host_potential& hp, const D1* hostp, const D1* in, D1 gms,
+                          const double* zr, const KVals& kv, D1* out) {
+  const D1 x[3] = {in[0], in[1], in[2]}, v[3] = {in[3], in[4], in[5]};
+  D1 H[9];
+  for (int k = 0; k < 9; ++k) H[k] = mk(0.0);
+  for (int c = 0; c < hp.n_components; ++c) comp_hess_d(hp.c[c].kind, hostp + 4 * c, x, H);
+  const D1 rad = dsqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2]);
+  const D1 rh[3] = {x[0] / rad, x[1] / rad, x[2] / rad};
+  D1 d2 = mk(0.0);
+  for (int a = 0; a < 3; ++a)
+    for (int b = 0; b < 3; ++b) d2 = d2 + rh[a] * H[3 * a + b] * rh[b];
+  const D1 L[3] = {x[1] * v[2] - x[2] * v[1], x[2] * v[0] - x[0] * v[2], x[0] * v[1] - x[1] * v[0]};
+  const D1 Ln = dsqrt(L[0] * L[0] + L[1] * L[1] + L[2] * L[2]);
+  const D1 om = Ln / (rad * rad);
+  const D1 rt = dcbrt(gms / (om * om - d2));
+  const D1 relv = om * rt;
+  const D1 zh[3] = {L[0] / Ln, L[1] / Ln, L[2] / Ln};
+  const D1 vr = v[0] * rh[0] + v[1] * rh[1] + v[2] * rh[2];
+  const D1 pv[3] = {v[0] - vr * rh[0], v[1] - vr * rh[1], v[2] - vr * rh[2]};
+  const D1 pn = dsqrt(pv[0] * pv[0] + pv[1] * pv[1] + pv[2] * pv[2]);
+  const double kr = kv.k[0] + zr[0] * kv.k[4];
+  const double kvphi = kr * (kv.k[1] + zr[1] * kv.k[5]);
+  const double kz = kv.k[2] + zr[2] * kv.k[6];
+  const double kvz = kv.k[3] + zr[3] * kv.k[7];
+  for (int a = 0; a < 3; ++a) {
+    const D1 ph = pv[a] / pn;
+    const D1 dpos = kr * (rh[a] * rt) + kz * (zh[a] * rt);
+    const D1 dvel = kvphi * (relv * ph) + kvz * (relv * zh[a]);
+    out[a] = x[a] - dpos;
+    out[3 + a] = x[a] + dpos;
+    out[6 + a] = v[a] - dvel;
+    out[9 + a] = v[a] + dvel;
+  }
+}
+
+__global__ __launch_bounds__(128) void release_vjp_kernel(
+    ssc_host_potential hp, const double* __restrict__ prog_w, const int32_t* __restrict__ rel_idx,
+    const double* __restrict__ g_msat, int nrel, int64_t seed, KVals kv,
+    const double* __restrict__ gpc, const double* __restrict__ gpf,
+    const double* __restrict__ gvc, const double* __restrict__ gvf, double* __restrict__ g_prog,
+    double* __restrict__ g_host_rel) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nrel) return;
+  const int nc = hp.n_components;
+  double zr[4];
+  const int64_t ridx = rel_idx ? (int64_t)rel_idx[i] : (int64_t)i;
+  for (int k = 0; k < 4; ++k) zr[k] = prng_normal(ridx * randint1000(seed, k));
+  double gout[12];
+  for (int a = 0; a < 3; ++a) {
+    gout[a] = gpc[3 * (int64_t)i + a];
+    gout[3 + a] = gpf[3 * (int64_t)i + a];
+    gout[6 + a] = gvc[3 * (int64_t)i + a];
+    gout[9 + a] = gvf[3 * (int64_t)i + a];
+  }
+  const int nin = 6 + 2 * nc;
+  for (int dir = 0; dir < nin; ++dir) {
+    D1 in[6], hostp[4 * SSC_MAX_COMPONENTS], out[12];
+    for (int k = 0; k < 6; ++k) in[k] = D1{prog_w[6 * (int64_t)i + k], dir == k ? 1.0 : 0.0};
+    for (int c = 0; c < nc; ++c)
+      for (int k = 0; k < 4; ++k)
+        hostp[4 * c + k] = D1{hp.c[c].p[k], (k < 2 && dir == 6 + 2 * c + k) ? 1.0 : 0.0};
+    release_d(hp, hostp, in, mk(g_msat[i]), zr, kv, out);
+    double acc = 0.0;
+    for (int o = 0; o < 12; ++o) acc += gout[o] * out[o].d;
+    if (dir < 6) g_prog[6 * (int64_t)i + dir] = acc;
+    else g_host_rel[(int64_t)i * 2 * nc + (dir - 6)] = acc;
+  }
+}
+
+__global__ void release_host_reduce(const double* __restrict__ g_host_rel, int nrel, int nc2,
+                                    double* __restrict__ g_host) {
+  const int c = threadIdx.x;
+  if (c >= nc2) return;
+  double a = 0.0;
+  for (int i = 0; i < nrel; ++i) a += g_host_rel[(int64_t)i * nc2 + c];
+  g_host[c] = a;
+}
 }  // namespace ssc
 
 using namespace ssc;
@@ -168,4 +339,28 @@ extern "C" int ssc_release(const ssc_host_potential* host, const double* prog_w,
   hipLaunchKernelGGL(release_kernel, dim3((nrel + 255) / 256), dim3(256), 0, (hipStream_t)stream,
                      *host, prog_w, rel_idx, g_msat, nrel, seed, kv, pos_close, pos_far, vel_close, vel_far);
   return check_launch("ssc_release");
+}
+
+extern "C" int ssc_release_vjp(const ssc_host_potential* host, const double* prog_w,
+                               const int32_t* rel_idx, const double* g_msat, int32_t nrel,
+                               int64_t seed, const double* kvals, const double* g_pos_close,
+                               const double* g_pos_far, const double* g_vel_close,
+                               const double* g_vel_far, double* g_prog, double* g_host,
+                               double* scratch, void* stream) {
+  int rc = validate_host(host);
+  if (rc) return rc;
+  if (nrel <= 0) return fail(SSC_E_ARG, "ssc_release_vjp: nrel must be positive");
+  if (!prog_w || !g_msat || !g_pos_close || !g_pos_far || !g_vel_close || !g_vel_far || !g_prog ||
+      !g_host || !scratch)
+    return fail(SSC_E_ARG, "ssc_release_vjp: NULL array argument");
+  KVals kv;
+  const double def[8] = {2.0, 0.3, 0.0, 0.0, 0.4, 0.4, 0.5, 0.5};
+  for (int k = 0; k < 8; ++k) kv.k[k] = kvals ? kvals[k] : def[k];
+  hipStream_t st = (hipStream_t)stream;
+  hipLaunchKernelGGL(release_vjp_kernel, dim3((nrel + 127) / 128), dim3(128), 0, st, *host, prog_w,
+                     rel_idx, g_msat, nrel, seed, kv, g_pos_close, g_pos_far, g_vel_close,
+                     g_vel_far, g_prog, scratch);
+  hipLaunchKernelGGL(release_host_reduce, dim3(1), dim3(64), 0, st, scratch, nrel,
+                     2 * host->n_components, g_host);
+  return check_launch("ssc_release_vjp");
 }

@@ -22,8 +22,10 @@ struct SourceLDS;
 // No sources.
 template <>
 struct SourceLDS<SSC_SRC_NONE> {
+  static constexpr int n = 0;
   __device__ void init(const ssc_sources&, char*) {}
   __device__ void stage(const ssc_sources&, int, int) {}
+  __device__ void stage_row(const ssc_sources&, int, int) {}
   template <int P>
   __device__ void add_grad(int, double, const double*, const double*, const double*, double*,
                            double*, double*, int, int) const {}
@@ -50,6 +52,7 @@ struct SourceLDS<SSC_SRC_STATIC> {
     __syncthreads();
   }
   __device__ void stage(const ssc_sources&, int, int) {}
+  __device__ void stage_row(const ssc_sources&, int, int) {}
   template <int P>
   __device__ void add_grad(int, double, const double* x, const double* y, const double* z,
                            double* gx, double* gy, double* gz, int j0, int js) const {
@@ -107,6 +110,11 @@ struct SourceLDS<SSC_SRC_TRACK> {
     }
     __syncthreads();
   }
+  // Make row k resident for a walk towards lower rows (block-uniform k; barriers inside).
+  __device__ void stage_row(const ssc_sources& s, int k, int nrows) {
+    if (kb != INT_MIN && k >= kb && k <= kb + chunk) return;
+    stage(s, k - chunk > 0 ? k - chunk : 0, nrows);
+  }
   template <int P>
   __device__ void add_grad(int k, double, const double* x, const double* y, const double* z,
                            double* gx, double* gy, double* gz, int j0, int js) const {
@@ -145,6 +153,7 @@ struct SourceLDS<SSC_SRC_LINE> {
     __syncthreads();
   }
   __device__ void stage(const ssc_sources&, int, int) {}
+  __device__ void stage_row(const ssc_sources&, int, int) {}
   template <int P>
   __device__ void add_grad(int, double t, const double* x, const double* y, const double* z,
                            double* gx, double* gy, double* gz, int j0, int js) const {

@@ -1,0 +1,124 @@
+"""GPU discrete adjoint: kernel parity against the oracle adjoint (checkpoint segments that do
+not divide the step count, ragged starts, forcing, TRACK sources), the release-model VJP
+against the oracle's dual-number Jacobian, and the full perturbed-stream gradient (halo mass
+and radius, subhalo masses, progenitor row) against central finite differences of the GPU
+forward loss."""
+import numpy as np
+import pytest
+import torch
+
+from test_gpu_core import bound_orbits
+
+pytestmark = pytest.mark.gpu
+
+G = 4.498502151469553e-12
+
+
+def _nrel(a, b):
+    a = a.detach().cpu().numpy() if isinstance(a, torch.Tensor) else np.asarray(a)
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
+
+
+@pytest.mark.parametrize("kind", ["track", "none"])
+def test_adjoint_kernel_vs_oracle(oracle, ssc, kind):
+    from streamsculptor_amd import workloads as W
+    from streamsculptor_amd.adjoint import LeapfrogAdjoint
+    from streamsculptor_amd.main import SourceSet
+    from streamsculptor_amd import _lib
+    rng = np.random.default_rng(31)
+    host = W.host_potential()
+    h = oracle.host(W.host_components())
+    n, nsteps, t0, dt = 700, 150, -150.0, 1.0
+    w0 = bound_orbits(rng, n)
+    start = np.sort(rng.integers(0, nsteps + 3, n)).astype(np.int32)
+    src = so = None
+    if kind == "track":
+        sw0, m, rs = W.subhalo_ics(n=16, seed=2)
+        _, tr = oracle.leapfrog(h, sw0, t0, dt, nsteps, traj=True)
+        pos = np.ascontiguousarray(tr[:, :, :3])
+        src = SourceSet(_lib.SRC_TRACK, gm=G * m * 30, rs=rs, pos=pos, grid=(t0, dt))
+        so = oracle.Sources(oracle.SRC_TRACK, gm=G * m * 30, rs=rs, pos=pos)
+    lamT = rng.normal(size=(n, 6))
+    F = rng.normal(scale=1e-3, size=(nsteps + 1, n, 6))
+    for i, s in enumerate(start):  # rows before a particle's start carry no forcing
+        F[:s, i] = 0.0
+    adj = LeapfrogAdjoint(host, w0, t0, dt, nsteps, start=start, checkpoint_every=37, src=src)
+    wT = adj.forward()
+    out = adj.backward(lamT, forcing=F)
+    ref = oracle.adjoint_leapfrog(h, w0, t0, dt, nsteps, lamT, start=start, src=so, forcing=F)
+    fwd = oracle.leapfrog(h, w0, t0, dt, nsteps, start=start, src=so)
+    assert _nrel(wT, fwd) < 1e-12
+    e = {"lam0": _nrel(out["lam0"], ref["lam0"]), "g_host": _nrel(out["g_host"], ref["g_host"])}
+    if kind == "track":
+        e["g_gm"] = _nrel(out["g_gm"], ref["g_gm"])
+        e["g_pos"] = _nrel(out["g_pos"], ref["g_pos"])
+    print(f"adjoint[{kind}] " + " ".join(f"{k} {v:.2e}" for k, v in e.items()))
+    assert all(v < 1e-9 for v in e.values()), e
+
+
+def test_release_vjp_vs_oracle(oracle, ssc):
+    from streamsculptor_amd import workloads as W
+    from streamsculptor_amd.adjoint import release_vjp
+    rng = np.random.default_rng(5)
+    host = W.host_potential()
+    h = oracle.host(W.host_components())
+    n = 500
+    w = bound_orbits(rng, n)
+    J = oracle.release_jacobian(h, w, 1e4, seed=7)          # [n][12][6 + 6]
+    g = rng.normal(size=(n, 12))
+    gp, gh = release_vjp(host, w, np.arange(n, dtype=np.int32), np.full(n, G * 1e4), 7,
+                         g[:, 0:3], g[:, 3:6], g[:, 6:9], g[:, 9:12])
+    ref = np.einsum("io,iok->ik", g, J)
+    assert _nrel(gp, ref[:, :6]) < 1e-10
+    assert _nrel(gh.reshape(-1), ref[:, 6:].sum(0)) < 1e-10
+
+
+def test_stream_gradient_against_finite_differences(ssc):
+    from streamsculptor_amd import workloads as W
+    from streamsculptor_amd.adjoint import StreamModel, stream_loss
+    nsteps = 300
+    ts = -nsteps * 1.0 + np.arange(nsteps + 1)
+    sw0, m, rs = W.subhalo_ics(n=8, seed=2)
+    sw0[:, :3] *= 0.4  # bring the subhalos near the stream so they matter
+    model = StreamModel(ts, sw0, rs, 1e4, per_step=3, seed=0, checkpoint_every=64)
+    prog_w0 = np.array([10.5, 2.0, 1.0, -0.05, 0.17, 0.06])
+    halo0 = np.array([5.4e11, 15.62])
+    m0 = m * 20
+    # target: the same stream from a perturbed halo
+    with torch.no_grad():
+        st = model.forward(halo0[0] * 1.05, halo0[1] * 0.97, m0, prog_w0)
+        target = st["wT"][:, :3].clone()
+    halo = torch.tensor(halo0, dtype=torch.float64, device="cuda", requires_grad=True)
+    masses = torch.tensor(m0, dtype=torch.float64, device="cuda", requires_grad=True)
+    pw = torch.tensor(prog_w0, dtype=torch.float64, device="cuda", requires_grad=True)
+    L = stream_loss(halo, masses, pw, model, target)
+    L.backward()
+
+    def loss_at(hh, mm, ww):
+        with torch.no_grad():
+            s = model.forward(hh[0], hh[1], mm, ww)
+            return float(((s["wT"][:, :3] - target) ** 2).sum().item())
+
+    checks = []
+    for k, rel in ((0, 1e-6), (1, 1e-6)):
+        e = halo0[k] * rel
+        hp, hm = halo0.copy(), halo0.copy()
+        hp[k] += e
+        hm[k] -= e
+        checks.append(("halo%d" % k, halo.grad[k].item(), (loss_at(hp, m0, prog_w0) - loss_at(hm, m0, prog_w0)) / (2 * e)))
+    for j in (0, 3, 6):
+        e = m0[j] * 1e-5
+        mp, mm = m0.copy(), m0.copy()
+        mp[j] += e
+        mm[j] -= e
+        checks.append(("m%d" % j, masses.grad[j].item(), (loss_at(halo0, mp, prog_w0) - loss_at(halo0, mm, prog_w0)) / (2 * e)))
+    for c in range(6):
+        e = 1e-6 if c < 3 else 1e-8
+        wp, wm = prog_w0.copy(), prog_w0.copy()
+        wp[c] += e
+        wm[c] -= e
+        checks.append(("w%d" % c, pw.grad[c].item(), (loss_at(halo0, m0, wp) - loss_at(halo0, m0, wm)) / (2 * e)))
+    for name, a, f in checks:
+        print(f"grad {name}: adjoint {a:.8e} fd {f:.8e} rel {abs(a - f) / max(abs(f), 1e-300):.2e}")
+    for name, a, f in checks:
+        assert abs(a - f) <= 2e-5 * abs(f) + 1e-12, name
