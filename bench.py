@@ -11,7 +11,10 @@ the GPU before the timed region); ``value`` is the surrogate rate when its media
 stream-position error against the plain run is <= 10 pc, else the plain rate.
 
 Other workloads: --workload plain (8000 particles, host only), subhalos (1e5 particles, 64
-live subhalos, direct sums), responses (1e5 particles + 64 first-order response vectors each).
+live subhalos, direct sums), responses (1e5 particles + 64 first-order response vectors each),
+adjoint (the subhalos stream's loss and its gradient with respect to the halo mass and radius,
+the 64 subhalo masses and the progenitor row: forward with checkpoints every 100 steps plus
+the full reverse chain).
 
 Prints ONE JSON line on stdout; progress goes to stderr.
 """
@@ -48,7 +51,7 @@ def build_workload(name, particles=None):
         pot = Potential_Combine([host, clumps], units=usys)
         total, nsrc = 1_000_000, 512
         extra["clumps"] = clumps
-    elif name in ("subhalos", "responses"):
+    elif name in ("subhalos", "responses", "adjoint"):
         sw0, m, rs = W.subhalo_ics()
         live = LiveSubhaloPotential.from_orbits(host, sw0, m, rs, float(ts[0]), W.DT, W.T_STEPS)
         pot = Potential_Combine([host, live], units=usys) if name == "subhalos" else host
@@ -86,6 +89,34 @@ def run_plain(wl, ev=None):
     if ev:
         ev[1].record()
     return w0, start, out
+
+
+def adjoint_setup(wl):
+    import numpy as np
+    from streamsculptor_amd import workloads as W
+    from streamsculptor_amd.adjoint import StreamModel
+    sw0, m, rs = W.subhalo_ics()
+    model = StreamModel(wl["ts"], sw0, rs, W.MSAT, wl["per_step"], seed=0, checkpoint_every=100)
+    prog_w0 = wl["prog_w0"].cpu().numpy()
+    halo = np.array([W.HOST["m_halo"], W.HOST["rs_halo"]])
+    st = model.forward(halo[0] * 1.05, halo[1] * 0.97, m, prog_w0)
+    target = st["wT"][:, :3].clone()
+    return dict(model=model, halo=halo, m=m, prog_w0=prog_w0, target=target)
+
+
+def run_adjoint(wl, ad, ev=None):
+    import torch
+    from streamsculptor_amd.adjoint import stream_loss
+    halo = torch.tensor(ad["halo"], dtype=torch.float64, device="cuda", requires_grad=True)
+    ms = torch.tensor(ad["m"], dtype=torch.float64, device="cuda", requires_grad=True)
+    pw = torch.tensor(ad["prog_w0"], dtype=torch.float64, device="cuda", requires_grad=True)
+    if ev:
+        ev[0].record()
+    L = stream_loss(halo, ms, pw, ad["model"], ad["target"])
+    L.backward()
+    if ev:
+        ev[1].record()
+    return L, halo.grad, ms.grad, pw.grad
 
 
 def run_surrogate(wl, sur, ev=None):
@@ -168,7 +199,7 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="clumps",
-                    choices=["clumps", "subhalos", "responses", "plain"])
+                    choices=["clumps", "subhalos", "responses", "adjoint", "plain"])
     ap.add_argument("--particles", type=int, default=None)
     ap.add_argument("--train-iters", type=int, default=8000)
     ap.add_argument("--no-cpu", action="store_true")
@@ -189,6 +220,30 @@ def main():
     wl = build_workload(args.workload, args.particles)
     ps = wl["psteps"]
     log(f"{wl['total']} particles, {ps} particle-steps per step, {wl['nsrc']} sources")
+
+    if args.workload == "adjoint":
+        ad = adjoint_setup(wl)
+        wall_a, kern_a, (L, gh, gm, gw) = timed(lambda ev: run_adjoint(wl, ad, ev), args.steps,
+                                                args.warmup, "adjoint")
+        fl_a = 5025
+        res = {"metric": "particle-steps/s of loss + gradient (discrete leapfrog adjoint)",
+               "value": ps * args.steps / wall_a, "unit": "particle-steps/s", "n_gpus": 1,
+               "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1e3 * wall_a / args.steps,
+               "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+               "data": "synthetic",
+               "config": {"workload": "adjoint of the subhalos stream", "particles": wl["total"],
+                          "sources": wl["nsrc"], "steps_per_orbit": 4000, "dt_myr": 1.0,
+                          "particle_steps": ps, "checkpoint_every": 100,
+                          "gradients": "halo (m, r_s), 64 subhalo masses, progenitor row"},
+               "roofline": {"bound": "fp64-valu", "achieved": fl_a * ps / kern_a / 1e12,
+                            "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s",
+                            "frac": fl_a * ps / kern_a / PEAK_FP64, "traffic": None,
+                            "kernel": "loss + backward (all kernels)",
+                            "flop_per_particle_step": fl_a},
+               "loss": float(L.item()), "grad_halo": gh.tolist(),
+               "grad_prog_w0": gw.tolist()}
+        print(json.dumps(res), flush=True)
+        return
 
     wall_p, kern_p, (w0, start, out_p) = timed(lambda ev: run_plain(wl, ev), args.steps,
                                                args.warmup, "plain")

@@ -169,40 +169,74 @@ __global__ __launch_bounds__(256) void sum_kernel(const float* __restrict__ part
   if (threadIdx.x == 0) out[0] = red[0];
 }
 
-// out[o] = sum_b D[b*ld + o] for o < ncol (block: 64 columns x 4 row phases)
-__global__ __launch_bounds__(256) void colsum_kernel(const float* __restrict__ D, int64_t ld,
-                                                     int64_t nb, int ncol, float* __restrict__ out) {
+// Column sums over the batch in two deterministic passes: blocks of (64 columns x 4 row
+// phases) over batch chunks of CS_ROWS rows write partials, a second kernel adds the chunks.
+constexpr int CS_ROWS = 512;
+
+__global__ __launch_bounds__(256) void colsum_partial_kernel(const float* __restrict__ D,
+                                                             int64_t ld, int64_t nb, int ncol,
+                                                             float* __restrict__ part) {
   __shared__ float red[4][64];
   const int c = blockIdx.x * 64 + (threadIdx.x & 63), ph = threadIdx.x >> 6;
+  const int64_t b0 = (int64_t)blockIdx.y * CS_ROWS, b1 = min(nb, b0 + CS_ROWS);
   float s = 0.0f;
   if (c < ncol)
-    for (int64_t b = ph; b < nb; b += 4) s += D[b * ld + c];
+    for (int64_t b = b0 + ph; b < b1; b += 4) s += D[b * ld + c];
   red[ph][threadIdx.x & 63] = s;
   __syncthreads();
   if (ph == 0 && c < ncol)
-    out[c] = red[0][threadIdx.x] + red[1][threadIdx.x] + red[2][threadIdx.x] + red[3][threadIdx.x];
+    part[(int64_t)blockIdx.y * ncol + c] =
+        red[0][threadIdx.x] + red[1][threadIdx.x] + red[2][threadIdx.x] + red[3][threadIdx.x];
 }
 
-// dW1[o][i] = sum_b dZ1[b][o] x[b][i]
-__global__ __launch_bounds__(256) void layer1_wgrad_kernel(const float* __restrict__ dZ,
-                                                           const float* __restrict__ x, int64_t nb,
-                                                           float* __restrict__ dW1) {
-  __shared__ float red[4][64][IN];
+// dW1[o][i] = sum_b dZ1[b][o] x[b][i] and db1[o] = sum_b dZ1[b][o], partials per batch chunk:
+// part[chunk][o][4] = (dW1[o][0..2], db1[o]).
+__global__ __launch_bounds__(256) void layer1_grad_partial_kernel(const float* __restrict__ dZ,
+                                                                  const float* __restrict__ x,
+                                                                  int64_t nb,
+                                                                  float* __restrict__ part) {
+  __shared__ float red[4][64][IN + 1];
   const int o = blockIdx.x * 64 + (threadIdx.x & 63), ph = threadIdx.x >> 6;
-  float s[IN] = {0.0f, 0.0f, 0.0f};
-  for (int64_t b = ph; b < nb; b += 4) {
+  const int64_t b0 = (int64_t)blockIdx.y * CS_ROWS, b1 = min(nb, b0 + CS_ROWS);
+  float s[IN + 1] = {0.0f, 0.0f, 0.0f, 0.0f};
+  for (int64_t b = b0 + ph; b < b1; b += 4) {
     const float d = dZ[b * H + o];
 #pragma unroll
     for (int i = 0; i < IN; ++i) s[i] = fmaf(d, x[b * IN + i], s[i]);
+    s[IN] += d;
   }
 #pragma unroll
-  for (int i = 0; i < IN; ++i) red[ph][threadIdx.x & 63][i] = s[i];
+  for (int i = 0; i <= IN; ++i) red[ph][threadIdx.x & 63][i] = s[i];
   __syncthreads();
   if (ph == 0)
 #pragma unroll
-    for (int i = 0; i < IN; ++i)
-      dW1[o * IN + i] = red[0][threadIdx.x][i] + red[1][threadIdx.x][i] + red[2][threadIdx.x][i] +
-                        red[3][threadIdx.x][i];
+    for (int i = 0; i <= IN; ++i)
+      part[((int64_t)blockIdx.y * H + o) * (IN + 1) + i] =
+          red[0][threadIdx.x][i] + red[1][threadIdx.x][i] + red[2][threadIdx.x][i] +
+          red[3][threadIdx.x][i];
+}
+
+// out[e] = sum_z part[z][e] (e < ne); with stride_out = IN + 1 split for the layer-1 case
+__global__ void chunk_sum_kernel(const float* __restrict__ part, int nz, int ne,
+                                 float* __restrict__ out) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= ne) return;
+  float s = 0.0f;
+  for (int z = 0; z < nz; ++z) s += part[(int64_t)z * ne + e];
+  out[e] = s;
+}
+
+__global__ void layer1_grad_finish(const float* __restrict__ part, int nz, float* __restrict__ dW1,
+                                   float* __restrict__ db1) {
+  const int o = blockIdx.x * blockDim.x + threadIdx.x;
+  if (o >= H) return;
+  float s[IN + 1] = {0.0f, 0.0f, 0.0f, 0.0f};
+  for (int z = 0; z < nz; ++z)
+#pragma unroll
+    for (int i = 0; i <= IN; ++i) s[i] += part[((int64_t)z * H + o) * (IN + 1) + i];
+#pragma unroll
+  for (int i = 0; i < IN; ++i) dW1[o * IN + i] = s[i];
+  db1[o] = s[IN];
 }
 
 __global__ void adam_kernel(float* __restrict__ p, const float* __restrict__ g,
@@ -287,7 +321,7 @@ int64_t ws_bytes(int64_t nb) {
   b += 3 * nb * H * 4;            // Z1..Z3
   b += nb * H * 4;                // dA / dZ (fp32)
   b += 2 * nb * OUT * 4;          // dY, (spare)
-  b += nz * H * H * 4;            // split-K partials
+  b += (nz * H * H + ((nb + CS_ROWS - 1) / CS_ROWS) * H * (IN + 1)) * 4;  // partials
   b += ((nb + 255) / 256 + 64) * 4;  // loss partials
   b += 3 * nb * H * 2;            // A1..A3
   b += nb * H * 2;                // dZ bf16
@@ -310,7 +344,7 @@ Workspace carve(void* base, int64_t nb) {
   w.Z3 = (float*)take(nb * H * 4);
   w.dA = (float*)take(nb * H * 4);
   w.dY = (float*)take(2 * nb * OUT * 4);
-  w.part = (float*)take(nz * H * H * 4);
+  w.part = (float*)take((nz * H * H + ((nb + CS_ROWS - 1) / CS_ROWS) * H * (IN + 1)) * 4);
   w.red = (float*)take(((nb + 255) / 256 + 64) * 4);
   w.A1 = (uint16_t*)take(nb * H * 2);
   w.A2 = (uint16_t*)take(nb * H * 2);
@@ -324,6 +358,14 @@ Workspace carve(void* base, int64_t nb) {
 }
 
 inline unsigned nblk(int64_t n, int b = 256) { return (unsigned)((n + b - 1) / b); }
+
+void colsum(const float* D, int64_t ld, int64_t nb, int ncol, float* out, float* scratch,
+            hipStream_t st) {
+  const int nz = (int)((nb + CS_ROWS - 1) / CS_ROWS);
+  hipLaunchKernelGGL(colsum_partial_kernel, dim3((ncol + 63) / 64, nz), dim3(256), 0, st, D, ld, nb,
+                     ncol, scratch);
+  hipLaunchKernelGGL(chunk_sum_kernel, dim3(nblk(ncol)), dim3(256), 0, st, scratch, nz, ncol, out);
+}
 
 void gemm(const uint16_t* A, int64_t sAm, int64_t sAk, const uint16_t* B, int64_t sBk,
           int64_t sBn, float* C, int M, int N, int K, const float* bias, hipStream_t st,
@@ -363,7 +405,7 @@ void backward(const float* x, int64_t nb, float* g, Workspace& w, hipStream_t st
   const int nz = (int)splitk_for(nb);
   // output layer
   gemm(w.dYb, 1, OUT, w.A3, H, 1, g + OW4, OUT, H, (int)nb, nullptr, st, w.part, nz);
-  hipLaunchKernelGGL(colsum_kernel, dim3(1), dim3(256), 0, st, w.dY, (int64_t)OUT, nb, OUT, g + OB4);
+  colsum(w.dY, OUT, nb, OUT, g + OB4, w.part, st);
   gemm(w.dYb, OUT, 1, w.W4b, H, 1, w.dA, (int)nb, H, OUT, nullptr, st);
   // hidden layers 3 and 2
   const float* Zs[2] = {w.Z3, w.Z2};
@@ -373,14 +415,16 @@ void backward(const float* x, int64_t nb, float* g, Workspace& w, hipStream_t st
   for (int s = 0; s < 2; ++s) {
     hipLaunchKernelGGL(act_bwd_kernel, dim3(nblk(nb * H)), dim3(256), 0, st, w.dA, Zs[s], w.dZb, nb * H);
     gemm(w.dZb, 1, H, Ain[s], H, 1, g + oW[s], H, H, (int)nb, nullptr, st, w.part, nz);
-    hipLaunchKernelGGL(colsum_kernel, dim3(H / 64), dim3(256), 0, st, w.dA, (int64_t)H, nb, H, g + oB[s]);
+    colsum(w.dA, H, nb, H, g + oB[s], w.part, st);
     gemm(w.dZb, H, 1, Wb[s], H, 1, w.dA, (int)nb, H, H, nullptr, st);
   }
   // first layer (fp32)
   hipLaunchKernelGGL(act_bwd_kernel, dim3(nblk(nb * H)), dim3(256), 0, st, w.dA, w.Z1,
                      (uint16_t*)nullptr, nb * H);
-  hipLaunchKernelGGL(layer1_wgrad_kernel, dim3(H / 64), dim3(256), 0, st, w.dA, x, nb, g + OW1);
-  hipLaunchKernelGGL(colsum_kernel, dim3(H / 64), dim3(256), 0, st, w.dA, (int64_t)H, nb, H, g + OB1);
+  const int nz1 = (int)((nb + CS_ROWS - 1) / CS_ROWS);
+  hipLaunchKernelGGL(layer1_grad_partial_kernel, dim3(H / 64, nz1), dim3(256), 0, st, w.dA, x, nb,
+                     w.part);
+  hipLaunchKernelGGL(layer1_grad_finish, dim3(1), dim3(H), 0, st, w.part, nz1, g + OW1, g + OB1);
 }
 
 }  // namespace

@@ -110,9 +110,7 @@ def perturb_leapfrog(base, src, w0, dw0, start, t0, dt, nsteps, with_radius):
     out = torch.empty_like(w0)
     dout = torch.empty(n, m, dd, dtype=torch.float64, device=w0.device)
     if src.kind == _lib.SRC_TRACK:
-        g0, gdt = src.grid
-        if abs(g0 - t0) > 1e-9 * max(1.0, abs(t0)) or abs(gdt - dt) > 1e-12 * max(1.0, abs(dt)):
-            raise ValueError("the subhalo track must be sampled on the integration grid")
+        src = src.on_grid(t0, dt, nsteps)
     sd = src.desc()
     _lib.call("ssc_perturb_leapfrog", _lib.ctypes.byref(h), _lib.ctypes.byref(sd), _lib.ptr(w0),
               _lib.ptr(d0), _lib.ptr(st), n, float(t0), float(dt), int(nsteps), int(with_radius),
@@ -135,17 +133,12 @@ def _integrate_perturbation(pertgen, w0, tsn, t0v, t1v, solver, dt0):
         d = d[..., :6].contiguous()
     dt, nsteps, kidx = leapfrog_grid(tsn, t0v, t1v, dt0)
     if src.kind == _lib.SRC_TRACK:
-        g0, gdt = src.grid
-        off = (t0v - g0) / gdt
-        if abs(off - round(off)) > 1e-9 or abs(gdt - dt) > 1e-12 * abs(dt):
-            raise ValueError("the subhalo track must be sampled on the integration grid")
-        src = _shift_track(src, int(round(off)), nsteps)
+        src = src.on_grid(t0v, dt, nsteps)
     ys_w, ys_d = [], []
     cur_w, cur_d, kprev = w, d, 0
     for k in kidx:
         if k > kprev:
-            seg = _shift_track(src, int(kprev), int(k - kprev)) if src.kind == _lib.SRC_TRACK else src
-            cur_w, cur_d = perturb_leapfrog(base, seg, cur_w, cur_d, None, t0v + kprev * dt, dt,
+            cur_w, cur_d = perturb_leapfrog(base, src, cur_w, cur_d, None, t0v + kprev * dt, dt,
                                             int(k - kprev), with_radius)
             kprev = int(k)
         ys_w.append(cur_w[0])
@@ -153,12 +146,3 @@ def _integrate_perturbation(pertgen, w0, tsn, t0v, t1v, solver, dt0):
         full[:, :cur_d.shape[-1]] = cur_d[0]
         ys_d.append(full if dd == 12 else full[:, :dd])
     return Solution(_lib.f64(tsn), [torch.stack(ys_w), torch.stack(ys_d)])
-
-
-def _shift_track(src, k0, nsteps):
-    """A TRACK source set whose row 0 is row k0 of ``src``."""
-    from .main import SourceSet
-    t = src.tensors()
-    g0, gdt = src.grid
-    return SourceSet(src.kind, gm=t["gm"], rs=t["rs"], pos=t["pos"][k0:k0 + nsteps + 1],
-                     grid=(g0 + k0 * gdt, gdt))

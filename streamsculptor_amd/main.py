@@ -98,6 +98,27 @@ class SourceSet:
             setattr(s, k, None if t[k] is None else t[k].data_ptr())
         return s
 
+    def on_grid(self, t0, dt, nsteps):
+        """This TRACK set re-indexed so that row k is the position at t0 + k*dt, k = 0..nsteps
+        (the integration step must be +- the track spacing and t0 a track row)."""
+        g0, gdt = self.grid
+        if abs(g0 - t0) <= 1e-9 * max(1.0, abs(t0)) and abs(gdt - dt) <= 1e-12 * max(1.0, abs(dt)):
+            return self
+        ratio = dt / gdt
+        r0f = (t0 - g0) / gdt
+        r0 = int(round(r0f))
+        sgn = int(round(ratio))
+        rows = self.tensors()["pos"].shape[0]
+        if (abs(abs(ratio) - 1.0) > 1e-12 or abs(r0f - r0) > 1e-9 * max(1.0, abs(r0f))
+                or not (0 <= r0 < rows) or not (0 <= r0 + sgn * nsteps < rows)):
+            raise ValueError("a source track must cover the integration grid with the same "
+                             f"spacing (track t0={g0}, dt={gdt}; integration t0={t0}, dt={dt}, "
+                             f"nsteps={nsteps})")
+        t = self.tensors()
+        idx = torch.arange(nsteps + 1, device=t["pos"].device) * sgn + r0
+        return SourceSet(self.kind, gm=t["gm"], rs=t["rs"], pos=t["pos"][idx].contiguous(),
+                         grid=(float(t0), float(dt)))
+
     def step_of(self, t):
         """Grid row of time t for TRACK sources."""
         if self.kind != _lib.SRC_TRACK:
@@ -262,10 +283,7 @@ class Potential:
         nsave = 0 if si is None else si.numel()
         ys = torch.empty(nsave, n, 6, dtype=torch.float64, device=dev) if nsave else None
         if src is not None and src.kind == _lib.SRC_TRACK:
-            g0, gdt = src.grid
-            if abs(g0 - t0) > 1e-9 * max(1.0, abs(t0)) or abs(gdt - dt) > 1e-12 * max(1.0, abs(dt)):
-                raise ValueError("a source track must be sampled on the integration grid "
-                                 f"(track t0={g0}, dt={gdt}; integration t0={t0}, dt={dt})")
+            src = src.on_grid(t0, dt, nsteps)
         sd = src.desc() if src is not None else None
         _lib.call("ssc_leapfrog", _lib.ctypes.byref(h),
                   None if sd is None else _lib.ctypes.byref(sd), _lib.ptr(w0), _lib.ptr(st), n,
