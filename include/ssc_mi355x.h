@@ -116,6 +116,34 @@ int ssc_leapfrog(const ssc_host_potential* host, const ssc_sources* src,
                  double t0, double dt, int32_t nsteps, double* w_out,
                  const int32_t* save_idx, int32_t nsave, double* ys, void* stream);
 
+/* Step-size control of ssc_dopri5 (reference PIDController(rtol, atol, dtmin, dtmax,
+ * force_dtmin=True, jump_ts), streamsculptor/main.py:146, fields.py:54). */
+typedef struct ssc_dopri5_options {
+  double rtol;
+  double atol;
+  double dtmin;         /* <= 0: no floor; steps clamped up to the floor are always accepted */
+  double dtmax;         /* <= 0: no ceiling */
+  double dt0;           /* <= 0: starting step from the error-based heuristic */
+  int32_t max_steps;    /* accepted + rejected steps per orbit before it fails */
+  int32_t n_stops;
+  const double* stops;  /* [n_stops] device: times at which a step must end (jump_ts) */
+} ssc_dopri5_options;
+
+/* Adaptive Dormand-Prince 5(4) for n independent orbits in host + sources (NONE, STATIC or
+ * LINE; TRACK is rejected).  Orbit i runs from t_start[i] (device, NULL = t0 for all) to t1,
+ * forwards or backwards.
+ *   w_out[n][6]      final rows (NULL allowed when ys is given)
+ *   ts[nts]          save times (device), monotone in the direction of integration; rows of
+ *                    ys[n][nts][6] outside an orbit's interval are NaN
+ *   stats[n][3]      NULL or {accepted, rejected, status}; status 1 = max_steps ran out, and
+ *                    then every output row of that orbit is +inf (the reference's throw=False)
+ * Replaces integrate_orbit / integrate_orbit_batch_vmapped / gen_stream_vmapped with
+ * solver=Dopri5 (reference streamsculptor/main.py:125-202,343-368). */
+int ssc_dopri5(const ssc_host_potential* host, const ssc_sources* src, const double* w0,
+               const double* t_start, int64_t n, double t0, double t1, const double* ts,
+               int32_t nts, const ssc_dopri5_options* opt, double* w_out, double* ys,
+               int32_t* stats, void* stream);
+
 /* Spray-stream release conditions (reference Potential.release_model + gen_stream_ics,
  * streamsculptor/main.py:209-306).  For each row i < nrel, from the progenitor row
  * prog_w[i][6] with satellite mass G*Msat = g_msat[i] and release index rel_idx[i]

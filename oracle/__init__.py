@@ -20,6 +20,13 @@ What pins each function (independently of this file) -- see tests/test_oracle_*.
   (JAX bit compatibility of the draws themselves is parity-unpinned: jax is not installable).
 * Release model: closed-form geometry on hand-built cases; its Jacobian against central
   finite differences of the release model itself.
+* Dopri5: the tableau and dense-output weights against scipy's RK45 arrays; one step and its
+  dense output against scipy's own RK step on the same right-hand side; converged orbits
+  against scipy DOP853 and the reference notebooks' printed orbits (tests.ipynb:302,1717,1747;
+  StreamSubhaloExample.ipynb:35).  The step-size controller (diffrax 0.7.0 PIDController,
+  not in the reference tree) is restated from its documentation: its step sequence is
+  parity-unpinned; its invariants (dtmin floor, max_steps failure, tolerance convergence)
+  are tested.
 """
 from __future__ import annotations
 
@@ -103,6 +110,10 @@ def _bind(L):
                                         ctypes.c_int]
     L.orc_perturb_leapfrog.argtypes = [vp, vp, vp, vp, vp, vp, i64, d, d, i32, ctypes.c_int,
                                        vp, vp, ctypes.c_int]
+    L.orc_dopri5.argtypes = [vp, vp, vp, vp, i64, d, d, vp, i32, d, d, d, d, d, i32, vp, i32, vp,
+                             vp, vp, ctypes.c_int]
+    L.orc_dp5_tableau.argtypes = [vp]
+    L.orc_dp5_step.argtypes = [vp, vp, d, d, vp, vp, vp, vp, i32, vp]
     return L
 
 
@@ -217,6 +228,77 @@ def leapfrog(h, w0, t0, dt, nsteps, start=None, src=None, traj=False, nthreads=0
     lib(fast).orc_leapfrog(ctypes.addressof(h), sp, w0.ctypes.data, _p(st), n, t0, dt, nsteps,
                        out.ctypes.data, _p(tr), nthreads)
     return (out, tr) if traj else out
+
+
+def dopri5(h, w0, t1, ts=(), t0=None, t_start=None, src=None, rtol=1e-7, atol=1e-7, dtmin=0.3,
+           dtmax=None, max_steps=10_000, stops=(), dt0=None, nthreads=0, fast=False):
+    """Adaptive Dormand-Prince 5(4) with diffrax's PID step control (restates integrate_orbit
+    with solver=Dopri5, main.py:125-163).  Returns (w_final (n, 6), ys (n, len(ts), 6),
+    stats (n, 3) = accepted, rejected, status)."""
+    w0 = np.ascontiguousarray(np.atleast_2d(w0), dtype=np.float64)
+    n = w0.shape[0]
+    ts = np.ascontiguousarray(np.atleast_1d(np.asarray(ts, dtype=np.float64)))
+    nts = ts.size
+    tsa = None if t_start is None else np.ascontiguousarray(
+        np.broadcast_to(np.asarray(t_start, dtype=np.float64), (n,)))
+    t0v = float(t0) if t0 is not None else (float(ts.min()) if nts else 0.0)
+    st = np.ascontiguousarray(np.asarray(stops, dtype=np.float64).reshape(-1))
+    out = np.zeros((n, 6))
+    ys = np.zeros((n, max(nts, 1), 6))
+    stats = np.zeros((n, 3), dtype=np.int32)
+    src, sp = _src_ptr(src)
+    lib(fast).orc_dopri5(ctypes.addressof(h), sp, w0.ctypes.data, _p(tsa), n, t0v, float(t1),
+                         ts.ctypes.data if nts else None, nts, rtol, atol,
+                         -1.0 if dtmin is None else float(dtmin),
+                         -1.0 if dtmax is None else float(dtmax),
+                         -1.0 if dt0 is None else float(dt0), int(max_steps),
+                         st.ctypes.data if st.size else None, st.size, out.ctypes.data,
+                         ys.ctypes.data, stats.ctypes.data, nthreads)
+    return out, ys[:, :nts], stats
+
+
+def stream_dopri5(h, prog_w0, ts, msat, seed, src=None, kvals=None, release_host=None,
+                  nthreads=0, fast=False, **tol):
+    """Spray stream with adaptive Dopri5 (restates gen_stream_vmapped with solver=Dopri5,
+    main.py:343-368): progenitor saved at every ts, release at every ts, particle i < T
+    integrated from ts[i] to ts[-1].  Returns (lead, trail, stats)."""
+    ts = np.asarray(ts, dtype=np.float64)
+    T = len(ts) - 1
+    _, prog, _ = dopri5(h, prog_w0, float(ts.max()), t0=float(ts.min()), ts=ts, src=src,
+                        nthreads=nthreads, fast=fast, **tol)
+    pc, pf, vc, vf = release(release_host or h, prog[0], ts, msat, seed, kvals)
+    w0 = np.empty((2 * T, 6))
+    w0[0::2, :3], w0[0::2, 3:] = pc[:T], vc[:T]
+    w0[1::2, :3], w0[1::2, 3:] = pf[:T], vf[:T]
+    out, _, stats = dopri5(h, w0, float(ts[-1]), t_start=np.repeat(ts[:T], 2), src=src,
+                           nthreads=nthreads, fast=fast, **tol)
+    return out[0::2], out[1::2], stats
+
+
+def dp5_tableau():
+    """(A (7, 7) lower triangle, c (7,), b (7,), b - bhat (7,), midpoint weights (7,))."""
+    t = np.zeros(15 + 28)
+    lib().orc_dp5_tableau(t.ctypes.data)
+    A = np.zeros((7, 7))
+    k = 0
+    for i in range(1, 6):
+        for j in range(i):
+            A[i, j] = t[k]
+            k += 1
+    return A, t[15:22], t[22:29], t[29:36], t[36:43]
+
+
+def dp5_step(h_pot, t, h, y, s=(), src=None):
+    """One Dopri5 step: (y1, error estimate, dense output at fractions s)."""
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    s = np.ascontiguousarray(np.atleast_1d(np.asarray(s, dtype=np.float64)))
+    yn, err = np.zeros(6), np.zeros(6)
+    dense = np.zeros((max(s.size, 1), 6))
+    src, sp = _src_ptr(src)
+    lib().orc_dp5_step(ctypes.addressof(h_pot), sp, float(t), float(h), y.ctypes.data,
+                       yn.ctypes.data, err.ctypes.data, s.ctypes.data if s.size else None,
+                       s.size, dense.ctypes.data)
+    return yn, err, dense[:s.size]
 
 
 def release(h, prog_w, ts, msat, seed, kvals=None):

@@ -8,6 +8,7 @@
 //
 // Build: oracle/Makefile (g++ -O2 -fopenmp, no -ffast-math, no FMA contraction).
 
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -413,6 +414,208 @@ void release_normals(int64_t seed, int64_t i, double z[4]) {
   for (int k = 0; k < 4; ++k) z[k] = prng_normal(i * ri[k]);
 }
 
+// ---------------------------------------------------------------------------------------
+// Adaptive Dormand-Prince 5(4) as the reference runs it through diffrax: diffeqsolve with
+// Dopri5 and PIDController(rtol, atol, dtmin, dtmax, force_dtmin=True, jump_ts), dt0=None,
+// SaveAt(ts=ts) (reference streamsculptor/main.py:139-162, fields.py:48-98).  diffrax 0.7.0
+// is not part of the reference tree; the step control restated here is its documented
+// integral controller (pcoeff = dcoeff = 0, icoeff = 1; safety 0.9, factor in [0.2, 10],
+// accepted steps never shrink, error order 5, RMS norm), Hairer's starting-step heuristic and
+// its end-of-interval clipping.
+//   Tableau: Dormand & Prince, J. Comput. Appl. Math. 6 (1980) 19-26.
+//   Dense output: the quartic through y0, y1, h f0, h f1 and Shampine's (1986) midpoint value.
+// Time runs in tau = dir * t so that the interval is always increasing.
+// ---------------------------------------------------------------------------------------
+namespace dp5 {
+const double A21 = 1.0 / 5;
+const double A31 = 3.0 / 40, A32 = 9.0 / 40;
+const double A41 = 44.0 / 45, A42 = -56.0 / 15, A43 = 32.0 / 9;
+const double A51 = 19372.0 / 6561, A52 = -25360.0 / 2187, A53 = 64448.0 / 6561, A54 = -212.0 / 729;
+const double A61 = 9017.0 / 3168, A62 = -355.0 / 33, A63 = 46732.0 / 5247, A64 = 49.0 / 176,
+             A65 = -5103.0 / 18656;
+const double C2 = 1.0 / 5, C3 = 3.0 / 10, C4 = 4.0 / 5, C5 = 8.0 / 9;
+const double B1 = 35.0 / 384, B3 = 500.0 / 1113, B4 = 125.0 / 192, B5 = -2187.0 / 6784,
+             B6 = 11.0 / 84;
+// b - bhat (bhat: the embedded 4th-order weights)
+const double E1 = 71.0 / 57600, E3 = -71.0 / 16695, E4 = 71.0 / 1920, E5 = -17253.0 / 339200,
+             E6 = 22.0 / 525, E7 = -1.0 / 40;
+// Shampine's midpoint weights: y(t + h/2) = y0 + h sum_i M_i k_i
+const double M1 = 6025192743.0 / 30085553152.0 / 2, M3 = 51252292925.0 / 65400821598.0 / 2,
+             M4 = -2691868925.0 / 45128329728.0 / 2, M5 = 187940372067.0 / 1594534317056.0 / 2,
+             M6 = -1776094331.0 / 19743644256.0 / 2, M7 = 11237099.0 / 235043384.0 / 2;
+}  // namespace dp5
+
+struct Dp5Opts {
+  double rtol, atol, dtmin, dtmax;  // dtmin/dtmax <= 0: none
+  double dt0;                       // <= 0: starting-step heuristic
+  int32_t max_steps;
+  const double* stops;  // tau-space, ascending
+  int32_t n_stops;
+};
+
+void dp5_rhs(const ssc_host_potential* hp, const ssc_sources* src, double dir, double tau,
+             const double y[6], double k[6]) {
+  double a[3];
+  accel(hp, src, y, 0, dir * tau, a);
+  for (int c = 0; c < 3; ++c) { k[c] = dir * y[3 + c]; k[3 + c] = dir * a[c]; }
+}
+
+double rms6(const double v[6]) {
+  double s = 0;
+  for (int c = 0; c < 6; ++c) s += v[c] * v[c];
+  return std::sqrt(s / 6.0);
+}
+
+// End the step at the first stop strictly after tau0 if the step would pass it.
+double clip_stops(const Dp5Opts& o, double tau0, double tau1) {
+  for (int j = 0; j < o.n_stops; ++j)
+    if (o.stops[j] > tau0) return tau1 > o.stops[j] ? o.stops[j] : tau1;
+  return tau1;
+}
+
+// One Dopri5 step of size h from (tau, y) with f0 = f(tau, y): the 5th-order solution yn, the
+// embedded error estimate err = h sum (b - bhat) k, f(tau + h, yn) (first stage of the next
+// step) and the dense-output quartic y(s) = y + s (F0 + s (c2 + s (b3 + s a4))), s in [0, 1].
+struct Dp5Step {
+  double yn[6], err[6], k7[6], F0[6], a4[6], b3[6], c2[6];
+};
+
+void dp5_step(const ssc_host_potential* hp, const ssc_sources* src, double dir, double tau,
+              double h, const double y[6], const double f0[6], Dp5Step& r) {
+  using namespace dp5;
+  double k1[6], k2[6], k3[6], k4[6], k5[6], k6[6], ys_[6];
+  for (int c = 0; c < 6; ++c) k1[c] = f0[c];
+  for (int c = 0; c < 6; ++c) ys_[c] = y[c] + h * (A21 * k1[c]);
+  dp5_rhs(hp, src, dir, tau + C2 * h, ys_, k2);
+  for (int c = 0; c < 6; ++c) ys_[c] = y[c] + h * (A31 * k1[c] + A32 * k2[c]);
+  dp5_rhs(hp, src, dir, tau + C3 * h, ys_, k3);
+  for (int c = 0; c < 6; ++c) ys_[c] = y[c] + h * (A41 * k1[c] + A42 * k2[c] + A43 * k3[c]);
+  dp5_rhs(hp, src, dir, tau + C4 * h, ys_, k4);
+  for (int c = 0; c < 6; ++c)
+    ys_[c] = y[c] + h * (A51 * k1[c] + A52 * k2[c] + A53 * k3[c] + A54 * k4[c]);
+  dp5_rhs(hp, src, dir, tau + C5 * h, ys_, k5);
+  for (int c = 0; c < 6; ++c)
+    ys_[c] = y[c] + h * (A61 * k1[c] + A62 * k2[c] + A63 * k3[c] + A64 * k4[c] + A65 * k5[c]);
+  dp5_rhs(hp, src, dir, tau + h, ys_, k6);
+  for (int c = 0; c < 6; ++c)
+    r.yn[c] = y[c] + h * (B1 * k1[c] + B3 * k3[c] + B4 * k4[c] + B5 * k5[c] + B6 * k6[c]);
+  dp5_rhs(hp, src, dir, tau + h, r.yn, r.k7);
+  const double* k7 = r.k7;
+  for (int c = 0; c < 6; ++c) {
+    r.err[c] = h * (E1 * k1[c] + E3 * k3[c] + E4 * k4[c] + E5 * k5[c] + E6 * k6[c] + E7 * k7[c]);
+    const double ym =
+        y[c] + h * (M1 * k1[c] + M3 * k3[c] + M4 * k4[c] + M5 * k5[c] + M6 * k6[c] + M7 * k7[c]);
+    // quartic with p(0) = y, p'(0) = F0, p(1) = yn, p'(1) = h k7, p(1/2) = ym
+    r.F0[c] = h * k1[c];
+    const double D1 = r.yn[c] - y[c] - r.F0[c], D2 = h * k7[c] - r.F0[c],
+                 D3 = ym - y[c] - 0.5 * r.F0[c];
+    r.a4[c] = 2.0 * D2 - 8.0 * D1 + 16.0 * D3;
+    r.b3[c] = D2 - 2.0 * D1 - 2.0 * r.a4[c];
+    r.c2[c] = D1 - r.a4[c] - r.b3[c];
+  }
+}
+
+void dp5_dense(const Dp5Step& r, const double y[6], double s, double out[6]) {
+  for (int c = 0; c < 6; ++c)
+    out[c] = y[c] + s * (r.F0[c] + s * (r.c2[c] + s * (r.b3[c] + s * r.a4[c])));
+}
+
+// One orbit from tau0 to tau1.  Saves: tau_s[j] (ascending) in [tau0, tau1] are written to
+// ys[j]; returns 0 on success, 1 when max_steps ran out (then every output is +inf).
+int dp5_orbit(const ssc_host_potential* hp, const ssc_sources* src, const Dp5Opts& o, double dir,
+              double tau0, double tau1, const double w0[6], const double* tau_s, int32_t nts,
+              double* ys, double w_out[6], int32_t stats[3]) {
+  using namespace dp5;
+  double y[6], f0[6];
+  for (int c = 0; c < 6; ++c) y[c] = w0[c];
+  int js = 0;
+  while (js < nts && tau_s[js] < tau0) {
+    for (int c = 0; c < 6; ++c) ys[6 * js + c] = NAN;  // before this orbit's start
+    ++js;
+  }
+  while (js < nts && tau_s[js] == tau0) {
+    for (int c = 0; c < 6; ++c) ys[6 * js + c] = y[c];
+    ++js;
+  }
+  int32_t nacc = 0, nrej = 0, nsteps = 0;
+  double tprev = tau0;
+  if (tau0 < tau1) {
+    dp5_rhs(hp, src, dir, tau0, y, f0);
+    // starting step (Hairer, Norsett & Wanner, Solving ODEs I, sec. II.4)
+    double sc[6], u[6];
+    for (int c = 0; c < 6; ++c) sc[c] = o.atol + std::fabs(y[c]) * o.rtol;
+    for (int c = 0; c < 6; ++c) u[c] = y[c] / sc[c];
+    const double d0 = rms6(u);
+    for (int c = 0; c < 6; ++c) u[c] = f0[c] / sc[c];
+    const double d1 = rms6(u);
+    const double h0 = (d0 < 1e-5 || d1 < 1e-5) ? 1e-6 : 0.01 * (d0 / d1);
+    double y1[6], f1[6];
+    for (int c = 0; c < 6; ++c) y1[c] = y[c] + h0 * f0[c];
+    dp5_rhs(hp, src, dir, tau0 + h0, y1, f1);
+    for (int c = 0; c < 6; ++c) u[c] = (f1[c] - f0[c]) / sc[c];
+    const double d2 = rms6(u) / h0;
+    const double md = d1 > d2 ? d1 : d2;
+    const double h1 = md <= 1e-15 ? std::max(1e-6, h0 * 1e-3) : std::pow(0.01 / md, 1.0 / 5.0);
+    double dt0 = o.dt0 > 0 ? o.dt0 : std::min(100.0 * h0, h1);
+    double tnext = clip_stops(o, tau0, tau0 + dt0);
+    if (tnext > tau1) tnext = tau1;
+    bool at_dtmin = false;
+    while (tprev < tau1 && nsteps < o.max_steps) {
+      const double h = tnext - tprev;
+      Dp5Step r;
+      dp5_step(hp, src, dir, tprev, h, y, f0, r);
+      for (int c = 0; c < 6; ++c)
+        u[c] = r.err[c] / (o.atol + o.rtol * std::max(std::fabs(y[c]), std::fabs(r.yn[c])));
+      const double e = rms6(u);
+      const bool keep = (e < 1.0) || at_dtmin;
+      const double fmin = keep ? 1.0 : 0.2;
+      double factor = 0.9 * std::pow(1.0 / e, 1.0 / 5.0);
+      factor = std::min(std::max(factor, fmin), 10.0);
+      double dtn = h * factor;
+      if (o.dtmax > 0 && dtn > o.dtmax) dtn = o.dtmax;
+      if (o.dtmin > 0) {
+        at_dtmin = dtn <= o.dtmin;
+        if (dtn < o.dtmin) dtn = o.dtmin;
+      }
+      double t_new = tprev;
+      if (keep) {
+        // dense output on (tprev, tnext]
+        while (js < nts && tau_s[js] <= tnext) {
+          if (tau_s[js] == tnext) {
+            for (int c = 0; c < 6; ++c) ys[6 * js + c] = r.yn[c];
+          } else {
+            dp5_dense(r, y, (tau_s[js] - tprev) / h, ys + 6 * js);
+          }
+          ++js;
+        }
+        for (int c = 0; c < 6; ++c) { y[c] = r.yn[c]; f0[c] = r.k7[c]; }
+        t_new = tnext;
+        ++nacc;
+      } else {
+        ++nrej;
+      }
+      double tn1 = clip_stops(o, t_new, t_new + dtn);
+      tprev = t_new < tau1 ? t_new : tau1;
+      if (tn1 > tau1 - 1e-10) tn1 = keep ? tau1 : tprev + 0.5 * (tau1 - tprev);
+      tnext = tn1;
+      ++nsteps;
+    }
+  }
+  stats[0] = nacc;
+  stats[1] = nrej;
+  stats[2] = tprev < tau1 ? 1 : 0;
+  if (stats[2]) {
+    for (int j = 0; j < nts; ++j)
+      for (int c = 0; c < 6; ++c) ys[6 * j + c] = INFINITY;
+    for (int c = 0; c < 6; ++c) w_out[c] = INFINITY;
+    return 1;
+  }
+  for (; js < nts; ++js)
+    for (int c = 0; c < 6; ++c) ys[6 * js + c] = NAN;  // after t1
+  for (int c = 0; c < 6; ++c) w_out[c] = y[c];
+  return 0;
+}
+
 }  // namespace
 
 // =========================================================================================
@@ -713,6 +916,66 @@ int orc_adjoint_leapfrog(const ssc_host_potential* hp, const ssc_sources* src, c
     }
   }
   return 0;
+}
+
+
+// Adaptive Dopri5 for n orbits (reference integrate_orbit / gen_stream_vmapped with
+// solver=Dopri5, main.py:125-163,343-368).  Orbit i runs from t_start[i] (t0 when t_start is
+// NULL) to t1 and is saved at ts[0..nts) (monotone in the direction of integration; entries
+// outside the orbit's interval are NaN).  stops[n_stops]: times where a step must end.
+// stats[i] = {accepted, rejected, status}; status 1 = max_steps reached (outputs +inf).
+int orc_dopri5(const ssc_host_potential* hp, const ssc_sources* src, const double* w0,
+               const double* t_start, int64_t n, double t0, double t1, const double* ts,
+               int32_t nts, double rtol, double atol, double dtmin, double dtmax, double dt0,
+               int32_t max_steps, const double* stops, int32_t n_stops, double* w_out,
+               double* ys, int32_t* stats, int nthreads) {
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+  int bad = 0;
+#pragma omp parallel
+  {
+    std::vector<double> tau_s(nts > 0 ? nts : 1), tstop(n_stops > 0 ? n_stops : 1);
+    std::vector<double> ysl(6 * (nts > 0 ? nts : 1));
+#pragma omp for schedule(dynamic, 8) reduction(+ : bad)
+    for (int64_t i = 0; i < n; ++i) {
+      const double ta = t_start ? t_start[i] : t0;
+      const double dir = t1 >= ta ? 1.0 : -1.0;
+      for (int j = 0; j < nts; ++j) tau_s[j] = dir * ts[j];
+      for (int j = 0; j < n_stops; ++j) tstop[j] = dir * stops[j];
+      std::sort(tstop.begin(), tstop.begin() + n_stops);
+      Dp5Opts o{rtol, atol, dtmin, dtmax, dt0, max_steps, tstop.data(), n_stops};
+      double wo[6];
+      int32_t st[3];
+      bad += dp5_orbit(hp, src, o, dir, dir * ta, dir * t1, w0 + 6 * i, tau_s.data(), nts,
+                       ysl.data(), wo, st);
+      if (ys) for (int k = 0; k < 6 * nts; ++k) ys[6 * nts * i + k] = ysl[k];
+      if (w_out) for (int c = 0; c < 6; ++c) w_out[6 * i + c] = wo[c];
+      if (stats) for (int c = 0; c < 3; ++c) stats[3 * i + c] = st[c];
+    }
+  }
+  return bad;
+}
+
+// Test hooks: the Dopri5 coefficients (A lower triangle row-major, c, b, b - bhat, midpoint
+// weights; 15 + 4 * 7 doubles) and one step with dense output at s[0..ns).
+void orc_dp5_tableau(double* out) {
+  using namespace dp5;
+  const double t[] = {A21, A31, A32, A41, A42, A43, A51, A52, A53, A54, A61, A62, A63, A64, A65,
+                      0.0, C2, C3, C4, C5, 1.0, 1.0,
+                      B1, 0.0, B3, B4, B5, B6, 0.0,
+                      E1, 0.0, E3, E4, E5, E6, E7,
+                      M1, 0.0, M3, M4, M5, M6, M7};
+  for (size_t i = 0; i < sizeof(t) / sizeof(t[0]); ++i) out[i] = t[i];
+}
+
+void orc_dp5_step(const ssc_host_potential* hp, const ssc_sources* src, double t, double h,
+                  const double* y, double* yn, double* err, const double* s, int32_t ns,
+                  double* dense) {
+  double f0[6];
+  dp5_rhs(hp, src, 1.0, t, y, f0);
+  Dp5Step r;
+  dp5_step(hp, src, 1.0, t, h, y, f0, r);
+  for (int c = 0; c < 6; ++c) { yn[c] = r.yn[c]; err[c] = r.err[c]; }
+  for (int j = 0; j < ns; ++j) dp5_dense(r, y, s[j], dense + 6 * j);
 }
 
 }  // extern "C"

@@ -10,5 +10,7 @@ from .main import Potential, Solution, SourceSet, UnitSystem, usys, dimensionles
 from . import solvers  # noqa: F401
 from .solvers import Leapfrog, Dopri5, Dopri8  # noqa: F401
 from . import potential  # noqa: F401
+from .streamhelpers import (gen_stream_ics_pert, gen_stream_vmapped_with_pert,  # noqa: F401
+                            gen_stream_vmapped_with_pert_fixed_prog)
 
 __version__ = "0.1.0"

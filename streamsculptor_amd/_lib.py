@@ -38,12 +38,20 @@ class Sources(ctypes.Structure):
                 ("vel", ctypes.c_void_p), ("t0", ctypes.c_void_p), ("t_window", ctypes.c_void_p)]
 
 
+class Dopri5Options(ctypes.Structure):
+    _fields_ = [("rtol", ctypes.c_double), ("atol", ctypes.c_double), ("dtmin", ctypes.c_double),
+                ("dtmax", ctypes.c_double), ("dt0", ctypes.c_double),
+                ("max_steps", ctypes.c_int32), ("n_stops", ctypes.c_int32),
+                ("stops", ctypes.c_void_p)]
+
+
 _vp, _i64, _i32, _d = ctypes.c_void_p, ctypes.c_int64, ctypes.c_int32, ctypes.c_double
 
 # name -> argtypes; every entry point returns int (0 = success)
 SIGNATURES = {
     "ssc_eval": [_vp, _vp, _vp, _i64, _d, _i32, _vp, _vp, _vp, _vp],
     "ssc_leapfrog": [_vp, _vp, _vp, _vp, _i64, _d, _d, _i32, _vp, _vp, _i32, _vp, _vp],
+    "ssc_dopri5": [_vp, _vp, _vp, _vp, _i64, _d, _d, _vp, _i32, _vp, _vp, _vp, _vp, _vp],
     "ssc_perturb_leapfrog": [_vp, _vp, _vp, _vp, _vp, _i64, _d, _d, _i32, _i32, _vp, _vp],
     "ssc_mlp_forward": [_vp, _vp, _i64, _vp, _vp, _i64],
     "ssc_mlp_loss_backward": [_vp, _vp, _vp, _vp, ctypes.c_float, _i64, _vp, _vp, _vp, _vp, _i64],

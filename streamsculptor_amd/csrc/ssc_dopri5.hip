@@ -1,0 +1,336 @@
+// Adaptive Dormand-Prince 5(4) orbit kernel: one orbit per lane, FP64 state and the seven
+// stage derivatives in VGPRs, Plummer source tables (STATIC or LINE) in LDS.  Every lane runs
+// its own step-size controller, so lanes of one wave may sit at different times; the wave
+// retires when its last lane reaches t1 or runs out of steps.
+//
+// Replaces integrate_orbit / gen_stream_vmapped with solver=Dopri5 (reference
+// streamsculptor/main.py:125-163,343-368): diffrax's diffeqsolve with
+// PIDController(rtol, atol, dtmin, dtmax, force_dtmin=True, jump_ts) and SaveAt(ts).  Step
+// control: integral controller on the RMS norm of err / (atol + rtol max(|y0|, |y1|)), factor
+// 0.9 (1/e)^(1/5) clipped to [1, 10] on accepted and [0.2, 10] on rejected steps; a step whose
+// size was clamped up to dtmin is accepted whatever its error; the starting step follows
+// Hairer, Norsett & Wanner (Solving ODEs I, sec. II.4); steps ending within 1e-10 of t1 are
+// stretched to t1 (or, when rejected, halved towards it).  Saved values between step ends come
+// from the quartic dense output through y0, y1, h f0, h f1 and Shampine's midpoint value.
+#include <hip/hip_runtime.h>
+
+#include <climits>
+#include <string>
+
+#include "ssc_device.h"
+#include "ssc_sources_lds.h"
+
+namespace ssc {
+namespace {
+
+constexpr int DP_BLOCK = 256;
+
+// Dormand & Prince (1980) tableau.
+constexpr double A21 = 1.0 / 5;
+constexpr double A31 = 3.0 / 40, A32 = 9.0 / 40;
+constexpr double A41 = 44.0 / 45, A42 = -56.0 / 15, A43 = 32.0 / 9;
+constexpr double A51 = 19372.0 / 6561, A52 = -25360.0 / 2187, A53 = 64448.0 / 6561,
+                 A54 = -212.0 / 729;
+constexpr double A61 = 9017.0 / 3168, A62 = -355.0 / 33, A63 = 46732.0 / 5247, A64 = 49.0 / 176,
+                 A65 = -5103.0 / 18656;
+constexpr double C2 = 1.0 / 5, C3 = 3.0 / 10, C4 = 4.0 / 5, C5 = 8.0 / 9;
+constexpr double B1 = 35.0 / 384, B3 = 500.0 / 1113, B4 = 125.0 / 192, B5 = -2187.0 / 6784,
+                 B6 = 11.0 / 84;
+// b - bhat
+constexpr double E1 = 71.0 / 57600, E3 = -71.0 / 16695, E4 = 71.0 / 1920,
+                 E5 = -17253.0 / 339200, E6 = 22.0 / 525, E7 = -1.0 / 40;
+// Shampine's midpoint weights (y at s = 1/2)
+constexpr double M1 = 6025192743.0 / 30085553152.0 / 2, M3 = 51252292925.0 / 65400821598.0 / 2,
+                 M4 = -2691868925.0 / 45128329728.0 / 2,
+                 M5 = 187940372067.0 / 1594534317056.0 / 2,
+                 M6 = -1776094331.0 / 19743644256.0 / 2, M7 = 11237099.0 / 235043384.0 / 2;
+
+struct Dp5Ctl {
+  double rtol, atol, dtmin, dtmax, dt0;  // dtmin / dtmax / dt0 <= 0: none / none / heuristic
+  int max_steps;
+  int n_stops;
+};
+
+// dy/dtau at tau (t = dir * tau) for the orbit ODE [v, -grad Phi].
+template <int SK>
+__device__ __forceinline__ void rhs(const ssc_host_potential& hp, const SourceLDS<SK>& lds,
+                                    double dir, double tau, const double* y, double* k) {
+  double gx, gy, gz;
+  host_grad(hp, y[0], y[1], y[2], gx, gy, gz);
+  lds.template add_grad<1>(0, dir * tau, &y[0], &y[1], &y[2], &gx, &gy, &gz, 0, 1);
+  k[0] = dir * y[3];
+  k[1] = dir * y[4];
+  k[2] = dir * y[5];
+  k[3] = -dir * gx;
+  k[4] = -dir * gy;
+  k[5] = -dir * gz;
+}
+
+__device__ __forceinline__ double rms6(const double* u) {
+  double s = 0.0;
+#pragma unroll
+  for (int c = 0; c < 6; ++c) s = fma(u[c], u[c], s);
+  return sqrt(s * (1.0 / 6.0));
+}
+
+// End a step at the first stop strictly after tau0 when the step would pass it.
+__device__ __forceinline__ double clip_stops(const double* __restrict__ stops, int n_stops,
+                                             double dir, double tau0, double tau1) {
+  double best = tau1;
+  for (int j = 0; j < n_stops; ++j) {
+    const double s = dir * stops[j];
+    if (s > tau0 && s < best) best = s;
+  }
+  return best;
+}
+
+template <int SK>
+__global__ __launch_bounds__(DP_BLOCK) void dopri5_kernel(
+    ssc_host_potential hp, ssc_sources src, const double* __restrict__ w0,
+    const double* __restrict__ t_start, int64_t n, double t0, double t1,
+    const double* __restrict__ ts, int nts, const double* __restrict__ stops, Dp5Ctl ctl,
+    double* __restrict__ w_out, double* __restrict__ ys, int32_t* __restrict__ stats) {
+  extern __shared__ __align__(16) char lds_raw[];
+  SourceLDS<SK> lds;
+  lds.init(src, lds_raw);  // the only barrier: every lane reaches it
+  const int64_t i = (int64_t)blockIdx.x * DP_BLOCK + threadIdx.x;
+  if (i >= n) return;
+
+  const double ta = t_start ? t_start[i] : t0;
+  const double dir = t1 >= ta ? 1.0 : -1.0;
+  const double tau0 = dir * ta, tau1 = dir * t1;
+  double y[6], f0[6];
+#pragma unroll
+  for (int c = 0; c < 6; ++c) y[c] = w0[6 * i + c];
+  double* __restrict__ yrow = ys ? ys + (int64_t)6 * nts * i : nullptr;
+  int js = 0;
+  for (; js < nts && dir * ts[js] < tau0; ++js)
+#pragma unroll
+    for (int c = 0; c < 6; ++c) yrow[6 * js + c] = __builtin_nan("");
+  for (; js < nts && dir * ts[js] == tau0; ++js)
+#pragma unroll
+    for (int c = 0; c < 6; ++c) yrow[6 * js + c] = y[c];
+
+  int nacc = 0, nrej = 0, nsteps = 0;
+  double tprev = tau0;
+  if (tau0 < tau1) {
+    rhs<SK>(hp, lds, dir, tau0, y, f0);
+    double tnext;
+    {
+      double sc[6], u[6];
+#pragma unroll
+      for (int c = 0; c < 6; ++c) sc[c] = ctl.atol + fabs(y[c]) * ctl.rtol;
+#pragma unroll
+      for (int c = 0; c < 6; ++c) u[c] = y[c] / sc[c];
+      const double d0 = rms6(u);
+#pragma unroll
+      for (int c = 0; c < 6; ++c) u[c] = f0[c] / sc[c];
+      const double d1 = rms6(u);
+      double dt0 = ctl.dt0;
+      if (!(dt0 > 0.0)) {
+        const double h0 = (d0 < 1e-5 || d1 < 1e-5) ? 1e-6 : 0.01 * (d0 / d1);
+        double y1[6], f1[6];
+#pragma unroll
+        for (int c = 0; c < 6; ++c) y1[c] = y[c] + h0 * f0[c];
+        rhs<SK>(hp, lds, dir, tau0 + h0, y1, f1);
+#pragma unroll
+        for (int c = 0; c < 6; ++c) u[c] = (f1[c] - f0[c]) / sc[c];
+        const double d2 = rms6(u) / h0;
+        const double md = fmax(d1, d2);
+        const double h1 = md <= 1e-15 ? fmax(1e-6, h0 * 1e-3) : pow(0.01 / md, 0.2);
+        dt0 = fmin(100.0 * h0, h1);
+      }
+      tnext = clip_stops(stops, ctl.n_stops, dir, tau0, tau0 + dt0);
+      if (tnext > tau1) tnext = tau1;
+    }
+    bool at_dtmin = false;
+    while (tprev < tau1 && nsteps < ctl.max_steps) {
+      const double h = tnext - tprev;
+      double k2[6], k3[6], k4[6], k5[6], k6[6], k7[6], yt[6], yn[6];
+#pragma unroll
+      for (int c = 0; c < 6; ++c) yt[c] = y[c] + h * (A21 * f0[c]);
+      rhs<SK>(hp, lds, dir, tprev + C2 * h, yt, k2);
+#pragma unroll
+      for (int c = 0; c < 6; ++c) yt[c] = y[c] + h * (A31 * f0[c] + A32 * k2[c]);
+      rhs<SK>(hp, lds, dir, tprev + C3 * h, yt, k3);
+#pragma unroll
+      for (int c = 0; c < 6; ++c) yt[c] = y[c] + h * (A41 * f0[c] + A42 * k2[c] + A43 * k3[c]);
+      rhs<SK>(hp, lds, dir, tprev + C4 * h, yt, k4);
+#pragma unroll
+      for (int c = 0; c < 6; ++c)
+        yt[c] = y[c] + h * (A51 * f0[c] + A52 * k2[c] + A53 * k3[c] + A54 * k4[c]);
+      rhs<SK>(hp, lds, dir, tprev + C5 * h, yt, k5);
+#pragma unroll
+      for (int c = 0; c < 6; ++c)
+        yt[c] = y[c] + h * (A61 * f0[c] + A62 * k2[c] + A63 * k3[c] + A64 * k4[c] + A65 * k5[c]);
+      rhs<SK>(hp, lds, dir, tnext, yt, k6);
+#pragma unroll
+      for (int c = 0; c < 6; ++c)
+        yn[c] = y[c] + h * (B1 * f0[c] + B3 * k3[c] + B4 * k4[c] + B5 * k5[c] + B6 * k6[c]);
+      rhs<SK>(hp, lds, dir, tnext, yn, k7);
+      double u[6];
+#pragma unroll
+      for (int c = 0; c < 6; ++c) {
+        const double err = h * (E1 * f0[c] + E3 * k3[c] + E4 * k4[c] + E5 * k5[c] + E6 * k6[c] +
+                                E7 * k7[c]);
+        u[c] = err / (ctl.atol + ctl.rtol * fmax(fabs(y[c]), fabs(yn[c])));
+      }
+      const double e = rms6(u);
+      const bool keep = (e < 1.0) || at_dtmin;
+      double factor = 0.9 * pow(1.0 / e, 0.2);
+      factor = fmin(fmax(factor, keep ? 1.0 : 0.2), 10.0);
+      double dtn = h * factor;
+      if (ctl.dtmax > 0.0 && dtn > ctl.dtmax) dtn = ctl.dtmax;
+      if (ctl.dtmin > 0.0) {
+        at_dtmin = dtn <= ctl.dtmin;
+        if (dtn < ctl.dtmin) dtn = ctl.dtmin;
+      }
+      double t_new = tprev;
+      if (keep) {
+        if (yrow && js < nts && dir * ts[js] <= tnext) {
+          // dense output on (tprev, tnext]
+          double F0[6], a4[6], b3[6], c2[6];
+#pragma unroll
+          for (int c = 0; c < 6; ++c) {
+            const double ym = y[c] + h * (M1 * f0[c] + M3 * k3[c] + M4 * k4[c] + M5 * k5[c] +
+                                          M6 * k6[c] + M7 * k7[c]);
+            F0[c] = h * f0[c];
+            const double D1 = yn[c] - y[c] - F0[c], D2 = h * k7[c] - F0[c],
+                         D3 = ym - y[c] - 0.5 * F0[c];
+            a4[c] = 2.0 * D2 - 8.0 * D1 + 16.0 * D3;
+            b3[c] = D2 - 2.0 * D1 - 2.0 * a4[c];
+            c2[c] = D1 - a4[c] - b3[c];
+          }
+          for (; js < nts && dir * ts[js] <= tnext; ++js) {
+            const double tj = dir * ts[js];
+            if (tj == tnext) {
+#pragma unroll
+              for (int c = 0; c < 6; ++c) yrow[6 * js + c] = yn[c];
+            } else {
+              const double s = (tj - tprev) / h;
+#pragma unroll
+              for (int c = 0; c < 6; ++c)
+                yrow[6 * js + c] = y[c] + s * (F0[c] + s * (c2[c] + s * (b3[c] + s * a4[c])));
+            }
+          }
+        }
+#pragma unroll
+        for (int c = 0; c < 6; ++c) {
+          y[c] = yn[c];
+          f0[c] = k7[c];
+        }
+        t_new = tnext;
+        ++nacc;
+      } else {
+        ++nrej;
+      }
+      double tn1 = clip_stops(stops, ctl.n_stops, dir, t_new, t_new + dtn);
+      tprev = t_new < tau1 ? t_new : tau1;
+      if (tn1 > tau1 - 1e-10) tn1 = keep ? tau1 : tprev + 0.5 * (tau1 - tprev);
+      tnext = tn1;
+      ++nsteps;
+    }
+  }
+  const bool failed = tprev < tau1;
+  if (stats) {
+    stats[3 * i] = nacc;
+    stats[3 * i + 1] = nrej;
+    stats[3 * i + 2] = failed ? 1 : 0;
+  }
+  if (failed) {
+    const double inf = __builtin_inf();
+    if (yrow)
+      for (int j = 0; j < nts; ++j)
+#pragma unroll
+        for (int c = 0; c < 6; ++c) yrow[6 * j + c] = inf;
+    if (w_out)
+#pragma unroll
+      for (int c = 0; c < 6; ++c) w_out[6 * i + c] = inf;
+    return;
+  }
+  if (yrow)
+    for (; js < nts; ++js)
+#pragma unroll
+      for (int c = 0; c < 6; ++c) yrow[6 * js + c] = __builtin_nan("");
+  if (w_out)
+#pragma unroll
+    for (int c = 0; c < 6; ++c) w_out[6 * i + c] = y[c];
+}
+
+int set_lds_dp(const void* fn, size_t lds) {
+  if (lds > 160 * 1024) return fail(SSC_E_LIMIT, "ssc_dopri5: source table exceeds LDS");
+  if (lds > 60 * 1024) {
+    hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess)
+      return fail(SSC_E_HIP, std::string("hipFuncSetAttribute: ") + hipGetErrorString(e));
+  }
+  return SSC_OK;
+}
+
+template <int SK>
+int launch_dopri5(const ssc_host_potential& hp, const ssc_sources& src, const double* w0,
+                  const double* t_start, int64_t n, double t0, double t1, const double* ts,
+                  int nts, const double* stops, const Dp5Ctl& ctl, double* w_out, double* ys,
+                  int32_t* stats, hipStream_t st) {
+  const size_t lds = SourceLDS<SK>::bytes(src);
+  int rc = set_lds_dp((const void*)dopri5_kernel<SK>, lds);
+  if (rc) return rc;
+  const int64_t blocks = (n + DP_BLOCK - 1) / DP_BLOCK;
+  hipLaunchKernelGGL(dopri5_kernel<SK>, dim3((unsigned)blocks), dim3(DP_BLOCK), lds, st, hp, src,
+                     w0, t_start, n, t0, t1, ts, nts, stops, ctl, w_out, ys, stats);
+  return check_launch("ssc_dopri5");
+}
+
+}  // namespace
+}  // namespace ssc
+
+using namespace ssc;
+
+extern "C" int ssc_dopri5(const ssc_host_potential* host, const ssc_sources* src,
+                          const double* w0, const double* t_start, int64_t n, double t0,
+                          double t1, const double* ts, int32_t nts,
+                          const ssc_dopri5_options* opt, double* w_out, double* ys,
+                          int32_t* stats, void* stream) {
+  int rc = validate_host(host);
+  if (rc) return rc;
+  if (!opt) return fail(SSC_E_ARG, "ssc_dopri5: options are NULL");
+  if (n < 0 || nts < 0 || opt->n_stops < 0) return fail(SSC_E_ARG, "ssc_dopri5: negative size");
+  if (n == 0) return SSC_OK;
+  if (!w0) return fail(SSC_E_ARG, "ssc_dopri5: w0 is NULL");
+  if (!w_out && !(ys && nts > 0)) return fail(SSC_E_ARG, "ssc_dopri5: no output (w_out, ys)");
+  if (nts > 0 && (!ts || !ys)) return fail(SSC_E_ARG, "ssc_dopri5: nts > 0 needs ts and ys");
+  if (opt->n_stops > 0 && !opt->stops) return fail(SSC_E_ARG, "ssc_dopri5: n_stops > 0 needs stops");
+  if (!(opt->rtol >= 0.0) || !(opt->atol >= 0.0) || !(opt->rtol + opt->atol > 0.0))
+    return fail(SSC_E_ARG, "ssc_dopri5: rtol/atol must be >= 0 and not both 0");
+  if (opt->max_steps < 1) return fail(SSC_E_ARG, "ssc_dopri5: max_steps < 1");
+  if ((n + DP_BLOCK - 1) / DP_BLOCK > (int64_t)INT_MAX)
+    return fail(SSC_E_LIMIT, "ssc_dopri5: too many orbits");
+  rc = validate_sources(src, 0);
+  if (rc) return rc;
+  ssc_sources s;
+  if (src) {
+    s = *src;
+  } else {
+    s = ssc_sources{};
+  }
+  if (s.n == 0) s.kind = SSC_SRC_NONE;
+  if (s.kind == SSC_SRC_TRACK)
+    return fail(SSC_E_ARG, "ssc_dopri5: TRACK sources are sampled on a fixed grid; adaptive steps "
+                           "need STATIC or LINE sources");
+  if (nts == 0) ys = nullptr;
+  Dp5Ctl ctl{opt->rtol, opt->atol, opt->dtmin, opt->dtmax, opt->dt0, opt->max_steps, opt->n_stops};
+  hipStream_t st = (hipStream_t)stream;
+  switch (s.kind) {
+    case SSC_SRC_NONE:
+      return launch_dopri5<SSC_SRC_NONE>(*host, s, w0, t_start, n, t0, t1, ts, nts, opt->stops,
+                                         ctl, w_out, ys, stats, st);
+    case SSC_SRC_STATIC:
+      return launch_dopri5<SSC_SRC_STATIC>(*host, s, w0, t_start, n, t0, t1, ts, nts, opt->stops,
+                                           ctl, w_out, ys, stats, st);
+    case SSC_SRC_LINE:
+      return launch_dopri5<SSC_SRC_LINE>(*host, s, w0, t_start, n, t0, t1, ts, nts, opt->stops,
+                                         ctl, w_out, ys, stats, st);
+    default:
+      return fail(SSC_E_ARG, "ssc_dopri5: bad source kind");
+  }
+}

@@ -94,7 +94,11 @@ def _pert_parts(pertgen):
     if pert._kernel_components():
         raise NotImplementedError("the perturbing potential must consist of Plummer sources")
     src = pert._kernel_sources()
-    radius = getattr(pertgen, "potential_structural", None) is not None
+    struct = getattr(pertgen, "potential_structural", None)
+    radius = struct is not None
+    if radius and hasattr(struct, "matches") and not struct.matches(pert):
+        raise ValueError("potential_structural must describe the same subhalos as "
+                         "potential_perturbation (the radius block is d/da of those subhalos)")
     return base, src, radius
 
 

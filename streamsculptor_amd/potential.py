@@ -127,6 +127,29 @@ class SubhaloLinePotential(Potential):
         return _per_source(self, xyz, t, "phi")
 
 
+class SubhaloLinePotential_dRadius(Potential):
+    """d Phi / d a of the straight-line Plummer subhalos of a SubhaloLinePotential (reference
+    streamsculptor/potential.py:852-904): the structural potential whose gradient drives the
+    radius block of the first-order responses.  The response kernel forms
+    -grad(d Phi_j / d a_j) = 3 G m_j a_j (x - s_j) / (|x - s_j|^2 + a_j^2)^(5/2) in closed form,
+    so this class carries the same subhalo description and is consumed by the perturbation
+    field; evaluating it on its own has no kernel form."""
+
+    def __init__(self, m, a, subhalo_x0, subhalo_v, subhalo_t0, t_window, units=None):
+        super().__init__(units, {"m": m, "a": a, "subhalo_x0": subhalo_x0, "subhalo_v": subhalo_v,
+                                 "subhalo_t0": subhalo_t0, "t_window": t_window})
+
+    def matches(self, pert):
+        """True when this describes the same subhalos as ``pert``."""
+        names = ("m", "a", "subhalo_x0", "subhalo_v", "subhalo_t0", "t_window")
+        try:
+            return all(np.allclose(np.asarray(_arr(getattr(self, k)), dtype=np.float64),
+                                   np.asarray(_arr(getattr(pert, k)), dtype=np.float64),
+                                   rtol=0, atol=0) for k in names)
+        except (AttributeError, ValueError):
+            return False
+
+
 class PlummerClumps(Potential):
     """Static Plummer clumps summed directly (the clumpy-halo substructure of the surrogate
     workload).  Equivalent to a SubhaloLinePotential with zero velocity and an infinite
@@ -216,4 +239,5 @@ class _SourceOnly(Potential):
 
 __all__ = ["MiyamotoNagaiDisk", "NFWPotential", "PlummerPotential", "HernquistPotential",
            "Potential_Combine", "GalaMilkyWayPotential", "CustomPotential",
-           "SubhaloLinePotential", "PlummerClumps", "LiveSubhaloPotential", "usys"]
+           "SubhaloLinePotential", "SubhaloLinePotential_dRadius", "PlummerClumps",
+           "LiveSubhaloPotential", "usys"]

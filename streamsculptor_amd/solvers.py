@@ -3,7 +3,7 @@ reference passes, e.g. ``diffrax.Dopri5(scan_kind='bounded')`` at reference
 streamsculptor/main.py:288,344).
 
 * ``Leapfrog()`` / ``"leapfrog"``: fixed-step kick-drift-kick leapfrog; needs ``dt0``.
-* ``Dopri5()`` / ``"dopri5"``: adaptive Dormand-Prince 5(4) with a PI step controller.
+* ``Dopri5()`` / ``"dopri5"``: adaptive Dormand-Prince 5(4) with an integral step-size controller.
 * ``Dopri8()``: the reference's default for ``integrate_orbit``; not built on this path, so it
   raises ``NotImplementedError`` instead of silently running another method.
 

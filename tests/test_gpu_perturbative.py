@@ -1,0 +1,130 @@
+"""GPU tests of the first-order perturbation pipeline (reference perturbative.py:24-135,
+243-296; streamhelpers.py:57-146) with the fixed-step leapfrog:
+
+* release Jacobian (twelve VJPs of the release kernel) == the oracle's dual-number Jacobian;
+* the whole pipeline (backward progenitor responses -> release Jacobian -> forward particle
+  responses) == central differences of the nonlinear perturbed stream in the subhalo masses
+  (the reference's own test method, examples/tests.ipynb:2959,3182) for the mass block, and
+  in the subhalo radius for the radius block;
+* the perturbed-stream helpers run the three potentials the reference assigns.
+"""
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+G = 4.498502151469553e-12
+DT = 1.0
+
+
+def _setup(ssc, scale=1.0, da=0.0):
+    from streamsculptor_amd import workloads as W
+    from streamsculptor_amd.potential import (SubhaloLinePotential,
+                                              SubhaloLinePotential_dRadius)
+    base = W.host_potential()
+    ts = np.arange(-600.0, 0.0 + DT / 2, 4 * DT)  # 151 release times on the dt grid
+    m = np.array([3e7, 1e7]) * scale
+    a = np.array([0.4, 0.25]) + da
+    x0 = np.array([[8.0, 4.0, 2.0], [-3.0, 9.0, 1.0]])
+    v = np.array([[0.05, -0.1, 0.02], [0.1, 0.03, -0.04]])
+    t0 = np.array([-300.0, -150.0])
+    tw = np.array([200.0, 400.0])
+    pert = SubhaloLinePotential(m, a, x0, v, t0, tw, units=ssc.usys)
+    struct = SubhaloLinePotential_dRadius(m, a, x0, v, t0, tw, units=ssc.usys)
+    return base, pert, struct, ts
+
+
+def test_release_jacobian_matches_oracle(oracle, ssc):
+    from streamsculptor_amd import workloads as W
+    from streamsculptor_amd.perturbative import BaseStreamModel
+    base, _, _, ts = _setup(ssc)
+    bs = BaseStreamModel(base, W.PROG_TODAY, ts, 1e4, 7, solver="leapfrog", dt0=DT)
+    J = bs.dRel_dIC.cpu().numpy()
+    prog = bs.prog_loc_fwd.cpu().numpy()
+    jac = oracle.release_jacobian(oracle.host(W.host_components()), prog, 1e4, 7)  # (n, 12, nin)
+    ref = np.empty_like(J)
+    ref[:, 0, :3], ref[:, 0, 3:] = jac[:, 0:3, :6], jac[:, 6:9, :6]
+    ref[:, 1, :3], ref[:, 1, 3:] = jac[:, 3:6, :6], jac[:, 9:12, :6]
+    assert np.abs(J - ref).max() <= 1e-9 * np.abs(ref).max()
+
+
+def _stream(ssc, base, pert, ts, prog_today):
+    """Nonlinear perturbed stream with the progenitor held at prog_today at ts[-1]."""
+    from streamsculptor_amd.potential import Potential_Combine
+    from streamsculptor_amd.streamhelpers import gen_stream_vmapped_with_pert
+    total = Potential_Combine([base, pert], units=ssc.usys)
+    ic = total.integrate_orbit(w0=prog_today, ts=[ts[-1], ts[0]], t0=ts[-1], t1=ts[0],
+                               solver="leapfrog", dt0=DT).ys[-1]
+    lead, trail = gen_stream_vmapped_with_pert(base, pert, ts, ic, 1e4, 7, solver="leapfrog",
+                                               dt0=DT)
+    return torch.cat([lead, trail]).cpu().numpy()
+
+
+def test_pipeline_matches_nonlinear_differences(ssc):
+    from streamsculptor_amd import workloads as W
+    from streamsculptor_amd.perturbative import BaseStreamModel, GenerateMassRadiusPerturbation
+    base, pert, struct, ts = _setup(ssc)
+    bs = BaseStreamModel(base, W.PROG_TODAY, ts, 1e4, 7, solver="leapfrog", dt0=DT)
+    print("[perturbative] base model ready", flush=True)
+    gen = GenerateMassRadiusPerturbation(base, pert, struct, BaseStreamModel=bs, dt0=DT)
+    (lw, ld), (tw_, td) = gen.compute_perturbation_OTF()
+    T = len(ts) - 1
+    assert lw.shape == (T, 6) and ld.shape == (T, 2, 12)
+    resp = torch.cat([ld, td]).cpu().numpy()  # (2T, Nsh, 12)
+    # the zeroth order equals the unperturbed stream
+    w0 = torch.cat([lw, tw_]).cpu().numpy()
+    lead0, trail0 = base.gen_stream_vmapped(ts=ts, prog_w0=W.PROG_TODAY, Msat=1e4, seed_num=7,
+                                            solver="leapfrog", dt0=DT)
+    np.testing.assert_allclose(w0, torch.cat([lead0, trail0]).cpu().numpy(), rtol=0, atol=1e-10)
+    # mass block: d(stream)/d(eps) of Phi_base + eps Phi_pert at eps = 0, i.e. a central
+    # difference in a common scale of both subhalo masses around 0
+    prog_today = bs.prog_loc_fwd[-1]
+    eps = 1e-3
+    plus = _stream(ssc, base, _setup(ssc, scale=eps)[1], ts, prog_today)
+    minus = _stream(ssc, base, _setup(ssc, scale=-eps)[1], ts, prog_today)
+    fd = (plus - minus) / (2 * eps)
+    lin = resp[:, :, :6].sum(axis=1)
+    scale = np.abs(lin).max(axis=0)
+    err = np.abs(fd - lin).max(axis=0) / scale
+    print(f"[perturbative] mass block vs nonlinear difference: {err}", flush=True)
+    assert err.max() < 1e-4
+    # radius block: d/da of the mass response (both radii shifted together)
+    da = 1e-4
+    gp = GenerateMassRadiusPerturbation(base, *(_setup(ssc, da=da)[1:3]), BaseStreamModel=bs,
+                                        dt0=DT)
+    gm_ = GenerateMassRadiusPerturbation(base, *(_setup(ssc, da=-da)[1:3]), BaseStreamModel=bs,
+                                         dt0=DT)
+    (_, lp), (_, tp) = gp.compute_perturbation_OTF()
+    (_, lm), (_, tm) = gm_.compute_perturbation_OTF()
+    rp = torch.cat([lp, tp]).cpu().numpy()
+    rm = torch.cat([lm, tm]).cpu().numpy()
+    fd_r = (rp[:, :, :6] - rm[:, :, :6]) / (2 * da)
+    lin_r = resp[:, :, 6:]
+    err_r = np.abs(fd_r - lin_r).max() / np.abs(lin_r).max()
+    print(f"[perturbative] radius block vs difference in a: {err_r:.2e}", flush=True)
+    assert err_r < 1e-4
+
+
+def test_stream_helpers_assign_potentials(ssc):
+    """gen_stream_vmapped_with_pert_fixed_prog keeps the progenitor in the base potential;
+    with a subhalo that is never active the perturbed helpers reduce to the base stream."""
+    from streamsculptor_amd import workloads as W
+    from streamsculptor_amd.potential import SubhaloLinePotential
+    from streamsculptor_amd.streamhelpers import (gen_stream_ics_pert,
+                                                  gen_stream_vmapped_with_pert,
+                                                  gen_stream_vmapped_with_pert_fixed_prog)
+    base, _, _, ts = _setup(ssc)
+    idle = SubhaloLinePotential(np.array([1e8]), np.array([0.5]), np.array([[5.0, 5.0, 5.0]]),
+                                np.array([[0.0, 0.0, 0.0]]), np.array([1e5]), np.array([1.0]),
+                                units=ssc.usys)
+    kw = dict(solver="leapfrog", dt0=DT)
+    l0, t0_ = base.gen_stream_vmapped(ts=ts, prog_w0=W.PROG_TODAY, Msat=1e4, seed_num=3, **kw)
+    for fn in (gen_stream_vmapped_with_pert, gen_stream_vmapped_with_pert_fixed_prog):
+        l1, t1 = fn(base, idle, ts, W.PROG_TODAY, 1e4, 3, **kw)
+        np.testing.assert_array_equal(l1.cpu().numpy(), l0.cpu().numpy())
+        np.testing.assert_array_equal(t1.cpu().numpy(), t0_.cpu().numpy())
+    ics = gen_stream_ics_pert(base, idle, ts, W.PROG_TODAY, 1e4, 3, **kw)
+    ics0 = base.gen_stream_ics(ts=ts, prog_w0=W.PROG_TODAY, Msat=1e4, seed_num=3, **kw)
+    for a, b in zip(ics, ics0):
+        np.testing.assert_array_equal(a.cpu().numpy(), b.cpu().numpy())
