@@ -74,15 +74,20 @@ enum { SSC_SRC_NONE = 0, SSC_SRC_STATIC = 1, SSC_SRC_TRACK = 2, SSC_SRC_LINE = 3
 
 typedef struct ssc_sources {
   int32_t kind;
-  int32_t n;            /* number of sources, 0..SSC_MAX_SOURCES */
+  int32_t n;            /* number of sources, 0..SSC_MAX_SOURCES (ungrouped) */
   int32_t n_track;      /* TRACK: rows in pos */
-  int32_t reserved;
+  int32_t group_size;   /* 0: every orbit sees all n sources.  > 0: the sources form
+                           consecutive groups of group_size and orbit i sees only group
+                           group[i] -- a batch of independent simulations in one call
+                           (ssc_dopri5 only; STATIC or LINE; n may then exceed
+                           SSC_MAX_SOURCES) */
   const double* gm;     /* [n] G*m (device) */
   const double* rs;     /* [n] Plummer scale radius (device) */
   const double* pos;    /* STATIC/LINE: [n][3]; TRACK: [n_track][n][3] (device) */
   const double* vel;    /* LINE: [n][3] (device) */
   const double* t0;     /* LINE: [n] (device) */
   const double* t_window; /* LINE: [n] (device) */
+  const int32_t* group; /* group_size > 0: [orbits] group index of each orbit (device) */
 } ssc_sources;
 
 /* Thread-local message for the last failing call on this thread. */
@@ -136,7 +141,8 @@ typedef struct ssc_dopri5_options {
  *   ts[nts]          save times (device), monotone in the direction of integration; rows of
  *                    ys[n][nts][6] outside an orbit's interval are NaN
  *   stats[n][3]      NULL or {accepted, rejected, status}; status 1 = max_steps ran out, and
- *                    then every output row of that orbit is +inf (the reference's throw=False)
+ *                    then every output row of that orbit is +inf (the reference's throw=False);
+ *                    status 2 = its source group index is out of range (rows are NaN)
  * Replaces integrate_orbit / integrate_orbit_batch_vmapped / gen_stream_vmapped with
  * solver=Dopri5 (reference streamsculptor/main.py:125-202,343-368). */
 int ssc_dopri5(const ssc_host_potential* host, const ssc_sources* src, const double* w0,

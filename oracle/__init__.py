@@ -59,9 +59,10 @@ class _Host(ctypes.Structure):
 
 class _Sources(ctypes.Structure):
     _fields_ = [("kind", ctypes.c_int32), ("n", ctypes.c_int32), ("n_track", ctypes.c_int32),
-                ("reserved", ctypes.c_int32),
+                ("group_size", ctypes.c_int32),
                 ("gm", ctypes.c_void_p), ("rs", ctypes.c_void_p), ("pos", ctypes.c_void_p),
-                ("vel", ctypes.c_void_p), ("t0", ctypes.c_void_p), ("t_window", ctypes.c_void_p)]
+                ("vel", ctypes.c_void_p), ("t0", ctypes.c_void_p), ("t_window", ctypes.c_void_p),
+                ("group", ctypes.c_void_p)]
 
 
 def build(force: bool = False) -> str:
@@ -144,7 +145,7 @@ class Sources:
     """Holds numpy arrays alive for a _Sources descriptor."""
 
     def __init__(self, kind=SRC_NONE, gm=None, rs=None, pos=None, vel=None, t0=None,
-                 t_window=None):
+                 t_window=None, group_size=0, group=None):
         self.kind = kind
         self.arrays = {}
         s = _Sources()
@@ -163,6 +164,10 @@ class Sources:
                 t_window = np.ascontiguousarray(t_window, dtype=np.float64)
                 self.arrays.update(vel=vel, t0=t0, t_window=t_window)
                 s.vel, s.t0, s.t_window = vel.ctypes.data, t0.ctypes.data, t_window.ctypes.data
+            if group_size:
+                group = np.ascontiguousarray(group, dtype=np.int32)
+                self.arrays.update(group=group)
+                s.group_size, s.group = int(group_size), group.ctypes.data
         self.desc = s
 
 

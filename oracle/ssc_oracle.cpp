@@ -945,7 +945,24 @@ int orc_dopri5(const ssc_host_potential* hp, const ssc_sources* src, const doubl
       Dp5Opts o{rtol, atol, dtmin, dtmax, dt0, max_steps, tstop.data(), n_stops};
       double wo[6];
       int32_t st[3];
-      bad += dp5_orbit(hp, src, o, dir, dir * ta, dir * t1, w0 + 6 * i, tau_s.data(), nts,
+      // grouped sources: orbit i sees only the group_size sources of group[i]
+      ssc_sources sub;
+      const ssc_sources* s_i = src;
+      if (src && src->group_size > 0) {
+        const int64_t j0 = (int64_t)src->group[i] * src->group_size;
+        sub = *src;
+        sub.n = src->group_size;
+        sub.group_size = 0;
+        sub.group = nullptr;
+        sub.gm = src->gm + j0;
+        sub.rs = src->rs + j0;
+        sub.pos = src->pos + 3 * j0;
+        if (src->vel) sub.vel = src->vel + 3 * j0;
+        if (src->t0) sub.t0 = src->t0 + j0;
+        if (src->t_window) sub.t_window = src->t_window + j0;
+        s_i = &sub;
+      }
+      bad += dp5_orbit(hp, s_i, o, dir, dir * ta, dir * t1, w0 + 6 * i, tau_s.data(), nts,
                        ysl.data(), wo, st);
       if (ys) for (int k = 0; k < 6 * nts; ++k) ys[6 * nts * i + k] = ysl[k];
       if (w_out) for (int c = 0; c < 6; ++c) w_out[6 * i + c] = wo[c];

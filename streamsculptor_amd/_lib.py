@@ -33,9 +33,10 @@ class HostPotential(ctypes.Structure):
 
 class Sources(ctypes.Structure):
     _fields_ = [("kind", ctypes.c_int32), ("n", ctypes.c_int32), ("n_track", ctypes.c_int32),
-                ("reserved", ctypes.c_int32),
+                ("group_size", ctypes.c_int32),
                 ("gm", ctypes.c_void_p), ("rs", ctypes.c_void_p), ("pos", ctypes.c_void_p),
-                ("vel", ctypes.c_void_p), ("t0", ctypes.c_void_p), ("t_window", ctypes.c_void_p)]
+                ("vel", ctypes.c_void_p), ("t0", ctypes.c_void_p), ("t_window", ctypes.c_void_p),
+                ("group", ctypes.c_void_p)]
 
 
 class Dopri5Options(ctypes.Structure):

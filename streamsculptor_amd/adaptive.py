@@ -35,12 +35,12 @@ def _sources(pot):
     return h, src
 
 
-def dopri5(pot, w0, t1, t0=0.0, t_start=None, ts=None, rtol=1e-7, atol=1e-7, dtmin=0.3,
-           dtmax=None, max_steps=10_000, jump_ts=None, dt0=None, throw=False):
-    """Run ssc_dopri5 on rows w0 (n, 6).  Orbit i runs from t_start[i] (default t0) to t1 and
-    is saved at ts (monotone in the direction of integration).  Returns (w_final (n, 6),
-    ys (n, len(ts), 6) or None, stats (n, 3) int32 = accepted, rejected, failed)."""
-    h, src = _sources(pot)
+def run(h, sd, w0, t1, t0=0.0, t_start=None, ts=None, rtol=1e-7, atol=1e-7, dtmin=0.3,
+        dtmax=None, max_steps=10_000, jump_ts=None, dt0=None, throw=False):
+    """ssc_dopri5 on rows w0 (n, 6) with a host descriptor ``h`` and a sources descriptor ``sd``
+    (or None).  Orbit i runs from t_start[i] (default t0) to t1 and is saved at ts (monotone in
+    the direction of integration).  Returns (w_final (n, 6), ys (n, len(ts), 6) or None,
+    stats (n, 3) int32 = accepted, rejected, status)."""
     w0 = _lib.f64(w0).reshape(-1, 6).contiguous()
     n = w0.shape[0]
     dev = w0.device
@@ -56,16 +56,24 @@ def dopri5(pot, w0, t1, t0=0.0, t_start=None, ts=None, rtol=1e-7, atol=1e-7, dtm
     ys = torch.empty(n, nts, 6, dtype=torch.float64, device=dev) if nts else None
     stats = torch.empty(n, 3, dtype=torch.int32, device=dev)
     opt = _options(rtol, atol, dtmin, dtmax, max_steps, stops, dt0)
-    sd = src.desc() if src is not None else None
     _lib.call("ssc_dopri5", _lib.ctypes.byref(h), None if sd is None else _lib.ctypes.byref(sd),
               _lib.ptr(w0), _lib.ptr(tst), n, float(t0), float(t1), _lib.ptr(tsd), nts,
               _lib.ctypes.byref(opt), _lib.ptr(out), _lib.ptr(ys), _lib.ptr(stats))
     if throw:
-        failed = int(stats[:, 2].sum().item())
+        failed = int((stats[:, 2] != 0).sum().item())
         if failed:
             raise RuntimeError(f"Dopri5: {failed} of {n} orbits reached max_steps={max_steps} "
                                "before t1")
     return out, ys, stats
+
+
+def dopri5(pot, w0, t1, t0=0.0, t_start=None, ts=None, rtol=1e-7, atol=1e-7, dtmin=0.3,
+           dtmax=None, max_steps=10_000, jump_ts=None, dt0=None, throw=False):
+    """``run`` with the descriptors of potential ``pot``."""
+    h, src = _sources(pot)
+    sd = src.desc() if src is not None else None
+    return run(h, sd, w0, t1, t0=t0, t_start=t_start, ts=ts, rtol=rtol, atol=atol, dtmin=dtmin,
+               dtmax=dtmax, max_steps=max_steps, jump_ts=jump_ts, dt0=dt0, throw=throw)
 
 
 def integrate_orbit(pot, w, tsn, t0v, t1v, rtol=1e-7, atol=1e-7, dtmin=0.3, dtmax=None,

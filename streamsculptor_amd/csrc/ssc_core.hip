@@ -40,10 +40,23 @@ int validate_host(const ssc_host_potential* host) {
   return SSC_OK;
 }
 
-int validate_sources(const ssc_sources* src, int32_t nsteps_needed) {
+int validate_sources(const ssc_sources* src, int32_t nsteps_needed, bool allow_groups) {
   if (!src || src->kind == SSC_SRC_NONE || src->n == 0) return SSC_OK;
-  if (src->n < 0 || src->n > SSC_MAX_SOURCES)
+  if (src->group_size != 0) {
+    if (!allow_groups)
+      return fail(SSC_E_ARG, "sources: grouped sources (group_size > 0) are only supported by "
+                             "ssc_dopri5");
+    if (src->group_size < 0 || src->group_size > SSC_MAX_SOURCES)
+      return fail(SSC_E_LIMIT, "sources: group_size out of range [1, " +
+                                   std::to_string(SSC_MAX_SOURCES) + "]");
+    if (src->n < 0 || src->n % src->group_size != 0)
+      return fail(SSC_E_ARG, "sources: n must be a multiple of group_size");
+    if (!src->group) return fail(SSC_E_ARG, "sources: group_size > 0 needs the group array");
+    if (src->kind != SSC_SRC_STATIC && src->kind != SSC_SRC_LINE)
+      return fail(SSC_E_ARG, "sources: grouped sources must be STATIC or LINE");
+  } else if (src->n < 0 || src->n > SSC_MAX_SOURCES) {
     return fail(SSC_E_LIMIT, "sources: n out of range [0, " + std::to_string(SSC_MAX_SOURCES) + "]");
+  }
   if (!src->gm || !src->rs || !src->pos) return fail(SSC_E_ARG, "sources: NULL gm/rs/pos");
   if (src->kind == SSC_SRC_TRACK && src->n_track < nsteps_needed + 1)
     return fail(SSC_E_ARG, "sources: TRACK table has " + std::to_string(src->n_track) +

@@ -18,7 +18,7 @@ void set_error(const std::string& msg);
 int fail(int code, const std::string& msg);
 int check_launch(const char* what);
 int validate_host(const ssc_host_potential* host);
-int validate_sources(const ssc_sources* src, int32_t nsteps_needed);
+int validate_sources(const ssc_sources* src, int32_t nsteps_needed, bool allow_groups = false);
 
 // ------------------------------------------------------------------------------------------
 // small vector helpers

@@ -51,13 +51,63 @@ struct Dp5Ctl {
   int n_stops;
 };
 
-// dy/dtau at tau (t = dir * tau) for the orbit ODE [v, -grad Phi].
+// One orbit's group of sources read straight from global memory: in a batch of independent
+// simulations (grouped sources) every orbit has its own few subhalos, so there is no shared
+// table to stage in LDS; the group's records stay in L1/L2 across the stages.
 template <int SK>
+struct GroupSources {
+  const double* __restrict__ gm;
+  const double* __restrict__ rs;
+  const double* __restrict__ pos;
+  const double* __restrict__ vel;
+  const double* __restrict__ t0;
+  const double* __restrict__ tw;
+  int n;
+  // False (and no source is ever read) when group[i] is outside [0, n / group_size).
+  __device__ bool init(const ssc_sources& s, int64_t i) {
+    n = s.group_size;
+    const int g = s.group[i];
+    if (g < 0 || g >= s.n / n) {
+      n = 0;
+      return false;
+    }
+    const int64_t j0 = (int64_t)g * n;
+    gm = s.gm + j0;
+    rs = s.rs + j0;
+    pos = s.pos + 3 * j0;
+    vel = SK == SSC_SRC_LINE ? s.vel + 3 * j0 : nullptr;
+    t0 = SK == SSC_SRC_LINE ? s.t0 + j0 : nullptr;
+    tw = SK == SSC_SRC_LINE ? s.t_window + j0 : nullptr;
+    return true;
+  }
+  __device__ __forceinline__ void add(double t, double x, double y, double z, double& gx,
+                                      double& gy, double& gz) const {
+    for (int j = 0; j < n; ++j) {
+      double sx = pos[3 * j], sy = pos[3 * j + 1], sz = pos[3 * j + 2];
+      if (SK == SSC_SRC_LINE) {
+        const double dt = t - t0[j];
+        if (!(fabs(dt) < tw[j])) continue;
+        sx = fma(vel[3 * j], dt, sx);
+        sy = fma(vel[3 * j + 1], dt, sy);
+        sz = fma(vel[3 * j + 2], dt, sz);
+      }
+      const double a = rs[j];
+      plummer_grad(x - sx, y - sy, z - sz, gm[j], a * a, gx, gy, gz);
+    }
+  }
+};
+
+// dy/dtau at tau (t = dir * tau) for the orbit ODE [v, -grad Phi].
+template <int SK, bool GR>
 __device__ __forceinline__ void rhs(const ssc_host_potential& hp, const SourceLDS<SK>& lds,
-                                    double dir, double tau, const double* y, double* k) {
+                                    const GroupSources<SK>& grp, double dir, double tau,
+                                    const double* y, double* k) {
   double gx, gy, gz;
   host_grad(hp, y[0], y[1], y[2], gx, gy, gz);
-  lds.template add_grad<1>(0, dir * tau, &y[0], &y[1], &y[2], &gx, &gy, &gz, 0, 1);
+  if (GR)
+    grp.add(dir * tau, y[0], y[1], y[2], gx, gy, gz);
+  else
+    lds.template add_grad<1>(0, dir * tau, &y[0], &y[1], &y[2], &gx, &gy, &gz, 0, 1);
   k[0] = dir * y[3];
   k[1] = dir * y[4];
   k[2] = dir * y[5];
@@ -84,7 +134,7 @@ __device__ __forceinline__ double clip_stops(const double* __restrict__ stops, i
   return best;
 }
 
-template <int SK>
+template <int SK, bool GR>
 __global__ __launch_bounds__(DP_BLOCK) void dopri5_kernel(
     ssc_host_potential hp, ssc_sources src, const double* __restrict__ w0,
     const double* __restrict__ t_start, int64_t n, double t0, double t1,
@@ -92,9 +142,23 @@ __global__ __launch_bounds__(DP_BLOCK) void dopri5_kernel(
     double* __restrict__ w_out, double* __restrict__ ys, int32_t* __restrict__ stats) {
   extern __shared__ __align__(16) char lds_raw[];
   SourceLDS<SK> lds;
-  lds.init(src, lds_raw);  // the only barrier: every lane reaches it
+  if (!GR) lds.init(src, lds_raw);  // the only barrier: every lane reaches it
   const int64_t i = (int64_t)blockIdx.x * DP_BLOCK + threadIdx.x;
   if (i >= n) return;
+  GroupSources<SK> grp;
+  if (GR && !grp.init(src, i)) {  // bad group index: status 2, NaN rows, nothing integrated
+    if (stats) {
+      stats[3 * i] = 0;
+      stats[3 * i + 1] = 0;
+      stats[3 * i + 2] = 2;
+    }
+    double* yr = ys ? ys + (int64_t)6 * nts * i : nullptr;
+    if (yr)
+      for (int j = 0; j < 6 * nts; ++j) yr[j] = __builtin_nan("");
+    if (w_out)
+      for (int c = 0; c < 6; ++c) w_out[6 * i + c] = __builtin_nan("");
+    return;
+  }
 
   const double ta = t_start ? t_start[i] : t0;
   const double dir = t1 >= ta ? 1.0 : -1.0;
@@ -114,7 +178,7 @@ __global__ __launch_bounds__(DP_BLOCK) void dopri5_kernel(
   int nacc = 0, nrej = 0, nsteps = 0;
   double tprev = tau0;
   if (tau0 < tau1) {
-    rhs<SK>(hp, lds, dir, tau0, y, f0);
+    rhs<SK, GR>(hp, lds, grp, dir, tau0, y, f0);
     double tnext;
     {
       double sc[6], u[6];
@@ -132,7 +196,7 @@ __global__ __launch_bounds__(DP_BLOCK) void dopri5_kernel(
         double y1[6], f1[6];
 #pragma unroll
         for (int c = 0; c < 6; ++c) y1[c] = y[c] + h0 * f0[c];
-        rhs<SK>(hp, lds, dir, tau0 + h0, y1, f1);
+        rhs<SK, GR>(hp, lds, grp, dir, tau0 + h0, y1, f1);
 #pragma unroll
         for (int c = 0; c < 6; ++c) u[c] = (f1[c] - f0[c]) / sc[c];
         const double d2 = rms6(u) / h0;
@@ -149,25 +213,25 @@ __global__ __launch_bounds__(DP_BLOCK) void dopri5_kernel(
       double k2[6], k3[6], k4[6], k5[6], k6[6], k7[6], yt[6], yn[6];
 #pragma unroll
       for (int c = 0; c < 6; ++c) yt[c] = y[c] + h * (A21 * f0[c]);
-      rhs<SK>(hp, lds, dir, tprev + C2 * h, yt, k2);
+      rhs<SK, GR>(hp, lds, grp, dir, tprev + C2 * h, yt, k2);
 #pragma unroll
       for (int c = 0; c < 6; ++c) yt[c] = y[c] + h * (A31 * f0[c] + A32 * k2[c]);
-      rhs<SK>(hp, lds, dir, tprev + C3 * h, yt, k3);
+      rhs<SK, GR>(hp, lds, grp, dir, tprev + C3 * h, yt, k3);
 #pragma unroll
       for (int c = 0; c < 6; ++c) yt[c] = y[c] + h * (A41 * f0[c] + A42 * k2[c] + A43 * k3[c]);
-      rhs<SK>(hp, lds, dir, tprev + C4 * h, yt, k4);
+      rhs<SK, GR>(hp, lds, grp, dir, tprev + C4 * h, yt, k4);
 #pragma unroll
       for (int c = 0; c < 6; ++c)
         yt[c] = y[c] + h * (A51 * f0[c] + A52 * k2[c] + A53 * k3[c] + A54 * k4[c]);
-      rhs<SK>(hp, lds, dir, tprev + C5 * h, yt, k5);
+      rhs<SK, GR>(hp, lds, grp, dir, tprev + C5 * h, yt, k5);
 #pragma unroll
       for (int c = 0; c < 6; ++c)
         yt[c] = y[c] + h * (A61 * f0[c] + A62 * k2[c] + A63 * k3[c] + A64 * k4[c] + A65 * k5[c]);
-      rhs<SK>(hp, lds, dir, tnext, yt, k6);
+      rhs<SK, GR>(hp, lds, grp, dir, tnext, yt, k6);
 #pragma unroll
       for (int c = 0; c < 6; ++c)
         yn[c] = y[c] + h * (B1 * f0[c] + B3 * k3[c] + B4 * k4[c] + B5 * k5[c] + B6 * k6[c]);
-      rhs<SK>(hp, lds, dir, tnext, yn, k7);
+      rhs<SK, GR>(hp, lds, grp, dir, tnext, yn, k7);
       double u[6];
 #pragma unroll
       for (int c = 0; c < 6; ++c) {
@@ -267,17 +331,17 @@ int set_lds_dp(const void* fn, size_t lds) {
   return SSC_OK;
 }
 
-template <int SK>
+template <int SK, bool GR>
 int launch_dopri5(const ssc_host_potential& hp, const ssc_sources& src, const double* w0,
                   const double* t_start, int64_t n, double t0, double t1, const double* ts,
                   int nts, const double* stops, const Dp5Ctl& ctl, double* w_out, double* ys,
                   int32_t* stats, hipStream_t st) {
-  const size_t lds = SourceLDS<SK>::bytes(src);
-  int rc = set_lds_dp((const void*)dopri5_kernel<SK>, lds);
+  const size_t lds = GR ? 0 : SourceLDS<SK>::bytes(src);
+  int rc = set_lds_dp((const void*)dopri5_kernel<SK, GR>, lds);
   if (rc) return rc;
   const int64_t blocks = (n + DP_BLOCK - 1) / DP_BLOCK;
-  hipLaunchKernelGGL(dopri5_kernel<SK>, dim3((unsigned)blocks), dim3(DP_BLOCK), lds, st, hp, src,
-                     w0, t_start, n, t0, t1, ts, nts, stops, ctl, w_out, ys, stats);
+  hipLaunchKernelGGL((dopri5_kernel<SK, GR>), dim3((unsigned)blocks), dim3(DP_BLOCK), lds, st, hp,
+                     src, w0, t_start, n, t0, t1, ts, nts, stops, ctl, w_out, ys, stats);
   return check_launch("ssc_dopri5");
 }
 
@@ -305,7 +369,7 @@ extern "C" int ssc_dopri5(const ssc_host_potential* host, const ssc_sources* src
   if (opt->max_steps < 1) return fail(SSC_E_ARG, "ssc_dopri5: max_steps < 1");
   if ((n + DP_BLOCK - 1) / DP_BLOCK > (int64_t)INT_MAX)
     return fail(SSC_E_LIMIT, "ssc_dopri5: too many orbits");
-  rc = validate_sources(src, 0);
+  rc = validate_sources(src, 0, true);
   if (rc) return rc;
   ssc_sources s;
   if (src) {
@@ -320,17 +384,15 @@ extern "C" int ssc_dopri5(const ssc_host_potential* host, const ssc_sources* src
   if (nts == 0) ys = nullptr;
   Dp5Ctl ctl{opt->rtol, opt->atol, opt->dtmin, opt->dtmax, opt->dt0, opt->max_steps, opt->n_stops};
   hipStream_t st = (hipStream_t)stream;
+  const bool grouped = s.kind != SSC_SRC_NONE && s.group_size > 0;
+#define SSC_DP(K, G)                                                                              \
+  launch_dopri5<K, G>(*host, s, w0, t_start, n, t0, t1, ts, nts, opt->stops, ctl, w_out, ys,      \
+                      stats, st)
   switch (s.kind) {
-    case SSC_SRC_NONE:
-      return launch_dopri5<SSC_SRC_NONE>(*host, s, w0, t_start, n, t0, t1, ts, nts, opt->stops,
-                                         ctl, w_out, ys, stats, st);
-    case SSC_SRC_STATIC:
-      return launch_dopri5<SSC_SRC_STATIC>(*host, s, w0, t_start, n, t0, t1, ts, nts, opt->stops,
-                                           ctl, w_out, ys, stats, st);
-    case SSC_SRC_LINE:
-      return launch_dopri5<SSC_SRC_LINE>(*host, s, w0, t_start, n, t0, t1, ts, nts, opt->stops,
-                                         ctl, w_out, ys, stats, st);
-    default:
-      return fail(SSC_E_ARG, "ssc_dopri5: bad source kind");
+    case SSC_SRC_NONE: return SSC_DP(SSC_SRC_NONE, false);
+    case SSC_SRC_STATIC: return grouped ? SSC_DP(SSC_SRC_STATIC, true) : SSC_DP(SSC_SRC_STATIC, false);
+    case SSC_SRC_LINE: return grouped ? SSC_DP(SSC_SRC_LINE, true) : SSC_DP(SSC_SRC_LINE, false);
+    default: return fail(SSC_E_ARG, "ssc_dopri5: bad source kind");
   }
+#undef SSC_DP
 }

@@ -98,6 +98,26 @@ class SourceSet:
             setattr(s, k, None if t[k] is None else t[k].data_ptr())
         return s
 
+    def desc_grouped(self, group_size, group):
+        """Descriptor of these sources as consecutive groups of ``group_size``; orbit i sees
+        group ``group[i]`` (a device int32 tensor the caller keeps alive for the call)."""
+        if self.kind not in (_lib.SRC_STATIC, _lib.SRC_LINE):
+            raise NotImplementedError("grouped sources must be STATIC or LINE")
+        t = self.tensors()
+        s = _lib.Sources()
+        s.kind = self.kind
+        s.n = t["gm"].numel()
+        if group_size < 1 or group_size > _lib.MAX_SOURCES or s.n % group_size:
+            raise ValueError("group_size must divide the number of sources and be <= "
+                             f"{_lib.MAX_SOURCES}")
+        s.group_size = int(group_size)
+        if group.dtype != torch.int32 or not group.is_contiguous():
+            raise ValueError("group must be a contiguous int32 tensor")
+        s.group = group.data_ptr()
+        for k in ("gm", "rs", "pos", "vel", "t0", "t_window"):
+            setattr(s, k, None if t[k] is None else t[k].data_ptr())
+        return s
+
     def on_grid(self, t0, dt, nsteps):
         """This TRACK set re-indexed so that row k is the position at t0 + k*dt, k = 0..nsteps
         (the integration step must be +- the track spacing and t0 a track row)."""

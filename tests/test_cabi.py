@@ -43,11 +43,43 @@ def test_library_exports_every_declared_symbol():
             assert n in _lib.SIGNATURES, n
 
 
-def test_struct_layout_matches_header():
+_LAYOUT_C = r"""
+#include <stddef.h>
+#include <stdio.h>
+#include "ssc_mi355x.h"
+#define F(T, m) printf(#T "." #m " %zu\n", offsetof(T, m));
+int main(void) {
+  printf("ssc_component %zu\nssc_host_potential %zu\nssc_sources %zu\n"
+         "ssc_dopri5_options %zu\n", sizeof(ssc_component), sizeof(ssc_host_potential),
+         sizeof(ssc_sources), sizeof(ssc_dopri5_options));
+  F(ssc_sources, group_size) F(ssc_sources, gm) F(ssc_sources, t_window) F(ssc_sources, group)
+  F(ssc_dopri5_options, dt0) F(ssc_dopri5_options, max_steps) F(ssc_dopri5_options, stops)
+  F(ssc_host_potential, c) F(ssc_component, p)
+  return 0;
+}
+"""
+
+
+def test_struct_layout_matches_header(tmp_path):
+    """The ctypes mirrors have the C compiler's sizes and offsets for every descriptor."""
+    import subprocess
     from streamsculptor_amd import _lib
-    assert ctypes.sizeof(_lib.Component) == 40
-    assert ctypes.sizeof(_lib.HostPotential) == 8 + 8 * 40
-    assert ctypes.sizeof(_lib.Sources) == 16 + 6 * 8
+    src = tmp_path / "layout.c"
+    src.write_text(_LAYOUT_C)
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)],
+                   check=True)
+    got = dict(line.split() for line in subprocess.run([str(exe)], check=True,
+                                                       capture_output=True, text=True)
+               .stdout.splitlines())
+    mirror = {"ssc_component": _lib.Component, "ssc_host_potential": _lib.HostPotential,
+              "ssc_sources": _lib.Sources, "ssc_dopri5_options": _lib.Dopri5Options}
+    for name, cls in mirror.items():
+        assert ctypes.sizeof(cls) == int(got[name]), name
+    for key, val in got.items():
+        if "." in key:
+            struct, field = key.split(".")
+            assert getattr(mirror[struct], field).offset == int(val), key
 
 
 def test_product_fails_loudly_without_gpu():
