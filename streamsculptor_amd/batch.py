@@ -98,12 +98,14 @@ def integrate_orbit_batch(pot_base, batch, w0, ts, t0=None, t1=None, rtol=1e-7, 
 
 def gen_stream_batch_with_pert(pot_base, batch, ts, prog_w0, Msat, seed_num, kval_arr=1.0,
                                rtol=1e-7, atol=1e-7, dtmin=0.1, dtmax=None, max_steps=10_000,
-                               throw=False, fixed_prog=False):
+                               throw=False, fixed_prog=False, events=None, return_stats=False):
     """B perturbed streams (reference streamhelpers.gen_stream_vmapped_with_pert, and
     gen_stream_vmapped_with_pert_fixed_prog when fixed_prog=True, streamhelpers.py:90-146) with
     Dopri5: progenitor b in pot_base + batch[b] (pot_base alone when fixed_prog), release model
     in pot_base, particles in pot_base + batch[b].  prog_w0: (6,) or (B, 6) at ts[0].
-    Returns (lead (B, T, 6), trail (B, T, 6)), T = len(ts) - 1."""
+    Returns (lead (B, T, 6), trail (B, T, 6)), T = len(ts) - 1, and with return_stats the
+    particles' (B, T, 2, 3) {accepted, rejected, status}.  ``events``: a pair of
+    torch.cuda.Event recorded around the particle launch."""
     h = _host(pot_base)
     tsn = _arr(ts).reshape(-1)
     T = tsn.size - 1
@@ -128,7 +130,13 @@ def gen_stream_batch_with_pert(pot_base, batch, ts, prog_w0, Msat, seed_num, kva
     w0[:, :, 1, :3], w0[:, :, 1, 3:] = pf, vf
     t_start = torch.as_tensor(np.repeat(tsn[:T], 2), device=dev).repeat(B)
     group = torch.arange(B, dtype=torch.int32, device=dev).repeat_interleave(2 * T)
-    out, _, _ = adaptive.run(h, batch.desc(group), w0.reshape(-1, 6), float(tsn[-1]),
-                             t_start=t_start, **kw)
+    sd = batch.desc(group)
+    if events:
+        events[0].record()
+    out, _, stats = adaptive.run(h, sd, w0.reshape(-1, 6), float(tsn[-1]), t_start=t_start, **kw)
+    if events:
+        events[1].record()
     out = out.reshape(B, T, 2, 6)
+    if return_stats:
+        return out[:, :, 0], out[:, :, 1], stats.reshape(B, T, 2, 3)
     return out[:, :, 0], out[:, :, 1]

@@ -11,12 +11,19 @@ FP64 sqrt, division, rsqrt and log1p expand to multi-instruction sequences on CD
                   (the squared-radius add of r_s^2 fuses)             = 18 per source
   host Hessian    (response and adjoint forms)                        = 96
   response vector per subhalo: unit-mass Plummer 16 + H dx 15 + source add 3 + KDK 18 = 52
+  Dopri5 step (accepted or rejected), per state component: stage combinations of 1..5
+                  terms (2m + 1 each: 35) + 5th-order solution 11 + error estimate 13 +
+                  scaled error 3 = 62, x 6 components = 372; RMS norm 14 + controller 10
+                  = 396, plus 6 right-hand sides of (host gradient + 18 per active source
+                  + 6 for [dir v, -dir grad])
 """
 HOST_GRAD = 57
 KDK = 18
 PLUMMER_SOURCE = 18
 HOST_HESSIAN = 96
 RESPONSE_PER_SUBHALO = 52
+HERNQUIST_GRAD = 16
+DOPRI5_STEP = 396
 
 
 def plain(n_sources: int = 0) -> int:
@@ -38,3 +45,10 @@ def mlp(widths=(3, 256, 256, 256, 4)) -> int:
 def surrogate() -> int:
     """Surrogate particle-step: MLP in BF16 MFMA plus the FP64 host force and KDK (75)."""
     return plain(0)
+
+
+def dopri5_step(host_grad: int = HOST_GRAD, n_sources: int = 0) -> int:
+    """One Dopri5 step (6 new right-hand sides, FSAL) in a host whose gradient costs
+    ``host_grad`` plus n active Plummer sources (3-component host: 774; with a Hernquist
+    nucleus: 870; and one active subhalo: 978)."""
+    return DOPRI5_STEP + 6 * (host_grad + PLUMMER_SOURCE * n_sources + 6)

@@ -11,6 +11,14 @@
 * ``clumps``      -- 512 static Plummer clumps, radii drawn from the NFW(r_s = 15.62) enclosed
                      mass within 60 kpc, isotropic (seed 1), masses log-uniform in [1e5, 1e7],
                      r_s = 1.62 (M/1e8)^0.5; 1e6 spray particles = 250 per release step.
+* ``gd1``         -- a batch of perturbed GD-1-like streams as the reference's GPU example runs
+                     them (examples/GD1_and_SubhaloFitting_GPU.ipynb, cells 7, 18-27): Milky Way
+                     host with nucleus, stream age 2500 Myr, stripping times
+                     linspace(-2500, -300, 2000) plus the observation time 0 (4000 particles),
+                     Msat = 1e4, seed 532, Dopri5 at rtol = atol = 1e-6 and dtmin = 0.05; each
+                     simulation has one straight-line Plummer subhalo (t_window = 100 Myr) drawn
+                     from the example's prior and placed at impact parameter b from the
+                     progenitor's orbit at the impact time (seed 8942).
 """
 from __future__ import annotations
 
@@ -100,3 +108,38 @@ def spray_layout(per_step):
 
 def particle_steps(per_step, nsteps=T_STEPS):
     return per_step * nsteps * (nsteps + 1) // 2
+
+
+# GD-1-like present-day point: the example's spline of the unperturbed stream at phi1 = -30
+# (examples/GD1_and_SubhaloFitting_GPU.ipynb, cell 17 output).
+GD1_TODAY = np.array([-12.19942006, 0.4483408, 6.52725897, -0.07686926, -0.2278697, -0.1179158])
+GD1_AGE = 2500.0
+
+
+def gd1_ts(n_strip=2000, t_dissolve=-300.0):
+    return np.concatenate([np.linspace(-GD1_AGE, t_dissolve, n_strip), [0.0]])
+
+
+def gd1_subhalo_prior(n, seed=8942):
+    """Per-simulation subhalo parameters from the example's prior (cell 24): log10 m in [6, 7],
+    log10 a in [log10 0.01, log10 0.2], log10 t_impact in [1, log10 2000], log10 b in
+    [-5, log10 0.5], v_impact in [-0.4, 0.4] kpc/Myr; directions isotropic."""
+    rng = np.random.default_rng(seed)
+    m = 10 ** rng.uniform(6.0, 7.0, n)
+    a = 10 ** rng.uniform(np.log10(0.01), np.log10(0.2), n)
+    t_imp = 10 ** rng.uniform(1.0, np.log10(2000.0), n)
+    b = 10 ** rng.uniform(-5.0, np.log10(0.5), n)
+    v = rng.uniform(-0.4, 0.4, n)
+    nb = rng.normal(size=(n, 3))
+    nb /= np.linalg.norm(nb, axis=1, keepdims=True)
+    nv = rng.normal(size=(n, 3))
+    nv /= np.linalg.norm(nv, axis=1, keepdims=True)
+    return dict(m=m, a=a, t_impact=t_imp, b=b, v=v, dir_b=nb, dir_v=nv)
+
+
+def gala_mw_components():
+    """reference GalaMilkyWayPotential (potential.py:390-414): disk, bulge, nucleus, halo."""
+    return [(1, [G * HOST["m_disk"], HOST["a_disk"], HOST["b_disk"]]),
+            (3, [G * HOST["m_bulge"], HOST["c_bulge"], 0.0]),
+            (3, [G * 1.71e9, 0.07, 0.0]),
+            (2, [G * HOST["m_halo"], HOST["rs_halo"]])]
