@@ -11,7 +11,7 @@ namespace mlp {
 constexpr int IN = 3;
 constexpr int H = 256;
 constexpr int OUT = 4;
-constexpr int OUTP = 32;  // output rows padded to one 32-row MFMA tile
+constexpr int OUTP = 16;  // output rows padded to one 16-row MFMA tile
 
 // Offsets (floats) of the fp32 master parameter vector:
 //   W1[H][IN] b1[H] W2[H][H] b2[H] W3[H][H] b3[H] W4[OUT][H] b4[OUT]
@@ -25,12 +25,14 @@ constexpr int64_t OW4 = OB3 + H;
 constexpr int64_t OB4 = OW4 + OUT * H;
 constexpr int64_t NPARAM = OB4 + OUT;
 
-// Packed inference form (bytes):
-//   fragments of W2, W3 (H/32 tiles x H/16 k-steps x 64 lanes x 8 bf16) and of W4 padded to
-//   32 rows (1 tile), all with the k order permuted by kperm (below);
-//   then fp32 W1 rows (in natural neuron order) and fp32 biases b1, b2, b3, b4.
-constexpr int64_t FRAG_L = (int64_t)(H / 32) * (H / 16) * 64 * 8;  // bf16 elements per H x H layer
-constexpr int64_t FRAG_4 = (int64_t)(H / 16) * 64 * 8;
+// Packed inference form (bytes), laid out for v_mfma_f32_16x16x32_bf16 with the points on the
+// MFMA N axis (16 per wave):
+//   fragments of W2 and W3: [tile m (H/16)][k-step j (H/32)][lane 64][8 bf16], lane l holding
+//   A[16 m + l % 16][32 j + 8 (l / 16) + e] with the k order permuted by kperm (below); W4 as
+//   one 16-row tile (rows >= OUT zero): [j][lane][8];
+//   then fp32 W1 rows (in natural neuron order) and fp32 biases b1, b2, b3, b4 (padded to 16).
+constexpr int64_t FRAG_L = (int64_t)(H / 16) * (H / 32) * 64 * 8;  // bf16 elements per H x H layer
+constexpr int64_t FRAG_4 = (int64_t)(H / 32) * 64 * 8;
 constexpr int64_t PK_W2 = 0;                            // bf16 element offsets
 constexpr int64_t PK_W3 = PK_W2 + FRAG_L;
 constexpr int64_t PK_W4 = PK_W3 + FRAG_L;
@@ -44,14 +46,14 @@ constexpr int64_t PK_B4 = PK_B3 + H;
 constexpr int64_t PK_F32_END = PK_B4 + OUTP;
 constexpr int64_t PACKED_BYTES = PK_F32_BASE_BYTES + PK_F32_END * 4;
 
-// K order of every chained layer: within each block of 16 inputs the middle two quads swap,
-// so that k-slot 8h + e of a B fragment (lane half h, element e) is exactly the neuron the
-// lane already holds in accumulator register r = 8u + e of the previous layer's 32x32 output
-// tile (rows 8(r/4) + 4h + r%4).  The permutation is its own inverse.
+// K order of every chained layer: within each block of 32 inputs, k-slot 8g + e of a B fragment
+// (lane group g = l / 16, element e) is the neuron that lane group already holds from the
+// previous layer's two 16x16 output tiles 2j and 2j + 1 (rows 4g + r of each):
+//   e < 4: neuron 4g + e;  e >= 4: neuron 16 + 4g + (e - 4).
 __host__ __device__ inline int kperm(int s) {
-  const int blk = s & ~15, w = s & 15;
-  const int h = w >> 3, e = w & 7;
-  return blk + 8 * (e >> 2) + 4 * h + (e & 3);
+  const int blk = s & ~31, w = s & 31;
+  const int g = w >> 3, e = w & 7;
+  return blk + (e < 4 ? 4 * g + e : 16 + 4 * g + (e - 4));
 }
 
 // SiLU z * sigmoid(z) with the hardware exp2 and reciprocal (about 1 ulp each; the result is
@@ -85,6 +87,7 @@ __host__ __device__ inline float bf2f(uint16_t b) {
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef short i16x8 __attribute__((ext_vector_type(8)));
 
 }  // namespace mlp

@@ -263,11 +263,12 @@ __global__ void pack_kernel(const float* __restrict__ p, uint16_t* __restrict__ 
     if (r < FRAG_L) { W = p + OW2; rows = H; }
     else if (r < 2 * FRAG_L) { r -= FRAG_L; W = p + OW3; rows = H; }
     else { r -= 2 * FRAG_L; W = p + OW4; rows = OUT; }
+    // fragment order [tile m][k-step j][lane][e] of v_mfma_f32_16x16x32_bf16's A operand
     const int e = (int)(r & 7), l = (int)((r >> 3) & 63);
-    const int64_t tq = r >> 9;
-    const int q = (int)(tq % (H / 16)), tt = (int)(tq / (H / 16));
-    const int row = 32 * tt + (l & 31);
-    const int col = kperm(16 * q + 8 * (l >> 5) + e);
+    const int64_t mj = r >> 9;
+    const int j = (int)(mj % (H / 32)), m = (int)(mj / (H / 32));
+    const int row = 16 * m + (l & 15);
+    const int col = kperm(32 * j + 8 * (l >> 4) + e);
     pk16[i] = row < rows ? f2bf(W[(int64_t)row * H + col]) : (uint16_t)0;
   }
   if (i < PK_F32_END) {
@@ -287,13 +288,13 @@ __global__ __launch_bounds__(256) void mlp_infer_kernel(const uint16_t* __restri
                                                         const float* __restrict__ pk32,
                                                         const float* __restrict__ x, int64_t nb,
                                                         float* __restrict__ y) {
-  const int64_t p = ((int64_t)blockIdx.x * 256 + threadIdx.x) / 64 * 32 + (threadIdx.x & 31);
+  const int64_t p = ((int64_t)blockIdx.x * 256 + threadIdx.x) / 64 * 16 + (threadIdx.x & 15);
   const bool ok = p < nb;
   const float x0 = ok ? x[3 * p] : 0.0f, x1 = ok ? x[3 * p + 1] : 0.0f, x2 = ok ? x[3 * p + 2] : 0.0f;
   float o[4];
   GlobalWeights gw{pk16};
   mlp_tile_forward(gw, pk32, x0, x1, x2, o);
-  if (ok && (threadIdx.x & 63) < 32)
+  if (ok && (threadIdx.x & 63) < 16)
 #pragma unroll
     for (int j = 0; j < 4; ++j) y[4 * p + j] = o[j];
 }
@@ -513,7 +514,7 @@ int ssc_mlp_infer(const void* packed, const float* x, int64_t batch, float* y, v
   if (batch == 0) return SSC_OK;
   const uint16_t* pk16 = (const uint16_t*)packed;
   const float* pk32 = (const float*)((const char*)packed + PK_F32_BASE_BYTES);
-  const int64_t blocks = (batch + 127) / 128;  // 4 waves x 32 points
+  const int64_t blocks = (batch + 63) / 64;  // 4 waves x 16 points
   hipLaunchKernelGGL(mlp_infer_kernel, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream,
                      pk16, pk32, x, batch, y);
   return check_launch("ssc_mlp_infer");

@@ -1,9 +1,14 @@
-// Register-chained MLP forward for 32 points per wave (shared by ssc_mlp_infer and the
-// surrogate integrator).  Lane l evaluates point l % 32; the upper half of the wave (l >= 32)
-// carries the same point and supplies the other half of every MFMA operand.  The weight
-// source is a policy: GlobalWeights reads fragments straight from the packed buffer, the
-// integrator's policy stages each layer in LDS first (a block-wide operation with barriers,
-// so every lane of the block must call the forward together).
+// Register-chained MLP forward for 16 points per wave (shared by ssc_mlp_infer and the
+// surrogate integrator), on v_mfma_f32_16x16x32_bf16 with the points on the MFMA N axis.
+// Lane l evaluates point l % 16 together with the three other lanes of its column
+// (l % 16 + 16 g, g = 0..3); lane group g = l / 16 holds a quarter of every activation vector:
+// 64 of the 256 hidden neurons, i.e. 8 BF16 fragments (32 VGPRs) per layer input and output.
+// A layer's output tile pair (2j, 2j + 1) leaves rows 4g + r of each in lane group g, which is
+// exactly k-block j of the next layer's B operand once the weights' k order is permuted
+// (kperm), so activations never leave the registers.  The weight source is a policy:
+// GlobalWeights reads fragments straight from the packed buffer, the integrator's policy stages
+// each layer in LDS first (a block-wide operation with barriers, so every lane of the block
+// must call the forward together).
 #pragma once
 
 #include "ssc_mlp.h"
@@ -18,6 +23,13 @@ __device__ __forceinline__ mlp::bf16x8 pack8(const float* v) {
   return r;
 }
 
+// An empty volatile statement that reads a finished fragment: the fragment must be computed
+// here, in program order.  Without it LLVM sinks a layer's MFMAs and activations to their use
+// in the next layer -- below the barrier that stages the next layer's weights in LDS -- while
+// the fragment loads have to stay above it, so all 128 weight fragments of a layer end up live
+// at once (and spilled).
+__device__ __forceinline__ void pin(mlp::bf16x8& v) { asm volatile("" : "+v"(v)); }
+
 struct GlobalWeights {
   const uint16_t* pk16;
   // fragments of chained layer 0 (W2), 1 (W3) or 2 (W4)
@@ -28,79 +40,78 @@ struct GlobalWeights {
 };
 
 // pk32: W1 (fp32, BF16-rounded values) and the biases, in the packed fp32 layout.
+// y[0..3]: the OUT outputs of point l % 16, valid in lane group 0 (lanes 0..15).
 template <class Weights>
 __device__ __forceinline__ void mlp_tile_forward(Weights& wsrc, const float* __restrict__ pk32,
                                                  float x0, float x1, float x2, float* y) {
   using namespace mlp;
-  const int l = threadIdx.x & 63, hh = l >> 5;
+  constexpr int KB = H / 32;  // k-blocks of a 256-wide input
+  // The lane index is made opaque to the optimiser: every weight, bias and fragment address
+  // below is a lane-dependent base plus a constant, and inside the integrator's step loop the
+  // compiler would otherwise hoist hundreds of such addresses out of the loop and keep them
+  // live (then spill them).  Recomputing them from one base per call costs a few adds.
+  int l = threadIdx.x & 63;
+  asm volatile("" : "+v"(l));
+  const int g = l >> 4;
   const float* w1 = pk32 + PK_W1;
-  bf16x8 bin[H / 16];
-  // layer 1 (VALU): k-slot 16q + 8hh + e of the layer-2 B operand is neuron kperm(.)
+  bf16x8 bin[KB];
+  // layer 1 (fp32 VALU): k-slot 32j + 8g + e of the layer-2 B operand is neuron kperm(.)
 #pragma unroll
-  for (int q = 0; q < H / 16; ++q) {
+  for (int j = 0; j < KB; ++j) {
     float v[8];
 #pragma unroll
     for (int e = 0; e < 8; ++e) {
-      const int n = kperm(16 * q + 8 * hh + e);
+      const int n = kperm(32 * j + 8 * g + e);
       float z = pk32[PK_B1 + n];
       z = fmaf(x0, w1[3 * n], z);
       z = fmaf(x1, w1[3 * n + 1], z);
       z = fmaf(x2, w1[3 * n + 2], z);
       v[e] = silu(z);
     }
-    bin[q] = pack8(v);
-    __builtin_amdgcn_sched_barrier(0);
+    bin[j] = pack8(v);
+    pin(bin[j]);
   }
-  // two chained 256x256 layers; output tiles are produced two at a time so that only two
-  // accumulators, the input fragments and the output fragments are live
+  // two chained 256x256 layers; output k-block j = tiles 2j and 2j + 1
 #pragma unroll
   for (int layer = 0; layer < 2; ++layer) {
     const bf16x8* W = wsrc.layer(layer);
     const float* bias = pk32 + (layer == 0 ? PK_B2 : PK_B3);
-    bf16x8 bout[H / 16];
+    bf16x8 bout[KB];
 #pragma unroll
-    for (int t = 0; t < H / 32; t += 2) {
-      f32x16 a0, a1;
+    for (int j = 0; j < KB; ++j) {
+      f32x4 a0, a1;
 #pragma unroll
-      for (int r = 0; r < 16; ++r) {
-        const int row = 8 * (r >> 2) + 4 * hh + (r & 3);
-        a0[r] = bias[32 * t + row];
-        a1[r] = bias[32 * (t + 1) + row];
+      for (int r = 0; r < 4; ++r) {
+        a0[r] = bias[32 * j + 4 * g + r];
+        a1[r] = bias[32 * j + 16 + 4 * g + r];
       }
 #pragma unroll
-      for (int q = 0; q < H / 16; ++q) {
-        a0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(W[(t * (H / 16) + q) * 64 + l], bin[q], a0, 0, 0, 0);
-        a1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(W[((t + 1) * (H / 16) + q) * 64 + l], bin[q], a1,
-                                                    0, 0, 0);
-        // bound how far the scheduler hoists fragment loads (register pressure)
-        if ((q & 3) == 3) __builtin_amdgcn_sched_barrier(0);
+      for (int q = 0; q < KB; ++q) {
+        a0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(W[((2 * j) * KB + q) * 64 + l], bin[q], a0, 0,
+                                                     0, 0);
+        a1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(W[((2 * j + 1) * KB + q) * 64 + l], bin[q],
+                                                     a1, 0, 0, 0);
       }
+      float v[8];
 #pragma unroll
-      for (int u = 0; u < 2; ++u) {
-        float v0[8], v1[8];
-#pragma unroll
-        for (int e = 0; e < 8; ++e) {
-          v0[e] = silu(a0[8 * u + e]);
-          v1[e] = silu(a1[8 * u + e]);
-        }
-        bout[2 * t + u] = pack8(v0);
-        bout[2 * (t + 1) + u] = pack8(v1);
+      for (int r = 0; r < 4; ++r) {
+        v[r] = silu(a0[r]);
+        v[4 + r] = silu(a1[r]);
       }
+      bout[j] = pack8(v);
+      pin(bout[j]);
     }
 #pragma unroll
-    for (int q = 0; q < H / 16; ++q) bin[q] = bout[q];
+    for (int q = 0; q < KB; ++q) bin[q] = bout[q];
   }
-  // output layer (4 rows of one padded 32-row tile)
+  // output layer: rows 0..OUT-1 of one padded 16-row tile land in lane group 0
   const bf16x8* W4 = wsrc.layer(2);
-  f32x16 out;
+  f32x4 out;
 #pragma unroll
-  for (int r = 0; r < 16; ++r) {
-    const int row = 8 * (r >> 2) + 4 * hh + (r & 3);
-    out[r] = row < OUT ? pk32[PK_B4 + row] : 0.0f;
-  }
+  for (int r = 0; r < 4; ++r) out[r] = pk32[PK_B4 + 4 * g + r];
 #pragma unroll
-  for (int q = 0; q < H / 16; ++q)
-    out = __builtin_amdgcn_mfma_f32_32x32x16_bf16(W4[q * 64 + l], bin[q], out, 0, 0, 0);
+  for (int q = 0; q < KB; ++q)
+    out = __builtin_amdgcn_mfma_f32_16x16x32_bf16(W4[q * 64 + l], bin[q], out, 0, 0, 0);
 #pragma unroll
   for (int j = 0; j < OUT; ++j) y[j] = out[j];
 }

@@ -1,11 +1,13 @@
 // Leapfrog with the learned clump-force surrogate (gfx950).
 //
-// A block of 8 waves integrates 256 particles for the whole grid.  The FP64 state of particle
-// p lives in lanes p and p + 32 of its wave (both halves compute the same update, so the MLP's
-// 64-lane MFMA operands need no exchange).  Every step:
-//   kick + drift (FP64) -> MLP(x) on the matrix cores: the three 256-wide layers run as BF16
-//   v_mfma_f32_32x32x16 with fp32 accumulation, the activations chained in VGPRs and each
-//   layer's weight fragments staged once per block in LDS -> clump force = g_scale * MLP
+// A block of 8 waves (two per SIMD, so one wave's MFMAs overlap the other's activations)
+// integrates 128 particles for the whole grid.  The FP64 state of particle p lives in the four
+// lanes p + 16 g of its wave (all four compute the same update, so the MLP's 64-lane MFMA
+// operands need no exchange).  Every step:
+//   kick + drift (FP64) -> MLP(x) on the matrix cores: the two 256x256 layers and the output
+//   layer run as BF16 v_mfma_f32_16x16x32 with fp32 accumulation, the activations chained in
+//   VGPRs and each layer's weight fragments staged once per block in LDS -> clump force =
+//   g_scale * MLP
 //   -> fallback: particles in flagged cells (or outside the training box) are compacted into an
 //   LDS list and get the exact clump sum, one wave per particle with the clumps split over
 //   lanes -> host force (FP64, analytic) -> kick.
@@ -31,8 +33,8 @@ struct SurrogateDesc {
   int32_t reserved;
 };
 
-// Block shapes: WAVES = 4 (one wave per SIMD, 512-register budget) or 8 (two per SIMD,
-// 256-register budget); SG_PARTS = 32 particles per wave.
+// Block shape: 8 waves (two per SIMD, 256-register budget each), 16 particles per wave.
+constexpr int SG_BLOCK_THREADS = 512;
 
 template <int SG_BLOCK>
 struct LdsWeights {
@@ -64,7 +66,7 @@ __global__ __launch_bounds__(SG_BLOCK) void surrogate_leapfrog_kernel(
     const double* __restrict__ w0, const int32_t* __restrict__ start, int64_t n, double t0,
     double dt, int nsteps, double* __restrict__ w_out, unsigned long long* __restrict__ counters) {
   using namespace mlp;
-  constexpr int SG_PARTS = SG_BLOCK / 2;
+  constexpr int SG_PARTS = SG_BLOCK / 4;
   extern __shared__ __align__(16) char lds_raw[];
   __shared__ int kmin_sh;
   __shared__ int nfb;
@@ -75,10 +77,10 @@ __global__ __launch_bounds__(SG_BLOCK) void surrogate_leapfrog_kernel(
   float* pk32 = reinterpret_cast<float*>(lds_raw + FRAG_L * 2 + FRAG_4 * 2);
 
   const int l = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int pl = wv * 32 + (l & 31);
+  const int pl = wv * 16 + (l & 15);
   const int64_t i = (int64_t)blockIdx.x * SG_PARTS + pl;
   const bool valid = i < n;
-  const bool owner = (l < 32) && valid;
+  const bool owner = (l < 16) && valid;
 
   double x = 10.0, y = 0.0, z = 0.0, vx = 0.0, vy = 0.0, vz = 0.0;
   int s = nsteps;
@@ -114,9 +116,9 @@ __global__ __launch_bounds__(SG_BLOCK) void surrogate_leapfrog_kernel(
     const float u2 = (float)((z - sd.center[2]) * sd.inv_half[2]);
     float o[4];
     mlp_tile_forward(lw, pk32, u0, u1, u2, o);
-    // outputs live on the lower half-wave; mirror them to the upper half
-    const float o1 = __shfl(o[1], l & 31, 64), o2 = __shfl(o[2], l & 31, 64),
-                o3 = __shfl(o[3], l & 31, 64);
+    // outputs live in lane group 0; copy them to the particle's other three lanes
+    const float o1 = __shfl(o[1], l & 15, 64), o2 = __shfl(o[2], l & 15, 64),
+                o3 = __shfl(o[3], l & 15, 64);
     double gx = sd.g_scale * (double)o1, gy = sd.g_scale * (double)o2, gz = sd.g_scale * (double)o3;
     // fallback test: outside the training box or a flagged cell
     const double cx = (x - sd.grid_lo[0]) * sd.inv_cell[0];
@@ -131,14 +133,14 @@ __global__ __launch_bounds__(SG_BLOCK) void surrogate_leapfrog_kernel(
     fb = fb && valid;
     if (__syncthreads_or(fb)) {
       int slot = -1;
-      if (fb && l < 32) {
+      if (fb && l < 16) {
         slot = atomicAdd(&nfb, 1);
         fb_buf[slot][0] = x;
         fb_buf[slot][1] = y;
         fb_buf[slot][2] = z;
         ++nfall;
       }
-      slot = __shfl(slot, l & 31, 64);
+      slot = __shfl(slot, l & 15, 64);
       __syncthreads();
       const int cnt = nfb;
       for (int f = wv; f < cnt; f += SG_BLOCK / 64) {
@@ -231,20 +233,16 @@ extern "C" int ssc_surrogate_leapfrog(const ssc_host_potential* host, const ssc_
   const size_t lds = (size_t)(mlp::FRAG_L + mlp::FRAG_4) * 2 + (size_t)mlp::PK_F32_END * 4;
   const uint16_t* pk16 = (const uint16_t*)packed;
   const float* pk32 = (const float*)((const char*)packed + mlp::PK_F32_BASE_BYTES);
-  const char* env = getenv("SSC_SURROGATE_WAVES");
-  const int waves = (env && atoi(env) == 8) ? 8 : 4;
-#define SSC_SG(B)                                                                                 \
-  {                                                                                               \
-    hipError_t e = hipFuncSetAttribute((const void*)surrogate_leapfrog_kernel<B>,                 \
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);     \
-    if (e != hipSuccess)                                                                          \
-      return fail(SSC_E_HIP, std::string("hipFuncSetAttribute: ") + hipGetErrorString(e));       \
-    const int64_t blocks = (n + B / 2 - 1) / (B / 2);                                             \
-    hipLaunchKernelGGL(surrogate_leapfrog_kernel<B>, dim3((unsigned)blocks), dim3(B), lds,        \
-                       (hipStream_t)stream, *host, *clumps, sd, pk16, pk32, flags, w0, start_step, \
-                       n, t0, dt, nsteps, w_out, fallback_count);                                 \
-  }
-  if (waves == 8) SSC_SG(512) else SSC_SG(256)
-#undef SSC_SG
+  constexpr int B = SG_BLOCK_THREADS;
+  hipError_t e = hipFuncSetAttribute((const void*)surrogate_leapfrog_kernel<B>,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  if (e != hipSuccess)
+    return fail(SSC_E_HIP, std::string("hipFuncSetAttribute: ") + hipGetErrorString(e));
+  const int64_t parts = B / 4;
+  const int64_t blocks = (n + parts - 1) / parts;
+  if (blocks > (int64_t)INT_MAX) return fail(SSC_E_LIMIT, "ssc_surrogate_leapfrog: too many particles");
+  hipLaunchKernelGGL(surrogate_leapfrog_kernel<B>, dim3((unsigned)blocks), dim3(B), lds,
+                     (hipStream_t)stream, *host, *clumps, sd, pk16, pk32, flags, w0, start_step, n,
+                     t0, dt, nsteps, w_out, fallback_count);
   return check_launch("ssc_surrogate_leapfrog");
 }
