@@ -30,12 +30,17 @@ __device__ __forceinline__ mlp::bf16x8 pack8(const float* v) {
 // at once (and spilled).
 __device__ __forceinline__ void pin(mlp::bf16x8& v) { asm volatile("" : "+v"(v)); }
 
+// Weight policies.  chunk(c), c = 0..3, returns the fragments of chunk c of the two chained
+// layers -- W2 tiles 0-7, W2 tiles 8-15, W3 tiles 0-7, W3 tiles 8-15 (64 KB each) -- and
+// out() the output layer's fragments.
 struct GlobalWeights {
   const uint16_t* pk16;
-  // fragments of chained layer 0 (W2), 1 (W3) or 2 (W4)
-  __device__ const mlp::bf16x8* layer(int which) const {
-    const int64_t off = which == 0 ? mlp::PK_W2 : (which == 1 ? mlp::PK_W3 : mlp::PK_W4);
+  __device__ const mlp::bf16x8* chunk(int c) const {
+    const int64_t off = ((c >> 1) == 0 ? mlp::PK_W2 : mlp::PK_W3) + (c & 1) * (mlp::FRAG_L / 2);
     return reinterpret_cast<const mlp::bf16x8*>(pk16 + off);
+  }
+  __device__ const mlp::bf16x8* out() const {
+    return reinterpret_cast<const mlp::bf16x8*>(pk16 + mlp::PK_W4);
   }
 };
 
@@ -71,41 +76,46 @@ __device__ __forceinline__ void mlp_tile_forward(Weights& wsrc, const float* __r
     bin[j] = pack8(v);
     pin(bin[j]);
   }
-  // two chained 256x256 layers; output k-block j = tiles 2j and 2j + 1
+  // two chained 256x256 layers, each in two chunks of 8 tiles; output k-block j = tiles 2j
+  // and 2j + 1 (tiles 2jj, 2jj + 1 of chunk j / 4)
 #pragma unroll
   for (int layer = 0; layer < 2; ++layer) {
-    const bf16x8* W = wsrc.layer(layer);
     const float* bias = pk32 + (layer == 0 ? PK_B2 : PK_B3);
     bf16x8 bout[KB];
 #pragma unroll
-    for (int j = 0; j < KB; ++j) {
-      f32x4 a0, a1;
+    for (int half = 0; half < 2; ++half) {
+      const bf16x8* W = wsrc.chunk(2 * layer + half);
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        a0[r] = bias[32 * j + 4 * g + r];
-        a1[r] = bias[32 * j + 16 + 4 * g + r];
-      }
+      for (int jj = 0; jj < KB / 2; ++jj) {
+        const int j = (KB / 2) * half + jj;
+        f32x4 a0, a1;
 #pragma unroll
-      for (int q = 0; q < KB; ++q) {
-        a0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(W[((2 * j) * KB + q) * 64 + l], bin[q], a0, 0,
-                                                     0, 0);
-        a1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(W[((2 * j + 1) * KB + q) * 64 + l], bin[q],
-                                                     a1, 0, 0, 0);
-      }
-      float v[8];
+        for (int r = 0; r < 4; ++r) {
+          a0[r] = bias[32 * j + 4 * g + r];
+          a1[r] = bias[32 * j + 16 + 4 * g + r];
+        }
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        v[r] = silu(a0[r]);
-        v[4 + r] = silu(a1[r]);
+        for (int q = 0; q < KB; ++q) {
+          a0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(W[((2 * jj) * KB + q) * 64 + l], bin[q], a0,
+                                                       0, 0, 0);
+          a1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(W[((2 * jj + 1) * KB + q) * 64 + l], bin[q],
+                                                       a1, 0, 0, 0);
+        }
+        float v[8];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          v[r] = silu(a0[r]);
+          v[4 + r] = silu(a1[r]);
+        }
+        bout[j] = pack8(v);
+        pin(bout[j]);
       }
-      bout[j] = pack8(v);
-      pin(bout[j]);
     }
 #pragma unroll
     for (int q = 0; q < KB; ++q) bin[q] = bout[q];
   }
   // output layer: rows 0..OUT-1 of one padded 16-row tile land in lane group 0
-  const bf16x8* W4 = wsrc.layer(2);
+  const bf16x8* W4 = wsrc.out();
   f32x4 out;
 #pragma unroll
   for (int r = 0; r < 4; ++r) out[r] = pk32[PK_B4 + 4 * g + r];

@@ -36,21 +36,35 @@ struct SurrogateDesc {
 // Block shape: 8 waves (two per SIMD, 256-register budget each), 16 particles per wave.
 constexpr int SG_BLOCK_THREADS = 512;
 
+// The two chained layers stream through two 64 KB LDS buffers in four chunks per step (W2 tiles
+// 0-7, W2 tiles 8-15, W3 tiles 0-7, W3 tiles 8-15).  Each thread holds its share (PIECES x
+// 16 B) of the next chunk in registers while the current chunk is computed, so the L2 latency
+// hides behind a chunk of MFMAs.  One barrier per chunk suffices: chunk c overwrites the buffer
+// that chunk c - 2 was read from, and every wave finished that read before it reached the
+// barrier of chunk c - 1, which the writer has passed.
 template <int SG_BLOCK>
 struct LdsWeights {
-  mlp::bf16x8* buf;        // one 256x256 layer in fragment order
+  static constexpr int CH = (int)(mlp::FRAG_L / 2 / 8);  // 16-byte fragments per chunk
+  static constexpr int PIECES = CH / SG_BLOCK;
+  static_assert(CH % SG_BLOCK == 0, "chunk must split evenly over the block");
+  mlp::bf16x8* buf;        // two chunk buffers of CH fragments
   const mlp::bf16x8* w4;   // output-layer fragments (resident)
-  const mlp::bf16x8* gw2;  // global fragments of W2 / W3
-  const mlp::bf16x8* gw3;
-  __device__ const mlp::bf16x8* layer(int which) {
-    if (which == 2) return w4;
-    const mlp::bf16x8* src = which == 0 ? gw2 : gw3;
-    __syncthreads();  // every wave is done with the previous contents
-    constexpr int N = (int)(mlp::FRAG_L / 8);
-    for (int i = threadIdx.x; i < N; i += SG_BLOCK) buf[i] = src[i];
-    __syncthreads();
-    return buf;
+  const mlp::bf16x8* gw;   // global fragments of W2 followed by W3
+  mlp::bf16x8 pre[PIECES];
+  __device__ void prefetch(int c) {
+    const mlp::bf16x8* src = gw + (int64_t)c * CH;
+#pragma unroll
+    for (int k = 0; k < PIECES; ++k) pre[k] = src[threadIdx.x + k * SG_BLOCK];
   }
+  __device__ const mlp::bf16x8* chunk(int c) {
+    mlp::bf16x8* dst = buf + (c & 1) * CH;
+#pragma unroll
+    for (int k = 0; k < PIECES; ++k) dst[threadIdx.x + k * SG_BLOCK] = pre[k];
+    __syncthreads();
+    prefetch((c + 1) & 3);
+    return dst;
+  }
+  __device__ const mlp::bf16x8* out() const { return w4; }
 };
 
 __device__ __forceinline__ double wave_sum64(double v) {
@@ -105,8 +119,8 @@ __global__ __launch_bounds__(SG_BLOCK) void surrogate_leapfrog_kernel(
   __syncthreads();
   const int kmin = kmin_sh;
 
-  LdsWeights<SG_BLOCK> lw{wbuf, w4, reinterpret_cast<const bf16x8*>(pk16 + PK_W2),
-                reinterpret_cast<const bf16x8*>(pk16 + PK_W3)};
+  LdsWeights<SG_BLOCK> lw{wbuf, w4, reinterpret_cast<const bf16x8*>(pk16 + PK_W2), {}};
+  lw.prefetch(0);
   unsigned long long nfall = 0;
 
   // total acceleration at (x, y, z); block-uniform call (barriers inside)
