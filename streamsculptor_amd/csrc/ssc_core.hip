@@ -220,16 +220,95 @@ __global__ __launch_bounds__(LF_BLOCK) void leapfrog_kernel(
     }
 }
 
+// G lanes per orbit: for batches too small to fill the chip one orbit per lane (1e5 particles
+// are 1.5 waves per SIMD), G consecutive lanes split the orbit's source sum (lane r takes
+// sources r, r + G, ...), combine it with a butterfly over the group and all carry the same
+// orbit state.  Block-uniform grid walk and TRACK staging as in leapfrog_kernel.
+template <int SK, int G>
+__device__ __forceinline__ void accel_group(const ssc_host_potential& hp, const SourceLDS<SK>& lds,
+                                            int k, double t, int r, double x, double y, double z,
+                                            double& ax, double& ay, double& az) {
+  double gx = 0.0, gy = 0.0, gz = 0.0;
+  lds.template add_grad<1>(k, t, &x, &y, &z, &gx, &gy, &gz, r, G);
+  gx = group_sum<G>(gx);
+  gy = group_sum<G>(gy);
+  gz = group_sum<G>(gz);
+  double hx, hy, hz;
+  host_grad(hp, x, y, z, hx, hy, hz);
+  ax = -(hx + gx);
+  ay = -(hy + gy);
+  az = -(hz + gz);
+}
+
+template <int SK, int G>
+__global__ __launch_bounds__(LF_BLOCK) void leapfrog_group_kernel(
+    ssc_host_potential hp, ssc_sources src, const double* __restrict__ w0,
+    const int32_t* __restrict__ start, int64_t n, double t0, double dt, int nsteps,
+    double* __restrict__ w_out, const int32_t* __restrict__ save_idx, int nsave,
+    double* __restrict__ ys) {
+  extern __shared__ __align__(16) char lds_raw[];
+  __shared__ int kmin_sh;
+  const int r = threadIdx.x % G;
+  const int64_t i = (int64_t)blockIdx.x * (LF_BLOCK / G) + threadIdx.x / G;
+  const bool valid = i < n, lead = valid && r == 0;
+  int s = nsteps;
+  double x = 1.0, y = 0.0, z = 0.0, vx = 0.0, vy = 0.0, vz = 0.0;
+  if (valid) {
+    const int sp = start ? start[i] : 0;
+    s = sp < 0 ? 0 : (sp > nsteps ? nsteps : sp);
+    const double* w = w0 + 6 * i;
+    x = w[0]; y = w[1]; z = w[2]; vx = w[3]; vy = w[4]; vz = w[5];
+  }
+  if (threadIdx.x == 0) kmin_sh = INT_MAX;
+  __syncthreads();
+  atomicMin(&kmin_sh, s);
+  SourceLDS<SK> lds;
+  lds.init(src, lds_raw);
+  __syncthreads();
+  const int kmin = kmin_sh;
+  int next = 0;
+  while (next < nsave && save_idx[next] <= kmin) {
+    if (lead) {
+      double* o = ys + ((int64_t)next * n + i) * 6;
+      o[0] = x; o[1] = y; o[2] = z; o[3] = vx; o[4] = vy; o[5] = vz;
+    }
+    ++next;
+  }
+  const double h = 0.5 * dt;
+  double ax = 0.0, ay = 0.0, az = 0.0;
+  if (kmin < nsteps) {
+    lds.stage(src, kmin, nsteps + 1);
+    accel_group<SK, G>(hp, lds, kmin, t0 + kmin * dt, r, x, y, z, ax, ay, az);
+  }
+  for (int k = kmin; k < nsteps; ++k) {
+    lds.stage(src, k, nsteps + 1);
+    const bool act = k >= s;
+    if (act) {
+      vx = vx + h * ax; vy = vy + h * ay; vz = vz + h * az;
+      x = x + dt * vx; y = y + dt * vy; z = z + dt * vz;
+    }
+    accel_group<SK, G>(hp, lds, k + 1, t0 + (k + 1) * dt, r, x, y, z, ax, ay, az);
+    if (act) {
+      vx = vx + h * ax; vy = vy + h * ay; vz = vz + h * az;
+    }
+    while (next < nsave && save_idx[next] == k + 1) {
+      if (lead) {
+        double* o = ys + ((int64_t)next * n + i) * 6;
+        o[0] = x; o[1] = y; o[2] = z; o[3] = vx; o[4] = vy; o[5] = vz;
+      }
+      ++next;
+    }
+  }
+  if (lead) {
+    double* o = w_out + 6 * i;
+    o[0] = x; o[1] = y; o[2] = z; o[3] = vx; o[4] = vy; o[5] = vz;
+  }
+}
+
 // One wave per orbit: for a handful of orbits in a large source table (the progenitor in a
 // clumpy halo) the 64 lanes split the source sum and combine it with a butterfly reduction;
 // the orbit state is replicated in every lane.
 constexpr int WAVE = 64;
-
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-  for (int m = WAVE / 2; m >= 1; m >>= 1) v += __shfl_xor(v, m, WAVE);
-  return v;
-}
 
 template <int SK>
 __device__ __forceinline__ void accel_wave(const ssc_host_potential& hp, const SourceLDS<SK>& lds,
@@ -237,9 +316,9 @@ __device__ __forceinline__ void accel_wave(const ssc_host_potential& hp, const S
                                            double& ax, double& ay, double& az) {
   double gx = 0.0, gy = 0.0, gz = 0.0;
   lds.template add_grad<1>(k, t, &x, &y, &z, &gx, &gy, &gz, threadIdx.x, WAVE);
-  gx = wave_sum(gx);
-  gy = wave_sum(gy);
-  gz = wave_sum(gz);
+  gx = group_sum<WAVE>(gx);
+  gy = group_sum<WAVE>(gy);
+  gz = group_sum<WAVE>(gz);
   double hx, hy, hz;
   host_grad(hp, x, y, z, hx, hy, hz);
   ax = -(hx + gx);
@@ -306,8 +385,20 @@ static int set_lds(const void* fn, size_t lds) {
   return SSC_OK;
 }
 
-// Orbits per lane of the block kernel: two once there are enough orbits and sources for the
-// halved LDS traffic to matter, one otherwise.
+// Lanes per orbit for the grouped kernel: enough to give the launch about six waves per SIMD
+// (6 x 1024 SIMDs x 64 lanes), at most 16, and at least four sources per lane.
+static int group_lanes(int64_t n, int nsrc) {
+  if (nsrc < 16 || n <= 0) return 1;
+  const double want = 393216.0 / (double)n;
+  int g = 1;
+  while (g < want && g < 16) g *= 2;
+  while (g > 1 && nsrc / g < 4) g /= 2;
+  return g;
+}
+
+// Launch shape: one wave per orbit for a handful of orbits in a large source table; two
+// particles per lane once the batch alone fills the chip; G lanes per orbit for mid-sized
+// batches with sources; one particle per lane otherwise.
 template <int SK>
 static int launch_leapfrog(const ssc_host_potential& hp, const ssc_sources& src, const double* w0,
                            const int32_t* start, int64_t n, double t0, double dt, int nsteps,
@@ -321,7 +412,7 @@ static int launch_leapfrog(const ssc_host_potential& hp, const ssc_sources& src,
                        w0, start, n, t0, dt, nsteps, w_out, save_idx, nsave, ys);
     return check_launch("ssc_leapfrog");
   }
-  if (SK != SSC_SRC_NONE && src.n >= 16 && n >= 65536) {
+  if (SK != SSC_SRC_NONE && src.n >= 16 && n >= 393216) {
     int rc = set_lds((const void*)leapfrog_kernel<SK, 2>, lds);
     if (rc) return rc;
     const int64_t blocks = (n + 2 * LF_BLOCK - 1) / (2 * LF_BLOCK);
@@ -329,6 +420,21 @@ static int launch_leapfrog(const ssc_host_potential& hp, const ssc_sources& src,
                        hp, src, w0, start, n, t0, dt, nsteps, w_out, save_idx, nsave, ys);
     return check_launch("ssc_leapfrog");
   }
+  const int g = SK == SSC_SRC_NONE ? 1 : group_lanes(n, src.n);
+#define SSC_GROUP(GV)                                                                              \
+  if (g == GV) {                                                                                   \
+    int rc = set_lds((const void*)leapfrog_group_kernel<SK, GV>, lds);                            \
+    if (rc) return rc;                                                                             \
+    const int64_t blocks = (n + LF_BLOCK / GV - 1) / (LF_BLOCK / GV);                              \
+    hipLaunchKernelGGL((leapfrog_group_kernel<SK, GV>), dim3((unsigned)blocks), dim3(LF_BLOCK),   \
+                       lds, st, hp, src, w0, start, n, t0, dt, nsteps, w_out, save_idx, nsave, ys); \
+    return check_launch("ssc_leapfrog");                                                          \
+  }
+  SSC_GROUP(2)
+  SSC_GROUP(4)
+  SSC_GROUP(8)
+  SSC_GROUP(16)
+#undef SSC_GROUP
   int rc = set_lds((const void*)leapfrog_kernel<SK, 1>, lds);
   if (rc) return rc;
   const int64_t blocks = (n + LF_BLOCK - 1) / LF_BLOCK;

@@ -38,6 +38,30 @@ __device__ __forceinline__ double rsq(double v) {
 }
 
 // ------------------------------------------------------------------------------------------
+// Sum of a double over aligned groups of G consecutive lanes (G = 1, 2, 4, ..., 64), the result
+// in every lane of the group.  Within a 16-lane row the exchanges are DPP moves (quad
+// permutations, then the half-row and row mirrors), which need no trip through the LDS unit;
+// the 32- and 64-lane steps use lane shuffles.  Every lane of the wave must be active.
+// ------------------------------------------------------------------------------------------
+template <int CTRL>
+__device__ __forceinline__ double dpp_move(double v) {
+  const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), CTRL, 0xF, 0xF, false);
+  return __hiloint2double(hi, lo);
+}
+
+template <int G>
+__device__ __forceinline__ double group_sum(double v) {
+  if (G >= 2) v += dpp_move<0xB1>(v);   // quad_perm [1,0,3,2]: lane ^ 1
+  if (G >= 4) v += dpp_move<0x4E>(v);   // quad_perm [2,3,0,1]: lane ^ 2
+  if (G >= 8) v += dpp_move<0x141>(v);  // row_half_mirror: the other quad of the 8-lane half
+  if (G >= 16) v += dpp_move<0x140>(v); // row_mirror: the other half of the 16-lane row
+  if (G >= 32) v += __shfl_xor(v, 16, 64);
+  if (G >= 64) v += __shfl_xor(v, 32, 64);
+  return v;
+}
+
+// ------------------------------------------------------------------------------------------
 // Host components, gradient only (the per-step force).  Formulas: reference
 // streamsculptor/potential.py:66-84 (MN, NFW), :124-138 (Plummer, Hernquist), differentiated
 // in closed form.

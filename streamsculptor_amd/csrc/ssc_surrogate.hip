@@ -67,11 +67,6 @@ struct LdsWeights {
   __device__ const mlp::bf16x8* out() const { return w4; }
 };
 
-__device__ __forceinline__ double wave_sum64(double v) {
-#pragma unroll
-  for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
-  return v;
-}
 
 template <int SG_BLOCK>
 __global__ __launch_bounds__(SG_BLOCK) void surrogate_leapfrog_kernel(
@@ -165,9 +160,9 @@ __global__ __launch_bounds__(SG_BLOCK) void surrogate_leapfrog_kernel(
           plummer_grad(px - clumps.pos[3 * j], py - clumps.pos[3 * j + 1], pz - clumps.pos[3 * j + 2],
                        clumps.gm[j], rs * rs, sx, sy, sz);
         }
-        sx = wave_sum64(sx);
-        sy = wave_sum64(sy);
-        sz = wave_sum64(sz);
+        sx = group_sum<64>(sx);
+        sy = group_sum<64>(sy);
+        sz = group_sum<64>(sz);
         if (l == 0) {
           fb_buf[f][0] = sx;
           fb_buf[f][1] = sy;

@@ -82,12 +82,9 @@ def spray_particles(prog_pot, release_pot, ts, prog_w0, Msat, seed_num, per_step
         pc, pf, vc, vf = release_pot._release_along(ws, T + 1, Msat, seed_num + s, kval_arr)
         table[s, 0, :, :3], table[s, 0, :, 3:] = pc, vc
         table[s, 1, :, :3], table[s, 1, :, 3:] = pf, vf
-    sel_s = np.empty((T, per_step), dtype=np.int64)
-    sel_a = np.empty((T, per_step), dtype=np.int64)
-    for i in range(T):
-        lay = even if i % 2 == 0 else odd
-        sel_s[i] = [p[0] for p in lay]
-        sel_a[i] = [p[1] for p in lay]
+    lay = np.array([even, odd], dtype=np.int64)            # (parity, per_step, [seed, arm])
+    par = np.arange(T) % 2
+    sel_s, sel_a = lay[par, :, 0], lay[par, :, 1]
     rel = np.repeat(np.arange(T), per_step)
     s_idx = torch.as_tensor(sel_s.reshape(-1), device=dev)
     a_idx = torch.as_tensor(sel_a.reshape(-1), device=dev)

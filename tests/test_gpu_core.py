@@ -117,17 +117,22 @@ def test_leapfrog_parity(oracle, ssc, kind):
     assert err <= 1e-9
 
 
-@pytest.mark.parametrize("path", ["wave_per_orbit", "two_per_lane", "one_per_lane"])
+@pytest.mark.parametrize("path", ["wave_per_orbit", "two_per_lane", "one_per_lane", "group_8",
+                                  "group_16", "group_4"])
 def test_leapfrog_launch_paths(oracle, ssc, path):
-    """The three launch shapes of ssc_leapfrog (one wave per orbit for few orbits in a large
-    source table; two particles per lane for large batches; one otherwise) against the oracle."""
+    """The launch shapes of ssc_leapfrog (one wave per orbit for few orbits in a large source
+    table; two particles per lane for large batches; G lanes per orbit for mid-sized batches with
+    sources; one particle per lane otherwise) against the oracle."""
     from streamsculptor_amd.potential import Potential_Combine, PlummerClumps
     from streamsculptor_amd import workloads as W
     rng = np.random.default_rng(21)
     h, host = _host_pair(oracle, ssc)
-    pos, m, rs = W.clump_table(n=512 if path == "wave_per_orbit" else 48, seed=1)
-    n, nsteps = {"wave_per_orbit": (7, 800), "two_per_lane": (70001, 120),
-                 "one_per_lane": (3001, 300)}[path]
+    nsrc = {"wave_per_orbit": 512, "one_per_lane": 12, "group_16": 64}.get(path, 48)
+    pos, m, rs = W.clump_table(n=nsrc, seed=1)
+    # sizes that select each shape (two per lane from 393,216 orbits; G from the wave target)
+    n, nsteps = {"wave_per_orbit": (7, 800), "two_per_lane": (400001, 40),
+                 "one_per_lane": (3001, 300), "group_8": (3001, 300), "group_16": (30001, 150),
+                 "group_4": (100001, 60)}[path]
     w0 = bound_orbits(rng, n)
     start = np.sort(rng.integers(0, nsteps, n)).astype(np.int32)
     src_o = oracle.Sources(oracle.SRC_STATIC, gm=G * m, rs=rs, pos=pos)
