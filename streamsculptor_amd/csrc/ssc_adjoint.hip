@@ -66,9 +66,11 @@ __device__ __forceinline__ void comp_grad_dp1(const ssc_component& c, double x, 
 }
 
 // Adjoint contribution at one grid point: lx -= H W, host-parameter gradients += (da/dp)^T W.
-template <int SK>
+// With G lanes per orbit, lane r takes sources r, r + G, ... and the group sums the source part
+// of H W; the host part is computed by every lane of the group.
+template <int SK, int G>
 __device__ __forceinline__ void adj_apply(const ssc_host_potential& hp, const SourceLDS<SK>& lds,
-                                          int k, double t, double x, double y, double z,
+                                          int k, double t, int rl, double x, double y, double z,
                                           const double* W, double* lx, double* gh) {
   double phi, g[3], H[9];
   host_full(hp, x, y, z, phi, g, H);
@@ -84,7 +86,8 @@ __device__ __forceinline__ void adj_apply(const ssc_host_potential& hp, const So
 #pragma unroll
   for (int r = 0; r < 3; ++r) hw[r] = H[3 * r] * W[0] + H[3 * r + 1] * W[1] + H[3 * r + 2] * W[2];
   if (SK != SSC_SRC_NONE) {
-    for (int j = 0; j < lds.n; ++j) {
+    double sw[3] = {0.0, 0.0, 0.0};
+    for (int j = rl; j < lds.n; j += G) {
       double sx, sy, sz, gm, q2;
       if (!lds.get(k, t, j, sx, sy, sz, gm, q2)) continue;
       const double dx = x - sx, dy = y - sy, dz = z - sz;
@@ -92,22 +95,40 @@ __device__ __forceinline__ void adj_apply(const ssc_host_potential& hp, const So
       const double is = rsq(s2);
       const double i3 = gm * is * is * is;
       const double c5 = 3.0 * i3 * is * is * (dx * W[0] + dy * W[1] + dz * W[2]);
-      hw[0] += i3 * W[0] - c5 * dx;
-      hw[1] += i3 * W[1] - c5 * dy;
-      hw[2] += i3 * W[2] - c5 * dz;
+      sw[0] += i3 * W[0] - c5 * dx;
+      sw[1] += i3 * W[1] - c5 * dy;
+      sw[2] += i3 * W[2] - c5 * dz;
     }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) hw[c] += group_sum<G>(sw[c]);
   }
   lx[0] -= hw[0];
   lx[1] -= hw[1];
   lx[2] -= hw[2];
 }
 
-// One checkpoint segment [k0, k1] for all particles (block-uniform grid walk).
+// Total acceleration of one orbit whose G lanes split the source sum (all lanes of the group
+// must be active: they always are, since the group shares one orbit).
+template <int SK, int G>
+__device__ __forceinline__ void accel_split(const ssc_host_potential& hp, const SourceLDS<SK>& lds,
+                                            int k, double t, int rl, double x, double y, double z,
+                                            double& ax, double& ay, double& az) {
+  double sx = 0.0, sy = 0.0, sz = 0.0;
+  lds.template add_grad<1>(k, t, &x, &y, &z, &sx, &sy, &sz, rl, G);
+  double gx, gy, gz;
+  host_grad(hp, x, y, z, gx, gy, gz);
+  ax = -(gx + group_sum<G>(sx));
+  ay = -(gy + group_sum<G>(sy));
+  az = -(gz + group_sum<G>(sz));
+}
+
+// One checkpoint segment [k0, k1] for all particles (block-uniform grid walk), G lanes per
+// particle (lane rl of the group takes sources rl, rl + G, ...; lane 0 writes the records).
 //   ckpt: state at row k0 (or the initial row for particles starting later)
 //   lam:  in/out adjoint of each particle's state (at k1 on entry, at max(k0, start) on exit)
 //   X, Wb: [k1 - k0 + 1][n][3] recorded positions and weights (zero where inactive)
 //   gpart: [gridDim.x][2*n_comp] per-block host-parameter partials (accumulated)
-template <int SK>
+template <int SK, int G>
 __global__ __launch_bounds__(AJ_BLOCK) void adjoint_segment_kernel(
     ssc_host_potential hp, ssc_sources src, const double* __restrict__ ckpt,
     const int32_t* __restrict__ start, int64_t n, double t0, double dt, int nsteps, int k0, int k1,
@@ -116,8 +137,9 @@ __global__ __launch_bounds__(AJ_BLOCK) void adjoint_segment_kernel(
   extern __shared__ __align__(16) char lds_raw[];
   __shared__ int kmin_sh;
   __shared__ double red[AJ_BLOCK / 64][AJ_MAXC];
-  const int64_t i = (int64_t)blockIdx.x * AJ_BLOCK + threadIdx.x;
-  const bool valid = i < n;
+  const int rl = threadIdx.x % G;
+  const int64_t i = (int64_t)blockIdx.x * (AJ_BLOCK / G) + threadIdx.x / G;
+  const bool valid = i < n, lead = valid && rl == 0;
   int s = nsteps;
   double x = 1.0, y = 0.0, z = 0.0, vx = 0.0, vy = 0.0, vz = 0.0;
   if (valid) {
@@ -138,7 +160,7 @@ __global__ __launch_bounds__(AJ_BLOCK) void adjoint_segment_kernel(
   const int rows = k1 - k0 + 1;
   const double h = 0.5 * dt;
   // zero the records of this particle (rows it does not reach stay zero)
-  if (valid)
+  if (lead)
     for (int r = 0; r < rows; ++r) {
       double* wb = Wb + ((int64_t)r * n + i) * 3;
       wb[0] = wb[1] = wb[2] = 0.0;
@@ -149,17 +171,12 @@ __global__ __launch_bounds__(AJ_BLOCK) void adjoint_segment_kernel(
   if (kmin < k1) {
     // ---- recompute the segment, recording positions
     double ax = 0.0, ay = 0.0, az = 0.0;
-    if (act) {
+    if (act && lead) {
       double* o = X + ((int64_t)(kb - k0) * n + i) * 3;
       o[0] = x; o[1] = y; o[2] = z;
     }
     lds.stage(src, kmin, nsteps + 1);
-    {
-      double gx, gy, gz;
-      host_grad(hp, x, y, z, gx, gy, gz);
-      lds.template add_grad<1>(kmin, t0 + kmin * dt, &x, &y, &z, &gx, &gy, &gz, 0, 1);
-      ax = -gx; ay = -gy; az = -gz;
-    }
+    accel_split<SK, G>(hp, lds, kmin, t0 + kmin * dt, rl, x, y, z, ax, ay, az);
     for (int k = kmin; k < k1; ++k) {
       lds.stage(src, k, nsteps + 1);
       const bool mv = act && k >= kb;
@@ -167,14 +184,13 @@ __global__ __launch_bounds__(AJ_BLOCK) void adjoint_segment_kernel(
         vx += h * ax; vy += h * ay; vz += h * az;
         x += dt * vx; y += dt * vy; z += dt * vz;
       }
-      double gx, gy, gz;
-      host_grad(hp, x, y, z, gx, gy, gz);
-      lds.template add_grad<1>(k + 1, t0 + (k + 1) * dt, &x, &y, &z, &gx, &gy, &gz, 0, 1);
-      ax = -gx; ay = -gy; az = -gz;
+      accel_split<SK, G>(hp, lds, k + 1, t0 + (k + 1) * dt, rl, x, y, z, ax, ay, az);
       if (mv) {
         vx += h * ax; vy += h * ay; vz += h * az;
-        double* o = X + ((int64_t)(k + 1 - k0) * n + i) * 3;
-        o[0] = x; o[1] = y; o[2] = z;
+        if (lead) {
+          double* o = X + ((int64_t)(k + 1 - k0) * n + i) * 3;
+          o[0] = x; o[1] = y; o[2] = z;
+        }
       }
     }
     // ---- reverse sweep
@@ -196,9 +212,11 @@ __global__ __launch_bounds__(AJ_BLOCK) void adjoint_segment_kernel(
       if (act) {
         const double* xp = X + ((int64_t)(k1 - k0) * n + i) * 3;
         const double W[3] = {h * lv[0], h * lv[1], h * lv[2]};
-        adj_apply<SK>(hp, lds, k1, t0 + k1 * dt, xp[0], xp[1], xp[2], W, lx, gh);
-        double* wb = Wb + ((int64_t)(k1 - k0) * n + i) * 3;
-        wb[0] = W[0]; wb[1] = W[1]; wb[2] = W[2];
+        adj_apply<SK, G>(hp, lds, k1, t0 + k1 * dt, rl, xp[0], xp[1], xp[2], W, lx, gh);
+        if (lead) {
+          double* wb = Wb + ((int64_t)(k1 - k0) * n + i) * 3;
+          wb[0] = W[0]; wb[1] = W[1]; wb[2] = W[2];
+        }
       }
     }
     for (int k = k1 - 1; k >= kmin; --k) {
@@ -212,17 +230,19 @@ __global__ __launch_bounds__(AJ_BLOCK) void adjoint_segment_kernel(
 #pragma unroll
         for (int c = 0; c < 3; ++c) W[c] = h * lh[c] + (k > s ? h * lv[c] : 0.0);
         const double* xp = X + ((int64_t)(k - k0) * n + i) * 3;
-        adj_apply<SK>(hp, lds, k, t0 + k * dt, xp[0], xp[1], xp[2], W, lx, gh);
-        double* wb = Wb + ((int64_t)(k - k0) * n + i) * 3;
-        wb[0] = W[0]; wb[1] = W[1]; wb[2] = W[2];
+        adj_apply<SK, G>(hp, lds, k, t0 + k * dt, rl, xp[0], xp[1], xp[2], W, lx, gh);
+        if (lead) {
+          double* wb = Wb + ((int64_t)(k - k0) * n + i) * 3;
+          wb[0] = W[0]; wb[1] = W[1]; wb[2] = W[2];
+        }
       }
     }
-    if (valid) {
+    if (lead) {
       double* L = lam + 6 * i;
 #pragma unroll
       for (int c = 0; c < 3; ++c) { L[c] = lx[c]; L[3 + c] = lv[c]; }
     }
-  } else if (valid && k1 == nsteps) {
+  } else if (lead && k1 == nsteps) {
     // nothing moves in the last segment: the adjoint of the (unmoved) row is lam_T + forcing
 #pragma unroll
     for (int c = 0; c < 6; ++c) {
@@ -230,10 +250,11 @@ __global__ __launch_bounds__(AJ_BLOCK) void adjoint_segment_kernel(
       lam[6 * i + c] = lam_T[6 * i + c] + f;
     }
   }
-  // block reduction of the host-parameter gradients (wave shuffles, then LDS)
+  // block reduction of the host-parameter gradients (wave shuffles, then LDS); every lane of a
+  // group holds the same values, so only lane 0 of each group contributes
   const int nc2 = 2 * hp.n_components;
   for (int c = 0; c < nc2; ++c) {
-    double v = gh[c];
+    double v = rl == 0 ? gh[c] : 0.0;
 #pragma unroll
     for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6][c] = v;
@@ -347,7 +368,8 @@ static int64_t nseg_of(int nsteps, int E) { return nsteps == 0 ? 0 : (nsteps + E
 
 static int64_t adj_bytes(int64_t n, int nsteps, int E, int m, AdjWs* w, char* base) {
   const int64_t nseg = nseg_of(nsteps, E);
-  const int64_t nblk = (n + AJ_BLOCK - 1) / AJ_BLOCK;
+  // segment-kernel blocks: G lanes per particle (the same choice adjoint_backward makes)
+  const int64_t nblk = (n * group_lanes(n, m) + AJ_BLOCK - 1) / AJ_BLOCK;
   const int64_t nchunk = (n + SR_CHUNK - 1) / SR_CHUNK;
   const int64_t rows = (int64_t)E + 1;
   int64_t off = 0;
@@ -380,12 +402,22 @@ static int adjoint_backward(const ssc_host_potential& hp, const ssc_sources& src
                             hipStream_t st) {
   const int m = SK == SSC_SRC_NONE ? 0 : src.n;
   const int64_t nseg = nseg_of(nsteps, E);
-  const unsigned nblk = (unsigned)((n + AJ_BLOCK - 1) / AJ_BLOCK);
+  const int G = SK == SSC_SRC_NONE ? 1 : group_lanes(n, m);
+  const unsigned nblk = (unsigned)((n * G + AJ_BLOCK - 1) / AJ_BLOCK);
   const int64_t nchunk = (n + SR_CHUNK - 1) / SR_CHUNK;
   const size_t lds = SourceLDS<SK>::bytes(src);
+  auto seg_kernel = [&](void) -> const void* {
+    switch (G) {
+      case 2: return (const void*)adjoint_segment_kernel<SK, 2>;
+      case 4: return (const void*)adjoint_segment_kernel<SK, 4>;
+      case 8: return (const void*)adjoint_segment_kernel<SK, 8>;
+      case 16: return (const void*)adjoint_segment_kernel<SK, 16>;
+      default: return (const void*)adjoint_segment_kernel<SK, 1>;
+    }
+  };
   if (lds > 60 * 1024) {
-    hipError_t e = hipFuncSetAttribute((const void*)adjoint_segment_kernel<SK>,
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    hipError_t e = hipFuncSetAttribute(seg_kernel(), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       (int)lds);
     if (e != hipSuccess) return fail(SSC_E_HIP, std::string("hipFuncSetAttribute: ") + hipGetErrorString(e));
   }
   hipLaunchKernelGGL(zero_kernel, dim3((nblk * AJ_MAXC + 255) / 256), dim3(256), 0, st, w.gpart,
@@ -398,9 +430,18 @@ static int adjoint_backward(const ssc_host_potential& hp, const ssc_sources& src
   for (int64_t c = nseg - 1; c >= 0; --c) {
     const int k0 = (int)(c * E), k1 = (int)std::min<int64_t>((c + 1) * E, nsteps);
     const int rows = k1 - k0 + 1;
-    hipLaunchKernelGGL(adjoint_segment_kernel<SK>, dim3(nblk), dim3(AJ_BLOCK), lds, st, hp, src,
-                       w.ckpt + c * n * 6, start, n, t0, dt, nsteps, k0, k1, lam_T, forcing, w.lam,
-                       w.X, w.Wb, w.gpart);
+#define SSC_SEG(GV)                                                                                \
+  hipLaunchKernelGGL((adjoint_segment_kernel<SK, GV>), dim3(nblk), dim3(AJ_BLOCK), lds, st, hp, src, \
+                     w.ckpt + c * n * 6, start, n, t0, dt, nsteps, k0, k1, lam_T, forcing, w.lam,    \
+                     w.X, w.Wb, w.gpart)
+    switch (G) {
+      case 2: SSC_SEG(2); break;
+      case 4: SSC_SEG(4); break;
+      case 8: SSC_SEG(8); break;
+      case 16: SSC_SEG(16); break;
+      default: SSC_SEG(1); break;
+    }
+#undef SSC_SEG
     if (m > 0 && (g_gm || g_pos)) {
       hipLaunchKernelGGL(source_adjoint_kernel<SK>, dim3((unsigned)nchunk, (unsigned)rows), dim3(64),
                          0, st, src, w.X, w.Wb, n, k0, rows, t0, dt, w.part);

@@ -385,17 +385,6 @@ static int set_lds(const void* fn, size_t lds) {
   return SSC_OK;
 }
 
-// Lanes per orbit for the grouped kernel: enough to give the launch about six waves per SIMD
-// (6 x 1024 SIMDs x 64 lanes), at most 16, and at least four sources per lane.
-static int group_lanes(int64_t n, int nsrc) {
-  if (nsrc < 16 || n <= 0) return 1;
-  const double want = 393216.0 / (double)n;
-  int g = 1;
-  while (g < want && g < 16) g *= 2;
-  while (g > 1 && nsrc / g < 4) g /= 2;
-  return g;
-}
-
 // Launch shape: one wave per orbit for a handful of orbits in a large source table; two
 // particles per lane once the batch alone fills the chip; G lanes per orbit for mid-sized
 // batches with sources; one particle per lane otherwise.

@@ -43,6 +43,17 @@ __device__ __forceinline__ double rsq(double v) {
 // permutations, then the half-row and row mirrors), which need no trip through the LDS unit;
 // the 32- and 64-lane steps use lane shuffles.  Every lane of the wave must be active.
 // ------------------------------------------------------------------------------------------
+// Lanes per orbit for the grouped kernels: enough to give the launch about six waves per SIMD
+// (6 x 1024 SIMDs x 64 lanes), at most 16, and at least four sources per lane.
+inline int group_lanes(int64_t n, int nsrc) {
+  if (nsrc < 16 || n <= 0) return 1;
+  const double want = 393216.0 / (double)n;
+  int g = 1;
+  while (g < want && g < 16) g *= 2;
+  while (g > 1 && nsrc / g < 4) g /= 2;
+  return g;
+}
+
 template <int CTRL>
 __device__ __forceinline__ double dpp_move(double v) {
   const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), CTRL, 0xF, 0xF, false);
