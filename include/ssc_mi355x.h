@@ -253,8 +253,20 @@ int ssc_mlp_adam(float* params, const float* grads, float* m, float* v, int64_t 
 /* Pack parameters into the inference form (ssc_mlp_packed_bytes() bytes, device). */
 int ssc_mlp_pack(const float* params, void* packed, void* stream);
 
-/* Inference with the packed form: one wave evaluates 32 points, activations stay in VGPRs. */
+/* Inference with the packed form: one wave evaluates 16 points, activations stay in VGPRs. */
 int ssc_mlp_infer(const void* packed, const float* x, int64_t batch, float* y, void* stream);
+
+/* Packed transposes W4^T, W3^T, W2^T for the input vector-Jacobian product
+ * (ssc_mlp_packed_vjp_bytes() bytes, device). */
+int64_t ssc_mlp_packed_vjp_bytes(void);
+int ssc_mlp_pack_vjp(const float* params, void* packed_vjp, void* stream);
+
+/* Input vector-Jacobian product dx[b] = d(dy[b] . y(x[b])) / dx[b] (fp32 [batch][3]) with the
+ * packed forms: the forward is recomputed in registers and the backward products run on the
+ * same BF16 MFMA chain (no counterpart in the reference: the derivative of the learned clump
+ * force that the discrete adjoint of the surrogate integrator needs). */
+int ssc_mlp_input_vjp(const void* packed, const void* packed_vjp, const float* x, const float* dy,
+                      int64_t batch, float* dx, void* stream);
 
 /* Leapfrog of n particles in host + clumps with the clump force from the packed MLP
  * (no counterpart in the reference).  scaling[17] (host doubles): input centre[3], input

@@ -61,6 +61,8 @@ SIGNATURES = {
                      ctypes.c_float, _i32, ctypes.c_float],
     "ssc_mlp_pack": [_vp, _vp],
     "ssc_mlp_infer": [_vp, _vp, _i64, _vp],
+    "ssc_mlp_pack_vjp": [_vp, _vp],
+    "ssc_mlp_input_vjp": [_vp, _vp, _vp, _vp, _i64, _vp],
     "ssc_surrogate_leapfrog": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _i64, _d, _d, _i32, _vp, _vp],
     "ssc_release_vjp": [_vp, _vp, _vp, _vp, _i32, ctypes.c_int64, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
                         _vp],
@@ -88,7 +90,7 @@ def load():
         L = ctypes.CDLL(LIB_PATH)
         L.ssc_last_error.restype = ctypes.c_char_p
         L.ssc_version.restype = ctypes.c_char_p
-        for name in ("ssc_mlp_num_params", "ssc_mlp_packed_bytes"):
+        for name in ("ssc_mlp_num_params", "ssc_mlp_packed_bytes", "ssc_mlp_packed_vjp_bytes"):
             getattr(L, name).restype = ctypes.c_int64
             getattr(L, name).argtypes = []
         L.ssc_mlp_workspace_bytes.restype = ctypes.c_int64

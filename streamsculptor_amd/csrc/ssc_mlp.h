@@ -46,6 +46,17 @@ constexpr int64_t PK_B4 = PK_B3 + H;
 constexpr int64_t PK_F32_END = PK_B4 + OUTP;
 constexpr int64_t PACKED_BYTES = PK_F32_BASE_BYTES + PK_F32_END * 4;
 
+// Packed transposes for the input vector-Jacobian product (BF16 element offsets):
+//   W4^T as 16 tiles of one 32-wide k-step (rows = layer-3 neurons, k < OUT real),
+//   W3^T and W2^T in the forward fragment order (rows = the layer's input neurons, k = its
+//   output neurons in kperm order).
+constexpr int64_t FRAG_4T = (int64_t)(H / 16) * 64 * 8;
+constexpr int64_t PKV_W4T = 0;
+constexpr int64_t PKV_W3T = PKV_W4T + FRAG_4T;
+constexpr int64_t PKV_W2T = PKV_W3T + FRAG_L;
+constexpr int64_t PKV_END = PKV_W2T + FRAG_L;
+constexpr int64_t PACKED_VJP_BYTES = PKV_END * 2;
+
 // K order of every chained layer: within each block of 32 inputs, k-slot 8g + e of a B fragment
 // (lane group g = l / 16, element e) is the neuron that lane group already holds from the
 // previous layer's two 16x16 output tiles 2j and 2j + 1 (rows 4g + r of each):

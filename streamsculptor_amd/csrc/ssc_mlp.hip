@@ -282,7 +282,48 @@ __global__ void pack_kernel(const float* __restrict__ p, uint16_t* __restrict__ 
   }
 }
 
+// Packed transposes of the input vector-Jacobian product (layout in ssc_mlp.h).
+__global__ void pack_vjp_kernel(const float* __restrict__ p, uint16_t* __restrict__ pkv) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= PKV_END) return;
+  const int e = (int)(i & 7), l = (int)((i >> 3) & 63);
+  if (i < PKV_W3T) {
+    // W4^T: tile m, one k-step; A[16 m + l % 16][k] = W4[k][16 m + l % 16] for k < OUT
+    const int m = (int)(i >> 9);
+    const int row = 16 * m + (l & 15), k = 8 * (l >> 4) + e;
+    pkv[i] = k < OUT ? f2bf(p[OW4 + (int64_t)k * H + row]) : (uint16_t)0;
+    return;
+  }
+  int64_t r = i - PKV_W3T;
+  const float* W = p + OW3;
+  if (r >= FRAG_L) { r -= FRAG_L; W = p + OW2; }
+  const int64_t mj = r >> 9;
+  const int j = (int)(mj % (H / 32)), m = (int)(mj / (H / 32));
+  const int row = 16 * m + (l & 15);                    // input neuron of the layer
+  const int col = kperm(32 * j + 8 * (l >> 4) + e);      // output neuron of the layer
+  pkv[i] = f2bf(W[(int64_t)col * H + row]);
+}
+
 }  // namespace mlp
+
+__global__ __launch_bounds__(256) void mlp_vjp_kernel(const uint16_t* __restrict__ pk16,
+                                                      const float* __restrict__ pk32,
+                                                      const uint16_t* __restrict__ pkv,
+                                                      const float* __restrict__ x,
+                                                      const float* __restrict__ dy, int64_t nb,
+                                                      float* __restrict__ dx) {
+  const int64_t p = ((int64_t)blockIdx.x * 256 + threadIdx.x) / 64 * 16 + (threadIdx.x & 15);
+  const bool ok = p < nb;
+  const float x0 = ok ? x[3 * p] : 0.0f, x1 = ok ? x[3 * p + 1] : 0.0f, x2 = ok ? x[3 * p + 2] : 0.0f;
+  float d[4], o[3];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) d[j] = ok ? dy[4 * p + j] : 0.0f;
+  GlobalWeights gw{pk16, pkv};
+  mlp_tile_input_vjp(gw, pk32, x0, x1, x2, d, o);
+  if (ok && (threadIdx.x & 63) < 16)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) dx[3 * p + j] = o[j];
+}
 
 __global__ __launch_bounds__(256) void mlp_infer_kernel(const uint16_t* __restrict__ pk16,
                                                         const float* __restrict__ pk32,
@@ -507,6 +548,28 @@ int ssc_mlp_pack(const float* params, void* packed, void* stream) {
   hipLaunchKernelGGL(pack_kernel, dim3(nblk(2 * FRAG_L + FRAG_4)), dim3(256), 0, (hipStream_t)stream,
                      params, pk16, pk32);
   return check_launch("ssc_mlp_pack");
+}
+
+int64_t ssc_mlp_packed_vjp_bytes(void) { return PACKED_VJP_BYTES; }
+
+int ssc_mlp_pack_vjp(const float* params, void* packed_vjp, void* stream) {
+  if (!params || !packed_vjp) return fail(SSC_E_ARG, "ssc_mlp_pack_vjp: NULL pointer");
+  hipLaunchKernelGGL(pack_vjp_kernel, dim3(nblk(PKV_END)), dim3(256), 0, (hipStream_t)stream,
+                     params, (uint16_t*)packed_vjp);
+  return check_launch("ssc_mlp_pack_vjp");
+}
+
+int ssc_mlp_input_vjp(const void* packed, const void* packed_vjp, const float* x, const float* dy,
+                      int64_t batch, float* dx, void* stream) {
+  if (!packed || !packed_vjp || !x || !dy || !dx || batch < 0)
+    return fail(SSC_E_ARG, "ssc_mlp_input_vjp: bad argument");
+  if (batch == 0) return SSC_OK;
+  const uint16_t* pk16 = (const uint16_t*)packed;
+  const float* pk32 = (const float*)((const char*)packed + PK_F32_BASE_BYTES);
+  const int64_t blocks = (batch + 63) / 64;  // 4 waves x 16 points
+  hipLaunchKernelGGL(mlp_vjp_kernel, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream,
+                     pk16, pk32, (const uint16_t*)packed_vjp, x, dy, batch, dx);
+  return check_launch("ssc_mlp_input_vjp");
 }
 
 int ssc_mlp_infer(const void* packed, const float* x, int64_t batch, float* y, void* stream) {

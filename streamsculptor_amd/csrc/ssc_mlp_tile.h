@@ -33,14 +33,24 @@ __device__ __forceinline__ void pin(mlp::bf16x8& v) { asm volatile("" : "+v"(v))
 // Weight policies.  chunk(c), c = 0..3, returns the fragments of chunk c of the two chained
 // layers -- W2 tiles 0-7, W2 tiles 8-15, W3 tiles 0-7, W3 tiles 8-15 (64 KB each) -- and
 // out() the output layer's fragments.
+// The vector-Jacobian product also uses chunkT(c) -- W3^T tiles 0-7, W3^T 8-15, W2^T 0-7,
+// W2^T 8-15 -- and outT(), the W4^T fragments.
 struct GlobalWeights {
   const uint16_t* pk16;
+  const uint16_t* pkv = nullptr;  // packed transposes (input VJP only)
   __device__ const mlp::bf16x8* chunk(int c) const {
     const int64_t off = ((c >> 1) == 0 ? mlp::PK_W2 : mlp::PK_W3) + (c & 1) * (mlp::FRAG_L / 2);
     return reinterpret_cast<const mlp::bf16x8*>(pk16 + off);
   }
   __device__ const mlp::bf16x8* out() const {
     return reinterpret_cast<const mlp::bf16x8*>(pk16 + mlp::PK_W4);
+  }
+  __device__ const mlp::bf16x8* chunkT(int c) const {
+    const int64_t off = ((c >> 1) == 0 ? mlp::PKV_W3T : mlp::PKV_W2T) + (c & 1) * (mlp::FRAG_L / 2);
+    return reinterpret_cast<const mlp::bf16x8*>(pkv + off);
+  }
+  __device__ const mlp::bf16x8* outT() const {
+    return reinterpret_cast<const mlp::bf16x8*>(pkv + mlp::PKV_W4T);
   }
 };
 
@@ -124,6 +134,162 @@ __device__ __forceinline__ void mlp_tile_forward(Weights& wsrc, const float* __r
     out = __builtin_amdgcn_mfma_f32_16x16x32_bf16(W4[q * 64 + l], bin[q], out, 0, 0, 0);
 #pragma unroll
   for (int j = 0; j < OUT; ++j) y[j] = out[j];
+}
+
+// Input vector-Jacobian product dx = d(dy . MLP(x)) / dx for the 16 points of a wave (same lane
+// layout as mlp_tile_forward; dy[0..3] of point l % 16 in every lane of its column; dx in every
+// lane of the column).  The forward pass keeps the SiLU derivatives of its 3 x 64 neurons per
+// lane as BF16 fragments in k-slot order; the backward products W4^T, W3^T, W2^T chain through
+// the same layout as the forward (an output tile pair of one product is the next product's
+// B fragment), and the last product, with the fp32 W1, runs on the VALU.
+template <class Weights>
+__device__ __forceinline__ void mlp_tile_input_vjp(Weights& wsrc, const float* __restrict__ pk32,
+                                                   float x0, float x1, float x2, const float* dy,
+                                                   float* dx) {
+  using namespace mlp;
+  constexpr int KB = H / 32;
+  int l = threadIdx.x & 63;
+  asm volatile("" : "+v"(l));
+  const int g = l >> 4;
+  const float* w1 = pk32 + PK_W1;
+  bf16x8 bin[KB], sg1[KB], sg2[KB], sg3[KB];
+#pragma unroll
+  for (int j = 0; j < KB; ++j) {
+    float v[8], d[8];
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      const int n = kperm(32 * j + 8 * g + e);
+      float z = pk32[PK_B1 + n];
+      z = fmaf(x0, w1[3 * n], z);
+      z = fmaf(x1, w1[3 * n + 1], z);
+      z = fmaf(x2, w1[3 * n + 2], z);
+      v[e] = silu(z);
+      d[e] = silu_grad(z);
+    }
+    bin[j] = pack8(v);
+    sg1[j] = pack8(d);
+    pin(bin[j]);
+    pin(sg1[j]);
+  }
+#pragma unroll
+  for (int layer = 0; layer < 2; ++layer) {
+    const float* bias = pk32 + (layer == 0 ? PK_B2 : PK_B3);
+    bf16x8* sgo = layer == 0 ? sg2 : sg3;
+    bf16x8 bout[KB];
+#pragma unroll
+    for (int half = 0; half < 2; ++half) {
+      const bf16x8* W = wsrc.chunk(2 * layer + half);
+#pragma unroll
+      for (int jj = 0; jj < KB / 2; ++jj) {
+        const int j = (KB / 2) * half + jj;
+        f32x4 a0, a1;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          a0[r] = bias[32 * j + 4 * g + r];
+          a1[r] = bias[32 * j + 16 + 4 * g + r];
+        }
+#pragma unroll
+        for (int q = 0; q < KB; ++q) {
+          a0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(W[((2 * jj) * KB + q) * 64 + l], bin[q], a0,
+                                                       0, 0, 0);
+          a1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(W[((2 * jj + 1) * KB + q) * 64 + l], bin[q],
+                                                       a1, 0, 0, 0);
+        }
+        float v[8], d[8];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          v[r] = silu(a0[r]);
+          v[4 + r] = silu(a1[r]);
+          d[r] = silu_grad(a0[r]);
+          d[4 + r] = silu_grad(a1[r]);
+        }
+        bout[j] = pack8(v);
+        sgo[j] = pack8(d);
+        pin(bout[j]);
+        pin(sgo[j]);
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < KB; ++q) bin[q] = bout[q];
+  }
+  // backward: the output cotangent enters as k-slots 0..3 of lane group 0
+  bf16x8 dcur[KB];
+  {
+    float bv[8];
+#pragma unroll
+    for (int e = 0; e < 8; ++e) bv[e] = (g == 0 && e < OUT) ? dy[e] : 0.0f;
+    const bf16x8 b = pack8(bv);
+    const bf16x8* W4T = wsrc.outT();
+    const f32x4 zero = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int j = 0; j < KB; ++j) {
+      const f32x4 a0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(W4T[(2 * j) * 64 + l], b, zero, 0, 0, 0);
+      const f32x4 a1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(W4T[(2 * j + 1) * 64 + l], b, zero,
+                                                                0, 0, 0);
+      float v[8];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        v[r] = a0[r] * (float)sg3[j][r];
+        v[4 + r] = a1[r] * (float)sg3[j][4 + r];
+      }
+      dcur[j] = pack8(v);
+      pin(dcur[j]);
+    }
+  }
+  float gx = 0.0f, gy = 0.0f, gz = 0.0f;
+#pragma unroll
+  for (int layer = 0; layer < 2; ++layer) {
+    const bf16x8* sgl = layer == 0 ? sg2 : sg1;
+    bf16x8 dnext[KB];
+#pragma unroll
+    for (int half = 0; half < 2; ++half) {
+      const bf16x8* W = wsrc.chunkT(2 * layer + half);
+#pragma unroll
+      for (int jj = 0; jj < KB / 2; ++jj) {
+        const int j = (KB / 2) * half + jj;
+        f32x4 a0 = {0.0f, 0.0f, 0.0f, 0.0f}, a1 = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int q = 0; q < KB; ++q) {
+          a0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(W[((2 * jj) * KB + q) * 64 + l], dcur[q], a0,
+                                                       0, 0, 0);
+          a1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(W[((2 * jj + 1) * KB + q) * 64 + l], dcur[q],
+                                                       a1, 0, 0, 0);
+        }
+        float v[8];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          v[r] = a0[r] * (float)sgl[j][r];
+          v[4 + r] = a1[r] * (float)sgl[j][4 + r];
+        }
+        if (layer == 0) {
+          dnext[j] = pack8(v);
+          pin(dnext[j]);
+        } else {
+#pragma unroll
+          for (int e = 0; e < 8; ++e) {
+            const int n = kperm(32 * j + 8 * g + e);
+            gx = fmaf(v[e], w1[3 * n], gx);
+            gy = fmaf(v[e], w1[3 * n + 1], gy);
+            gz = fmaf(v[e], w1[3 * n + 2], gz);
+          }
+        }
+      }
+    }
+    if (layer == 0) {
+#pragma unroll
+      for (int q = 0; q < KB; ++q) dcur[q] = dnext[q];
+    }
+  }
+  // sum over the four lane groups of the column
+  gx += __shfl_xor(gx, 16, 64);
+  gy += __shfl_xor(gy, 16, 64);
+  gz += __shfl_xor(gz, 16, 64);
+  gx += __shfl_xor(gx, 32, 64);
+  gy += __shfl_xor(gy, 32, 64);
+  gz += __shfl_xor(gz, 32, 64);
+  dx[0] = gx;
+  dx[1] = gy;
+  dx[2] = gz;
 }
 
 }  // namespace ssc

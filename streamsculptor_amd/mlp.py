@@ -48,6 +48,7 @@ class MLP:
         self.step = 0
         self._ws, self._ws_n = None, 0
         self.packed = None
+        self.packed_vjp = None
 
     def _workspace(self, n):
         if self._ws is None or self._ws_n < n:
@@ -100,6 +101,7 @@ class MLP:
                   ctypes.c_float(beta2), ctypes.c_float(eps), self.step,
                   ctypes.c_float(weight_decay))
         self.packed = None
+        self.packed_vjp = None
 
     def pack(self):
         if self.packed is None:
@@ -107,6 +109,23 @@ class MLP:
             self.packed = torch.empty(nbytes, dtype=torch.uint8, device=self.params.device)
             _lib.call("ssc_mlp_pack", _lib.ptr(self.params), _lib.ptr(self.packed))
         return self.packed
+
+    def pack_vjp(self):
+        if self.packed_vjp is None:
+            nbytes = _lib.load().ssc_mlp_packed_vjp_bytes()
+            self.packed_vjp = torch.empty(nbytes, dtype=torch.uint8, device=self.params.device)
+            _lib.call("ssc_mlp_pack_vjp", _lib.ptr(self.params), _lib.ptr(self.packed_vjp))
+        return self.packed_vjp
+
+    def input_vjp(self, x, dy):
+        """d(sum(dy * y(x))) / dx with the packed, register-chained kernel (fp32 [n, 3])."""
+        x = _f32(x).reshape(-1, IN)
+        dy = _f32(dy).reshape(-1, OUT)
+        n = x.shape[0]
+        dx = torch.empty(n, IN, dtype=torch.float32, device=x.device)
+        _lib.call("ssc_mlp_input_vjp", _lib.ptr(self.pack()), _lib.ptr(self.pack_vjp()),
+                  _lib.ptr(x), _lib.ptr(dy), n, _lib.ptr(dx))
+        return dx
 
     def infer(self, x):
         """Inference with the packed, register-chained kernel."""

@@ -38,7 +38,7 @@ def test_library_exports_every_declared_symbol():
     # every declared compute entry point has a ctypes signature in the binding
     for n in _declared():
         if n not in ("ssc_last_error", "ssc_version", "ssc_mlp_num_params",
-                     "ssc_mlp_packed_bytes", "ssc_mlp_workspace_bytes",
+                     "ssc_mlp_packed_bytes", "ssc_mlp_packed_vjp_bytes", "ssc_mlp_workspace_bytes",
                      "ssc_adjoint_workspace_bytes"):
             assert n in _lib.SIGNATURES, n
 

@@ -101,3 +101,26 @@ def test_training_reduces_loss(ssc):
     final = float(((y.cpu().numpy() - t) ** 2).sum(1).mean())
     print(f"training: first batch loss {first:.3e}, final full loss {final:.3e}")
     assert final < 0.05 * first
+
+
+def test_input_vjp_matches_oracle(ssc):
+    """Register-chained input VJP (packed W4^T, W3^T, W2^T) against the BF16-emulating oracle:
+    random cotangents, each output alone (one Jacobian row per point: a transposed or permuted
+    W4^T fragment would mix the rows), and a ragged batch."""
+    from streamsculptor_amd.mlp import MLP
+    p, x, _, _ = _case(n=2500, seed=4)
+    rng = np.random.default_rng(5)
+    net = MLP(params=p)
+    dy = rng.normal(size=(len(x), 4)).astype(np.float32)
+    ref = OM.input_vjp(p, x, dy, emulate=True)
+    got = net.input_vjp(x, dy)
+    e = _nrel(got, ref)
+    print(f"input vjp: {e:.2e}")
+    assert e < 2e-3
+    for k in range(4):
+        ek = np.zeros_like(dy)
+        ek[:, k] = 1.0
+        ek_ref = OM.input_vjp(p, x, ek, emulate=True)
+        ek_got = net.input_vjp(x, ek)
+        assert _nrel(ek_got, ek_ref) < 2e-3, k
+    assert _nrel(net.input_vjp(x[:77], dy[:77]), ref[:77]) < 2e-3

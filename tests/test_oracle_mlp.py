@@ -58,3 +58,36 @@ def test_bf16_rounding_matches_torch():
     a = (rng.normal(size=10000) * 10 ** rng.uniform(-5, 5, 10000)).astype(np.float32)
     ref = torch.tensor(a).to(torch.bfloat16).to(torch.float32).numpy()
     np.testing.assert_array_equal(M.bf16(a), ref)
+
+
+def test_input_vjp_against_torch_and_fd():
+    """d(dy . MLP(x))/dx: plain float64 form = torch autograd; the BF16-emulating form stays
+    within BF16 rounding of it."""
+    rng = np.random.default_rng(4)
+    p = M.init_params(2).astype(np.float64) + rng.normal(scale=0.01, size=M.NPARAM)
+    x = rng.uniform(-1, 1, (48, 3))
+    dy = rng.normal(size=(48, 4))
+    P, o = {}, 0
+    for name, s in M.SHAPES:
+        n = int(np.prod(s))
+        P[name] = torch.tensor(p[o:o + n].reshape(s), dtype=torch.float64)
+        o += n
+    xt = torch.tensor(x, dtype=torch.float64, requires_grad=True)
+    h = xt
+    for li in (1, 2, 3):
+        h = torch.nn.functional.silu(h @ P[f"W{li}"].T + P[f"b{li}"])
+    y = h @ P["W4"].T + P["b4"]
+    (y * torch.tensor(dy)).sum().backward()
+    ref = xt.grad.numpy()
+    got = M.input_vjp(p, x, dy, emulate=False)
+    np.testing.assert_allclose(got, ref, rtol=1e-10, atol=1e-12 * np.abs(ref).max())
+    # central differences of dy . y along each input axis
+    e = 1e-6
+    for c in range(3):
+        xp, xm = x.copy(), x.copy()
+        xp[:, c] += e
+        xm[:, c] -= e
+        fd = ((M.forward(p, xp, emulate=False)[0] - M.forward(p, xm, emulate=False)[0]) * dy).sum(1) / (2 * e)
+        np.testing.assert_allclose(got[:, c], fd, rtol=1e-6, atol=1e-8)
+    em = M.input_vjp(p.astype(np.float32), x.astype(np.float32), dy, emulate=True)
+    assert np.linalg.norm(em - ref) / np.linalg.norm(ref) < 2e-2
