@@ -115,6 +115,7 @@ def _bind(L):
                              vp, vp, ctypes.c_int]
     L.orc_dp5_tableau.argtypes = [vp]
     L.orc_dp5_step.argtypes = [vp, vp, d, d, vp, vp, vp, vp, i32, vp]
+    L.orc_host_dgrad.argtypes = [vp, vp, i64, vp]
     return L
 
 
@@ -221,6 +222,15 @@ def evaluate(h, xyz, t=0.0, step=0, src=None):
     lib().orc_eval(ctypes.addressof(h), sp, xyz.ctypes.data, n, t, step,
                    phi.ctypes.data, grad.ctypes.data, hess.ctypes.data)
     return phi, grad, hess.reshape(n, 3, 3)
+
+
+def host_dgrad(h, xyz):
+    """(n, n_comp, 2, 3): d(grad Phi_c)/d(G m_c) and d(grad Phi_c)/d(p1_c) at each point."""
+    xyz = np.ascontiguousarray(np.atleast_2d(xyz), dtype=np.float64)
+    n = xyz.shape[0]
+    out = np.zeros((n, max(h.n_components, 1), 2, 3))
+    lib().orc_host_dgrad(ctypes.addressof(h), xyz.ctypes.data, n, out.ctypes.data)
+    return out[:, :h.n_components]
 
 
 def leapfrog(h, w0, t0, dt, nsteps, start=None, src=None, traj=False, nthreads=0, fast=False):

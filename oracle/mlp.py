@@ -117,19 +117,18 @@ def loss_and_grad(p, x, target, weight=None, emulate=True):
     return loss, flatten(g), y
 
 
-def input_vjp(p, x, dy, emulate=True):
+def input_vjp(p, x, dy, emulate=True, return_y=False):
     """dx = d(dy . y)/dx for y = MLP(x): the vector-Jacobian product with respect to the inputs
     (used by the adjoint through the surrogate).  emulate=True rounds where the GPU kernel does:
-    BF16 weights, BF16 output cotangents, SiLU derivatives stored in BF16, BF16 hidden-layer
-    cotangents between the matrix products; fp32 accumulation; the last product with the fp32
-    (BF16-valued) W1 in fp32."""
+    BF16 weights, fp32 output cotangents (the W4^T product runs in fp32), SiLU derivatives
+    stored in BF16, BF16 hidden-layer cotangents between the matrix products; the last product
+    with the fp32 (BF16-valued) W1.  return_y=True also returns the forward output."""
     y, c = forward(p, x, emulate)
     f = np.float64
     P = unflatten(np.asarray(p, dtype=np.float64 if not emulate else np.float32))
     rnd = bf16 if emulate else (lambda a: a)
-    dy = np.asarray(dy, dtype=f)
-    dyr = rnd(dy.astype(np.float32)).astype(f) if emulate else dy
-    dA = dyr @ rnd(P["W4"]).astype(f)
+    dy = np.asarray(dy, dtype=np.float32 if emulate else f).astype(f)
+    dA = dy @ rnd(P["W4"]).astype(f)
     for li, Wn in ((2, "W3"), (1, "W2")):
         sg = rnd(silu_grad(c["Z"][li]).astype(np.float32)).astype(f) if emulate else silu_grad(c["Z"][li])
         dZ = dA * sg
@@ -138,7 +137,8 @@ def input_vjp(p, x, dy, emulate=True):
         dA = dZ @ rnd(P[Wn]).astype(f)
     sg = rnd(silu_grad(c["Z"][0]).astype(np.float32)).astype(f) if emulate else silu_grad(c["Z"][0])
     dZ1 = dA * sg
-    return dZ1 @ rnd(P["W1"]).astype(f)
+    dx = dZ1 @ rnd(P["W1"]).astype(f)
+    return (dx, y) if return_y else dx
 
 
 def adam(p, g, m, v, step, lr, b1=0.9, b2=0.999, eps=1e-8, wd=0.0):

@@ -645,6 +645,26 @@ int orc_eval(const ssc_host_potential* hp, const ssc_sources* src, const double*
   return 0;
 }
 
+// Host-parameter partials of the gradient at n points: out[i][c] = (d grad Phi_c / d(G m_c),
+// d grad Phi_c / d p1_c), 6 doubles per component (the same partials the adjoint accumulates).
+int orc_host_dgrad(const ssc_host_potential* hp, const double* xyz, int64_t n, double* out) {
+  const int nc = hp->n_components;
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < n; ++i) {
+    for (int c = 0; c < nc; ++c) {
+      double gc[3] = {0, 0, 0}, dp[3];
+      comp_eval<double>(hp->c[c].kind, hp->c[c].p, xyz + 3 * i, nullptr, gc, nullptr);
+      comp_grad_dp1(hp->c[c].kind, hp->c[c].p, xyz + 3 * i, dp);
+      double* o = out + ((int64_t)i * nc + c) * 6;
+      for (int k = 0; k < 3; ++k) {
+        o[k] = gc[k] / hp->c[c].p[0];
+        o[3 + k] = dp[k];
+      }
+    }
+  }
+  return 0;
+}
+
 // Kick-drift-kick leapfrog on the grid t_k = t0 + k dt (Verlet 1967; Hairer, Lubich & Wanner,
 // Geometric Numerical Integration, sec. I.1.4).  The reference has no fixed-step solver: this
 // restates the north-star fixed-step replacement of integrate_orbit (main.py:125-163).

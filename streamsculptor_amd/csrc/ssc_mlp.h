@@ -46,16 +46,15 @@ constexpr int64_t PK_B4 = PK_B3 + H;
 constexpr int64_t PK_F32_END = PK_B4 + OUTP;
 constexpr int64_t PACKED_BYTES = PK_F32_BASE_BYTES + PK_F32_END * 4;
 
-// Packed transposes for the input vector-Jacobian product (BF16 element offsets):
-//   W4^T as 16 tiles of one 32-wide k-step (rows = layer-3 neurons, k < OUT real),
-//   W3^T and W2^T in the forward fragment order (rows = the layer's input neurons, k = its
-//   output neurons in kperm order).
-constexpr int64_t FRAG_4T = (int64_t)(H / 16) * 64 * 8;
-constexpr int64_t PKV_W4T = 0;
-constexpr int64_t PKV_W3T = PKV_W4T + FRAG_4T;
+// Packed transposes for the input vector-Jacobian product: W3^T and W2^T fragments in the
+// forward fragment order (rows = the layer's input neurons, k = its output neurons in kperm
+// order; BF16 element offsets), then W4^T as fp32 [H][OUT] (BF16-rounded values; this small
+// product runs on the VALU so the output cotangent stays fp32).
+constexpr int64_t PKV_W3T = 0;
 constexpr int64_t PKV_W2T = PKV_W3T + FRAG_L;
-constexpr int64_t PKV_END = PKV_W2T + FRAG_L;
-constexpr int64_t PACKED_VJP_BYTES = PKV_END * 2;
+constexpr int64_t PKV_BF16_END = PKV_W2T + FRAG_L;
+constexpr int64_t PKV_W4T_BYTES = PKV_BF16_END * 2;  // byte offset of the fp32 W4^T table
+constexpr int64_t PACKED_VJP_BYTES = PKV_W4T_BYTES + (int64_t)H * OUT * 4;
 
 // K order of every chained layer: within each block of 32 inputs, k-slot 8g + e of a B fragment
 // (lane group g = l / 16, element e) is the neuron that lane group already holds from the

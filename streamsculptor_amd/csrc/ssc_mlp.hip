@@ -283,17 +283,15 @@ __global__ void pack_kernel(const float* __restrict__ p, uint16_t* __restrict__ 
 }
 
 // Packed transposes of the input vector-Jacobian product (layout in ssc_mlp.h).
-__global__ void pack_vjp_kernel(const float* __restrict__ p, uint16_t* __restrict__ pkv) {
+__global__ void pack_vjp_kernel(const float* __restrict__ p, uint16_t* __restrict__ pkv,
+                                float* __restrict__ w4t) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= PKV_END) return;
-  const int e = (int)(i & 7), l = (int)((i >> 3) & 63);
-  if (i < PKV_W3T) {
-    // W4^T: tile m, one k-step; A[16 m + l % 16][k] = W4[k][16 m + l % 16] for k < OUT
-    const int m = (int)(i >> 9);
-    const int row = 16 * m + (l & 15), k = 8 * (l >> 4) + e;
-    pkv[i] = k < OUT ? f2bf(p[OW4 + (int64_t)k * H + row]) : (uint16_t)0;
-    return;
+  if (i < H * OUT) {  // W4^T [n][o] = W4[o][n], BF16-rounded values in fp32
+    const int n = (int)(i / OUT), o = (int)(i % OUT);
+    w4t[i] = bf2f(f2bf(p[OW4 + (int64_t)o * H + n]));
   }
+  if (i >= PKV_BF16_END) return;
+  const int e = (int)(i & 7), l = (int)((i >> 3) & 63);
   int64_t r = i - PKV_W3T;
   const float* W = p + OW3;
   if (r >= FRAG_L) { r -= FRAG_L; W = p + OW2; }
@@ -315,11 +313,11 @@ __global__ __launch_bounds__(256) void mlp_vjp_kernel(const uint16_t* __restrict
   const int64_t p = ((int64_t)blockIdx.x * 256 + threadIdx.x) / 64 * 16 + (threadIdx.x & 15);
   const bool ok = p < nb;
   const float x0 = ok ? x[3 * p] : 0.0f, x1 = ok ? x[3 * p + 1] : 0.0f, x2 = ok ? x[3 * p + 2] : 0.0f;
-  float d[4], o[3];
+  float d[4], yo[4], o[3];
 #pragma unroll
   for (int j = 0; j < 4; ++j) d[j] = ok ? dy[4 * p + j] : 0.0f;
   GlobalWeights gw{pk16, pkv};
-  mlp_tile_input_vjp(gw, pk32, x0, x1, x2, d, o);
+  mlp_tile_input_vjp(gw, pk32, x0, x1, x2, d, yo, o);
   if (ok && (threadIdx.x & 63) < 16)
 #pragma unroll
     for (int j = 0; j < 3; ++j) dx[3 * p + j] = o[j];
@@ -554,8 +552,8 @@ int64_t ssc_mlp_packed_vjp_bytes(void) { return PACKED_VJP_BYTES; }
 
 int ssc_mlp_pack_vjp(const float* params, void* packed_vjp, void* stream) {
   if (!params || !packed_vjp) return fail(SSC_E_ARG, "ssc_mlp_pack_vjp: NULL pointer");
-  hipLaunchKernelGGL(pack_vjp_kernel, dim3(nblk(PKV_END)), dim3(256), 0, (hipStream_t)stream,
-                     params, (uint16_t*)packed_vjp);
+  hipLaunchKernelGGL(pack_vjp_kernel, dim3(nblk(PKV_BF16_END)), dim3(256), 0, (hipStream_t)stream,
+                     params, (uint16_t*)packed_vjp, (float*)((char*)packed_vjp + PKV_W4T_BYTES));
   return check_launch("ssc_mlp_pack_vjp");
 }
 

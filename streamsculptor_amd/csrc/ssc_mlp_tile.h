@@ -34,7 +34,7 @@ __device__ __forceinline__ void pin(mlp::bf16x8& v) { asm volatile("" : "+v"(v))
 // layers -- W2 tiles 0-7, W2 tiles 8-15, W3 tiles 0-7, W3 tiles 8-15 (64 KB each) -- and
 // out() the output layer's fragments.
 // The vector-Jacobian product also uses chunkT(c) -- W3^T tiles 0-7, W3^T 8-15, W2^T 0-7,
-// W2^T 8-15 -- and outT(), the W4^T fragments.
+// W2^T 8-15 -- and w4t(), the fp32 W4^T table [H][OUT].
 struct GlobalWeights {
   const uint16_t* pk16;
   const uint16_t* pkv = nullptr;  // packed transposes (input VJP only)
@@ -49,8 +49,8 @@ struct GlobalWeights {
     const int64_t off = ((c >> 1) == 0 ? mlp::PKV_W3T : mlp::PKV_W2T) + (c & 1) * (mlp::FRAG_L / 2);
     return reinterpret_cast<const mlp::bf16x8*>(pkv + off);
   }
-  __device__ const mlp::bf16x8* outT() const {
-    return reinterpret_cast<const mlp::bf16x8*>(pkv + mlp::PKV_W4T);
+  __device__ const float* w4t() const {
+    return reinterpret_cast<const float*>(reinterpret_cast<const char*>(pkv) + mlp::PKV_W4T_BYTES);
   }
 };
 
@@ -136,16 +136,17 @@ __device__ __forceinline__ void mlp_tile_forward(Weights& wsrc, const float* __r
   for (int j = 0; j < OUT; ++j) y[j] = out[j];
 }
 
-// Input vector-Jacobian product dx = d(dy . MLP(x)) / dx for the 16 points of a wave (same lane
-// layout as mlp_tile_forward; dy[0..3] of point l % 16 in every lane of its column; dx in every
-// lane of the column).  The forward pass keeps the SiLU derivatives of its 3 x 64 neurons per
-// lane as BF16 fragments in k-slot order; the backward products W4^T, W3^T, W2^T chain through
-// the same layout as the forward (an output tile pair of one product is the next product's
-// B fragment), and the last product, with the fp32 W1, runs on the VALU.
+// Input vector-Jacobian product dx = d(dy . MLP(x)) / dx for the 16 points of a wave, together
+// with the forward output y (same lane layout as mlp_tile_forward: dy[0..3] of point l % 16 in
+// every lane of its column; y valid in lane group 0, dx in every lane of the column).  The
+// forward keeps the SiLU derivatives of its 3 x 64 neurons per lane as BF16 fragments in k-slot
+// order.  Backward: W4^T dy on the VALU in fp32 (each lane needs only its own 64 neurons), then
+// W3^T and W2^T on the MFMA chain in the forward's layout (an output tile pair of one product is
+// the next product's B fragment), and the last product, with the fp32 W1, on the VALU.
 template <class Weights>
 __device__ __forceinline__ void mlp_tile_input_vjp(Weights& wsrc, const float* __restrict__ pk32,
                                                    float x0, float x1, float x2, const float* dy,
-                                                   float* dx) {
+                                                   float* y, float* dx) {
   using namespace mlp;
   constexpr int KB = H / 32;
   int l = threadIdx.x & 63;
@@ -212,25 +213,32 @@ __device__ __forceinline__ void mlp_tile_input_vjp(Weights& wsrc, const float* _
 #pragma unroll
     for (int q = 0; q < KB; ++q) bin[q] = bout[q];
   }
-  // backward: the output cotangent enters as k-slots 0..3 of lane group 0
+  {  // forward output layer (rows 0..OUT-1 land in lane group 0)
+    const bf16x8* W4 = wsrc.out();
+    f32x4 out;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) out[r] = pk32[PK_B4 + 4 * g + r];
+#pragma unroll
+    for (int q = 0; q < KB; ++q)
+      out = __builtin_amdgcn_mfma_f32_16x16x32_bf16(W4[q * 64 + l], bin[q], out, 0, 0, 0);
+#pragma unroll
+    for (int j = 0; j < OUT; ++j) y[j] = out[j];
+  }
+  // backward: dZ3 = (W4^T dy) * silu'(Z3) for this lane's 64 neurons (fp32 VALU)
   bf16x8 dcur[KB];
   {
-    float bv[8];
-#pragma unroll
-    for (int e = 0; e < 8; ++e) bv[e] = (g == 0 && e < OUT) ? dy[e] : 0.0f;
-    const bf16x8 b = pack8(bv);
-    const bf16x8* W4T = wsrc.outT();
-    const f32x4 zero = {0.0f, 0.0f, 0.0f, 0.0f};
+    const float4* wt = reinterpret_cast<const float4*>(wsrc.w4t());
 #pragma unroll
     for (int j = 0; j < KB; ++j) {
-      const f32x4 a0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(W4T[(2 * j) * 64 + l], b, zero, 0, 0, 0);
-      const f32x4 a1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(W4T[(2 * j + 1) * 64 + l], b, zero,
-                                                                0, 0, 0);
       float v[8];
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        v[r] = a0[r] * (float)sg3[j][r];
-        v[4 + r] = a1[r] * (float)sg3[j][4 + r];
+      for (int e = 0; e < 8; ++e) {
+        const float4 w = wt[kperm(32 * j + 8 * g + e)];
+        float a = w.x * dy[0];
+        a = fmaf(w.y, dy[1], a);
+        a = fmaf(w.z, dy[2], a);
+        a = fmaf(w.w, dy[3], a);
+        v[e] = a * (float)sg3[j][e];
       }
       dcur[j] = pack8(v);
       pin(dcur[j]);
