@@ -281,6 +281,33 @@ int ssc_surrogate_leapfrog(const ssc_host_potential* host, const ssc_sources* cl
                            double dt, int32_t nsteps, double* w_out,
                            unsigned long long* fallback_count, void* stream);
 
+/* Discrete adjoint of ssc_surrogate_leapfrog (no counterpart in the reference: it replaces
+ * jax.grad through the stream integration when the clump force is the learned surrogate).
+ * Same model arguments as ssc_surrogate_leapfrog; the caller's device workspace holds the
+ * checkpoints (ceil(nsteps / checkpoint_every) + 1 rows of (x, v, a) per particle) between the two
+ * calls.  The forward also returns w_out.  The backward walks the leapfrog backwards from each
+ * checkpoint (the map is explicitly invertible) and applies the clump Hessian through the MLP
+ * input VJP (packed_vjp from ssc_mlp_pack_vjp), or the exact clump sum in flagged cells:
+ * lam_T[n][6] = dL/dw_out, forcing[(nsteps + 1)][n][6] (may be NULL) = dL/d(row k), outputs
+ * lam_0[n][6] = dL/d(w0) and g_host[n_components][2] = dL/d(G m_c), dL/d(p1_c) (host
+ * parameters).  fallback_count (device, may be NULL) accumulates the rows that used the exact
+ * sum. */
+int64_t ssc_surrogate_adjoint_workspace_bytes(int64_t n, int32_t nsteps, int32_t checkpoint_every);
+int ssc_surrogate_adjoint_forward(const ssc_host_potential* host, const ssc_sources* clumps,
+                                  const double* scaling, const void* packed, const uint8_t* flags,
+                                  const double* w0, const int32_t* start_step, int64_t n,
+                                  double t0, double dt, int32_t nsteps, int32_t checkpoint_every,
+                                  double* w_out, unsigned long long* fallback_count,
+                                  void* workspace, int64_t workspace_bytes, void* stream);
+int ssc_surrogate_adjoint_backward(const ssc_host_potential* host, const ssc_sources* clumps,
+                                   const double* scaling, const void* packed,
+                                   const void* packed_vjp, const uint8_t* flags,
+                                   const int32_t* start_step, int64_t n, double dt,
+                                   int32_t nsteps, int32_t checkpoint_every, const double* lam_T,
+                                   const double* forcing, double* lam_0, double* g_host,
+                                   unsigned long long* fallback_count, void* workspace,
+                                   int64_t workspace_bytes, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -70,6 +70,10 @@ SIGNATURES = {
     "ssc_adjoint_backward": [_vp, _vp, _vp, _i64, _d, _d, _i32, _i32, _vp, _vp, _vp, _vp, _vp, _vp,
                              _vp, _i64],
     "ssc_release": [_vp, _vp, _vp, _vp, _i32, ctypes.c_int64, _vp, _vp, _vp, _vp, _vp, _vp],
+    "ssc_surrogate_adjoint_forward": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _i64, _d, _d, _i32, _i32,
+                                      _vp, _vp, _vp, _i64],
+    "ssc_surrogate_adjoint_backward": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _i64, _d, _i32, _i32,
+                                       _vp, _vp, _vp, _vp, _vp, _vp, _i64],
 }
 
 _lib = None
@@ -95,6 +99,9 @@ def load():
             getattr(L, name).argtypes = []
         L.ssc_mlp_workspace_bytes.restype = ctypes.c_int64
         L.ssc_mlp_workspace_bytes.argtypes = [ctypes.c_int64]
+        L.ssc_surrogate_adjoint_workspace_bytes.restype = ctypes.c_int64
+        L.ssc_surrogate_adjoint_workspace_bytes.argtypes = [ctypes.c_int64, ctypes.c_int32,
+                                                            ctypes.c_int32]
         L.ssc_adjoint_workspace_bytes.restype = ctypes.c_int64
         L.ssc_adjoint_workspace_bytes.argtypes = [ctypes.c_int64, ctypes.c_int32, ctypes.c_int32,
                                                   ctypes.c_int32]

@@ -245,6 +245,48 @@ __device__ __forceinline__ void host_full(const ssc_host_potential& hp, double x
   H[7] = H[5];
 }
 
+// d(grad Phi_c)/d(p1) for one component (MN a, NFW r_s, Hernquist c, Plummer r_s).
+__device__ __forceinline__ void comp_grad_dp1(const ssc_component& c, double x, double y, double z,
+                                              double* out) {
+  const double gm = c.p[0];
+  out[0] = out[1] = out[2] = 0.0;
+  switch (c.kind) {
+    case SSC_COMP_MIYAMOTO_NAGAI: {
+      const double a = c.p[1], b = c.p[2];
+      const double zeta = sqrt(z * z + b * b), S = a + zeta;
+      const double D2 = x * x + y * y + S * S, D = sqrt(D2);
+      const double iD3 = 1.0 / (D2 * D), iD5 = iD3 / D2;
+      out[0] = -3.0 * gm * x * S * iD5;
+      out[1] = -3.0 * gm * y * S * iD5;
+      out[2] = gm * (z / zeta) * iD3 - 3.0 * gm * (z * S / zeta) * S * iD5;
+      break;
+    }
+    case SSC_COMP_NFW: {
+      const double rs = c.p[1];
+      const double r = sqrt(x * x + y * y + z * z), rp = r + rs;
+      const double f = gm * (-1.0 / (r * rs * rp) + 1.0 / (r * rp * rp)) / r;
+      out[0] = f * x; out[1] = f * y; out[2] = f * z;
+      break;
+    }
+    case SSC_COMP_HERNQUIST: {
+      const double cc = c.p[1], soft = c.p[2];
+      const double r = sqrt(x * x + y * y + z * z + soft), rc = r + cc;
+      const double f = -2.0 * gm / (r * rc * rc * rc);
+      out[0] = f * x; out[1] = f * y; out[2] = f * z;
+      break;
+    }
+    case SSC_COMP_PLUMMER: {
+      const double rs = c.p[1];
+      const double s2 = x * x + y * y + z * z + rs * rs, s = sqrt(s2);
+      const double f = -3.0 * gm * rs / (s2 * s2 * s);
+      out[0] = f * x; out[1] = f * y; out[2] = f * z;
+      break;
+    }
+    default:
+      break;
+  }
+}
+
 // Plummer source (x - s): gradient contribution, the N x M hot loop body.
 __device__ __forceinline__ void plummer_grad(double dx, double dy, double dz, double gm,
                                              double rs2, double& gx, double& gy, double& gz) {

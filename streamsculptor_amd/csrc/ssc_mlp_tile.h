@@ -139,10 +139,11 @@ __device__ __forceinline__ void mlp_tile_forward(Weights& wsrc, const float* __r
 // Input vector-Jacobian product dx = d(dy . MLP(x)) / dx for the 16 points of a wave, together
 // with the forward output y (same lane layout as mlp_tile_forward: dy[0..3] of point l % 16 in
 // every lane of its column; y valid in lane group 0, dx in every lane of the column).  The
-// forward keeps the SiLU derivatives of its 3 x 64 neurons per lane as BF16 fragments in k-slot
-// order.  Backward: W4^T dy on the VALU in fp32 (each lane needs only its own 64 neurons), then
-// W3^T and W2^T on the MFMA chain in the forward's layout (an output tile pair of one product is
-// the next product's B fragment), and the last product, with the fp32 W1, on the VALU.
+// forward keeps the SiLU derivatives of layers 2 and 3 (64 neurons per lane) as BF16 fragments
+// in k-slot order; layer 1's are recomputed in the last backward step (three FMAs a neuron).
+// Backward: W4^T dy on the VALU in fp32 (each lane needs only its own 64 neurons), then W3^T and
+// W2^T on the MFMA chain in the forward's layout (an output tile pair of one product is the next
+// product's B fragment), and the last product, with the fp32 W1, on the VALU.
 template <class Weights>
 __device__ __forceinline__ void mlp_tile_input_vjp(Weights& wsrc, const float* __restrict__ pk32,
                                                    float x0, float x1, float x2, const float* dy,
@@ -153,10 +154,10 @@ __device__ __forceinline__ void mlp_tile_input_vjp(Weights& wsrc, const float* _
   asm volatile("" : "+v"(l));
   const int g = l >> 4;
   const float* w1 = pk32 + PK_W1;
-  bf16x8 bin[KB], sg1[KB], sg2[KB], sg3[KB];
+  bf16x8 bin[KB], sg2[KB], sg3[KB];
 #pragma unroll
   for (int j = 0; j < KB; ++j) {
-    float v[8], d[8];
+    float v[8];
 #pragma unroll
     for (int e = 0; e < 8; ++e) {
       const int n = kperm(32 * j + 8 * g + e);
@@ -165,12 +166,9 @@ __device__ __forceinline__ void mlp_tile_input_vjp(Weights& wsrc, const float* _
       z = fmaf(x1, w1[3 * n + 1], z);
       z = fmaf(x2, w1[3 * n + 2], z);
       v[e] = silu(z);
-      d[e] = silu_grad(z);
     }
     bin[j] = pack8(v);
-    sg1[j] = pack8(d);
     pin(bin[j]);
-    pin(sg1[j]);
   }
 #pragma unroll
   for (int layer = 0; layer < 2; ++layer) {
@@ -245,9 +243,11 @@ __device__ __forceinline__ void mlp_tile_input_vjp(Weights& wsrc, const float* _
     }
   }
   float gx = 0.0f, gy = 0.0f, gz = 0.0f;
+  // opaque copies of the inputs: the layer-1 pre-activations are recomputed below rather than
+  // kept live (64 registers) from the forward
+  asm volatile("" : "+v"(x0), "+v"(x1), "+v"(x2));
 #pragma unroll
   for (int layer = 0; layer < 2; ++layer) {
-    const bf16x8* sgl = layer == 0 ? sg2 : sg1;
     bf16x8 dnext[KB];
 #pragma unroll
     for (int half = 0; half < 2; ++half) {
@@ -263,22 +263,28 @@ __device__ __forceinline__ void mlp_tile_input_vjp(Weights& wsrc, const float* _
           a1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(W[((2 * jj + 1) * KB + q) * 64 + l], dcur[q],
                                                        a1, 0, 0, 0);
         }
-        float v[8];
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          v[r] = a0[r] * (float)sgl[j][r];
-          v[4 + r] = a1[r] * (float)sgl[j][4 + r];
-        }
         if (layer == 0) {
+          float v[8];
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            v[r] = a0[r] * (float)sg2[j][r];
+            v[4 + r] = a1[r] * (float)sg2[j][4 + r];
+          }
           dnext[j] = pack8(v);
           pin(dnext[j]);
         } else {
 #pragma unroll
           for (int e = 0; e < 8; ++e) {
             const int n = kperm(32 * j + 8 * g + e);
-            gx = fmaf(v[e], w1[3 * n], gx);
-            gy = fmaf(v[e], w1[3 * n + 1], gy);
-            gz = fmaf(v[e], w1[3 * n + 2], gz);
+            float z = pk32[PK_B1 + n];
+            z = fmaf(x0, w1[3 * n], z);
+            z = fmaf(x1, w1[3 * n + 1], z);
+            z = fmaf(x2, w1[3 * n + 2], z);
+            const float sgr = bf2f(f2bf(silu_grad(z)));  // stored-BF16 semantics
+            const float v = (e < 4 ? a0[e] : a1[e - 4]) * sgr;
+            gx = fmaf(v, w1[3 * n], gx);
+            gy = fmaf(v, w1[3 * n + 1], gy);
+            gz = fmaf(v, w1[3 * n + 2], gz);
           }
         }
       }

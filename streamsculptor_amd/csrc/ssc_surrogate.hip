@@ -19,61 +19,24 @@
 #include "ssc_device.h"
 #include "ssc_mlp.h"
 #include "ssc_mlp_tile.h"
+#include "ssc_surrogate.h"
 
 namespace ssc {
-
-struct SurrogateDesc {
-  double center[3];
-  double inv_half[3];
-  double phi_scale;
-  double g_scale;
-  double grid_lo[3];
-  double inv_cell[3];
-  int32_t grid_n[3];
-  int32_t reserved;
-};
 
 // Block shape: 8 waves (two per SIMD, 256-register budget each), 16 particles per wave.
 constexpr int SG_BLOCK_THREADS = 512;
 
-// The two chained layers stream through two 64 KB LDS buffers in four chunks per step (W2 tiles
-// 0-7, W2 tiles 8-15, W3 tiles 0-7, W3 tiles 8-15).  Each thread holds its share (PIECES x
-// 16 B) of the next chunk in registers while the current chunk is computed, so the L2 latency
-// hides behind a chunk of MFMAs.  One barrier per chunk suffices: chunk c overwrites the buffer
-// that chunk c - 2 was read from, and every wave finished that read before it reached the
-// barrier of chunk c - 1, which the writer has passed.
-template <int SG_BLOCK>
-struct LdsWeights {
-  static constexpr int CH = (int)(mlp::FRAG_L / 2 / 8);  // 16-byte fragments per chunk
-  static constexpr int PIECES = CH / SG_BLOCK;
-  static_assert(CH % SG_BLOCK == 0, "chunk must split evenly over the block");
-  mlp::bf16x8* buf;        // two chunk buffers of CH fragments
-  const mlp::bf16x8* w4;   // output-layer fragments (resident)
-  const mlp::bf16x8* gw;   // global fragments of W2 followed by W3
-  mlp::bf16x8 pre[PIECES];
-  __device__ void prefetch(int c) {
-    const mlp::bf16x8* src = gw + (int64_t)c * CH;
-#pragma unroll
-    for (int k = 0; k < PIECES; ++k) pre[k] = src[threadIdx.x + k * SG_BLOCK];
-  }
-  __device__ const mlp::bf16x8* chunk(int c) {
-    mlp::bf16x8* dst = buf + (c & 1) * CH;
-#pragma unroll
-    for (int k = 0; k < PIECES; ++k) dst[threadIdx.x + k * SG_BLOCK] = pre[k];
-    __syncthreads();
-    prefetch((c + 1) & 3);
-    return dst;
-  }
-  __device__ const mlp::bf16x8* out() const { return w4; }
-};
 
-
-template <int SG_BLOCK>
+// CK: also write checkpoint rows ckpt[r][n][9] = (x, v, a) at grid rows min(r E, nsteps),
+// r = 0..ceil(nsteps / E) (a particle that has not started holds its initial row and the
+// acceleration there) for the discrete adjoint.
+template <int SG_BLOCK, bool CK>
 __global__ __launch_bounds__(SG_BLOCK) void surrogate_leapfrog_kernel(
     ssc_host_potential hp, ssc_sources clumps, SurrogateDesc sd, const uint16_t* __restrict__ pk16,
     const float* __restrict__ pk32g, const uint8_t* __restrict__ flags,
     const double* __restrict__ w0, const int32_t* __restrict__ start, int64_t n, double t0,
-    double dt, int nsteps, double* __restrict__ w_out, unsigned long long* __restrict__ counters) {
+    double dt, int nsteps, double* __restrict__ w_out, unsigned long long* __restrict__ counters,
+    int E, double* __restrict__ ckpt) {
   using namespace mlp;
   constexpr int SG_PARTS = SG_BLOCK / 4;
   extern __shared__ __align__(16) char lds_raw[];
@@ -114,7 +77,8 @@ __global__ __launch_bounds__(SG_BLOCK) void surrogate_leapfrog_kernel(
   __syncthreads();
   const int kmin = kmin_sh;
 
-  LdsWeights<SG_BLOCK> lw{wbuf, w4, reinterpret_cast<const bf16x8*>(pk16 + PK_W2), {}};
+  LdsWeights<SG_BLOCK, 4> lw{wbuf, w4, nullptr, reinterpret_cast<const bf16x8*>(pk16 + PK_W2),
+                             nullptr, {}};
   lw.prefetch(0);
   unsigned long long nfall = 0;
 
@@ -130,16 +94,7 @@ __global__ __launch_bounds__(SG_BLOCK) void surrogate_leapfrog_kernel(
                 o3 = __shfl(o[3], l & 15, 64);
     double gx = sd.g_scale * (double)o1, gy = sd.g_scale * (double)o2, gz = sd.g_scale * (double)o3;
     // fallback test: outside the training box or a flagged cell
-    const double cx = (x - sd.grid_lo[0]) * sd.inv_cell[0];
-    const double cy = (y - sd.grid_lo[1]) * sd.inv_cell[1];
-    const double cz = (z - sd.grid_lo[2]) * sd.inv_cell[2];
-    bool fb = !(cx >= 0.0 && cy >= 0.0 && cz >= 0.0 && cx < sd.grid_n[0] && cy < sd.grid_n[1] &&
-                cz < sd.grid_n[2]);
-    if (!fb) {
-      const int64_t cell = ((int64_t)cx * sd.grid_n[1] + (int64_t)cy) * sd.grid_n[2] + (int64_t)cz;
-      fb = flags[cell] != 0;
-    }
-    fb = fb && valid;
+    const bool fb = valid && surrogate_fallback(sd, flags, x, y, z);
     if (__syncthreads_or(fb)) {
       int slot = -1;
       if (fb && l < 16) {
@@ -187,7 +142,17 @@ __global__ __launch_bounds__(SG_BLOCK) void surrogate_leapfrog_kernel(
 
   const double h = 0.5 * dt;
   double ax = 0.0, ay = 0.0, az = 0.0;
+  const int nrow = CK ? (nsteps + E - 1) / E : 0;  // last checkpoint row (grid row nsteps)
+  auto save = [&](int r) {
+    if (owner) {
+      double* o = ckpt + ((int64_t)r * n + i) * 9;
+      o[0] = x; o[1] = y; o[2] = z; o[3] = vx; o[4] = vy; o[5] = vz;
+      o[6] = ax; o[7] = ay; o[8] = az;
+    }
+  };
   if (kmin < nsteps) force(ax, ay, az);
+  if (CK)  // rows at or before the block's first step: nothing has moved yet
+    for (int r = 0; r <= nrow && (r == nrow ? nsteps : r * E) <= kmin; ++r) save(r);
   for (int k = kmin; k < nsteps; ++k) {
     const bool act = k >= s;
     if (act) {
@@ -198,6 +163,7 @@ __global__ __launch_bounds__(SG_BLOCK) void surrogate_leapfrog_kernel(
     if (act) {
       vx += h * ax; vy += h * ay; vz += h * az;
     }
+    if (CK && ((k + 1) % E == 0 || k + 1 == nsteps)) save(k + 1 == nsteps ? nrow : (k + 1) / E);
   }
   if (owner) {
     double* o = w_out + 6 * i;
@@ -210,48 +176,60 @@ __global__ __launch_bounds__(SG_BLOCK) void surrogate_leapfrog_kernel(
 
 using namespace ssc;
 
+namespace ssc {
+
+// Launch of the surrogate leapfrog (checkpoints when ckpt != NULL); arguments validated here.
+int launch_surrogate_leapfrog(const ssc_host_potential* host, const ssc_sources* clumps,
+                              const double* scaling, const void* packed, const uint8_t* flags,
+                              const double* w0, const int32_t* start_step, int64_t n, double t0,
+                              double dt, int32_t nsteps, double* w_out,
+                              unsigned long long* fallback_count, int E, double* ckpt,
+                              hipStream_t stream, const char* who) {
+  int rc = validate_host(host);
+  if (rc) return rc;
+  if (!clumps || clumps->kind != SSC_SRC_STATIC || clumps->n <= 0)
+    return fail(SSC_E_ARG, std::string(who) + ": needs the STATIC clump table for the fallback");
+  rc = validate_sources(clumps, nsteps);
+  if (rc) return rc;
+  if (!scaling || !packed || !flags) return fail(SSC_E_ARG, std::string(who) + ": NULL argument");
+  if (n < 0 || nsteps < 0) return fail(SSC_E_ARG, std::string(who) + ": negative size");
+  if (n == 0) return SSC_OK;
+  if (!w0 || !w_out) return fail(SSC_E_ARG, std::string(who) + ": NULL state");
+  SurrogateDesc sd;
+  rc = parse_scaling(scaling, sd, who);
+  if (rc) return rc;
+  const size_t lds = (size_t)(mlp::FRAG_L + mlp::FRAG_4) * 2 + (size_t)mlp::PK_F32_END * 4;
+  const uint16_t* pk16 = (const uint16_t*)packed;
+  const float* pk32 = (const float*)((const char*)packed + mlp::PK_F32_BASE_BYTES);
+  constexpr int B = SG_BLOCK_THREADS;
+  const void* kern = ckpt ? (const void*)surrogate_leapfrog_kernel<B, true>
+                          : (const void*)surrogate_leapfrog_kernel<B, false>;
+  hipError_t e = hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  if (e != hipSuccess)
+    return fail(SSC_E_HIP, std::string("hipFuncSetAttribute: ") + hipGetErrorString(e));
+  const int64_t parts = B / 4;
+  const int64_t blocks = (n + parts - 1) / parts;
+  if (blocks > (int64_t)INT_MAX) return fail(SSC_E_LIMIT, std::string(who) + ": too many particles");
+  if (ckpt)
+    hipLaunchKernelGGL((surrogate_leapfrog_kernel<B, true>), dim3((unsigned)blocks), dim3(B), lds,
+                       stream, *host, *clumps, sd, pk16, pk32, flags, w0, start_step, n, t0, dt,
+                       nsteps, w_out, fallback_count, E, ckpt);
+  else
+    hipLaunchKernelGGL((surrogate_leapfrog_kernel<B, false>), dim3((unsigned)blocks), dim3(B), lds,
+                       stream, *host, *clumps, sd, pk16, pk32, flags, w0, start_step, n, t0, dt,
+                       nsteps, w_out, fallback_count, 1, nullptr);
+  return check_launch(who);
+}
+
+}  // namespace ssc
+
 extern "C" int ssc_surrogate_leapfrog(const ssc_host_potential* host, const ssc_sources* clumps,
                                       const double* scaling, const void* packed,
                                       const uint8_t* flags, const double* w0,
                                       const int32_t* start_step, int64_t n, double t0, double dt,
                                       int32_t nsteps, double* w_out,
                                       unsigned long long* fallback_count, void* stream) {
-  int rc = validate_host(host);
-  if (rc) return rc;
-  if (!clumps || clumps->kind != SSC_SRC_STATIC || clumps->n <= 0)
-    return fail(SSC_E_ARG, "ssc_surrogate_leapfrog: needs the STATIC clump table for the fallback");
-  rc = validate_sources(clumps, nsteps);
-  if (rc) return rc;
-  if (!scaling || !packed || !flags) return fail(SSC_E_ARG, "ssc_surrogate_leapfrog: NULL argument");
-  if (n < 0 || nsteps < 0) return fail(SSC_E_ARG, "ssc_surrogate_leapfrog: negative size");
-  if (n == 0) return SSC_OK;
-  if (!w0 || !w_out) return fail(SSC_E_ARG, "ssc_surrogate_leapfrog: NULL state");
-  SurrogateDesc sd;
-  for (int k = 0; k < 3; ++k) {
-    sd.center[k] = scaling[k];
-    sd.inv_half[k] = 1.0 / scaling[3 + k];
-    sd.grid_lo[k] = scaling[8 + k];
-    sd.inv_cell[k] = 1.0 / scaling[11 + k];
-    sd.grid_n[k] = (int32_t)scaling[14 + k];
-    if (!(scaling[3 + k] > 0) || !(scaling[11 + k] > 0) || sd.grid_n[k] < 1)
-      return fail(SSC_E_ARG, "ssc_surrogate_leapfrog: bad scaling / grid");
-  }
-  sd.phi_scale = scaling[6];
-  sd.g_scale = scaling[7];
-  sd.reserved = 0;
-  const size_t lds = (size_t)(mlp::FRAG_L + mlp::FRAG_4) * 2 + (size_t)mlp::PK_F32_END * 4;
-  const uint16_t* pk16 = (const uint16_t*)packed;
-  const float* pk32 = (const float*)((const char*)packed + mlp::PK_F32_BASE_BYTES);
-  constexpr int B = SG_BLOCK_THREADS;
-  hipError_t e = hipFuncSetAttribute((const void*)surrogate_leapfrog_kernel<B>,
-                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-  if (e != hipSuccess)
-    return fail(SSC_E_HIP, std::string("hipFuncSetAttribute: ") + hipGetErrorString(e));
-  const int64_t parts = B / 4;
-  const int64_t blocks = (n + parts - 1) / parts;
-  if (blocks > (int64_t)INT_MAX) return fail(SSC_E_LIMIT, "ssc_surrogate_leapfrog: too many particles");
-  hipLaunchKernelGGL(surrogate_leapfrog_kernel<B>, dim3((unsigned)blocks), dim3(B), lds,
-                     (hipStream_t)stream, *host, *clumps, sd, pk16, pk32, flags, w0, start_step, n,
-                     t0, dt, nsteps, w_out, fallback_count);
-  return check_launch("ssc_surrogate_leapfrog");
+  return launch_surrogate_leapfrog(host, clumps, scaling, packed, flags, w0, start_step, n, t0, dt,
+                                   nsteps, w_out, fallback_count, 1, nullptr,
+                                   (hipStream_t)stream, "ssc_surrogate_leapfrog");
 }

@@ -101,6 +101,10 @@ class ClumpSurrogate:
                   _lib.ptr(out), _lib.ptr(cnt))
         return out, cnt
 
+    def adjoint(self, w0, t0, dt, nsteps, start=None, checkpoint_every=100, flags=None):
+        """Checkpointed discrete adjoint of ``leapfrog`` (SurrogateAdjoint)."""
+        return SurrogateAdjoint(self, w0, t0, dt, nsteps, start, checkpoint_every, flags)
+
     # ------------------------------------------------------------------ training
     @classmethod
     def train(cls, host, clumps, positions, n_train=1 << 22, n_val=1 << 18, batch=8192,
@@ -173,6 +177,57 @@ class ClumpSurrogate:
                            val_residual_max=float(res.max().item()),
                            flagged_cells=int(flags.sum().item()), cells=n ** 3,
                            val_fallback_fraction=float(fb.double().mean().item()))
+
+
+class SurrogateAdjoint:
+    """Forward (with checkpoints every ``checkpoint_every`` steps) and backward of the surrogate
+    leapfrog for rows w0 (ssc_surrogate_adjoint_forward / _backward).  The backward returns
+    dL/dw0 and dL/d(host parameters) for dL/dw_final = lam_T and optional per-row forcing."""
+
+    def __init__(self, sur, w0, t0, dt, nsteps, start=None, checkpoint_every=100, flags=None):
+        self.sur = sur
+        self.h = sur.host._host_only()
+        self.src = sur.clumps._kernel_sources()   # keeps the descriptor's tensors alive
+        self.sd = self.src.desc()
+        self.w0 = _lib.f64(w0).reshape(-1, 6).contiguous()
+        self.n = self.w0.shape[0]
+        self.start = None if start is None else _lib.i32(start)
+        self.t0, self.dt, self.nsteps, self.E = float(t0), float(dt), int(nsteps), int(checkpoint_every)
+        fl = sur.flags if flags is None else flags
+        self.flags = fl.to(device=self.w0.device, dtype=torch.uint8).contiguous()
+        self.sv = np.ascontiguousarray(sur.scaling_vector())
+        nbytes = _lib.load().ssc_surrogate_adjoint_workspace_bytes(self.n, self.nsteps, self.E)
+        self.ws = torch.empty(max(nbytes, 1), dtype=torch.uint8, device=self.w0.device)
+        self.w_out = None
+
+    def _svp(self):
+        return self.sv.ctypes.data_as(_lib.ctypes.c_void_p)
+
+    def forward(self):
+        self.w_out = torch.empty_like(self.w0)
+        cnt = torch.zeros(1, dtype=torch.int64, device=self.w0.device)
+        _lib.call("ssc_surrogate_adjoint_forward", _lib.ctypes.byref(self.h),
+                  _lib.ctypes.byref(self.sd), self._svp(), _lib.ptr(self.sur.mlp.pack()),
+                  _lib.ptr(self.flags), _lib.ptr(self.w0), _lib.ptr(self.start), self.n, self.t0,
+                  self.dt, self.nsteps, self.E, _lib.ptr(self.w_out), _lib.ptr(cnt),
+                  _lib.ptr(self.ws), self.ws.numel())
+        self.fallback_forward = cnt
+        return self.w_out
+
+    def backward(self, lam_T, forcing=None):
+        lam_T = _lib.f64(lam_T).reshape(self.n, 6).contiguous()
+        F = None if forcing is None else _lib.f64(forcing).contiguous()
+        lam0 = torch.empty_like(lam_T)
+        nc = self.h.n_components
+        gh = torch.zeros(max(nc, 1), 2, dtype=torch.float64, device=lam_T.device)
+        cnt = torch.zeros(1, dtype=torch.int64, device=lam_T.device)
+        mlp = self.sur.mlp
+        _lib.call("ssc_surrogate_adjoint_backward", _lib.ctypes.byref(self.h),
+                  _lib.ctypes.byref(self.sd), self._svp(), _lib.ptr(mlp.pack()),
+                  _lib.ptr(mlp.pack_vjp()), _lib.ptr(self.flags), _lib.ptr(self.start), self.n,
+                  self.dt, self.nsteps, self.E, _lib.ptr(lam_T), _lib.ptr(F), _lib.ptr(lam0),
+                  _lib.ptr(gh), _lib.ptr(cnt), _lib.ptr(self.ws), self.ws.numel())
+        return dict(lam0=lam0, g_host=gh[:nc], fallback=cnt)
 
 
 def _eval_chunked(pot, xyz, chunk=1 << 20):
