@@ -17,7 +17,9 @@ adjoint (the subhalos stream's loss and its gradient with respect to the halo ma
 the 64 subhalo masses and the progenitor row: forward with checkpoints every 100 steps plus
 the full reverse chain), gd1 (a batch of 700 perturbed GD-1-like streams with adaptive Dopri5,
 one straight-line subhalo each: the workload of the reference's GPU example, reported as
-simulations/s and particle-steps/s).
+simulations/s and particle-steps/s), ensemble (the clumps potential and surrogate with 2000
+progenitors of 2-6 Gyr streams, 4e6 particles: loss + gradient with respect to the halo and
+every progenitor's initial row through the surrogate adjoint, checkpoints every 100 steps).
 
 Prints ONE JSON line on stdout; progress goes to stderr.
 """
@@ -306,13 +308,126 @@ def gd1_main(args):
     print(json.dumps(res), flush=True)
 
 
+def ensemble_main(args):
+    """Config: the clumps potential and surrogate with a differentiable ensemble of streams
+    from many progenitors of different ages (loss + gradient through the surrogate adjoint)."""
+    import numpy as np
+    import torch
+    from streamsculptor_amd import flops
+    from streamsculptor_amd import workloads as W
+    from streamsculptor_amd.ensemble import SurrogateStreamEnsemble, ensemble_loss
+    from streamsculptor_amd.main import usys
+    from streamsculptor_amd.potential import PlummerClumps
+    from streamsculptor_amd.surrogate import ClumpSurrogate
+    S = args.streams
+    lengths, prog = W.ensemble_setup(S)
+    host = W.host_potential()
+    cpos, cm, crs = W.clump_table()
+    clumps = PlummerClumps(cm, crs, cpos, units=usys)
+    ens = SurrogateStreamEnsemble(None, lengths, n_release=args.releases, msat=W.MSAT, seed=0,
+                                  checkpoint_every=100, clumps=clumps)
+    mh, rsh = W.HOST["m_halo"], W.HOST["rs_halo"]
+    ps = ens.particle_steps
+    log(f"ensemble: {S} progenitors (lengths {lengths.min()}-{lengths.max()} steps), "
+        f"{ens.n} particles, {ps} particle-steps per loss")
+    t0 = time.perf_counter()
+    P = ens.training_positions(ens.host(mh, rsh), prog)
+    sur = ClumpSurrogate.train(host, clumps, P, iters=args.train_iters, log=log)
+    ens.sur = sur
+    torch.cuda.synchronize()
+    t_train = time.perf_counter() - t0
+    rep = {k: v for k, v in sur.report.items() if k != "losses"}
+    log(f"surrogate trained on {P.shape[0]} ensemble positions in {t_train:.1f} s: {rep}")
+    del P
+    target = ens.forward(mh * 1.05, rsh * 0.97, prog)["wT"][:, :3].clone()
+    torch.cuda.synchronize()
+
+    def step(ev, exact=False):
+        halo = torch.tensor([mh, rsh], dtype=torch.float64, device="cuda", requires_grad=True)
+        pw = torch.tensor(prog, dtype=torch.float64, device="cuda", requires_grad=True)
+        L = ensemble_loss(halo, pw, ens, target, exact=exact, events=ev)
+        L.backward()
+        return L.detach(), halo.grad, pw.grad
+
+    wall, kern, (L, gh, gw) = timed(step, args.steps, args.warmup, "ensemble loss + gradient")
+    fl = flops.mlp_input_vjp()
+    achieved = fl * ps / kern
+    res = {"metric": "particle-steps/s of loss + gradient through the surrogate (ensemble)",
+           "value": ps * args.steps / wall, "unit": "particle-steps/s", "n_gpus": 1,
+           "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1e3 * wall / args.steps,
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "bf16-mfma/f64", "data": "synthetic",
+           "config": {"workload": "surrogate ensemble adjoint", "progenitors": S,
+                      "stream_lengths": [int(lengths.min()), int(lengths.max())],
+                      "releases_per_stream": args.releases, "particles": ens.n,
+                      "particle_steps": ps, "checkpoint_every": 100, "clumps": 512,
+                      "gradients": "halo (m, r_s) and every progenitor's initial row"},
+           "roofline": {"bound": "mfma", "achieved": achieved / 1e12, "peak": PEAK_BF16 / 1e12,
+                        "unit": "TFLOP/s", "frac": achieved / PEAK_BF16, "traffic": None,
+                        "kernel": "surrogate_adjoint_segment_kernel (particles' reverse sweep)",
+                        "kernel_ms": kern * 1e3, "flop_per_particle_step": fl},
+           "train_seconds": t_train, "train": rep, "loss": float(L.item()),
+           "grad_halo": gh.tolist()}
+    log(f"ensemble: {res['value']:.3e} particle-steps/s (loss + gradient), reverse sweep "
+        f"{kern * 1e3:.0f} ms = {achieved / 1e12:.0f} TFLOP/s")
+    if not args.no_exact:
+        # the same loss and gradient with the exact clump sum for the particles (plain adjoint)
+        t1 = time.perf_counter()
+        Le, ghe, gwe = step(None, exact=True)
+        torch.cuda.synchronize()
+        t_exact = time.perf_counter() - t1
+        st_s = ens.forward(mh, rsh, prog)["wT"]
+        st_e = ens.forward(mh, rsh, prog, exact=True)["wT"]
+        med_pc = float(torch.linalg.norm(st_s[:, :3] - st_e[:, :3], dim=1).median().item()) * 1e3
+        rel_h = (torch.abs(gh - ghe) / torch.abs(ghe)).tolist()
+        rel_w = float((torch.linalg.norm(gw - gwe) / torch.linalg.norm(gwe)).item())
+        res["exact"] = {"seconds_loss_and_gradient": t_exact,
+                        "particle_steps_per_s": ps / t_exact, "loss": float(Le.item()),
+                        "grad_halo": ghe.tolist(), "median_final_position_err_pc": med_pc,
+                        "grad_halo_rel_diff": rel_h, "grad_progenitors_rel_diff": rel_w}
+        log(f"exact clump sum: {t_exact:.1f} s per loss + gradient; surrogate median position "
+            f"error {med_pc:.2f} pc, halo gradient rel. diff {rel_h}, progenitor rows {rel_w:.2e}")
+    if not args.no_cpu:
+        import threadpoolctl
+        import oracle as O
+        import oracle.surrogate as OS
+        sc = sur.scaling
+        h = O.host(W.host_components())
+        osrc = O.Sources(O.SRC_STATIC, gm=W.G * cm, rs=crs, pos=cpos)
+        om = OS.Model(h, osrc, sur.mlp.params.cpu().numpy(), sc.center, sc.half, sc.g_scale,
+                      sur.grid_lo, sur.grid_cell, sur.grid_n, sur.flags.cpu().numpy())
+        parts = ens.particles(ens.host(mh, rsh), prog)
+        sel = np.arange(0, ens.n, ens.n // args.cpu_particles)[:args.cpu_particles]
+        start = np.concatenate([ens.rel_step.reshape(-1)] * 2)[sel]
+        w0s = parts["w0"][torch.as_tensor(sel, device="cuda")].cpu().numpy()
+        lam = np.random.default_rng(0).normal(size=(len(sel), 6))
+        with threadpoolctl.threadpool_limits(1):
+            tc = time.perf_counter()
+            OS.adjoint(om, w0s, 1.0, ens.T, lam, start=start)
+            el = time.perf_counter() - tc
+        psteps = int((ens.T - start).sum())
+        res["cpu_baseline"] = {"value": psteps / el, "unit": "particle-steps/s", "cores": 1,
+                               "kind": "port",
+                               "sample": f"{len(sel)} particles of the ensemble (every "
+                                         f"{ens.n // args.cpu_particles}th), forward + adjoint "
+                                         f"through the oracle surrogate (numpy, BF16 emulation), "
+                                         f"{psteps} particle-steps, {el:.1f} s"}
+        log(f"cpu baseline {psteps / el:.3e} particle-steps/s (1 core)")
+    print(json.dumps(res), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="clumps",
-                    choices=["clumps", "subhalos", "responses", "adjoint", "plain", "gd1"])
+                    choices=["clumps", "subhalos", "responses", "adjoint", "plain", "gd1",
+                             "ensemble"])
+    ap.add_argument("--streams", type=int, default=2000, help="ensemble: progenitors")
+    ap.add_argument("--releases", type=int, default=1000, help="ensemble: release steps each")
+    ap.add_argument("--cpu-particles", type=int, default=8, help="ensemble: CPU sample size")
+    ap.add_argument("--no-exact", action="store_true", help="ensemble: skip the exact run")
     ap.add_argument("--sims", type=int, default=700, help="gd1: simulations per batch")
     ap.add_argument("--cpu-sims", type=int, default=4, help="gd1: simulations in the CPU sample")
     ap.add_argument("--particles", type=int, default=None)
@@ -334,6 +449,9 @@ def main():
     log(f"device {torch.cuda.get_device_name(0)}; building workload {args.workload}")
     if args.workload == "gd1":
         gd1_main(args)
+        return
+    if args.workload == "ensemble":
+        ensemble_main(args)
         return
     wl = build_workload(args.workload, args.particles)
     ps = wl["psteps"]

@@ -31,12 +31,14 @@ class SurrogateStreamEnsemble:
     """Fixed inputs of the ensemble: the trained ``ClumpSurrogate`` (its host is replaced by
     the halo parameters of each call), per-progenitor stream lengths in grid steps,
     ``n_release`` release steps per progenitor (two particles each), Msat and the release
-    seed."""
+    seed.  The progenitor rows are given at each progenitor's own first grid row
+    (t = -length dt)."""
 
     def __init__(self, sur, lengths, n_release=1000, msat=1e4, seed=0, dt=1.0, halo_index=2,
-                 base_components=None, checkpoint_every=100, kval_arr=1.0):
+                 base_components=None, checkpoint_every=100, kval_arr=1.0, clumps=None):
         from . import workloads as W
-        self.sur = sur
+        self.sur = sur            # may be set after construction (e.g. trained on this ensemble)
+        self.clumps = clumps      # clump Potential when sur is None
         L = np.asarray(lengths, dtype=np.int64)
         if (L < n_release).any():
             raise ValueError("every stream needs at least n_release steps")
@@ -84,11 +86,12 @@ class SurrogateStreamEnsemble:
         s.host = host
         return s
 
-    def forward(self, m_halo, rs_halo, prog_w0):
-        """Final particle rows (in ensemble order: lead particles of every (stream, release),
-        then the trail particles) and everything the backward pass needs."""
-        host = self.host(m_halo, rs_halo)
-        src = self.sur.clumps._kernel_sources()
+    def particles(self, host, prog_w0, want_positions=False):
+        """Progenitor orbits (host + exact clump sum, every grid row kept) and the released
+        particle rows in ensemble order: the lead particle of every (stream, release), then
+        the trail particles.  want_positions adds the progenitors' positions at every row
+        after their starts."""
+        src = self.sur_clumps()._kernel_sources()
         total = _with_sources(host, src)
         prog_w0 = _lib.f64(prog_w0).reshape(self.S, 6).contiguous()
         dev = prog_w0.device
@@ -98,28 +101,71 @@ class SurrogateStreamEnsemble:
         rows = torch.as_tensor(self.rel_step.reshape(-1), device=dev)
         sidx = torch.arange(self.S, device=dev).repeat_interleave(self.R)
         prog_rows = ys[rows, sidx].contiguous()                              # (S R, 6)
+        prog_pos = None
+        if want_positions:
+            keep = (torch.arange(T + 1, device=dev)[:, None]
+                    >= torch.as_tensor(self.prog_start, device=dev)[None, :])
+            prog_pos = ys[..., :3][keep]
         del ys
         rel_idx = torch.arange(self.R, dtype=torch.int32, device=dev).repeat(self.S)
         gms = torch.full((self.S * self.R,), G * self.msat, dtype=torch.float64, device=dev)
         pc, pf, vc, vf = host._release(prog_rows, rel_idx, gms, self.seed, self.kval_arr)
         w0 = torch.cat([torch.cat([pc, vc], 1), torch.cat([pf, vf], 1)], 0)
-        order = torch.as_tensor(self.order, device=dev)
-        sadj = self._surrogate_for(host).adjoint(w0[order], self.t0, self.dt, T,
-                                                 start=self.start_sorted,
-                                                 checkpoint_every=self.E)
-        wT_sorted = sadj.forward()
+        return dict(src=src, total=total, prog_w0=prog_w0, prog_rows=prog_rows, rows=rows,
+                    sidx=sidx, rel_idx=rel_idx, gms=gms, w0=w0, prog_pos=prog_pos)
+
+    def sur_clumps(self):
+        return self.clumps if self.sur is None else self.sur.clumps
+
+    def training_positions(self, host, prog_w0, n_sample=1 << 16, stride=16, seed=0):
+        """Positions along plain-method (exact clump sum) orbits of the ensemble: every row of
+        every progenitor orbit and every ``stride``-th row of ``n_sample`` random particles."""
+        from .surrogate import orbit_positions
+        P = self.particles(host, prog_w0, want_positions=True)
+        rng = np.random.default_rng(seed)
+        sel = np.sort(rng.choice(self.n, size=min(n_sample, self.n), replace=False))
+        start = np.concatenate([self.rel_step.reshape(-1)] * 2).astype(np.int32)
+        pos = orbit_positions(P["total"], P["w0"][torch.as_tensor(sel, device=P["w0"].device)],
+                              start[sel], self.t0, self.dt, self.T, stride=stride)
+        return torch.cat([P["prog_pos"], pos], 0)
+
+    def forward(self, m_halo, rs_halo, prog_w0, exact=False):
+        """Final particle rows (ensemble order) and everything the backward pass needs.
+        exact=True integrates the particles with the exact clump sum (plain adjoint) instead
+        of the surrogate."""
+        host = self.host(m_halo, rs_halo)
+        st = self.particles(host, prog_w0)
+        st.pop("prog_pos")
+        order = torch.as_tensor(self.order, device=st["w0"].device)
+        w0s = st.pop("w0")[order]
+        if exact:
+            adj = LeapfrogAdjoint(st["total"], w0s, self.t0, self.dt, self.T,
+                                  start=self.start_sorted, checkpoint_every=self.E, src=st["src"])
+        else:
+            adj = self._surrogate_for(host).adjoint(w0s, self.t0, self.dt, self.T,
+                                                    start=self.start_sorted,
+                                                    checkpoint_every=self.E)
+        del w0s
+        wT_sorted = adj.forward()
         wT = torch.empty_like(wT_sorted)
         wT[order] = wT_sorted
-        return dict(host=host, src=src, total=total, prog_rows=prog_rows, rel_idx=rel_idx,
-                    gms=gms, sadj=sadj, order=order, wT=wT, prog_w0=prog_w0, rows=rows,
-                    sidx=sidx)
+        st.update(host=host, padj=adj, exact=exact, order=order, wT=wT)
+        return st
 
-    def backward(self, st, lam_T):
-        """(dL/dm_halo, dL/drs_halo, dL/dprog_w0 (S, 6)) for dL/d(final rows) = lam_T."""
+    def backward(self, st, lam_T, events=None):
+        """(dL/dm_halo, dL/drs_halo, dL/dprog_w0 (S, 6)) for dL/d(final rows) = lam_T.
+        ``events`` (start, end) are recorded around the particles' adjoint."""
         T, S, R = self.T, self.S, self.R
         hi = self.halo_index
         order = st["order"]
-        A = st["sadj"].backward(lam_T[order])
+        if events:
+            events[0].record()
+        if st["exact"]:
+            A = st["padj"].backward(lam_T[order], want_sources=False)
+        else:
+            A = st["padj"].backward(lam_T[order])
+        if events:
+            events[1].record()
         lam0 = torch.empty_like(A["lam0"])
         lam0[order] = A["lam0"]
         g_host = A["g_host"].clone()
@@ -142,24 +188,26 @@ class SurrogateStreamEnsemble:
 
 class _EnsembleLoss(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, halo, prog_w0, model, target):
-        st = model.forward(halo[0], halo[1], prog_w0)
+    def forward(ctx, halo, prog_w0, model, target, exact, events):
+        st = model.forward(halo[0], halo[1], prog_w0, exact=exact)
         d = st["wT"][:, :3] - target
         loss = (d * d).sum()
-        ctx.model, ctx.st, ctx.d = model, st, d
+        ctx.model, ctx.st, ctx.d, ctx.events = model, st, d, events
         return loss
 
     @staticmethod
     def backward(ctx, gout):
         lam_T = torch.zeros_like(ctx.st["wT"])
         lam_T[:, :3] = 2.0 * ctx.d
-        gm_h, grs_h, g_w0 = ctx.model.backward(ctx.st, lam_T)
+        gm_h, grs_h, g_w0 = ctx.model.backward(ctx.st, lam_T, events=ctx.events)
+        ctx.st = None
         go = gout.to(torch.float64)
-        return torch.stack([gm_h, grs_h]) * go, g_w0 * go, None, None
+        return torch.stack([gm_h, grs_h]) * go, g_w0 * go, None, None, None, None
 
 
-def ensemble_loss(halo, prog_w0, model, target):
+def ensemble_loss(halo, prog_w0, model, target, exact=False, events=None):
     """L = sum_i |x_i(0) - target_i|^2 over the ensemble's particles, differentiable with respect
     to halo = (m_halo, r_s_halo) and the progenitors' initial rows (S, 6) (torch tensors on the
-    GPU)."""
-    return _EnsembleLoss.apply(halo, prog_w0, model, _lib.f64(target))
+    GPU).  exact=True: particles in the exact clump sum instead of the surrogate.  ``events``
+    (start, end) are recorded around the particles' adjoint in the backward pass."""
+    return _EnsembleLoss.apply(halo, prog_w0, model, _lib.f64(target), exact, events)

@@ -42,6 +42,13 @@ def mlp(widths=(3, 256, 256, 256, 4)) -> int:
     return sum(2 * a * b for a, b in zip(widths[:-1], widths[1:]))
 
 
+def mlp_input_vjp(widths=(3, 256, 256, 256, 4)) -> int:
+    """Forward plus the input vector-Jacobian product (the same products transposed), 2 FLOP
+    per multiply-accumulate (3-256-256-256-4: 531,456); the per-row work of the surrogate
+    adjoint's reverse sweep."""
+    return 2 * mlp(widths)
+
+
 def surrogate() -> int:
     """Surrogate particle-step: MLP in BF16 MFMA plus the FP64 host force and KDK (75)."""
     return plain(0)

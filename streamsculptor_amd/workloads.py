@@ -19,6 +19,10 @@
                      simulation has one straight-line Plummer subhalo (t_window = 100 Myr) drawn
                      from the example's prior and placed at impact parameter b from the
                      progenitor's orbit at the impact time (seed 8942).
+* ``ensemble``    -- the clumps potential with 2000 progenitors of stream lengths 2000-6000
+                     steps (ensemble_setup, seed 4), 1000 release steps each (2 particles per
+                     step): 4e6 particles on one 6000-step grid, differentiated with respect
+                     to the halo (m, r_s) and every progenitor's initial row.
 """
 from __future__ import annotations
 
@@ -135,6 +139,27 @@ def gd1_subhalo_prior(n, seed=8942):
     nv = rng.normal(size=(n, 3))
     nv /= np.linalg.norm(nv, axis=1, keepdims=True)
     return dict(m=m, a=a, t_impact=t_imp, b=b, v=v, dir_b=nb, dir_v=nv)
+
+
+def ensemble_setup(n_streams=2000, seed=4, lmin=2000, lmax=6000, rmin=10.0, rmax=25.0):
+    """Progenitors of the stream ensemble: stream lengths (grid steps) uniform in [lmin, lmax]
+    and initial rows at each progenitor's own start (t = -length): radius uniform in
+    [rmin, rmax], isotropic direction, a tangential velocity of 0.75-1.0 times the circular
+    speed of the spherically averaged host (disk as a point mass) in a random tangential
+    direction, plus a 0.02 kpc/Myr Gaussian part."""
+    rng = np.random.default_rng(seed)
+    lengths = rng.integers(lmin, lmax + 1, n_streams)
+    r = rng.uniform(rmin, rmax, n_streams)
+    u = rng.normal(size=(n_streams, 3))
+    u /= np.linalg.norm(u, axis=1, keepdims=True)
+    t = np.cross(u, rng.normal(size=(n_streams, 3)))
+    t /= np.linalg.norm(t, axis=1, keepdims=True)
+    m_enc = (HOST["m_halo"] * nfw_mass_profile(r, HOST["rs_halo"])
+             + HOST["m_bulge"] * r ** 2 / (r + HOST["c_bulge"]) ** 2 + HOST["m_disk"])
+    vc = np.sqrt(G * m_enc / r)
+    v = (rng.uniform(0.75, 1.0, n_streams) * vc)[:, None] * t
+    v += rng.normal(scale=0.02, size=(n_streams, 3))
+    return lengths, np.hstack([r[:, None] * u, v])
 
 
 def gala_mw_components():
