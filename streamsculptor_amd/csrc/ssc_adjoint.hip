@@ -295,14 +295,6 @@ __global__ void gm_rows_reduce(const double* __restrict__ gm_rows, int rows, int
   g_gm[j] += a;
 }
 
-__global__ void host_partial_reduce(const double* __restrict__ gpart, int nblk, int nc2,
-                                    double* __restrict__ g_host) {
-  const int c = threadIdx.x;
-  if (c >= nc2) return;
-  double a = 0.0;
-  for (int b = 0; b < nblk; ++b) a += gpart[(int64_t)b * AJ_MAXC + c];
-  g_host[c] = a;
-}
 
 __global__ void ckpt_index_kernel(int32_t* __restrict__ idx, int64_t nseg, int E, int nsteps) {
   const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -412,8 +404,9 @@ static int adjoint_backward(const ssc_host_potential& hp, const ssc_sources& src
     }
   }
   hipMemcpyAsync(lam_0, w.lam, n * 6 * 8, hipMemcpyDeviceToDevice, st);
-  hipLaunchKernelGGL(host_partial_reduce, dim3(1), dim3(64), 0, st, w.gpart, (int)nblk,
-                     2 * hp.n_components, g_host);
+  if (hp.n_components > 0)
+    hipLaunchKernelGGL(column_sum_kernel<256>, dim3(2 * hp.n_components), dim3(256), 0, st,
+                       w.gpart, (int64_t)nblk, AJ_MAXC, g_host);
   return check_launch("ssc_adjoint_backward");
 }
 

@@ -287,6 +287,25 @@ __device__ __forceinline__ void comp_grad_dp1(const ssc_component& c, double x, 
   }
 }
 
+// dst[c] = sum_i src[i * stride + c] for c = blockIdx.x (launch one block of BLOCK threads per
+// column): strided partial sums, then a tree in LDS in a fixed order.
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void column_sum_kernel(const double* __restrict__ src,
+                                                           int64_t rows, int stride,
+                                                           double* __restrict__ dst) {
+  __shared__ double part[BLOCK];
+  const int c = blockIdx.x;
+  double a = 0.0;
+  for (int64_t i = threadIdx.x; i < rows; i += BLOCK) a += src[i * stride + c];
+  part[threadIdx.x] = a;
+  __syncthreads();
+  for (int w = BLOCK / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) part[threadIdx.x] += part[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) dst[c] = part[0];
+}
+
 // Plummer source (x - s): gradient contribution, the N x M hot loop body.
 __device__ __forceinline__ void plummer_grad(double dx, double dy, double dz, double gm,
                                              double rs2, double& gx, double& gy, double& gz) {

@@ -311,14 +311,6 @@ __global__ __launch_bounds__(128) void release_vjp_kernel(
   }
 }
 
-__global__ void release_host_reduce(const double* __restrict__ g_host_rel, int nrel, int nc2,
-                                    double* __restrict__ g_host) {
-  const int c = threadIdx.x;
-  if (c >= nc2) return;
-  double a = 0.0;
-  for (int i = 0; i < nrel; ++i) a += g_host_rel[(int64_t)i * nc2 + c];
-  g_host[c] = a;
-}
 }  // namespace ssc
 
 using namespace ssc;
@@ -360,7 +352,8 @@ extern "C" int ssc_release_vjp(const ssc_host_potential* host, const double* pro
   hipLaunchKernelGGL(release_vjp_kernel, dim3((nrel + 127) / 128), dim3(128), 0, st, *host, prog_w,
                      rel_idx, g_msat, nrel, seed, kv, g_pos_close, g_pos_far, g_vel_close,
                      g_vel_far, g_prog, scratch);
-  hipLaunchKernelGGL(release_host_reduce, dim3(1), dim3(64), 0, st, scratch, nrel,
-                     2 * host->n_components, g_host);
+  if (host->n_components > 0)
+    hipLaunchKernelGGL(column_sum_kernel<256>, dim3(2 * host->n_components), dim3(256), 0, st,
+                       scratch, (int64_t)nrel, 2 * host->n_components, g_host);
   return check_launch("ssc_release_vjp");
 }

@@ -67,8 +67,8 @@ __device__ __forceinline__ bool surrogate_fallback(const SurrogateDesc& sd,
 template <int BLOCK, int NCH>
 struct LdsWeights {
   static constexpr int CH = (int)(mlp::FRAG_L / 2 / 8);  // 16-byte fragments per chunk
-  static constexpr int PIECES = CH / BLOCK;
-  static_assert(CH % BLOCK == 0, "chunk must split evenly over the block");
+  static constexpr int PIECES = (CH + BLOCK - 1) / BLOCK;
+  static constexpr bool EVEN = CH % BLOCK == 0;           // else the last piece is partial
   static_assert(NCH == 4 || NCH == 8, "four forward chunks, or forward and transposed");
   mlp::bf16x8* buf;        // two chunk buffers of CH fragments
   const mlp::bf16x8* w4;   // output-layer fragments (resident)
@@ -83,14 +83,16 @@ struct LdsWeights {
     asm volatile("" : "+v"(t));
     const mlp::bf16x8* src = c < 4 ? gw + (int64_t)c * CH : gwt + (int64_t)(c - 4) * CH;
 #pragma unroll
-    for (int k = 0; k < PIECES; ++k) pre[k] = src[t + k * BLOCK];
+    for (int k = 0; k < PIECES; ++k)
+      if (EVEN || k + 1 < PIECES || t + k * BLOCK < CH) pre[k] = src[t + k * BLOCK];
   }
   __device__ const mlp::bf16x8* chunk(int c) {
     int t = threadIdx.x;
     asm volatile("" : "+v"(t));
     mlp::bf16x8* dst = buf + (c & 1) * CH;
 #pragma unroll
-    for (int k = 0; k < PIECES; ++k) dst[t + k * BLOCK] = pre[k];
+    for (int k = 0; k < PIECES; ++k)
+      if (EVEN || k + 1 < PIECES || t + k * BLOCK < CH) dst[t + k * BLOCK] = pre[k];
     __syncthreads();
     prefetch((c + 1) & (NCH - 1));
     return dst;

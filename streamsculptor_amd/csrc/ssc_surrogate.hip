@@ -23,8 +23,8 @@
 
 namespace ssc {
 
-// Block shape: 8 waves (two per SIMD, 256-register budget each), 16 particles per wave.
-constexpr int SG_BLOCK_THREADS = 512;
+// Block shape: 12 waves (three per SIMD, 168-register budget each), 16 particles per wave.
+constexpr int SG_BLOCK_THREADS = 768;
 
 
 // CK: also write checkpoint rows ckpt[r][n][9] = (x, v, a) at grid rows min(r E, nsteps),

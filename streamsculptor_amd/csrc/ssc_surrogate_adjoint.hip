@@ -268,14 +268,6 @@ __global__ void sa_zero_kernel(double* __restrict__ p, int64_t n) {
   if (i < n) p[i] = 0.0;
 }
 
-__global__ void sa_host_reduce(const double* __restrict__ gpart, int nblk, int nc2,
-                               double* __restrict__ g_host) {
-  const int c = threadIdx.x;
-  if (c >= nc2) return;
-  double a = 0.0;
-  for (int b = 0; b < nblk; ++b) a += gpart[(int64_t)b * SA_MAXC + c];
-  g_host[c] = a;
-}
 
 // ------------------------------------------------------------------------------------------
 // workspace: checkpoints [(nseg + 1)][n][9], adjoint state [n][6], block partials
@@ -385,7 +377,9 @@ int ssc_surrogate_adjoint_backward(const ssc_host_potential* host, const ssc_sou
                        forcing, w.lam, w.gpart, fallback_count);
   }
   hipMemcpyAsync(lam_0, w.lam, n * 6 * 8, hipMemcpyDeviceToDevice, st);
-  hipLaunchKernelGGL(sa_host_reduce, dim3(1), dim3(64), 0, st, w.gpart, (int)nblk, 2 * nc, g_host);
+  if (nc > 0)
+    hipLaunchKernelGGL(column_sum_kernel<256>, dim3(2 * nc), dim3(256), 0, st, w.gpart, nblk,
+                       SA_MAXC, g_host);
   return check_launch(who);
 }
 
