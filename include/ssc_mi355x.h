@@ -150,6 +150,19 @@ int ssc_dopri5(const ssc_host_potential* host, const ssc_sources* src, const dou
                int32_t nts, const ssc_dopri5_options* opt, double* w_out, double* ys,
                int32_t* stats, void* stream);
 
+/* ssc_dopri5 with dense output instead of save times (reference integrate_orbit(dense=True),
+ * SaveAt(dense=True), streamsculptor/main.py:126,141, and gen_stream_vmapped_dense,
+ * main.py:409-430).  dense[n][dense_cap][20] (device; dense_cap >= max_steps + 1) receives one
+ * record per accepted step of orbit i, in tau = sign(t1 - t_start[i]) t:
+ *   {tau at the step start, step h, y0[6], f0[6] = dy/dtau at y0, Shampine midpoint value[6]}
+ * followed by a closing record {tau1, 0, y1[6], f1[6], 0[6]}; stats[i][0] accepted steps (so
+ * stats[i][0] + 1 records).  On step k the quartic through y0, y1 (record k + 1), h f0, h f1 and
+ * the midpoint value interpolates the orbit (the same interpolant ssc_dopri5 saves with). */
+int ssc_dopri5_dense(const ssc_host_potential* host, const ssc_sources* src, const double* w0,
+                     const double* t_start, int64_t n, double t0, double t1,
+                     const ssc_dopri5_options* opt, double* w_out, double* dense,
+                     int32_t dense_cap, int32_t* stats, void* stream);
+
 /* Spray-stream release conditions (reference Potential.release_model + gen_stream_ics,
  * streamsculptor/main.py:209-306).  For each row i < nrel, from the progenitor row
  * prog_w[i][6] with satellite mass G*Msat = g_msat[i] and release index rel_idx[i]

@@ -3,9 +3,10 @@
 Drop-in for integrate_orbit / gen_stream_vmapped with ``solver=Dopri5()`` (reference
 streamsculptor/main.py:125-163,343-368): diffrax's PIDController(rtol, atol, dtmin, dtmax,
 force_dtmin=True, jump_ts) with dt0=None and SaveAt(ts=ts).  One orbit per lane, each with its
-own step-size controller; saved rows between step ends come from the Dopri5 dense output.
-Failures (max_steps exhausted) give +inf rows, as the reference does with throw=False, or raise
-with throw=True.
+own step-size controller; saved rows between step ends come from the Dopri5 dense output, and
+``dense=True`` returns the step polynomials themselves (``DenseSolution``).  Failures
+(max_steps exhausted) give +inf rows, as the reference does with throw=False, or raise with
+throw=True.
 """
 from __future__ import annotations
 
@@ -67,6 +68,113 @@ def run(h, sd, w0, t1, t0=0.0, t_start=None, ts=None, rtol=1e-7, atol=1e-7, dtmi
     return out, ys, stats
 
 
+REC = 20  # doubles per dense-output record (tau, h, y0[6], f0[6], ymid[6])
+
+
+class DenseSolution:
+    """Result of ``integrate_orbit(dense=True)`` (the reference's diffrax Solution with
+    SaveAt(dense=True), streamsculptor/main.py:141): ``evaluate(t)`` and ``derivative(t)`` through
+    the Dopri5 step polynomials -- on each accepted step the quartic through y0, y1, h f0, h f1
+    and Shampine's midpoint value, the interpolant of the kernel's saved rows.  ``ts`` and ``ys``
+    are None as in the reference.  Times outside an orbit's interval use its first or last
+    step's polynomial.  Orbits that ran out of steps evaluate to +inf.
+
+    Single orbit: evaluate(t) is (6,) for a scalar t and (k, 6) for k times; a batch of n orbits
+    gives (n, 6) and (n, k, 6)."""
+
+    def __init__(self, rec, stats, tau_dir, batched, t0, t1):
+        self.rec = rec                    # (n, cap, REC)
+        self.stats = stats                # (n, 3) int32: accepted, rejected, status
+        self.dir = tau_dir                # (n,) +-1
+        self.batched = batched
+        self.t0, self.t1 = t0, t1
+        self.ts = None
+        self.ys = None
+        n, cap, _ = rec.shape
+        cnt = (stats[:, 0].long() + 1).clamp(max=cap)
+        starts = rec[:, :, 0].clone()
+        starts[torch.arange(cap, device=rec.device)[None, :] >= cnt[:, None]] = float("inf")
+        self._starts = starts
+        self._cnt = cnt
+
+    def _pieces(self, t):
+        t = _lib.f64(t)
+        scalar = t.ndim == 0
+        t = t.reshape(-1)
+        n = self.rec.shape[0]
+        tau = self.dir[:, None] * t[None, :]                               # (n, k)
+        k = torch.searchsorted(self._starts, tau.contiguous(), right=True) - 1
+        last = (self._cnt - 2).clamp(min=0)[:, None]
+        k = torch.minimum(k.clamp(min=0), last)
+        k1 = torch.minimum(k + 1, (self._cnt - 1)[:, None])
+        rows = torch.arange(n, device=t.device)[:, None]
+        r0, r1 = self.rec[rows, k], self.rec[rows, k1]                     # (n, k, REC)
+        h = r0[..., 1:2]
+        s = torch.where(h != 0, (tau[..., None] - r0[..., 0:1]) / torch.where(h != 0, h, 1.0), 0.0)
+        y0, f0, ym = r0[..., 2:8], r0[..., 8:14], r0[..., 14:20]
+        y1, f1 = r1[..., 2:8], r1[..., 8:14]
+        F0, F1 = h * f0, h * f1
+        D1 = y1 - y0 - F0
+        D2 = F1 - F0
+        D3 = ym - y0 - 0.5 * F0
+        a4 = 2.0 * D2 - 8.0 * D1 + 16.0 * D3
+        b3 = D2 - 2.0 * D1 - 2.0 * a4
+        c2 = D1 - a4 - b3
+        return scalar, s, h, y0, F0, c2, b3, a4, f0
+
+    def _finish(self, v, scalar):
+        v = torch.where((self.stats[:, 2] != 0)[:, None, None], float("inf"), v)
+        if scalar:
+            v = v[:, 0]
+        return v if self.batched else v[0]
+
+    def evaluate(self, t0, t1=None, left=True):
+        """y(t0), or y(t1) - y(t0) when t1 is given (diffrax's evaluate(t0, t1))."""
+        if t1 is not None:
+            return self.evaluate(t1) - self.evaluate(t0)
+        scalar, s, h, y0, F0, c2, b3, a4, _ = self._pieces(t0)
+        y = y0 + s * (F0 + s * (c2 + s * (b3 + s * a4)))
+        return self._finish(y, scalar)
+
+    def derivative(self, t, left=True):
+        """dy/dt of the interpolant."""
+        scalar, s, h, y0, F0, c2, b3, a4, f0 = self._pieces(t)
+        dtau = (F0 + s * (2.0 * c2 + s * (3.0 * b3 + s * 4.0 * a4))) / torch.where(h != 0, h, 1.0)
+        dtau = torch.where(h != 0, dtau, f0)
+        return self._finish(self.dir[:, None, None] * dtau, scalar)
+
+
+def run_dense(h, sd, w0, t1, t0=0.0, t_start=None, rtol=1e-7, atol=1e-7, dtmin=0.3, dtmax=None,
+              max_steps=10_000, jump_ts=None, dt0=None, throw=False, batched=True):
+    """ssc_dopri5_dense: like ``run`` but returns a DenseSolution over [t_start[i], t1]."""
+    w0 = _lib.f64(w0).reshape(-1, 6).contiguous()
+    n = w0.shape[0]
+    dev = w0.device
+    tst = None if t_start is None else _lib.f64(t_start).reshape(-1).contiguous()
+    if tst is not None and tst.numel() != n:
+        raise ValueError("t_start needs one time per orbit")
+    stops = None
+    if jump_ts is not None:
+        stops = _lib.f64(np.sort(np.asarray(jump_ts, dtype=np.float64).reshape(-1)))
+    cap = int(max_steps) + 1
+    rec = torch.empty(n, cap, REC, dtype=torch.float64, device=dev)
+    out = torch.empty_like(w0)
+    stats = torch.empty(n, 3, dtype=torch.int32, device=dev)
+    opt = _options(rtol, atol, dtmin, dtmax, max_steps, stops, dt0)
+    _lib.call("ssc_dopri5_dense", _lib.ctypes.byref(h),
+              None if sd is None else _lib.ctypes.byref(sd), _lib.ptr(w0), _lib.ptr(tst), n,
+              float(t0), float(t1), _lib.ctypes.byref(opt), _lib.ptr(out), _lib.ptr(rec), cap,
+              _lib.ptr(stats))
+    if throw:
+        failed = int((stats[:, 2] != 0).sum().item())
+        if failed:
+            raise RuntimeError(f"Dopri5: {failed} of {n} orbits reached max_steps={max_steps} "
+                               "before t1")
+    ta = tst if tst is not None else torch.full((n,), float(t0), dtype=torch.float64, device=dev)
+    tau_dir = torch.where(float(t1) >= ta, 1.0, -1.0).to(torch.float64)
+    return DenseSolution(rec, stats, tau_dir, batched, t0, t1)
+
+
 def dopri5(pot, w0, t1, t0=0.0, t_start=None, ts=None, rtol=1e-7, atol=1e-7, dtmin=0.3,
            dtmax=None, max_steps=10_000, jump_ts=None, dt0=None, throw=False):
     """``run`` with the descriptors of potential ``pot``."""
@@ -92,6 +200,17 @@ def integrate_orbit(pot, w, tsn, t0v, t1v, rtol=1e-7, atol=1e-7, dtmin=0.3, dtma
         ys = ys[:, inv]
     ts_t = _lib.f64(tsn)
     return Solution(ts_t, ys if batched else ys[0])
+
+
+def integrate_orbit_dense(pot, w, t0v, t1v, rtol=1e-7, atol=1e-7, dtmin=0.3, dtmax=None,
+                          max_steps=10_000, jump_ts=None, throw=False, dt0=None, batched=False,
+                          t_start=None):
+    """DenseSolution for Potential.integrate_orbit(dense=True) with solver=Dopri5."""
+    h, src = _sources(pot)
+    sd = src.desc() if src is not None else None
+    return run_dense(h, sd, w.reshape(-1, 6), t1v, t0=t0v, t_start=t_start, rtol=rtol, atol=atol,
+                     dtmin=dtmin, dtmax=dtmax, max_steps=max_steps, jump_ts=jump_ts, dt0=dt0,
+                     throw=throw, batched=batched)
 
 
 def integrate_to(pot, w0, t_start, t1, rtol=1e-7, atol=1e-7, dtmin=0.3, dtmax=None,

@@ -24,6 +24,10 @@ namespace ssc {
 namespace {
 
 constexpr int DP_BLOCK = 256;
+// Dense-output record of one accepted step, in tau = dir t: tau at the step start, the step h,
+// y0[6], f0 = dy/dtau at y0 [6], Shampine's midpoint value ym[6].  The record after the last
+// step holds (tau1, 0, y1, f1, 0): the quartic of step k needs y1 and f1 from record k + 1.
+constexpr int DP_REC = 20;
 
 // Dormand & Prince (1980) tableau.
 constexpr double A21 = 1.0 / 5;
@@ -139,7 +143,8 @@ __global__ __launch_bounds__(DP_BLOCK) void dopri5_kernel(
     ssc_host_potential hp, ssc_sources src, const double* __restrict__ w0,
     const double* __restrict__ t_start, int64_t n, double t0, double t1,
     const double* __restrict__ ts, int nts, const double* __restrict__ stops, Dp5Ctl ctl,
-    double* __restrict__ w_out, double* __restrict__ ys, int32_t* __restrict__ stats) {
+    double* __restrict__ w_out, double* __restrict__ ys, int32_t* __restrict__ stats,
+    double* __restrict__ dense, int dense_cap) {
   extern __shared__ __align__(16) char lds_raw[];
   SourceLDS<SK> lds;
   if (!GR) lds.init(src, lds_raw);  // the only barrier: every lane reaches it
@@ -165,8 +170,12 @@ __global__ __launch_bounds__(DP_BLOCK) void dopri5_kernel(
   const double tau0 = dir * ta, tau1 = dir * t1;
   double y[6], f0[6];
 #pragma unroll
-  for (int c = 0; c < 6; ++c) y[c] = w0[6 * i + c];
+  for (int c = 0; c < 6; ++c) {
+    y[c] = w0[6 * i + c];
+    f0[c] = 0.0;
+  }
   double* __restrict__ yrow = ys ? ys + (int64_t)6 * nts * i : nullptr;
+  double* __restrict__ drow = dense ? dense + (int64_t)dense_cap * DP_REC * i : nullptr;
   int js = 0;
   for (; js < nts && dir * ts[js] < tau0; ++js)
 #pragma unroll
@@ -250,6 +259,18 @@ __global__ __launch_bounds__(DP_BLOCK) void dopri5_kernel(
         if (dtn < ctl.dtmin) dtn = ctl.dtmin;
       }
       double t_new = tprev;
+      if (keep && drow) {
+        double* r = drow + (int64_t)nacc * DP_REC;
+        r[0] = tprev;
+        r[1] = h;
+#pragma unroll
+        for (int c = 0; c < 6; ++c) {
+          r[2 + c] = y[c];
+          r[8 + c] = f0[c];
+          r[14 + c] = y[c] + h * (M1 * f0[c] + M3 * k3[c] + M4 * k4[c] + M5 * k5[c] +
+                                  M6 * k6[c] + M7 * k7[c]);
+        }
+      }
       if (keep) {
         if (yrow && js < nts && dir * ts[js] <= tnext) {
           // dense output on (tprev, tnext]
@@ -319,6 +340,17 @@ __global__ __launch_bounds__(DP_BLOCK) void dopri5_kernel(
   if (w_out)
 #pragma unroll
     for (int c = 0; c < 6; ++c) w_out[6 * i + c] = y[c];
+  if (drow) {  // closing record: the end point of the last step
+    double* r = drow + (int64_t)nacc * DP_REC;
+    r[0] = tprev;
+    r[1] = 0.0;
+#pragma unroll
+    for (int c = 0; c < 6; ++c) {
+      r[2 + c] = y[c];
+      r[8 + c] = f0[c];
+      r[14 + c] = 0.0;
+    }
+  }
 }
 
 int set_lds_dp(const void* fn, size_t lds) {
@@ -335,13 +367,14 @@ template <int SK, bool GR>
 int launch_dopri5(const ssc_host_potential& hp, const ssc_sources& src, const double* w0,
                   const double* t_start, int64_t n, double t0, double t1, const double* ts,
                   int nts, const double* stops, const Dp5Ctl& ctl, double* w_out, double* ys,
-                  int32_t* stats, hipStream_t st) {
+                  int32_t* stats, double* dense, int dense_cap, hipStream_t st) {
   const size_t lds = GR ? 0 : SourceLDS<SK>::bytes(src);
   int rc = set_lds_dp((const void*)dopri5_kernel<SK, GR>, lds);
   if (rc) return rc;
   const int64_t blocks = (n + DP_BLOCK - 1) / DP_BLOCK;
   hipLaunchKernelGGL((dopri5_kernel<SK, GR>), dim3((unsigned)blocks), dim3(DP_BLOCK), lds, st, hp,
-                     src, w0, t_start, n, t0, t1, ts, nts, stops, ctl, w_out, ys, stats);
+                     src, w0, t_start, n, t0, t1, ts, nts, stops, ctl, w_out, ys, stats, dense,
+                     dense_cap);
   return check_launch("ssc_dopri5");
 }
 
@@ -350,23 +383,25 @@ int launch_dopri5(const ssc_host_potential& hp, const ssc_sources& src, const do
 
 using namespace ssc;
 
-extern "C" int ssc_dopri5(const ssc_host_potential* host, const ssc_sources* src,
-                          const double* w0, const double* t_start, int64_t n, double t0,
-                          double t1, const double* ts, int32_t nts,
-                          const ssc_dopri5_options* opt, double* w_out, double* ys,
-                          int32_t* stats, void* stream) {
+static int dopri5_entry(const ssc_host_potential* host, const ssc_sources* src, const double* w0,
+                        const double* t_start, int64_t n, double t0, double t1, const double* ts,
+                        int32_t nts, const ssc_dopri5_options* opt, double* w_out, double* ys,
+                        int32_t* stats, double* dense, int32_t dense_cap, void* stream) {
   int rc = validate_host(host);
   if (rc) return rc;
   if (!opt) return fail(SSC_E_ARG, "ssc_dopri5: options are NULL");
   if (n < 0 || nts < 0 || opt->n_stops < 0) return fail(SSC_E_ARG, "ssc_dopri5: negative size");
   if (n == 0) return SSC_OK;
   if (!w0) return fail(SSC_E_ARG, "ssc_dopri5: w0 is NULL");
-  if (!w_out && !(ys && nts > 0)) return fail(SSC_E_ARG, "ssc_dopri5: no output (w_out, ys)");
+  if (!w_out && !(ys && nts > 0) && !dense)
+    return fail(SSC_E_ARG, "ssc_dopri5: no output (w_out, ys, dense)");
   if (nts > 0 && (!ts || !ys)) return fail(SSC_E_ARG, "ssc_dopri5: nts > 0 needs ts and ys");
   if (opt->n_stops > 0 && !opt->stops) return fail(SSC_E_ARG, "ssc_dopri5: n_stops > 0 needs stops");
   if (!(opt->rtol >= 0.0) || !(opt->atol >= 0.0) || !(opt->rtol + opt->atol > 0.0))
     return fail(SSC_E_ARG, "ssc_dopri5: rtol/atol must be >= 0 and not both 0");
   if (opt->max_steps < 1) return fail(SSC_E_ARG, "ssc_dopri5: max_steps < 1");
+  if (dense && dense_cap < opt->max_steps + 1)
+    return fail(SSC_E_ARG, "ssc_dopri5_dense: dense_cap must be at least max_steps + 1");
   if ((n + DP_BLOCK - 1) / DP_BLOCK > (int64_t)INT_MAX)
     return fail(SSC_E_LIMIT, "ssc_dopri5: too many orbits");
   rc = validate_sources(src, 0, true);
@@ -387,7 +422,7 @@ extern "C" int ssc_dopri5(const ssc_host_potential* host, const ssc_sources* src
   const bool grouped = s.kind != SSC_SRC_NONE && s.group_size > 0;
 #define SSC_DP(K, G)                                                                              \
   launch_dopri5<K, G>(*host, s, w0, t_start, n, t0, t1, ts, nts, opt->stops, ctl, w_out, ys,      \
-                      stats, st)
+                      stats, dense, dense_cap, st)
   switch (s.kind) {
     case SSC_SRC_NONE: return SSC_DP(SSC_SRC_NONE, false);
     case SSC_SRC_STATIC: return grouped ? SSC_DP(SSC_SRC_STATIC, true) : SSC_DP(SSC_SRC_STATIC, false);
@@ -395,4 +430,22 @@ extern "C" int ssc_dopri5(const ssc_host_potential* host, const ssc_sources* src
     default: return fail(SSC_E_ARG, "ssc_dopri5: bad source kind");
   }
 #undef SSC_DP
+}
+
+extern "C" int ssc_dopri5(const ssc_host_potential* host, const ssc_sources* src,
+                          const double* w0, const double* t_start, int64_t n, double t0,
+                          double t1, const double* ts, int32_t nts,
+                          const ssc_dopri5_options* opt, double* w_out, double* ys,
+                          int32_t* stats, void* stream) {
+  return dopri5_entry(host, src, w0, t_start, n, t0, t1, ts, nts, opt, w_out, ys, stats, nullptr,
+                      0, stream);
+}
+
+extern "C" int ssc_dopri5_dense(const ssc_host_potential* host, const ssc_sources* src,
+                                const double* w0, const double* t_start, int64_t n, double t0,
+                                double t1, const ssc_dopri5_options* opt, double* w_out,
+                                double* dense, int32_t dense_cap, int32_t* stats, void* stream) {
+  if (!dense || !stats) return fail(SSC_E_ARG, "ssc_dopri5_dense: dense and stats are required");
+  return dopri5_entry(host, src, w0, t_start, n, t0, t1, nullptr, 0, opt, w_out, nullptr, stats,
+                      dense, dense_cap, stream);
 }

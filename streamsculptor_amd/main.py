@@ -316,16 +316,24 @@ class Potential:
                         jump_ts=None, throw=False, dt0=None):
         """Integrate w0 (6,) -- or a batch (N, 6) -- from t0 (default min(ts)) to t1 (default
         max(ts)), saving at ts (reference main.py:125-163).  Returns a ``Solution`` with ys of
-        shape (len(ts), 6) (or (N, len(ts), 6))."""
+        shape (len(ts), 6) (or (N, len(ts), 6)); with dense=True (Dopri5) a DenseSolution whose
+        ``evaluate(t)`` interpolates the orbit anywhere in [t0, t1]."""
         method = _solvers.resolve(solver)
-        if dense:
-            raise NotImplementedError("dense=True (diffrax interpolants) is not available")
         ts_t = _lib.f64(ts).reshape(-1)
         tsn = ts_t.cpu().numpy()
         t0v = float(tsn.min()) if t0 is None else float(_as_float(t0))
         t1v = float(tsn.max()) if t1 is None else float(_as_float(t1))
         w = _lib.f64(w0)
         batched = w.ndim == 2
+        if dense:
+            if method == "leapfrog":
+                raise NotImplementedError("dense=True needs an adaptive solver (Dopri5): the "
+                                          "fixed-step leapfrog has no dense interpolant")
+            from . import adaptive
+            return adaptive.integrate_orbit_dense(self, w, t0v, t1v, rtol=rtol, atol=atol,
+                                                  dtmin=dtmin, dtmax=dtmax, max_steps=max_steps,
+                                                  jump_ts=jump_ts, throw=throw, dt0=dt0,
+                                                  batched=batched)
         if method == "leapfrog":
             from ._grid import leapfrog_grid
             dt, nsteps, kidx = leapfrog_grid(tsn, t0v, t1v, dt0, steps=steps)
@@ -408,3 +416,17 @@ class Potential:
                           max_steps=max_steps, throw=throw, dt0=dt0)
 
     gen_stream_scan = gen_stream_vmapped
+
+    def gen_stream_vmapped_dense(self, ts=None, prog_w0=None, Msat=None, seed_num=None,
+                                 solver=None, kval_arr=1.0, rtol=1e-7, atol=1e-7, dtmin=0.3,
+                                 dtmax=None, max_steps=10_000):
+        """Dense spray stream (reference main.py:409-430): every particle released at ts[i],
+        i < len(ts)-1, integrated to ts[-1] with dense output.  Returns a batched
+        DenseSolution over the 2 (len(ts)-1) particles ordered (release, arm) with arm 0 =
+        lead; evaluate it with ``streamhelpers.eval_dense_stream``."""
+        from .streams import gen_stream_dense
+        return gen_stream_dense(self, self, ts, prog_w0, Msat, seed_num, solver=solver,
+                                kval_arr=kval_arr, rtol=rtol, atol=atol, dtmin=dtmin, dtmax=dtmax,
+                                max_steps=max_steps)
+
+    gen_stream_scan_dense = gen_stream_vmapped_dense

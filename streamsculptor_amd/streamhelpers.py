@@ -55,3 +55,19 @@ def gen_stream_vmapped_with_pert_fixed_prog(pot_base=None, pot_pert=None, ts=Non
     return gen_stream(pot_base, pot_base, ts, prog_w0, Msat, seed_num, solver=solver,
                       kval_arr=kval_arr, rtol=rtol, atol=atol, dtmin=dtmin, max_steps=max_steps,
                       throw=throw, dt0=dt0, particle_pot=total)
+
+
+def eval_dense_stream(t_eval=None, dense_stream=None):
+    """Leading and trailing arm of a dense stream at time t_eval (reference
+    streamhelpers.py:24-30): (lead (T, 6), trail (T, 6))."""
+    w = dense_stream.evaluate(t_eval)
+    return w[0::2], w[1::2]
+
+
+def eval_dense_stream_id(time=None, interp_func=None, idx=None, lead=True):
+    """Trajectory of particle ``idx`` of one arm of a dense stream at ``time`` (reference
+    streamhelpers.py:33-53)."""
+    w = interp_func.evaluate(time)
+    i = 2 * int(idx) + (0 if lead else 1)
+    return w[i]
+

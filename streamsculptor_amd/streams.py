@@ -61,6 +61,28 @@ def gen_stream(prog_pot, release_pot, ts, prog_w0, Msat, seed_num, solver=None, 
     return out[0::2], out[1::2]
 
 
+def gen_stream_dense(prog_pot, release_pot, ts, prog_w0, Msat, seed_num, solver=None,
+                     kval_arr=1.0, rtol=1e-7, atol=1e-7, dtmin=0.3, dtmax=None, max_steps=10_000):
+    """Dense spray stream: a batched DenseSolution over 2 (len(ts) - 1) particles, particle
+    2i + arm released at ts[i] (arm 0 = lead / close, 1 = trail / far) and integrated to ts[-1]
+    with Dopri5 (reference gen_stream_vmapped_dense, main.py:409-430)."""
+    if _solvers.resolve(solver) == "leapfrog":
+        raise NotImplementedError("dense streams need an adaptive solver (Dopri5)")
+    from . import adaptive
+    adaptive_kw = dict(rtol=rtol, atol=atol, dtmin=dtmin, dtmax=dtmax, max_steps=max_steps)
+    (pc, pf, vc, vf), _, ts_np = stream_ics(prog_pot, release_pot, ts, prog_w0, Msat, seed_num,
+                                            solver=solver, kval_arr=kval_arr, **adaptive_kw)
+    T = len(ts_np) - 1
+    w0 = torch.empty(2 * T, 6, dtype=torch.float64, device=pc.device)
+    w0[0::2, :3], w0[0::2, 3:] = pc[:T], vc[:T]
+    w0[1::2, :3], w0[1::2, 3:] = pf[:T], vf[:T]
+    t_start = np.repeat(ts_np[:T], 2)
+    h, src = adaptive._sources(prog_pot)
+    sd = src.desc() if src is not None else None
+    return adaptive.run_dense(h, sd, w0, float(ts_np[-1]), t0=float(ts_np[0]), t_start=t_start,
+                              batched=True, **adaptive_kw)
+
+
 def spray_particles(prog_pot, release_pot, ts, prog_w0, Msat, seed_num, per_step, dt0,
                     kval_arr=1.0):
     """Initial rows of a many-particle spray stream with ``per_step`` particles released at

@@ -132,3 +132,89 @@ def test_failure_rows_and_throw(oracle, ssc):
                                 units=ssc.usys)
     with pytest.raises(NotImplementedError):
         adaptive.dopri5(live, w0, 10.0)
+
+
+def test_dense_output_matches_oracle_saves(oracle, ssc):
+    """integrate_orbit(dense=True): evaluate(t) anywhere in the interval equals the oracle's
+    Dopri5 saved at those times (same steps, same interpolant), derivative(t) matches the RHS at
+    step ends and a central difference of evaluate inside steps; the notebook's sol.evaluate(30)
+    (tests.ipynb:12,17, dense=True) within the printed digits at tight tolerance."""
+    from streamsculptor_amd.potential import NFWPotential
+    h, mw = _mw(oracle, ssc)
+    rng = np.random.default_rng(9)
+    w0 = bound_orbits(rng, 40)
+    sol = mw.integrate_orbit(w0=w0, ts=np.array([0.0, 2000.0]), solver="dopri5", dense=True)
+    assert sol.ys is None
+    tq = np.sort(rng.uniform(0.0, 2000.0, 37))
+    got = sol.evaluate(tq).cpu().numpy()                      # (40, 37, 6)
+    _, ys_o, _ = oracle.dopri5(h, w0, 2000.0, t0=0.0, ts=tq)
+    e = rel_err_rows(got.reshape(-1, 6), ys_o.reshape(-1, 6))
+    print(f"dense evaluate vs oracle saves: {e:.2e}")
+    assert e < TOL
+    # the end points are the integrator's rows
+    _, ys_end, _ = oracle.dopri5(h, w0, 2000.0, t0=0.0, ts=np.array([2000.0]))
+    assert rel_err_rows(sol.evaluate(2000.0).cpu().numpy(), ys_end[:, 0]) < TOL
+    np.testing.assert_allclose(sol.evaluate(0.0).cpu().numpy(), w0, rtol=0, atol=1e-14)
+    # derivative: RHS at the start, and a central difference of the interpolant inside steps
+    d0 = sol.derivative(0.0).cpu().numpy()
+    _, g, _ = oracle.evaluate(h, w0[:, :3])
+    np.testing.assert_allclose(d0[:, :3], w0[:, 3:], rtol=1e-13)
+    np.testing.assert_allclose(d0[:, 3:], -g, rtol=1e-12)
+    tm = tq[5]
+    fd = (sol.evaluate(tm + 1e-4) - sol.evaluate(tm - 1e-4)).cpu().numpy() / 2e-4
+    dm = sol.derivative(tm).cpu().numpy()
+    assert np.abs(fd - dm).max() / np.abs(dm).max() < 1e-6
+    # single orbit, the reference notebook's call
+    pot = NFWPotential(1e12, 20.0, units=ssc.usys)
+    s1 = pot.integrate_orbit(w0=np.array([20.0, 15.0, 20.0, 0.08, 0.1, -0.05]),
+                             ts=np.linspace(0, 3000, 500), dense=True, solver="dopri5",
+                             rtol=1e-12, atol=1e-12, dtmin=None, max_steps=200_000)
+    golden = np.array([21.9793661, 17.67654451, 18.10530132, 0.051923, 0.07815599, -0.07552167])
+    v = s1.evaluate(30.0).cpu().numpy()
+    assert v.shape == (6,)
+    np.testing.assert_allclose(v, golden, rtol=0, atol=6e-9)
+
+
+def test_dense_backwards_batch_failure_and_stream(oracle, ssc):
+    from streamsculptor_amd import adaptive
+    from streamsculptor_amd import workloads as W
+    from streamsculptor_amd.streamhelpers import eval_dense_stream
+    h, mw = _mw(oracle, ssc)
+    rng = np.random.default_rng(10)
+    w0 = bound_orbits(rng, 16)
+    # backwards from per-orbit start times
+    tst = rng.uniform(-300.0, 0.0, 16)
+    hh, src = adaptive._sources(mw)
+    sol = adaptive.run_dense(hh, None, w0, -1500.0, t_start=tst)
+    tq = np.array([-400.0, -1000.0, -1400.0])   # saves run in the direction of integration
+    got = sol.evaluate(tq).cpu().numpy()
+    for i in range(16):
+        _, ys_o, _ = oracle.dopri5(h, w0[i], -1500.0, t0=tst[i], ts=tq)
+        assert rel_err_rows(got[i], ys_o[0]) < TOL
+    # an orbit that runs out of steps evaluates to +inf
+    bad = adaptive.run_dense(hh, None, w0, 3000.0, max_steps=20, dtmin=None)
+    assert torch.isinf(bad.evaluate(100.0)).all()
+    # dense stream: at ts[-1] it equals gen_stream_vmapped's particles
+    pot = W.host_potential()
+    ts = np.linspace(-600.0, 0.0, 121)
+    dens = pot.gen_stream_vmapped_dense(ts=ts, prog_w0=W.PROG_TODAY, Msat=1e4, seed_num=0,
+                                        solver="dopri5")
+    lead_d, trail_d = eval_dense_stream(0.0, dens)
+    lead, trail = pot.gen_stream_vmapped(ts=ts, prog_w0=W.PROG_TODAY, Msat=1e4, seed_num=0,
+                                         solver="dopri5")
+    assert rel_err_rows(lead_d.cpu().numpy(), lead.cpu().numpy()) < 1e-12
+    assert rel_err_rows(trail_d.cpu().numpy(), trail.cpu().numpy()) < 1e-12
+    # mid-way: particles released by then equal the oracle's stream orbits saved at that time
+    lead_m, trail_m = eval_dense_stream(-300.0, dens)
+    assert lead_m.shape == (120, 6)
+    hs = oracle.host(W.host_components())
+    _, prog, _ = oracle.dopri5(hs, W.PROG_TODAY, 0.0, t0=-600.0, ts=ts)
+    pc, pf, vc, vf = oracle.release(hs, prog[0], ts, 1e4, 0)
+    wp = np.empty((240, 6))
+    wp[0::2, :3], wp[0::2, 3:] = pc[:120], vc[:120]
+    wp[1::2, :3], wp[1::2, 3:] = pf[:120], vf[:120]
+    _, ys_m, _ = oracle.dopri5(hs, wp, 0.0, t_start=np.repeat(ts[:120], 2),
+                               ts=np.array([-300.0]))
+    rel = ts[:120] <= -300.0
+    assert rel_err_rows(lead_m.cpu().numpy()[rel], ys_m[0::2, 0][rel]) < TOL
+    assert rel_err_rows(trail_m.cpu().numpy()[rel], ys_m[1::2, 0][rel]) < TOL
