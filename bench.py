@@ -35,6 +35,10 @@ sys.path.insert(0, ROOT)
 PEAK_FP64 = 78.6e12      # MI355X FP64 vector, FLOP/s
 PEAK_BF16 = 2.5e15       # MI355X dense BF16 MFMA
 ERR_LIMIT_PC = 10.0
+# HBM bytes per launch of the plain clumps leapfrog kernel at the default 1e6 particles, from
+# rocprofv3 FETCH_SIZE (x 2.0, calibrated on a 1 GiB copy) + WRITE_SIZE
+TRAFFIC_CLUMPS_PLAIN = 100.4e6
+CPU_SECONDS = 10.0       # target CPU time of the baseline sample
 
 
 def log(msg):
@@ -165,7 +169,8 @@ def train_surrogate(wl, iters):
 
 def cpu_baseline(wl, w0, start, out, sample_every):
     """The oracle (-O3 x86-64-v3 + OpenMP build of the same algorithm) on a strided sample of
-    the same particles; also returns the sample's max row error against the GPU result."""
+    the same particles, repeated until about CPU_SECONDS of CPU work; also returns the sample's
+    max row error against the GPU result."""
     import numpy as np
     import oracle as O
     from streamsculptor_amd import workloads as W
@@ -182,20 +187,24 @@ def cpu_baseline(wl, w0, start, out, sample_every):
     w0s = w0[::sample_every].cpu().numpy()
     sts = start[::sample_every]
     nsteps = len(wl["ts"]) - 1
-    t0 = time.perf_counter()
-    if wl["name"] == "responses":
-        ref, _ = O.perturb_leapfrog(h, src, w0s, float(wl["ts"][0]), W.DT, nsteps, start=sts,
-                                    nthreads=cores)
-    else:
-        ref = O.leapfrog(h, w0s, float(wl["ts"][0]), W.DT, nsteps, start=sts, src=src,
-                         nthreads=cores, fast=True)
-    el = time.perf_counter() - t0
+    reps, el = 0, 0.0
+    while reps == 0 or (el < CPU_SECONDS and reps < 50):
+        t0 = time.perf_counter()
+        if wl["name"] == "responses":
+            ref, _ = O.perturb_leapfrog(h, src, w0s, float(wl["ts"][0]), W.DT, nsteps, start=sts,
+                                        nthreads=cores)
+        else:
+            ref = O.leapfrog(h, w0s, float(wl["ts"][0]), W.DT, nsteps, start=sts, src=src,
+                             nthreads=cores, fast=True)
+        el += time.perf_counter() - t0
+        reps += 1
     psteps = int((nsteps - sts).sum())
     g = out[::sample_every].cpu().numpy()
     err = float((np.linalg.norm(g - ref, axis=1) / np.linalg.norm(ref, axis=1)).max())
-    return dict(value=psteps / el, unit="particle-steps/s", cores=cores, kind="port",
-                sample=f"every {sample_every}th particle of the same stream ({len(w0s)} particles, "
-                       f"{psteps} particle-steps, {el:.1f} s, oracle OpenMP)"), err
+    every = f"every {sample_every}th particle" if sample_every > 1 else "every particle"
+    return dict(value=reps * psteps / el, unit="particle-steps/s", cores=cores, kind="port",
+                sample=f"{every} of the same stream ({len(w0s)} particles, {psteps} "
+                       f"particle-steps) x {reps} passes, {el:.1f} s, oracle OpenMP"), err
 
 
 def gd1_setup(B):
@@ -426,15 +435,16 @@ def main():
                              "ensemble"])
     ap.add_argument("--streams", type=int, default=2000, help="ensemble: progenitors")
     ap.add_argument("--releases", type=int, default=1000, help="ensemble: release steps each")
-    ap.add_argument("--cpu-particles", type=int, default=8, help="ensemble: CPU sample size")
+    ap.add_argument("--cpu-particles", type=int, default=16, help="ensemble: CPU sample size")
     ap.add_argument("--no-exact", action="store_true", help="ensemble: skip the exact run")
     ap.add_argument("--sims", type=int, default=700, help="gd1: simulations per batch")
-    ap.add_argument("--cpu-sims", type=int, default=4, help="gd1: simulations in the CPU sample")
+    ap.add_argument("--cpu-sims", type=int, default=600, help="gd1: simulations in the CPU sample")
     ap.add_argument("--particles", type=int, default=None)
     ap.add_argument("--train-iters", type=int, default=32000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-surrogate", action="store_true")
-    ap.add_argument("--cpu-sample-every", type=int, default=100)
+    ap.add_argument("--cpu-sample-every", type=int, default=None,
+                    help="CPU baseline stride (default: 16 for clumps, 1 otherwise)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: particles are independent, so a multi-GPU run is "
@@ -491,7 +501,9 @@ def main():
              "kernel_ms": kern_p * 1e3,
              "roofline": {"bound": "fp64-valu", "achieved": fl_p * ps / kern_p / 1e12,
                           "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s",
-                          "frac": fl_p * ps / kern_p / PEAK_FP64, "traffic": None,
+                          "frac": fl_p * ps / kern_p / PEAK_FP64,
+                          "traffic": (TRAFFIC_CLUMPS_PLAIN if args.workload == "clumps"
+                                      and wl["total"] == 1_000_000 else None),
                           "kernel": "leapfrog_kernel" if args.workload != "responses" else "perturb_kernel",
                           "flop_per_particle_step": fl_p}}
     result = {
@@ -549,7 +561,8 @@ def main():
 
     if not args.no_cpu:
         log("timing the CPU baseline (oracle on a sample)")
-        cb, err = cpu_baseline(wl, w0, start, out_p, args.cpu_sample_every)
+        every = args.cpu_sample_every or (16 if args.workload == "clumps" else 1)
+        cb, err = cpu_baseline(wl, w0, start, out_p, every)
         result["cpu_baseline"] = cb
         result["parity_sample_max_rel_err"] = err
         log(f"cpu baseline {cb['value']:.3e} particle-steps/s on {cb['cores']} cores; "
