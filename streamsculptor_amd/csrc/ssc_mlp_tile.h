@@ -30,23 +30,24 @@ __device__ __forceinline__ mlp::bf16x8 pack8(const float* v) {
 // at once (and spilled).
 __device__ __forceinline__ void pin(mlp::bf16x8& v) { asm volatile("" : "+v"(v)); }
 
-// Weight policies.  chunk(c), c = 0..3, returns the fragments of chunk c of the two chained
-// layers -- W2 tiles 0-7, W2 tiles 8-15, W3 tiles 0-7, W3 tiles 8-15 (64 KB each) -- and
-// out() the output layer's fragments.
-// The vector-Jacobian product also uses chunkT(c) -- W3^T tiles 0-7, W3^T 8-15, W2^T 0-7,
-// W2^T 8-15 -- and w4t(), the fp32 W4^T table [H][OUT].
+// Weight policies.  A chained 256x256 layer is consumed in NQ chunks of 16 / NQ tiles each:
+// chunk(c), c = 0 .. 2 NQ - 1, returns the fragments of chunk c of the two chained layers (W2
+// first, then W3), and out() the output layer's fragments.  The vector-Jacobian product also
+// uses chunkT(c) -- the same split of W3^T, then W2^T -- and w4t(), the fp32 W4^T table
+// [H][OUT].
 struct GlobalWeights {
+  static constexpr int NQ = 2;
   const uint16_t* pk16;
   const uint16_t* pkv = nullptr;  // packed transposes (input VJP only)
   __device__ const mlp::bf16x8* chunk(int c) const {
-    const int64_t off = ((c >> 1) == 0 ? mlp::PK_W2 : mlp::PK_W3) + (c & 1) * (mlp::FRAG_L / 2);
+    const int64_t off = (c / NQ == 0 ? mlp::PK_W2 : mlp::PK_W3) + (c % NQ) * (mlp::FRAG_L / NQ);
     return reinterpret_cast<const mlp::bf16x8*>(pk16 + off);
   }
   __device__ const mlp::bf16x8* out() const {
     return reinterpret_cast<const mlp::bf16x8*>(pk16 + mlp::PK_W4);
   }
   __device__ const mlp::bf16x8* chunkT(int c) const {
-    const int64_t off = ((c >> 1) == 0 ? mlp::PKV_W3T : mlp::PKV_W2T) + (c & 1) * (mlp::FRAG_L / 2);
+    const int64_t off = (c / NQ == 0 ? mlp::PKV_W3T : mlp::PKV_W2T) + (c % NQ) * (mlp::FRAG_L / NQ);
     return reinterpret_cast<const mlp::bf16x8*>(pkv + off);
   }
   __device__ const float* w4t() const {
@@ -86,18 +87,18 @@ __device__ __forceinline__ void mlp_tile_forward(Weights& wsrc, const float* __r
     bin[j] = pack8(v);
     pin(bin[j]);
   }
-  // two chained 256x256 layers, each in two chunks of 8 tiles; output k-block j = tiles 2j
-  // and 2j + 1 (tiles 2jj, 2jj + 1 of chunk j / 4)
+  // two chained 256x256 layers, each in NQ chunks of 16 / NQ tiles; output k-block j = tiles
+  // 2j and 2j + 1 (tiles 2jj, 2jj + 1 of chunk j / (8 / NQ))
 #pragma unroll
   for (int layer = 0; layer < 2; ++layer) {
     const float* bias = pk32 + (layer == 0 ? PK_B2 : PK_B3);
     bf16x8 bout[KB];
 #pragma unroll
-    for (int half = 0; half < 2; ++half) {
-      const bf16x8* W = wsrc.chunk(2 * layer + half);
+    for (int qc = 0; qc < Weights::NQ; ++qc) {
+      const bf16x8* W = wsrc.chunk(Weights::NQ * layer + qc);
 #pragma unroll
-      for (int jj = 0; jj < KB / 2; ++jj) {
-        const int j = (KB / 2) * half + jj;
+      for (int jj = 0; jj < KB / Weights::NQ; ++jj) {
+        const int j = (KB / Weights::NQ) * qc + jj;
         f32x4 a0, a1;
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
@@ -176,11 +177,11 @@ __device__ __forceinline__ void mlp_tile_input_vjp(Weights& wsrc, const float* _
     bf16x8* sgo = layer == 0 ? sg2 : sg3;
     bf16x8 bout[KB];
 #pragma unroll
-    for (int half = 0; half < 2; ++half) {
-      const bf16x8* W = wsrc.chunk(2 * layer + half);
+    for (int qc = 0; qc < Weights::NQ; ++qc) {
+      const bf16x8* W = wsrc.chunk(Weights::NQ * layer + qc);
 #pragma unroll
-      for (int jj = 0; jj < KB / 2; ++jj) {
-        const int j = (KB / 2) * half + jj;
+      for (int jj = 0; jj < KB / Weights::NQ; ++jj) {
+        const int j = (KB / Weights::NQ) * qc + jj;
         f32x4 a0, a1;
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
@@ -250,11 +251,11 @@ __device__ __forceinline__ void mlp_tile_input_vjp(Weights& wsrc, const float* _
   for (int layer = 0; layer < 2; ++layer) {
     bf16x8 dnext[KB];
 #pragma unroll
-    for (int half = 0; half < 2; ++half) {
-      const bf16x8* W = wsrc.chunkT(2 * layer + half);
+    for (int qc = 0; qc < Weights::NQ; ++qc) {
+      const bf16x8* W = wsrc.chunkT(Weights::NQ * layer + qc);
 #pragma unroll
-      for (int jj = 0; jj < KB / 2; ++jj) {
-        const int j = (KB / 2) * half + jj;
+      for (int jj = 0; jj < KB / Weights::NQ; ++jj) {
+        const int j = (KB / Weights::NQ) * qc + jj;
         f32x4 a0 = {0.0f, 0.0f, 0.0f, 0.0f}, a1 = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
         for (int q = 0; q < KB; ++q) {

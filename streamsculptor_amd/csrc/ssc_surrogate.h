@@ -55,33 +55,36 @@ __device__ __forceinline__ bool surrogate_fallback(const SurrogateDesc& sd,
   return flags[cell] != 0;
 }
 
-// The chained 256x256 layers stream through two 64 KB LDS buffers in chunks of eight 16-row
-// tiles.  Each thread holds its share (PIECES x 16 B) of the next chunk in registers while the
-// current chunk is computed, so the L2 latency hides behind a chunk of MFMAs.  One barrier per
-// chunk suffices: chunk c overwrites the buffer that chunk c - 2 was read from, and every wave
-// finished that read before it reached the barrier of chunk c - 1, which the writer has passed.
+// The chained 256x256 layers stream through two LDS buffers in chunks of 16 / NQ tiles
+// (128 / NQ KB each).  Each thread holds its share (PIECES x 16 B) of the next chunk in
+// registers while the current chunk is computed, so the L2 latency hides behind a chunk of
+// MFMAs.  One barrier per chunk suffices: chunk c overwrites the buffer that chunk c - 2 was read
+// from, and every wave finished that read before it reached the barrier of chunk c - 1, which the
+// writer has passed (the number of chunks per call is even, so the buffer parity carries over
+// between calls).
 //
-// NCH = 4: the forward's chunks W2 tiles 0-7, W2 8-15, W3 0-7, W3 8-15 (gw = W2 | W3).
-// NCH = 8: the input VJP's sequence, the same four followed by W3^T 0-7, W3^T 8-15, W2^T 0-7,
-// W2^T 8-15 (gwt = W3^T | W2^T); chunkT(c) is chunk 4 + c.
-template <int BLOCK, int NCH>
+// VJP = false: the forward's chunks, W2 then W3 (gw = W2 | W3), 2 NQ per call.
+// VJP = true: the input VJP's sequence, the same followed by W3^T then W2^T (gwt = W3^T | W2^T),
+// 4 NQ per call; chunkT(c) is chunk 2 NQ + c.
+template <int BLOCK, bool VJP, int NQ_ = 2>
 struct LdsWeights {
-  static constexpr int CH = (int)(mlp::FRAG_L / 2 / 8);  // 16-byte fragments per chunk
+  static constexpr int NQ = NQ_;
+  static constexpr int CH = (int)(mlp::FRAG_L / NQ / 8);  // 16-byte fragments per chunk
+  static constexpr int NCH = (VJP ? 4 : 2) * NQ;          // chunks per network call
   static constexpr int PIECES = (CH + BLOCK - 1) / BLOCK;
   static constexpr bool EVEN = CH % BLOCK == 0;           // else the last piece is partial
-  static_assert(NCH == 4 || NCH == 8, "four forward chunks, or forward and transposed");
   mlp::bf16x8* buf;        // two chunk buffers of CH fragments
   const mlp::bf16x8* w4;   // output-layer fragments (resident)
-  const float* w4t_;       // fp32 W4^T table (resident; NCH = 8 only)
+  const float* w4t_;       // fp32 W4^T table (resident; VJP only)
   const mlp::bf16x8* gw;   // global fragments of W2 followed by W3
-  const mlp::bf16x8* gwt;  // global fragments of W3^T followed by W2^T (NCH = 8 only)
+  const mlp::bf16x8* gwt;  // global fragments of W3^T followed by W2^T (VJP only)
   mlp::bf16x8 pre[PIECES];
   // The thread index is made opaque so the PIECES x NCH source addresses (loop-invariant in an
   // integrator's step loop) are recomputed per call instead of being hoisted and kept live.
   __device__ void prefetch(int c) {
     int t = threadIdx.x;
     asm volatile("" : "+v"(t));
-    const mlp::bf16x8* src = c < 4 ? gw + (int64_t)c * CH : gwt + (int64_t)(c - 4) * CH;
+    const mlp::bf16x8* src = c < 2 * NQ ? gw + (int64_t)c * CH : gwt + (int64_t)(c - 2 * NQ) * CH;
 #pragma unroll
     for (int k = 0; k < PIECES; ++k)
       if (EVEN || k + 1 < PIECES || t + k * BLOCK < CH) pre[k] = src[t + k * BLOCK];
@@ -94,10 +97,10 @@ struct LdsWeights {
     for (int k = 0; k < PIECES; ++k)
       if (EVEN || k + 1 < PIECES || t + k * BLOCK < CH) dst[t + k * BLOCK] = pre[k];
     __syncthreads();
-    prefetch((c + 1) & (NCH - 1));
+    prefetch(c + 1 == NCH ? 0 : c + 1);
     return dst;
   }
-  __device__ const mlp::bf16x8* chunkT(int c) { return chunk(4 + c); }
+  __device__ const mlp::bf16x8* chunkT(int c) { return chunk(2 * NQ + c); }
   __device__ const mlp::bf16x8* out() const { return w4; }
   __device__ const float* w4t() const { return w4t_; }
 };

@@ -77,8 +77,8 @@ __global__ __launch_bounds__(SG_BLOCK) void surrogate_leapfrog_kernel(
   __syncthreads();
   const int kmin = kmin_sh;
 
-  LdsWeights<SG_BLOCK, 4> lw{wbuf, w4, nullptr, reinterpret_cast<const bf16x8*>(pk16 + PK_W2),
-                             nullptr, {}};
+  LdsWeights<SG_BLOCK, false> lw{wbuf, w4, nullptr, reinterpret_cast<const bf16x8*>(pk16 + PK_W2),
+                                 nullptr, {}};
   lw.prefetch(0);
   unsigned long long nfall = 0;
 

@@ -35,10 +35,11 @@ int launch_surrogate_leapfrog(const ssc_host_potential* host, const ssc_sources*
 
 constexpr int SA_BLOCK = 256;            // 4 waves
 constexpr int SA_PARTS = SA_BLOCK / 4;   // 64 particles per block
+constexpr int SA_NQ = 4;                 // weight chunks per chained layer
 constexpr int SA_MAXC = 2 * SSC_MAX_COMPONENTS;
 
 // dynamic LDS layout (bytes)
-constexpr size_t SA_LDS_W = (size_t)mlp::FRAG_L * 2;                    // two chunk buffers
+constexpr size_t SA_LDS_W = (size_t)mlp::FRAG_L * 2 * 2 / SA_NQ;       // two chunk buffers
 constexpr size_t SA_LDS_W4 = SA_LDS_W;                                  // W4 fragments
 constexpr size_t SA_LDS_W4T = SA_LDS_W4 + (size_t)mlp::FRAG_4 * 2;      // fp32 W4^T
 constexpr size_t SA_LDS_PK32 = SA_LDS_W4T + (size_t)mlp::H * mlp::OUT * 4;
@@ -123,8 +124,8 @@ __global__ __launch_bounds__(B) void surrogate_adjoint_segment_kernel(
       for (int e = threadIdx.x; e < (int)PK_F32_END; e += B) pk32[e] = pk32g[e];
     }
     __syncthreads();
-    LdsWeights<B, 8> lw{wbuf, w4, w4t, reinterpret_cast<const bf16x8*>(pk16 + PK_W2),
-                        reinterpret_cast<const bf16x8*>(pkv16 + PKV_W3T), {}};
+    LdsWeights<B, true, SA_NQ> lw{wbuf, w4, w4t, reinterpret_cast<const bf16x8*>(pk16 + PK_W2),
+                                  reinterpret_cast<const bf16x8*>(pkv16 + PKV_W3T), {}};
     lw.prefetch(0);
     const double h = 0.5 * dt;
 
