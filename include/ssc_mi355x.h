@@ -61,14 +61,19 @@ typedef struct ssc_host_potential {
   ssc_component c[SSC_MAX_COMPONENTS];
 } ssc_host_potential;
 
-/* Plummer point sources summed directly (N particles x M sources).
+/* Point sources summed directly (N particles x M sources), Plummer profiles unless stated.
  *   STATIC: pos[n][3], fixed in time (clumps).
  *   TRACK:  pos[n_track][n][3], source positions sampled on the integration grid: row k is the
  *           position at t_k = t0 + k*dt (subhalos on their own orbits).  n_track >= nsteps + 1.
  *   LINE:   straight-line subhalos, position pos[j] + vel[j]*(t - t0[j]), active only while
  *           |t - t0[j]| < t_window[j] (reference SubhaloLinePotential,
- *           streamsculptor/potential.py:802-830).                                            */
-enum { SSC_SRC_NONE = 0, SSC_SRC_STATIC = 1, SSC_SRC_TRACK = 2, SSC_SRC_LINE = 3 };
+ *           streamsculptor/potential.py:802-830).
+ *   LINE_HERNQUIST: LINE geometry with Hernquist profiles, Phi = -G m / (r + rs) (reference
+ *           SubhaloLinePotentialCustom_fromFunc with func=HernquistPotential,
+ *           streamsculptor/potential.py:1161-1213).  Evaluation, leapfrog and Dopri5 only: the
+ *           response and adjoint kernels differentiate Plummer sources.                      */
+enum { SSC_SRC_NONE = 0, SSC_SRC_STATIC = 1, SSC_SRC_TRACK = 2, SSC_SRC_LINE = 3,
+       SSC_SRC_LINE_HERNQUIST = 4 };
 
 #define SSC_MAX_SOURCES 1024
 
@@ -79,14 +84,14 @@ typedef struct ssc_sources {
   int32_t group_size;   /* 0: every orbit sees all n sources.  > 0: the sources form
                            consecutive groups of group_size and orbit i sees only group
                            group[i] -- a batch of independent simulations in one call
-                           (ssc_dopri5 only; STATIC or LINE; n may then exceed
-                           SSC_MAX_SOURCES) */
+                           (ssc_dopri5 only; STATIC, LINE or LINE_HERNQUIST; n may
+                           then exceed SSC_MAX_SOURCES) */
   const double* gm;     /* [n] G*m (device) */
-  const double* rs;     /* [n] Plummer scale radius (device) */
-  const double* pos;    /* STATIC/LINE: [n][3]; TRACK: [n_track][n][3] (device) */
-  const double* vel;    /* LINE: [n][3] (device) */
-  const double* t0;     /* LINE: [n] (device) */
-  const double* t_window; /* LINE: [n] (device) */
+  const double* rs;     /* [n] scale radius of the profile (device) */
+  const double* pos;    /* STATIC/LINE*: [n][3]; TRACK: [n_track][n][3] (device) */
+  const double* vel;    /* LINE*: [n][3] (device) */
+  const double* t0;     /* LINE*: [n] (device) */
+  const double* t_window; /* LINE*: [n] (device) */
   const int32_t* group; /* group_size > 0: [orbits] group index of each orbit (device) */
 } ssc_sources;
 

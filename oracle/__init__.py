@@ -43,7 +43,7 @@ _FAST_PATH = os.path.join(_HERE, "libssc_oracle_fast.so")
 G = 4.498502151469553e-12  # kpc^3 / (Msun Myr^2)
 
 COMP_MN, COMP_NFW, COMP_HERNQUIST, COMP_PLUMMER = 1, 2, 3, 4
-SRC_NONE, SRC_STATIC, SRC_TRACK, SRC_LINE = 0, 1, 2, 3
+SRC_NONE, SRC_STATIC, SRC_TRACK, SRC_LINE, SRC_LINE_HERNQUIST = 0, 1, 2, 3, 4
 MAX_COMPONENTS = 8
 
 
@@ -159,7 +159,7 @@ class Sources:
             s.n_track = pos.shape[0] if kind == SRC_TRACK else 0
             self.arrays.update(gm=gm, rs=rs, pos=pos)
             s.gm, s.rs, s.pos = gm.ctypes.data, rs.ctypes.data, pos.ctypes.data
-            if kind == SRC_LINE:
+            if kind in (SRC_LINE, SRC_LINE_HERNQUIST):
                 vel = np.ascontiguousarray(vel, dtype=np.float64)
                 t0 = np.ascontiguousarray(t0, dtype=np.float64)
                 t_window = np.ascontiguousarray(t_window, dtype=np.float64)

@@ -216,7 +216,7 @@ bool source_pos(const ssc_sources* s, int j, int step, double t, double out[3]) 
     for (int i = 0; i < 3; ++i) out[i] = row[i];
     return true;
   }
-  if (s->kind == SSC_SRC_LINE) {
+  if (s->kind == SSC_SRC_LINE || s->kind == SSC_SRC_LINE_HERNQUIST) {
     double dtt = t - s->t0[j];
     if (!(std::fabs(dtt) < s->t_window[j])) return false;
     for (int i = 0; i < 3; ++i) out[i] = s->pos[3 * j + i] + s->vel[3 * j + i] * dtt;
@@ -225,8 +225,9 @@ bool source_pos(const ssc_sources* s, int j, int step, double t, double out[3]) 
   return false;
 }
 
-// Direct N x M Plummer sum (reference SubhaloLinePotential.potential,
-// streamsculptor/potential.py:813-830, which sums Plummer spheres over subhalos).
+// Direct N x M sum over Plummer spheres (reference SubhaloLinePotential.potential,
+// streamsculptor/potential.py:813-830) or, for LINE_HERNQUIST, Hernquist spheres (reference
+// SubhaloLinePotentialCustom_fromFunc with func=HernquistPotential, potential.py:1161-1190).
 void sources_eval(const ssc_sources* s, const double x[3], int step, double t,
                   double* phi, double g[3], double H[9]) {
   if (!s || s->kind == SSC_SRC_NONE) return;
@@ -234,7 +235,12 @@ void sources_eval(const ssc_sources* s, const double x[3], int step, double t,
     double sp[3];
     if (!source_pos(s, j, step, t, sp)) continue;
     double d[3] = {x[0] - sp[0], x[1] - sp[1], x[2] - sp[2]};
-    plummer_eval<double>(s->gm[j], s->rs[j], d, phi, g, H);
+    if (s->kind == SSC_SRC_LINE_HERNQUIST) {
+      const double p[4] = {s->gm[j], s->rs[j], 0.0, 0.0};
+      hernquist_eval<double>(p, d, phi, g, H);
+    } else {
+      plummer_eval<double>(s->gm[j], s->rs[j], d, phi, g, H);
+    }
   }
 }
 

@@ -16,7 +16,7 @@ LIB_PATH = os.path.join(HERE, "libssc_mi355x.so")
 G_USYS = 4.498502151469553e-12  # G in kpc^3 / (Msun Myr^2)
 
 COMP_MIYAMOTO_NAGAI, COMP_NFW, COMP_HERNQUIST, COMP_PLUMMER = 1, 2, 3, 4
-SRC_NONE, SRC_STATIC, SRC_TRACK, SRC_LINE = 0, 1, 2, 3
+SRC_NONE, SRC_STATIC, SRC_TRACK, SRC_LINE, SRC_LINE_HERNQUIST = 0, 1, 2, 3, 4
 MAX_COMPONENTS = 8
 MAX_SOURCES = 1024
 

@@ -23,11 +23,17 @@ def _arr(x, dtype=np.float64):
 
 
 class SubhaloLineBatch:
-    """B simulations with nsh straight-line Plummer subhalos each (one SubhaloLinePotential per
-    simulation, reference streamsculptor/potential.py:802-830).  Arrays: m, a, subhalo_t0,
-    t_window (B, nsh); subhalo_x0, subhalo_v (B, nsh, 3)."""
+    """B simulations with nsh straight-line subhalos each (one SubhaloLinePotential per
+    simulation, reference streamsculptor/potential.py:802-830; profile="hernquist" gives
+    Hernquist subhalos, as SubhaloLinePotentialCustom_fromFunc with HernquistPotential,
+    :1161-1213).  Arrays: m, a, subhalo_t0, t_window (B, nsh); subhalo_x0, subhalo_v
+    (B, nsh, 3)."""
 
-    def __init__(self, m, a, subhalo_x0, subhalo_v, subhalo_t0, t_window, units=None):
+    def __init__(self, m, a, subhalo_x0, subhalo_v, subhalo_t0, t_window, units=None,
+                 profile="plummer"):
+        if profile not in ("plummer", "hernquist"):
+            raise NotImplementedError("subhalo profiles with a kernel form: plummer, hernquist")
+        self.profile = profile
         m = _arr(m)
         if m.ndim != 2:
             raise ValueError("m must have shape (B, nsh)")
@@ -48,13 +54,19 @@ class SubhaloLineBatch:
         return self.B
 
     def __getitem__(self, b):
-        """The SubhaloLinePotential of simulation b."""
+        """The subhalo potential of simulation b."""
+        if self.profile == "hernquist":
+            from .potential import HernquistPotential, SubhaloLinePotentialCustom_fromFunc
+            return SubhaloLinePotentialCustom_fromFunc(
+                HernquistPotential, self.m[b], self.a[b], self.subhalo_x0[b], self.subhalo_v[b],
+                self.subhalo_t0[b], self.t_window[b], units=self.units)
         return SubhaloLinePotential(self.m[b], self.a[b], self.subhalo_x0[b], self.subhalo_v[b],
                                     self.subhalo_t0[b], self.t_window[b], units=self.units)
 
     def sources(self):
         if self._src is None:
-            self._src = SourceSet(_lib.SRC_LINE, gm=self._G * self.m.reshape(-1),
+            kind = _lib.SRC_LINE_HERNQUIST if self.profile == "hernquist" else _lib.SRC_LINE
+            self._src = SourceSet(kind, gm=self._G * self.m.reshape(-1),
                                   rs=self.a.reshape(-1), pos=self.subhalo_x0.reshape(-1, 3),
                                   vel=self.subhalo_v.reshape(-1, 3),
                                   t0=self.subhalo_t0.reshape(-1),

@@ -471,7 +471,8 @@ int ssc_adjoint_backward(const ssc_host_potential* host, const ssc_sources* src,
       return adjoint_backward<SSC_SRC_TRACK>(*host, s, start_step, n, t0, dt, nsteps, checkpoint_every,
                                              lam_T, forcing, lam_0, g_host, g_gm, g_pos, w, st);
     default:
-      return fail(SSC_E_ARG, "ssc_adjoint_backward: LINE sources are not differentiated");
+      return fail(SSC_E_ARG, "ssc_adjoint_backward: LINE sources (Plummer or Hernquist) are not "
+                             "differentiated");
   }
 }
 

@@ -52,8 +52,9 @@ int validate_sources(const ssc_sources* src, int32_t nsteps_needed, bool allow_g
     if (src->n < 0 || src->n % src->group_size != 0)
       return fail(SSC_E_ARG, "sources: n must be a multiple of group_size");
     if (!src->group) return fail(SSC_E_ARG, "sources: group_size > 0 needs the group array");
-    if (src->kind != SSC_SRC_STATIC && src->kind != SSC_SRC_LINE)
-      return fail(SSC_E_ARG, "sources: grouped sources must be STATIC or LINE");
+    if (src->kind != SSC_SRC_STATIC && src->kind != SSC_SRC_LINE &&
+        src->kind != SSC_SRC_LINE_HERNQUIST)
+      return fail(SSC_E_ARG, "sources: grouped sources must be STATIC, LINE or LINE_HERNQUIST");
   } else if (src->n < 0 || src->n > SSC_MAX_SOURCES) {
     return fail(SSC_E_LIMIT, "sources: n out of range [0, " + std::to_string(SSC_MAX_SOURCES) + "]");
   }
@@ -61,9 +62,10 @@ int validate_sources(const ssc_sources* src, int32_t nsteps_needed, bool allow_g
   if (src->kind == SSC_SRC_TRACK && src->n_track < nsteps_needed + 1)
     return fail(SSC_E_ARG, "sources: TRACK table has " + std::to_string(src->n_track) +
                                " rows, the integration needs " + std::to_string(nsteps_needed + 1));
-  if (src->kind == SSC_SRC_LINE && (!src->vel || !src->t0 || !src->t_window))
+  if ((src->kind == SSC_SRC_LINE || src->kind == SSC_SRC_LINE_HERNQUIST) &&
+      (!src->vel || !src->t0 || !src->t_window))
     return fail(SSC_E_ARG, "sources: LINE needs vel, t0 and t_window");
-  if (src->kind < SSC_SRC_NONE || src->kind > SSC_SRC_LINE)
+  if (src->kind < SSC_SRC_NONE || src->kind > SSC_SRC_LINE_HERNQUIST)
     return fail(SSC_E_ARG, "sources: unknown kind " + std::to_string(src->kind));
   return SSC_OK;
 }
@@ -92,7 +94,10 @@ __global__ __launch_bounds__(256) void eval_kernel(ssc_host_potential hp, ssc_so
       double sx, sy, sz;
       if (!source_position(src, j, step, t, sx, sy, sz)) continue;
       const double rs = src.rs[j];
-      plummer_full(x - sx, y - sy, z - sz, src.gm[j], rs * rs, phi, g, H);
+      if (src.kind == SSC_SRC_LINE_HERNQUIST)
+        hernquist_full(x - sx, y - sy, z - sz, src.gm[j], rs, phi, g, H);
+      else
+        plummer_full(x - sx, y - sy, z - sz, src.gm[j], rs * rs, phi, g, H);
     }
   }
   if (phi_out) phi_out[i] = phi;
@@ -483,6 +488,7 @@ int ssc_leapfrog(const ssc_host_potential* host, const ssc_sources* src, const d
     case SSC_SRC_STATIC: return SSC_LF(SSC_SRC_STATIC);
     case SSC_SRC_TRACK: return SSC_LF(SSC_SRC_TRACK);
     case SSC_SRC_LINE: return SSC_LF(SSC_SRC_LINE);
+    case SSC_SRC_LINE_HERNQUIST: return SSC_LF(SSC_SRC_LINE_HERNQUIST);
     default: return fail(SSC_E_ARG, "ssc_leapfrog: bad source kind");
   }
 #undef SSC_LF

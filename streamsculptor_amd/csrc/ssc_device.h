@@ -339,6 +339,42 @@ __device__ __forceinline__ void plummer_full(double dx, double dy, double dz, do
   H[8] += f + k * dz * dz;
 }
 
+// Hernquist source (x - s), Phi = -G m / (r + a): gradient contribution.
+__device__ __forceinline__ void hernquist_grad(double dx, double dy, double dz, double gm,
+                                               double a, double& gx, double& gy, double& gz) {
+  const double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
+  const double ir = rsq(r2);
+  const double rc = fma(r2, ir, a);
+  const double f = gm * ir / (rc * rc);
+  gx += f * dx;
+  gy += f * dy;
+  gz += f * dz;
+}
+
+// Hernquist source: potential, gradient and full Hessian contribution
+// (H = f I + k d d^T, f = G m / (r (r + a)^2), k = -f (3 r + a) / (r^2 (r + a))).
+__device__ __forceinline__ void hernquist_full(double dx, double dy, double dz, double gm,
+                                               double a, double& phi, double* g, double* H) {
+  const double r2 = dx * dx + dy * dy + dz * dz;
+  const double r = sqrt(r2);
+  const double rc = r + a;
+  const double f = gm / (r * rc * rc);
+  phi -= gm / rc;
+  g[0] += f * dx;
+  g[1] += f * dy;
+  g[2] += f * dz;
+  const double k = -f * (3.0 * r + a) / (r2 * rc);
+  H[0] += f + k * dx * dx;
+  H[1] += k * dx * dy;
+  H[2] += k * dx * dz;
+  H[3] += k * dx * dy;
+  H[4] += f + k * dy * dy;
+  H[5] += k * dy * dz;
+  H[6] += k * dx * dz;
+  H[7] += k * dy * dz;
+  H[8] += f + k * dz * dz;
+}
+
 // Position of source j at grid row `step` / time t; false when inactive (LINE window,
 // reference streamsculptor/potential.py:826).
 __device__ __forceinline__ bool source_position(const ssc_sources& s, int j, int step, double t,
@@ -356,7 +392,7 @@ __device__ __forceinline__ bool source_position(const ssc_sources& s, int j, int
     sz = row[2];
     return true;
   }
-  if (s.kind == SSC_SRC_LINE) {
+  if (s.kind == SSC_SRC_LINE || s.kind == SSC_SRC_LINE_HERNQUIST) {
     const double dt = t - s.t0[j];
     if (!(fabs(dt) < s.t_window[j])) return false;
     sx = s.pos[3 * j] + s.vel[3 * j] * dt;

@@ -79,16 +79,17 @@ struct GroupSources {
     gm = s.gm + j0;
     rs = s.rs + j0;
     pos = s.pos + 3 * j0;
-    vel = SK == SSC_SRC_LINE ? s.vel + 3 * j0 : nullptr;
-    t0 = SK == SSC_SRC_LINE ? s.t0 + j0 : nullptr;
-    tw = SK == SSC_SRC_LINE ? s.t_window + j0 : nullptr;
+    vel = LINE ? s.vel + 3 * j0 : nullptr;
+    t0 = LINE ? s.t0 + j0 : nullptr;
+    tw = LINE ? s.t_window + j0 : nullptr;
     return true;
   }
+  static constexpr bool LINE = SK == SSC_SRC_LINE || SK == SSC_SRC_LINE_HERNQUIST;
   __device__ __forceinline__ void add(double t, double x, double y, double z, double& gx,
                                       double& gy, double& gz) const {
     for (int j = 0; j < n; ++j) {
       double sx = pos[3 * j], sy = pos[3 * j + 1], sz = pos[3 * j + 2];
-      if (SK == SSC_SRC_LINE) {
+      if (LINE) {
         const double dt = t - t0[j];
         if (!(fabs(dt) < tw[j])) continue;
         sx = fma(vel[3 * j], dt, sx);
@@ -96,7 +97,10 @@ struct GroupSources {
         sz = fma(vel[3 * j + 2], dt, sz);
       }
       const double a = rs[j];
-      plummer_grad(x - sx, y - sy, z - sz, gm[j], a * a, gx, gy, gz);
+      if (SK == SSC_SRC_LINE_HERNQUIST)
+        hernquist_grad(x - sx, y - sy, z - sz, gm[j], a, gx, gy, gz);
+      else
+        plummer_grad(x - sx, y - sy, z - sz, gm[j], a * a, gx, gy, gz);
     }
   }
 };
@@ -427,6 +431,8 @@ static int dopri5_entry(const ssc_host_potential* host, const ssc_sources* src, 
     case SSC_SRC_NONE: return SSC_DP(SSC_SRC_NONE, false);
     case SSC_SRC_STATIC: return grouped ? SSC_DP(SSC_SRC_STATIC, true) : SSC_DP(SSC_SRC_STATIC, false);
     case SSC_SRC_LINE: return grouped ? SSC_DP(SSC_SRC_LINE, true) : SSC_DP(SSC_SRC_LINE, false);
+    case SSC_SRC_LINE_HERNQUIST:
+      return grouped ? SSC_DP(SSC_SRC_LINE_HERNQUIST, true) : SSC_DP(SSC_SRC_LINE_HERNQUIST, false);
     default: return fail(SSC_E_ARG, "ssc_dopri5: bad source kind");
   }
 #undef SSC_DP

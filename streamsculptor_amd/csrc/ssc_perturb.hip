@@ -226,5 +226,8 @@ extern "C" int ssc_perturb_leapfrog(const ssc_host_potential* host, const ssc_so
     }
   }
 #undef SSC_PT
+  if (s.kind == SSC_SRC_LINE_HERNQUIST)
+    return fail(SSC_E_ARG, "ssc_perturb_leapfrog: responses are built for Plummer sources; "
+                           "Hernquist subhalos are supported by the forward integrators only");
   return fail(SSC_E_ARG, "ssc_perturb_leapfrog: bad source kind");
 }

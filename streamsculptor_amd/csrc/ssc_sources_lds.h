@@ -2,6 +2,7 @@
 //   STATIC: rec[j] = (x, y, z, G*m), rs2[j]           (loaded once)
 //   TRACK:  rows kb..kb+chunk of the trajectory table, rec[r][j] = (x, y, z, G*m)
 //   LINE:   (x0, v, G*m, r_s^2, t0, t_window) per source, position formed per step
+//   LINE_HERNQUIST: the same with r_s in place of r_s^2 and the Hernquist term
 // add_grad(j0, js) sums sources j0, j0 + js, ... so a wave can also split one orbit's sum;
 // get(k, t, j, ...) returns one source's position/G*m/r_s^2 and whether it is active.
 #pragma once
@@ -135,11 +136,12 @@ struct SourceLDS<SSC_SRC_TRACK> {
 };
 
 struct alignas(16) LineRec {
-  double x0, y0, z0, gm, vx, vy, vz, rs2, t0, tw;
+  double x0, y0, z0, gm, vx, vy, vz, q, t0, tw;  // q: r_s^2 (Plummer) or r_s (Hernquist)
 };
 
-template <>
-struct SourceLDS<SSC_SRC_LINE> {
+// Straight-line sources with Plummer (HQ = false) or Hernquist (HQ = true) profiles.
+template <bool HQ>
+struct LineLDS {
   LineRec* rec;
   int n;
   static size_t bytes(const ssc_sources& s) { return (size_t)s.n * sizeof(LineRec); }
@@ -148,8 +150,8 @@ struct SourceLDS<SSC_SRC_LINE> {
     rec = reinterpret_cast<LineRec*>(lds);
     for (int j = threadIdx.x; j < n; j += blockDim.x)
       rec[j] = LineRec{s.pos[3 * j], s.pos[3 * j + 1], s.pos[3 * j + 2], s.gm[j],
-                       s.vel[3 * j], s.vel[3 * j + 1], s.vel[3 * j + 2], s.rs[j] * s.rs[j],
-                       s.t0[j], s.t_window[j]};
+                       s.vel[3 * j], s.vel[3 * j + 1], s.vel[3 * j + 2],
+                       HQ ? s.rs[j] : s.rs[j] * s.rs[j], s.t0[j], s.t_window[j]};
     __syncthreads();
   }
   __device__ void stage(const ssc_sources&, int, int) {}
@@ -163,18 +165,29 @@ struct SourceLDS<SSC_SRC_LINE> {
       if (!(fabs(dt) < r.tw)) continue;
       const double sx = r.x0 + r.vx * dt, sy = r.y0 + r.vy * dt, sz = r.z0 + r.vz * dt;
 #pragma unroll
-      for (int p = 0; p < P; ++p) plummer_grad(x[p] - sx, y[p] - sy, z[p] - sz, r.gm, r.rs2, gx[p], gy[p], gz[p]);
+      for (int p = 0; p < P; ++p) {
+        if (HQ)
+          hernquist_grad(x[p] - sx, y[p] - sy, z[p] - sz, r.gm, r.q, gx[p], gy[p], gz[p]);
+        else
+          plummer_grad(x[p] - sx, y[p] - sy, z[p] - sz, r.gm, r.q, gx[p], gy[p], gz[p]);
+      }
     }
   }
+  // Plummer only (the response and adjoint kernels): position, G m, r_s^2, active.
   __device__ bool get(int, double t, int j, double& sx, double& sy, double& sz, double& gm,
                       double& q) const {
     const LineRec r = rec[j];
     const double dt = t - r.t0;
     sx = r.x0 + r.vx * dt; sy = r.y0 + r.vy * dt; sz = r.z0 + r.vz * dt;
-    gm = r.gm; q = r.rs2;
+    gm = r.gm; q = r.q;
     return fabs(dt) < r.tw;
   }
 };
 
+template <>
+struct SourceLDS<SSC_SRC_LINE> : LineLDS<false> {};
+
+template <>
+struct SourceLDS<SSC_SRC_LINE_HERNQUIST> : LineLDS<true> {};
 
 }  // namespace ssc

@@ -101,7 +101,7 @@ class SourceSet:
     def desc_grouped(self, group_size, group):
         """Descriptor of these sources as consecutive groups of ``group_size``; orbit i sees
         group ``group[i]`` (a device int32 tensor the caller keeps alive for the call)."""
-        if self.kind not in (_lib.SRC_STATIC, _lib.SRC_LINE):
+        if self.kind not in (_lib.SRC_STATIC, _lib.SRC_LINE, _lib.SRC_LINE_HERNQUIST):
             raise NotImplementedError("grouped sources must be STATIC or LINE")
         t = self.tensors()
         s = _lib.Sources()

@@ -127,6 +127,45 @@ class SubhaloLinePotential(Potential):
         return _per_source(self, xyz, t, "phi")
 
 
+class SubhaloLinePotentialCustom_fromFunc(Potential):
+    """Straight-line subhalos with the profile of a potential class (reference
+    streamsculptor/potential.py:1161-1213): ``func`` is ``PlummerPotential`` or
+    ``HernquistPotential`` (the profiles with a kernel form; others raise), instantiated per
+    subhalo as func(m=m_j, r_s=r_s_j) at x - (x0_j + v_j (t - t0_j)) while |t - t0_j| < t_window.
+    Hernquist subhalos run in the evaluation, leapfrog and Dopri5 kernels; the first-order
+    response and adjoint kernels differentiate Plummer subhalos only."""
+
+    def __init__(self, func, m, r_s, subhalo_x0, subhalo_v, subhalo_t0, t_window, units=None):
+        super().__init__(units, {"func": func, "m": m, "r_s": r_s, "subhalo_x0": subhalo_x0,
+                                 "subhalo_v": subhalo_v, "subhalo_t0": subhalo_t0,
+                                 "t_window": t_window})
+        if func is PlummerPotential:
+            self._kind = _lib.SRC_LINE
+        elif func is HernquistPotential:
+            self._kind = _lib.SRC_LINE_HERNQUIST
+        else:
+            raise NotImplementedError(
+                f"subhalo profile {getattr(func, '__name__', func)!r} has no kernel form; "
+                "use PlummerPotential or HernquistPotential")
+
+    def _kernel_components(self):
+        return []
+
+    def _kernel_sources(self):
+        m = np.atleast_1d(_arr(self.m))
+        n = m.shape[0]
+        return SourceSet(self._kind, gm=self._G * m,
+                         rs=np.broadcast_to(np.atleast_1d(_arr(self.r_s)), (n,)).copy(),
+                         pos=_arr(self.subhalo_x0).reshape(n, 3),
+                         vel=_arr(self.subhalo_v).reshape(n, 3),
+                         t0=np.broadcast_to(np.atleast_1d(_arr(self.subhalo_t0)), (n,)).copy(),
+                         t_window=np.broadcast_to(np.atleast_1d(_arr(self.t_window)), (n,)).copy())
+
+    def potential_per_SH(self, xyz, t):
+        """Per-subhalo potential at xyz (reference streamsculptor/potential.py:1193-1213)."""
+        return _per_source(self, xyz, t, "phi")
+
+
 class SubhaloLinePotential_dRadius(Potential):
     """d Phi / d a of the straight-line Plummer subhalos of a SubhaloLinePotential (reference
     streamsculptor/potential.py:852-904): the structural potential whose gradient drives the
@@ -239,5 +278,6 @@ class _SourceOnly(Potential):
 
 __all__ = ["MiyamotoNagaiDisk", "NFWPotential", "PlummerPotential", "HernquistPotential",
            "Potential_Combine", "GalaMilkyWayPotential", "CustomPotential",
-           "SubhaloLinePotential", "SubhaloLinePotential_dRadius", "PlummerClumps",
+           "SubhaloLinePotential", "SubhaloLinePotentialCustom_fromFunc",
+           "SubhaloLinePotential_dRadius", "PlummerClumps",
            "LiveSubhaloPotential", "usys"]

@@ -180,3 +180,30 @@ def test_sources_superposition_and_symmetry(oracle):
     np.testing.assert_allclose(gl, oracle.evaluate(h0, pts, src=moved)[1], rtol=1e-13)
     _, gout, _ = oracle.evaluate(h0, pts, t=60.0, src=line)
     assert np.all(gout == 0)
+
+
+def test_hernquist_line_source_equals_shifted_component(oracle):
+    """A LINE_HERNQUIST source at rest inside its window is a Hernquist component centred on it
+    (the component's Phi, gradient and Hessian are pinned above); outside the window it is
+    absent; a moving one sits at x0 + v (t - t0)."""
+    import numpy as np
+    rng = np.random.default_rng(21)
+    G = 4.498502151469553e-12
+    gm, a = G * 3e8, 0.7
+    x0 = np.array([1.5, -2.0, 0.5])
+    v = np.array([0.1, 0.05, -0.2])
+    src = oracle.Sources(oracle.SRC_LINE_HERNQUIST, gm=np.array([gm]), rs=np.array([a]),
+                         pos=x0[None], vel=v[None], t0=np.array([10.0]),
+                         t_window=np.array([50.0]))
+    empty = oracle.host([])
+    comp = oracle.host([(oracle.COMP_HERNQUIST, [gm, a, 0.0])])
+    x = rng.normal(scale=4.0, size=(40, 3))
+    for t in (10.0, 35.0):
+        c = x0 + v * (t - 10.0)
+        p1, g1, H1 = oracle.evaluate(empty, x, t=t, src=src)
+        p2, g2, H2 = oracle.evaluate(comp, x - c)
+        np.testing.assert_allclose(p1, p2, rtol=1e-14)
+        np.testing.assert_allclose(g1, g2, rtol=1e-13)
+        np.testing.assert_allclose(H1, H2, rtol=1e-12, atol=1e-30)
+    p3, g3, _ = oracle.evaluate(empty, x, t=70.0, src=src)
+    assert np.all(p3 == 0.0) and np.all(g3 == 0.0)
