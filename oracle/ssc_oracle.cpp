@@ -761,7 +761,9 @@ int orc_release_jacobian(const ssc_host_potential* hp, const double* prog_w, con
 // KDK map of the perturbed system Phi_base + sum_j eps_j Phi_j at eps = 0 (restates
 // MassRadiusPerturbation_OTF.term, reference streamsculptor/fields.py:175-206: the base flow
 // [v0, a0], the mass block d(dx)/dt = dv, d(dv)/dt = -grad Phi_j(x0) - H(x0) dx, and, when
-// with_radius, the radius block driven by -grad(dPhi_j/da_j)).  State per particle: w (6) and
+// with_radius, the radius block driven by -grad(dPhi_j/da_j)); the sources are Plummer spheres
+// or, for LINE_HERNQUIST, Hernquist spheres (reference SubhaloLinePotentialCustom_fromFunc and
+// SubhaloLinePotentialCustom_dRadius_fromFunc, potential.py:1161-1268).  State per particle: w (6) and
 // dw[j] = (dx, dv[, d2x/da, d2v/da]) for j < m.  H is the Hessian of the base potential.
 int orc_perturb_leapfrog(const ssc_host_potential* hp, const ssc_sources* base_src,
                          const ssc_sources* pert, const double* w0, const double* dw0,
@@ -788,10 +790,21 @@ int orc_perturb_leapfrog(const ssc_host_potential* hp, const ssc_sources* base_s
         double gj[3] = {0, 0, 0}, rj[3] = {0, 0, 0};
         if (source_pos(pert, j, k, t, sp)) {
           double dv3[3] = {x[0] - sp[0], x[1] - sp[1], x[2] - sp[2]};
-          plummer_eval<double>(pert->gm[j], pert->rs[j], dv3, nullptr, gj, nullptr);
           double A = pert->rs[j];
-          double s2 = dv3[0] * dv3[0] + dv3[1] * dv3[1] + dv3[2] * dv3[2] + A * A;
-          double c5 = -3.0 * pert->gm[j] * A / (s2 * s2 * std::sqrt(s2));  // grad dPhi/da
+          double d2 = dv3[0] * dv3[0] + dv3[1] * dv3[1] + dv3[2] * dv3[2];
+          double c5;
+          if (pert->kind == SSC_SRC_LINE_HERNQUIST) {
+            // Hernquist: dPhi/da = G m / (r + a)^2, grad = -2 G m (x - s) / (r (r + a)^3)
+            const double p[4] = {pert->gm[j], A, 0.0, 0.0};
+            hernquist_eval<double>(p, dv3, nullptr, gj, nullptr);
+            double r = std::sqrt(d2), ra = r + A;
+            c5 = -2.0 * pert->gm[j] / (r * ra * ra * ra);
+          } else {
+            // Plummer: dPhi/da = G m a / s^3, grad = -3 G m a (x - s) / s^5
+            plummer_eval<double>(pert->gm[j], A, dv3, nullptr, gj, nullptr);
+            double s2 = d2 + A * A;
+            c5 = -3.0 * pert->gm[j] * A / (s2 * s2 * std::sqrt(s2));
+          }
           for (int c = 0; c < 3; ++c) rj[c] = c5 * dv3[c];
         }
         const double* dx = &d[(size_t)j * dd];

@@ -1,12 +1,15 @@
 // First-order perturbation responses (the fields-interface OTF perturbation ODE) on gfx950.
 //
 // Per particle the state is the unperturbed row w = (x, v) in the host potential plus, for
-// every perturbing Plummer source j, the response dw_j = d w / d eps_j at eps = 0 (mass block)
+// every perturbing source j (Plummer spheres, or Hernquist spheres for straight-line Hernquist
+// subhalos), the response dw_j = d w / d eps_j at eps = 0 (mass block)
 // and optionally d^2 w / d a_j d eps_j (radius block).  Discretisation: the exact derivative of
 // the kick-drift-kick map of Phi_host + sum_j eps_j Phi_j, i.e. the responses follow
 //   d(dx)/dt = dv,  d(dv)/dt = -grad Phi_j(x) - H(x) dx          (mass)
 //   d(rx)/dt = rv,  d(rv)/dt = -grad(dPhi_j/da_j)(x) - H(x) rx    (radius)
-// as in reference MassRadiusPerturbation_OTF.term (streamsculptor/fields.py:175-206).
+// as in reference MassRadiusPerturbation_OTF.term (streamsculptor/fields.py:175-206), with
+// dPhi_j/da_j the structural potential of SubhaloLinePotential_dRadius (potential.py:852-904)
+// or SubhaloLinePotentialCustom_dRadius_fromFunc (potential.py:1215-1268).
 //
 // Layout: L = ceil(m / G) lanes per particle, each owning G responses; the base orbit and the
 // host Hessian are recomputed by each of the L lanes (cheaper than exchanging them).  Velocities
@@ -37,6 +40,7 @@ __global__ __launch_bounds__(PT_BLOCK) void perturb_kernel(
   extern __shared__ __align__(16) char lds_raw[];
   __shared__ int kmin_sh;
   constexpr int DD = RAD ? 12 : 6;
+  constexpr bool HQ = SK == SSC_SRC_LINE_HERNQUIST;
   const int ppb = PT_BLOCK / L;
   const int lp = threadIdx.x / L, grp = threadIdx.x - lp * L;
   const int64_t i = (int64_t)blockIdx.x * ppb + lp;
@@ -93,13 +97,25 @@ __global__ __launch_bounds__(PT_BLOCK) void perturb_kernel(
       double fx = 0.0, fy = 0.0, fz = 0.0, ux = 0.0, uy = 0.0, uz = 0.0;
       if (act) {
         const double ex = x - sx, ey = y - sy, ez = z - sz;
-        const double s2 = fma(ex, ex, fma(ey, ey, fma(ez, ez, q2)));
-        const double is = rsq(s2);
-        const double f = gm * is * is * is;
-        fx = f * ex; fy = f * ey; fz = f * ez;
-        if (RAD) {  // grad of dPhi_j/da_j = -3 G m a e / s^5
-          const double r5 = -3.0 * f * sqrt(q2) * is * is;
-          ux = r5 * ex; uy = r5 * ey; uz = r5 * ez;
+        if (HQ) {  // Hernquist (q2 holds a): f = G m / (r (r + a)^2)
+          const double r2 = fma(ex, ex, fma(ey, ey, ez * ez));
+          const double ir = rsq(r2);
+          const double ia = 1.0 / fma(r2, ir, q2);
+          const double f = gm * ir * ia * ia;
+          fx = f * ex; fy = f * ey; fz = f * ez;
+          if (RAD) {  // grad of dPhi_j/da_j = -2 G m e / (r (r + a)^3)
+            const double r5 = -2.0 * f * ia;
+            ux = r5 * ex; uy = r5 * ey; uz = r5 * ez;
+          }
+        } else {  // Plummer (q2 holds a^2)
+          const double s2 = fma(ex, ex, fma(ey, ey, fma(ez, ez, q2)));
+          const double is = rsq(s2);
+          const double f = gm * is * is * is;
+          fx = f * ex; fy = f * ey; fz = f * ez;
+          if (RAD) {  // grad of dPhi_j/da_j = -3 G m a e / s^5
+            const double r5 = -3.0 * f * sqrt(q2) * is * is;
+            ux = r5 * ex; uy = r5 * ey; uz = r5 * ez;
+          }
         }
       }
       if (move) {
@@ -217,17 +233,16 @@ extern "C" int ssc_perturb_leapfrog(const ssc_host_potential* host, const ssc_so
       case SSC_SRC_STATIC: return SSC_PT(SSC_SRC_STATIC, 1);
       case SSC_SRC_TRACK: return SSC_PT(SSC_SRC_TRACK, 1);
       case SSC_SRC_LINE: return SSC_PT(SSC_SRC_LINE, 1);
+      case SSC_SRC_LINE_HERNQUIST: return SSC_PT(SSC_SRC_LINE_HERNQUIST, 1);
     }
   } else {
     switch (s.kind) {
       case SSC_SRC_STATIC: return SSC_PT(SSC_SRC_STATIC, 0);
       case SSC_SRC_TRACK: return SSC_PT(SSC_SRC_TRACK, 0);
       case SSC_SRC_LINE: return SSC_PT(SSC_SRC_LINE, 0);
+      case SSC_SRC_LINE_HERNQUIST: return SSC_PT(SSC_SRC_LINE_HERNQUIST, 0);
     }
   }
 #undef SSC_PT
-  if (s.kind == SSC_SRC_LINE_HERNQUIST)
-    return fail(SSC_E_ARG, "ssc_perturb_leapfrog: responses are built for Plummer sources; "
-                           "Hernquist subhalos are supported by the forward integrators only");
   return fail(SSC_E_ARG, "ssc_perturb_leapfrog: bad source kind");
 }

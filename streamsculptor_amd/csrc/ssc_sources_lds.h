@@ -173,7 +173,8 @@ struct LineLDS {
       }
     }
   }
-  // Plummer only (the response and adjoint kernels): position, G m, r_s^2, active.
+  // One source at time t (the response and adjoint kernels): position, G m, q (r_s^2 for
+  // Plummer, r_s for Hernquist), active.
   __device__ bool get(int, double t, int j, double& sx, double& sy, double& sz, double& gm,
                       double& q) const {
     const LineRec r = rec[j];

@@ -92,7 +92,7 @@ def _pert_parts(pertgen):
     if base._kernel_sources() is not None:
         raise NotImplementedError("the base potential of a perturbation must be analytic")
     if pert._kernel_components():
-        raise NotImplementedError("the perturbing potential must consist of Plummer sources")
+        raise NotImplementedError("the perturbing potential must consist of subhalo sources")
     src = pert._kernel_sources()
     struct = getattr(pertgen, "potential_structural", None)
     radius = struct is not None

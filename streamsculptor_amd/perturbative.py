@@ -89,9 +89,10 @@ class BaseStreamModel:
 class GenerateMassRadiusPerturbation:
     """Perturbation generator (reference perturbative.py:24-135).
 
-    potential_base: smooth potential; potential_perturbation: the subhalos (Plummer sources,
-    e.g. SubhaloLinePotential); potential_structural: their d Phi / d a
-    (SubhaloLinePotential_dRadius) or None for the mass block alone."""
+    potential_base: smooth potential; potential_perturbation: the subhalos (Plummer or
+    Hernquist sources: SubhaloLinePotential, SubhaloLinePotentialCustom_fromFunc, ...);
+    potential_structural: their d Phi / d r_s (SubhaloLinePotential_dRadius,
+    SubhaloLinePotentialCustom_dRadius_fromFunc) or None for the mass block alone."""
 
     def __init__(self, potential_base, potential_perturbation, potential_structural,
                  BaseStreamModel=None, units=None, solver="leapfrog", dt0=None, **kwargs):
@@ -103,7 +104,7 @@ class GenerateMassRadiusPerturbation:
         self.gradientPotentialPerturbation = potential_perturbation.gradient
         src = potential_perturbation._kernel_sources()
         if src is None:
-            raise NotImplementedError("the perturbing potential must consist of Plummer sources")
+            raise NotImplementedError("the perturbing potential must consist of subhalo sources")
         self.num_pert = int(src.tensors()["gm"].numel())
         self.jump_ts = None
         if BaseStreamModel is None:

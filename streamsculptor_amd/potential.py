@@ -127,26 +127,30 @@ class SubhaloLinePotential(Potential):
         return _per_source(self, xyz, t, "phi")
 
 
+def _profile_kind(func):
+    """Source kind of a subhalo profile class (the profiles with a kernel form)."""
+    if func is PlummerPotential:
+        return _lib.SRC_LINE
+    if func is HernquistPotential:
+        return _lib.SRC_LINE_HERNQUIST
+    raise NotImplementedError(
+        f"subhalo profile {getattr(func, '__name__', func)!r} has no kernel form; "
+        "use PlummerPotential or HernquistPotential")
+
+
 class SubhaloLinePotentialCustom_fromFunc(Potential):
     """Straight-line subhalos with the profile of a potential class (reference
     streamsculptor/potential.py:1161-1213): ``func`` is ``PlummerPotential`` or
     ``HernquistPotential`` (the profiles with a kernel form; others raise), instantiated per
     subhalo as func(m=m_j, r_s=r_s_j) at x - (x0_j + v_j (t - t0_j)) while |t - t0_j| < t_window.
-    Hernquist subhalos run in the evaluation, leapfrog and Dopri5 kernels; the first-order
-    response and adjoint kernels differentiate Plummer subhalos only."""
+    Hernquist subhalos run in the evaluation, leapfrog, Dopri5 and first-order response
+    kernels; the adjoint kernels differentiate Plummer subhalos only."""
 
     def __init__(self, func, m, r_s, subhalo_x0, subhalo_v, subhalo_t0, t_window, units=None):
         super().__init__(units, {"func": func, "m": m, "r_s": r_s, "subhalo_x0": subhalo_x0,
                                  "subhalo_v": subhalo_v, "subhalo_t0": subhalo_t0,
                                  "t_window": t_window})
-        if func is PlummerPotential:
-            self._kind = _lib.SRC_LINE
-        elif func is HernquistPotential:
-            self._kind = _lib.SRC_LINE_HERNQUIST
-        else:
-            raise NotImplementedError(
-                f"subhalo profile {getattr(func, '__name__', func)!r} has no kernel form; "
-                "use PlummerPotential or HernquistPotential")
+        self._kind = _profile_kind(func)
 
     def _kernel_components(self):
         return []
@@ -185,6 +189,37 @@ class SubhaloLinePotential_dRadius(Potential):
             return all(np.allclose(np.asarray(_arr(getattr(self, k)), dtype=np.float64),
                                    np.asarray(_arr(getattr(pert, k)), dtype=np.float64),
                                    rtol=0, atol=0) for k in names)
+        except (AttributeError, ValueError):
+            return False
+
+
+class SubhaloLinePotentialCustom_dRadius_fromFunc(Potential):
+    """d Phi / d r_s of straight-line subhalos with the profile of a potential class (reference
+    streamsculptor/potential.py:1215-1268, which takes jax.grad of func(m, r_s).potential in
+    r_s): the structural potential of the radius block of the first-order responses.  The
+    response kernel forms its gradient in closed form -- Plummer
+    -3 G m r_s (x - s) / (|x - s|^2 + r_s^2)^(5/2), Hernquist -2 G m (x - s) / (r (r + r_s)^3)
+    with r = |x - s| -- so this class carries the subhalo description for the perturbation
+    field; evaluating it on its own has no kernel form."""
+
+    def __init__(self, func, m, r_s, subhalo_x0, subhalo_v, subhalo_t0, t_window, units=None):
+        super().__init__(units, {"func": func, "m": m, "r_s": r_s, "subhalo_x0": subhalo_x0,
+                                 "subhalo_v": subhalo_v, "subhalo_t0": subhalo_t0,
+                                 "t_window": t_window})
+        self._kind = _profile_kind(func)
+
+    def potential_per_SH(self, xyz, t):
+        self._kernel_components()
+
+    def matches(self, pert):
+        """True when this describes the same subhalos (and profile) as ``pert``."""
+        if getattr(pert, "_kind", None) != self._kind:
+            return False
+        names = ("m", "r_s", "subhalo_x0", "subhalo_v", "subhalo_t0", "t_window")
+        try:
+            return all(np.array_equal(np.asarray(_arr(getattr(self, k)), dtype=np.float64),
+                                      np.asarray(_arr(getattr(pert, k)), dtype=np.float64))
+                       for k in names)
         except (AttributeError, ValueError):
             return False
 
@@ -279,5 +314,6 @@ class _SourceOnly(Potential):
 __all__ = ["MiyamotoNagaiDisk", "NFWPotential", "PlummerPotential", "HernquistPotential",
            "Potential_Combine", "GalaMilkyWayPotential", "CustomPotential",
            "SubhaloLinePotential", "SubhaloLinePotentialCustom_fromFunc",
-           "SubhaloLinePotential_dRadius", "PlummerClumps",
+           "SubhaloLinePotential_dRadius", "SubhaloLinePotentialCustom_dRadius_fromFunc",
+           "PlummerClumps",
            "LiveSubhaloPotential", "usys"]

@@ -3,7 +3,8 @@ HernquistPotential, reference streamsculptor/potential.py:1161-1213; source kind
 LINE_HERNQUIST) against the oracle, whose Hernquist line source is pinned to the Hernquist
 component in test_oracle_potentials.py: evaluation (Phi, gradient, Hessian), fixed-step
 leapfrog at 1e-9 with ragged starts on every launch path, adaptive Dopri5 at 1e-6 single and
-as a grouped batch of simulations; the response kernel refuses them."""
+as a grouped batch of simulations.  Their first-order responses are covered in
+test_gpu_perturb.py and test_gpu_perturbative.py."""
 import numpy as np
 import pytest
 
@@ -97,14 +98,3 @@ def test_dopri5_single_and_batch(oracle, ssc):
                             t_window=tw[b * nsh:(b + 1) * nsh])
         _, yo, _ = oracle.dopri5(h, wb, 1000.0, t0=0.0, ts=[1000.0], src=sb)
         assert rel_err_rows(ys[b, 0].cpu().numpy()[None], yo[0]) < 1e-6
-
-
-def test_responses_refuse_hernquist(oracle, ssc):
-    from streamsculptor_amd import workloads as W
-    from streamsculptor_amd.fields import perturb_leapfrog
-    rng = np.random.default_rng(34)
-    _, sub, _, _ = _pair(oracle, ssc, rng, 4, -100.0, 0.0)
-    w0 = bound_orbits(rng, 8)
-    with pytest.raises(RuntimeError, match="Hernquist"):
-        perturb_leapfrog(W.host_potential(), sub._kernel_sources(), w0, None,
-                         np.zeros(8, np.int32), -100.0, 1.0, 100, False)

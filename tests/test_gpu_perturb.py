@@ -19,7 +19,8 @@ def _rel(a, b):
 
 
 @pytest.mark.parametrize("kind,radius", [("track", False), ("static", False), ("line", True),
-                                         ("track", True)])
+                                         ("track", True), ("hernquist", False),
+                                         ("hernquist", True)])
 def test_perturb_parity(oracle, ssc, kind, radius):
     from streamsculptor_amd import workloads as W
     from streamsculptor_amd.fields import perturb_leapfrog
@@ -49,15 +50,17 @@ def test_perturb_parity(oracle, ssc, kind, radius):
             vel = rng.normal(scale=0.2, size=(m, 3))
             tc = rng.uniform(t0, 0, m)
             tw = rng.uniform(50, 300, m)
-            src = SourceSet(_lib.SRC_LINE, gm=gm, rs=rs, pos=pos, vel=vel, t0=tc, t_window=tw)
-            so = oracle.Sources(oracle.SRC_LINE, gm=gm, rs=rs, pos=pos, vel=vel, t0=tc, t_window=tw)
+            kk, ko = ((_lib.SRC_LINE_HERNQUIST, oracle.SRC_LINE_HERNQUIST) if kind == "hernquist"
+                      else (_lib.SRC_LINE, oracle.SRC_LINE))
+            src = SourceSet(kk, gm=gm, rs=rs, pos=pos, vel=vel, t0=tc, t_window=tw)
+            so = oracle.Sources(ko, gm=gm, rs=rs, pos=pos, vel=vel, t0=tc, t_window=tw)
     dd = 12 if radius else 6
     dw0 = rng.normal(scale=1e-3, size=(n, m, dd))
     w, dw = perturb_leapfrog(host, src, w0, dw0, start, t0, dt, nsteps, radius)
     wr, dwr = oracle.perturb_leapfrog(h, so, w0, t0, dt, nsteps, dw0=dw0, start=start,
                                       with_radius=radius)
     e1, e2 = _rel(w, wr), _rel(dw, dwr)
-    print(f"perturb[{kind}, radius={radius}] base {e1:.2e} responses {e2:.2e}")
+    print(f"perturb[{kind}, radius={radius}] base {e1:.2e} responses {e2:.2e}", flush=True)
     assert e1 <= 1e-9 and e2 <= 1e-9
 
 

@@ -5,7 +5,7 @@
 * the whole pipeline (backward progenitor responses -> release Jacobian -> forward particle
   responses) == central differences of the nonlinear perturbed stream in the subhalo masses
   (the reference's own test method, examples/tests.ipynb:2959,3182) for the mass block, and
-  in the subhalo radius for the radius block;
+  in the subhalo radius for the radius block, with Plummer and with Hernquist subhalos;
 * the perturbed-stream helpers run the three potentials the reference assigns.
 """
 import numpy as np
@@ -18,10 +18,12 @@ G = 4.498502151469553e-12
 DT = 1.0
 
 
-def _setup(ssc, scale=1.0, da=0.0):
+def _setup(ssc, scale=1.0, da=0.0, profile="plummer"):
     from streamsculptor_amd import workloads as W
-    from streamsculptor_amd.potential import (SubhaloLinePotential,
-                                              SubhaloLinePotential_dRadius)
+    from streamsculptor_amd.potential import (HernquistPotential, SubhaloLinePotential,
+                                              SubhaloLinePotential_dRadius,
+                                              SubhaloLinePotentialCustom_dRadius_fromFunc,
+                                              SubhaloLinePotentialCustom_fromFunc)
     base = W.host_potential()
     ts = np.arange(-600.0, 0.0 + DT / 2, 4 * DT)  # 151 release times on the dt grid
     m = np.array([3e7, 1e7]) * scale
@@ -30,8 +32,14 @@ def _setup(ssc, scale=1.0, da=0.0):
     v = np.array([[0.05, -0.1, 0.02], [0.1, 0.03, -0.04]])
     t0 = np.array([-300.0, -150.0])
     tw = np.array([200.0, 400.0])
-    pert = SubhaloLinePotential(m, a, x0, v, t0, tw, units=ssc.usys)
-    struct = SubhaloLinePotential_dRadius(m, a, x0, v, t0, tw, units=ssc.usys)
+    if profile == "hernquist":  # the reference's own subhalo workflow (generate_derivs.py:181-197)
+        pert = SubhaloLinePotentialCustom_fromFunc(HernquistPotential, m, a, x0, v, t0, tw,
+                                                   units=ssc.usys)
+        struct = SubhaloLinePotentialCustom_dRadius_fromFunc(HernquistPotential, m, a, x0, v, t0,
+                                                             tw, units=ssc.usys)
+    else:
+        pert = SubhaloLinePotential(m, a, x0, v, t0, tw, units=ssc.usys)
+        struct = SubhaloLinePotential_dRadius(m, a, x0, v, t0, tw, units=ssc.usys)
     return base, pert, struct, ts
 
 
@@ -61,10 +69,12 @@ def _stream(ssc, base, pert, ts, prog_today):
     return torch.cat([lead, trail]).cpu().numpy()
 
 
-def test_pipeline_matches_nonlinear_differences(ssc):
+@pytest.mark.parametrize("profile", ["plummer", "hernquist"])
+def test_pipeline_matches_nonlinear_differences(ssc, profile):
     from streamsculptor_amd import workloads as W
     from streamsculptor_amd.perturbative import BaseStreamModel, GenerateMassRadiusPerturbation
-    base, pert, struct, ts = _setup(ssc)
+    print(f"[perturbative] pipeline, {profile} subhalos", flush=True)
+    base, pert, struct, ts = _setup(ssc, profile=profile)
     bs = BaseStreamModel(base, W.PROG_TODAY, ts, 1e4, 7, solver="leapfrog", dt0=DT)
     print("[perturbative] base model ready", flush=True)
     gen = GenerateMassRadiusPerturbation(base, pert, struct, BaseStreamModel=bs, dt0=DT)
@@ -81,8 +91,8 @@ def test_pipeline_matches_nonlinear_differences(ssc):
     # difference in a common scale of both subhalo masses around 0
     prog_today = bs.prog_loc_fwd[-1]
     eps = 1e-3
-    plus = _stream(ssc, base, _setup(ssc, scale=eps)[1], ts, prog_today)
-    minus = _stream(ssc, base, _setup(ssc, scale=-eps)[1], ts, prog_today)
+    plus = _stream(ssc, base, _setup(ssc, scale=eps, profile=profile)[1], ts, prog_today)
+    minus = _stream(ssc, base, _setup(ssc, scale=-eps, profile=profile)[1], ts, prog_today)
     fd = (plus - minus) / (2 * eps)
     lin = resp[:, :, :6].sum(axis=1)
     scale = np.abs(lin).max(axis=0)
@@ -91,10 +101,10 @@ def test_pipeline_matches_nonlinear_differences(ssc):
     assert err.max() < 1e-4
     # radius block: d/da of the mass response (both radii shifted together)
     da = 1e-4
-    gp = GenerateMassRadiusPerturbation(base, *(_setup(ssc, da=da)[1:3]), BaseStreamModel=bs,
-                                        dt0=DT)
-    gm_ = GenerateMassRadiusPerturbation(base, *(_setup(ssc, da=-da)[1:3]), BaseStreamModel=bs,
-                                         dt0=DT)
+    gp = GenerateMassRadiusPerturbation(base, *(_setup(ssc, da=da, profile=profile)[1:3]),
+                                        BaseStreamModel=bs, dt0=DT)
+    gm_ = GenerateMassRadiusPerturbation(base, *(_setup(ssc, da=-da, profile=profile)[1:3]),
+                                         BaseStreamModel=bs, dt0=DT)
     (_, lp), (_, tp) = gp.compute_perturbation_OTF()
     (_, lm), (_, tm) = gm_.compute_perturbation_OTF()
     rp = torch.cat([lp, tp]).cpu().numpy()

@@ -4,8 +4,9 @@ The reference checks its perturbation output against the nonlinear stream differ
 the subhalo mass (streamsculptor/examples/tests.ipynb:2959,3182,2745).  Here: the response
 equals the central difference in eps of the oracle's own leapfrog with the subhalo's mass
 scaled by (1 + eps) -- an independent code path (direct force sums, no Hessians) -- to
-O(eps^2); the radius block equals the central difference of the mass response in the Plummer
-radius; responses are linear in the initial response."""
+O(eps^2); the radius block equals the central difference of the mass response in the scale
+radius (Plummer and Hernquist subhalos; the oracle's Hernquist line source is itself pinned to
+the Hernquist component); responses are linear in the initial response."""
 import numpy as np
 import pytest
 
@@ -27,11 +28,12 @@ def _setup(oracle, kind):
         vel = rng.normal(scale=0.1, size=(m, 3))
         tc = np.full(m, -150.0)
         tw = np.full(m, 1e4)
-        mk = lambda g, r: oracle.Sources(oracle.SRC_LINE, gm=g, rs=r, pos=pos, vel=vel, t0=tc, t_window=tw)
+        sk = oracle.SRC_LINE_HERNQUIST if kind == "hernquist" else oracle.SRC_LINE
+        mk = lambda g, r: oracle.Sources(sk, gm=g, rs=r, pos=pos, vel=vel, t0=tc, t_window=tw)
     return h, gm, rs, mk, w0
 
 
-@pytest.mark.parametrize("kind", ["static", "line"])
+@pytest.mark.parametrize("kind", ["static", "line", "hernquist"])
 def test_mass_response_is_derivative_of_the_map(oracle, kind):
     h, gm, rs, mk, w0 = _setup(oracle, kind)
     nsteps, t0, dt = 300, -300.0, 1.0
@@ -52,8 +54,9 @@ def test_mass_response_is_derivative_of_the_map(oracle, kind):
         assert np.abs(dw[:, j, :] - fd).max() <= 1e-6 * np.abs(fd).max()
 
 
-def test_radius_block_is_mixed_derivative(oracle):
-    h, gm, rs, mk, w0 = _setup(oracle, "static")
+@pytest.mark.parametrize("kind", ["static", "hernquist"])
+def test_radius_block_is_mixed_derivative(oracle, kind):
+    h, gm, rs, mk, w0 = _setup(oracle, kind)
     nsteps, t0, dt = 200, -200.0, 1.0
     _, dw = oracle.perturb_leapfrog(h, mk(gm, rs), w0, t0, dt, nsteps, with_radius=True)
     _, dm = oracle.perturb_leapfrog(h, mk(gm, rs), w0, t0, dt, nsteps)
