@@ -197,6 +197,27 @@ int ssc_perturb_leapfrog(const ssc_host_potential* host, const ssc_sources* pert
                          int64_t n, double t0, double dt, int32_t nsteps, int32_t with_radius,
                          double* w_out, double* dw_out, void* stream);
 
+/* Base orbits plus first-order responses integrated with adaptive Dormand-Prince 5(4): the
+ * reference's Runge-Kutta path of GenerateMassRadiusPerturbation (the progenitor's backward
+ * field solve, streamsculptor/perturbative.py:52-60, and compute_perturbation_OTF, :95-135,
+ * through fields.integrate_field, fields.py:34-99, on MassRadiusPerturbation_OTF.term,
+ * :175-206).  Particle i's state [w (6), dw (m, D)] is one ODE with one step controller whose
+ * error norm is the RMS over all 6 + m D components (D = 12 with with_radius, else 6); step
+ * control as ssc_dopri5 (opt->n_stops must be 0).
+ *   pert            the m perturbing sources: STATIC, LINE or LINE_HERNQUIST, not grouped;
+ *                   m <= 512 (256 with with_radius)
+ *   w0[n][6], dw0[n][m][D] (NULL = zeros) at t_start[i] (device, NULL = t0 for all), run to t1
+ *   w_out[n][6], dw_out[n][m][D]  final values (NULL allowed when saving)
+ *   ts[nts], ys[n][nts][6], dys[n][nts][m][D]  saves (quartic dense output; NaN outside an
+ *                   orbit's interval)
+ *   stats[n][3]     NULL or {accepted, rejected, status}; status 1 = max_steps ran out, and then
+ *                   every output of that particle is +inf. */
+int ssc_perturb_dopri5(const ssc_host_potential* host, const ssc_sources* pert,
+                       const double* w0, const double* dw0, const double* t_start, int64_t n,
+                       double t0, double t1, const double* ts, int32_t nts, int32_t with_radius,
+                       const ssc_dopri5_options* opt, double* w_out, double* dw_out, double* ys,
+                       double* dys, int32_t* stats, void* stream);
+
 /* Vector-Jacobian product of the release model: given dL/d(pos_close, pos_far, vel_close,
  * vel_far) [nrel][3] each, writes dL/d(progenitor row) g_prog[nrel][6] and dL/d(G m_c, p1_c) of
  * the host components into g_host[n_components][2] (summed over releases).  scratch:

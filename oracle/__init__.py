@@ -113,6 +113,8 @@ def _bind(L):
                                        vp, vp, ctypes.c_int]
     L.orc_dopri5.argtypes = [vp, vp, vp, vp, i64, d, d, vp, i32, d, d, d, d, d, i32, vp, i32, vp,
                              vp, vp, ctypes.c_int]
+    L.orc_perturb_dopri5.argtypes = [vp, vp, vp, vp, vp, i64, d, d, vp, i32, ctypes.c_int, d, d,
+                                     d, d, d, i32, vp, vp, vp, vp, vp, ctypes.c_int]
     L.orc_dp5_tableau.argtypes = [vp]
     L.orc_dp5_step.argtypes = [vp, vp, d, d, vp, vp, vp, vp, i32, vp]
     L.orc_host_dgrad.argtypes = [vp, vp, i64, vp]
@@ -377,6 +379,40 @@ def perturb_leapfrog(h, pert, w0, t0, dt, nsteps, dw0=None, start=None, base_src
                                w0.ctypes.data, _p(d0), _p(st), n, t0, dt, nsteps,
                                int(with_radius), out.ctypes.data, dw.ctypes.data, nthreads)
     return out, dw
+
+
+def perturb_dopri5(h, pert, w0, t1, ts=(), t0=None, t_start=None, dw0=None, with_radius=False,
+                   rtol=1e-6, atol=1e-6, dtmin=0.05, dtmax=None, max_steps=10_000, dt0=None,
+                   nthreads=0):
+    """Base orbits + first-order responses with adaptive Dopri5 on the whole pytree state (one
+    controller, RMS norm over all 6 + m D components).  Returns (w (n, 6), dw (n, m, D),
+    ys (n, len(ts), 6), dys (n, len(ts), m, D), stats (n, 3))."""
+    w0 = np.ascontiguousarray(np.atleast_2d(w0), dtype=np.float64)
+    n = w0.shape[0]
+    m = pert.desc.n
+    dd = 12 if with_radius else 6
+    ts = np.ascontiguousarray(np.atleast_1d(np.asarray(ts, dtype=np.float64)))
+    nts = ts.size
+    tsa = None if t_start is None else np.ascontiguousarray(
+        np.broadcast_to(np.asarray(t_start, dtype=np.float64), (n,)))
+    t0v = float(t0) if t0 is not None else (float(ts.min()) if nts else 0.0)
+    d0 = None if dw0 is None else np.ascontiguousarray(dw0, dtype=np.float64).reshape(n, m, dd)
+    out = np.zeros((n, 6))
+    dw = np.zeros((n, m, dd))
+    ys = np.zeros((n, max(nts, 1), 6))
+    dys = np.zeros((n, max(nts, 1), m, dd))
+    stats = np.zeros((n, 3), dtype=np.int32)
+    rc = lib().orc_perturb_dopri5(ctypes.addressof(h), ctypes.addressof(pert.desc),
+                                  w0.ctypes.data, _p(d0), _p(tsa), n, t0v, float(t1),
+                                  ts.ctypes.data if nts else None, nts, int(with_radius),
+                                  rtol, atol, -1.0 if dtmin is None else float(dtmin),
+                                  -1.0 if dtmax is None else float(dtmax),
+                                  -1.0 if dt0 is None else float(dt0), int(max_steps),
+                                  out.ctypes.data, dw.ctypes.data, ys.ctypes.data,
+                                  dys.ctypes.data, stats.ctypes.data, nthreads)
+    if rc < 0:
+        raise ValueError("perturb_dopri5: TRACK sources are not supported")
+    return out, dw, ys[:, :nts], dys[:, :nts], stats
 
 
 def adjoint_leapfrog(h, w0, t0, dt, nsteps, lam_T, start=None, src=None, forcing=None,

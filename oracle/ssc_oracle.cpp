@@ -622,6 +622,157 @@ int dp5_orbit(const ssc_host_potential* hp, const ssc_sources* src, const Dp5Opt
   return 0;
 }
 
+// Source terms of the first-order responses to perturbing source j at (x, grid row k, t):
+// gj = grad Phi_j(x) (mass block) and rj = grad(dPhi_j/da_j)(x) (radius block), both zero when
+// the source is inactive.  Plummer: dPhi/da = G m a / s^3, grad = -3 G m a (x - s) / s^5;
+// Hernquist (LINE_HERNQUIST): dPhi/da = G m / (r + a)^2, grad = -2 G m (x - s) / (r (r + a)^3).
+void pert_terms(const ssc_sources* pert, int j, int k, double t, const double x[3], double gj[3],
+                double rj[3]) {
+  for (int c = 0; c < 3; ++c) gj[c] = rj[c] = 0.0;
+  double sp[3];
+  if (!source_pos(pert, j, k, t, sp)) return;
+  double dv3[3] = {x[0] - sp[0], x[1] - sp[1], x[2] - sp[2]};
+  double A = pert->rs[j];
+  double d2 = dv3[0] * dv3[0] + dv3[1] * dv3[1] + dv3[2] * dv3[2];
+  double c5;
+  if (pert->kind == SSC_SRC_LINE_HERNQUIST) {
+    const double p[4] = {pert->gm[j], A, 0.0, 0.0};
+    hernquist_eval<double>(p, dv3, nullptr, gj, nullptr);
+    double r = std::sqrt(d2), ra = r + A;
+    c5 = -2.0 * pert->gm[j] / (r * ra * ra * ra);
+  } else {
+    plummer_eval<double>(pert->gm[j], A, dv3, nullptr, gj, nullptr);
+    double s2 = d2 + A * A;
+    c5 = -3.0 * pert->gm[j] * A / (s2 * s2 * std::sqrt(s2));
+  }
+  for (int c = 0; c < 3; ++c) rj[c] = c5 * dv3[c];
+}
+
+// Dopri5 with the controller of dp5_orbit on an N-dimensional state y (tau space; f(tau, y, k)
+// gives dy/dtau), the error norm being the RMS over all N components (diffrax's RMS norm over
+// every leaf of a pytree state).  Saves at tau_s[0..nts) into ys[nts][N]; returns 0 on success,
+// 1 when max_steps ran out (then ys rows and y are +inf).  No jump times.
+template <class F>
+int dp5_system(F&& f, int N, const Dp5Opts& o, double tau0, double tau1, std::vector<double>& y,
+               const double* tau_s, int32_t nts, double* ys, int32_t stats[3]) {
+  using namespace dp5;
+  std::vector<double> f0(N), k2(N), k3(N), k4(N), k5(N), k6(N), k7(N), yt(N), yn(N), sc(N);
+  auto rms = [&](const std::vector<double>& u) {
+    double s = 0;
+    for (int c = 0; c < N; ++c) s += u[c] * u[c];
+    return std::sqrt(s / N);
+  };
+  int js = 0;
+  while (js < nts && tau_s[js] < tau0) {
+    for (int c = 0; c < N; ++c) ys[(size_t)N * js + c] = NAN;
+    ++js;
+  }
+  while (js < nts && tau_s[js] == tau0) {
+    for (int c = 0; c < N; ++c) ys[(size_t)N * js + c] = y[c];
+    ++js;
+  }
+  int32_t nacc = 0, nrej = 0, nsteps = 0;
+  double tprev = tau0;
+  if (tau0 < tau1) {
+    f(tau0, y.data(), f0.data());
+    std::vector<double> u(N);
+    for (int c = 0; c < N; ++c) sc[c] = o.atol + std::fabs(y[c]) * o.rtol;
+    for (int c = 0; c < N; ++c) u[c] = y[c] / sc[c];
+    const double d0 = rms(u);
+    for (int c = 0; c < N; ++c) u[c] = f0[c] / sc[c];
+    const double d1 = rms(u);
+    double dt0 = o.dt0;
+    if (!(dt0 > 0)) {
+      const double h0 = (d0 < 1e-5 || d1 < 1e-5) ? 1e-6 : 0.01 * (d0 / d1);
+      for (int c = 0; c < N; ++c) yt[c] = y[c] + h0 * f0[c];
+      f(tau0 + h0, yt.data(), k2.data());
+      for (int c = 0; c < N; ++c) u[c] = (k2[c] - f0[c]) / sc[c];
+      const double d2 = rms(u) / h0;
+      const double md = d1 > d2 ? d1 : d2;
+      const double h1 = md <= 1e-15 ? std::max(1e-6, h0 * 1e-3) : std::pow(0.01 / md, 1.0 / 5.0);
+      dt0 = std::min(100.0 * h0, h1);
+    }
+    double tnext = tau0 + dt0;
+    if (tnext > tau1) tnext = tau1;
+    bool at_dtmin = false;
+    while (tprev < tau1 && nsteps < o.max_steps) {
+      const double h = tnext - tprev;
+      for (int c = 0; c < N; ++c) yt[c] = y[c] + h * (A21 * f0[c]);
+      f(tprev + C2 * h, yt.data(), k2.data());
+      for (int c = 0; c < N; ++c) yt[c] = y[c] + h * (A31 * f0[c] + A32 * k2[c]);
+      f(tprev + C3 * h, yt.data(), k3.data());
+      for (int c = 0; c < N; ++c) yt[c] = y[c] + h * (A41 * f0[c] + A42 * k2[c] + A43 * k3[c]);
+      f(tprev + C4 * h, yt.data(), k4.data());
+      for (int c = 0; c < N; ++c)
+        yt[c] = y[c] + h * (A51 * f0[c] + A52 * k2[c] + A53 * k3[c] + A54 * k4[c]);
+      f(tprev + C5 * h, yt.data(), k5.data());
+      for (int c = 0; c < N; ++c)
+        yt[c] = y[c] + h * (A61 * f0[c] + A62 * k2[c] + A63 * k3[c] + A64 * k4[c] + A65 * k5[c]);
+      f(tnext, yt.data(), k6.data());
+      for (int c = 0; c < N; ++c)
+        yn[c] = y[c] + h * (B1 * f0[c] + B3 * k3[c] + B4 * k4[c] + B5 * k5[c] + B6 * k6[c]);
+      f(tnext, yn.data(), k7.data());
+      for (int c = 0; c < N; ++c) {
+        const double err = h * (E1 * f0[c] + E3 * k3[c] + E4 * k4[c] + E5 * k5[c] + E6 * k6[c] +
+                                E7 * k7[c]);
+        u[c] = err / (o.atol + o.rtol * std::max(std::fabs(y[c]), std::fabs(yn[c])));
+      }
+      const double e = rms(u);
+      const bool keep = (e < 1.0) || at_dtmin;
+      double factor = 0.9 * std::pow(1.0 / e, 1.0 / 5.0);
+      factor = std::min(std::max(factor, keep ? 1.0 : 0.2), 10.0);
+      double dtn = h * factor;
+      if (o.dtmax > 0 && dtn > o.dtmax) dtn = o.dtmax;
+      if (o.dtmin > 0) {
+        at_dtmin = dtn <= o.dtmin;
+        if (dtn < o.dtmin) dtn = o.dtmin;
+      }
+      double t_new = tprev;
+      if (keep) {
+        for (; js < nts && tau_s[js] <= tnext; ++js) {
+          double* row = ys + (size_t)N * js;
+          if (tau_s[js] == tnext) {
+            for (int c = 0; c < N; ++c) row[c] = yn[c];
+            continue;
+          }
+          const double sj = (tau_s[js] - tprev) / h;
+          for (int c = 0; c < N; ++c) {
+            const double ym = y[c] + h * (M1 * f0[c] + M3 * k3[c] + M4 * k4[c] + M5 * k5[c] +
+                                          M6 * k6[c] + M7 * k7[c]);
+            const double F0 = h * f0[c];
+            const double D1 = yn[c] - y[c] - F0, D2 = h * k7[c] - F0, D3 = ym - y[c] - 0.5 * F0;
+            const double a4 = 2.0 * D2 - 8.0 * D1 + 16.0 * D3;
+            const double b3 = D2 - 2.0 * D1 - 2.0 * a4;
+            const double c2 = D1 - a4 - b3;
+            row[c] = y[c] + sj * (F0 + sj * (c2 + sj * (b3 + sj * a4)));
+          }
+        }
+        for (int c = 0; c < N; ++c) { y[c] = yn[c]; f0[c] = k7[c]; }
+        t_new = tnext;
+        ++nacc;
+      } else {
+        ++nrej;
+      }
+      double tn1 = t_new + dtn;
+      tprev = t_new < tau1 ? t_new : tau1;
+      if (tn1 > tau1 - 1e-10) tn1 = keep ? tau1 : tprev + 0.5 * (tau1 - tprev);
+      tnext = tn1;
+      ++nsteps;
+    }
+  }
+  stats[0] = nacc;
+  stats[1] = nrej;
+  stats[2] = tprev < tau1 ? 1 : 0;
+  if (stats[2]) {
+    for (size_t k = 0; k < (size_t)N * nts; ++k) ys[k] = INFINITY;
+    for (int c = 0; c < N; ++c) y[c] = INFINITY;
+    return 1;
+  }
+  for (; js < nts; ++js)
+    for (int c = 0; c < N; ++c) ys[(size_t)N * js + c] = NAN;
+  return 0;
+}
+
 }  // namespace
 
 // =========================================================================================
@@ -786,27 +937,8 @@ int orc_perturb_leapfrog(const ssc_host_potential* hp, const ssc_sources* base_s
       total_eval(hp, base_src, x, k, t, nullptr, g, H);
       for (int c = 0; c < 3; ++c) a[c] = -g[c];
       for (int j = 0; j < m; ++j) {
-        double sp[3];
-        double gj[3] = {0, 0, 0}, rj[3] = {0, 0, 0};
-        if (source_pos(pert, j, k, t, sp)) {
-          double dv3[3] = {x[0] - sp[0], x[1] - sp[1], x[2] - sp[2]};
-          double A = pert->rs[j];
-          double d2 = dv3[0] * dv3[0] + dv3[1] * dv3[1] + dv3[2] * dv3[2];
-          double c5;
-          if (pert->kind == SSC_SRC_LINE_HERNQUIST) {
-            // Hernquist: dPhi/da = G m / (r + a)^2, grad = -2 G m (x - s) / (r (r + a)^3)
-            const double p[4] = {pert->gm[j], A, 0.0, 0.0};
-            hernquist_eval<double>(p, dv3, nullptr, gj, nullptr);
-            double r = std::sqrt(d2), ra = r + A;
-            c5 = -2.0 * pert->gm[j] / (r * ra * ra * ra);
-          } else {
-            // Plummer: dPhi/da = G m a / s^3, grad = -3 G m a (x - s) / s^5
-            plummer_eval<double>(pert->gm[j], A, dv3, nullptr, gj, nullptr);
-            double s2 = d2 + A * A;
-            c5 = -3.0 * pert->gm[j] * A / (s2 * s2 * std::sqrt(s2));
-          }
-          for (int c = 0; c < 3; ++c) rj[c] = c5 * dv3[c];
-        }
+        double gj[3], rj[3];
+        pert_terms(pert, j, k, t, x, gj, rj);
         const double* dx = &d[(size_t)j * dd];
         double* o = &ad[(size_t)j * dd / 2];
         for (int r = 0; r < 3; ++r) {
@@ -851,6 +983,71 @@ int orc_perturb_leapfrog(const ssc_host_potential* hp, const ssc_sources* base_s
     for (int j = 0; j < m * dd; ++j) dw_out[(size_t)i * m * dd + j] = d[j];
   }
   return 0;
+}
+
+// First-order responses integrated with adaptive Dopri5 (reference
+// GenerateMassRadiusPerturbation with a Runge-Kutta solver: the progenitor's backward field
+// solve, perturbative.py:52-60, and compute_perturbation_OTF, :95-135, through
+// fields.integrate_field, fields.py:34-99, on MassRadiusPerturbation_OTF.term, :175-206).
+// Particle i's state is [w (6), dw (m, D)] (D = 12 with with_radius, else 6), integrated from
+// t_start[i] (t0 when NULL) to t1 as one ODE with one step controller (dp5_system: RMS norm
+// over all 6 + m D components).  Saves at ts into ys[n][nts][6], dys[n][nts][m][D]; finals into
+// w_out[n][6], dw_out[n][m][D]; stats[n][3] as orc_dopri5.
+int orc_perturb_dopri5(const ssc_host_potential* hp, const ssc_sources* pert, const double* w0,
+                       const double* dw0, const double* t_start, int64_t n, double t0, double t1,
+                       const double* ts, int32_t nts, int with_radius, double rtol, double atol,
+                       double dtmin, double dtmax, double dt0, int32_t max_steps, double* w_out,
+                       double* dw_out, double* ys, double* dys, int32_t* stats, int nthreads) {
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+  if (pert->kind == SSC_SRC_TRACK) return -1;  // sampled on a fixed grid only
+  const int m = pert->n;
+  const int dd = with_radius ? 12 : 6;
+  const int N = 6 + m * dd;
+  int bad = 0;
+#pragma omp parallel for schedule(dynamic, 1) reduction(+ : bad)
+  for (int64_t i = 0; i < n; ++i) {
+    const double ta = t_start ? t_start[i] : t0;
+    const double dir = t1 >= ta ? 1.0 : -1.0;
+    std::vector<double> tau_s(nts > 0 ? nts : 1), y(N), ysl((size_t)N * (nts > 0 ? nts : 1));
+    for (int j = 0; j < nts; ++j) tau_s[j] = dir * ts[j];
+    for (int c = 0; c < 6; ++c) y[c] = w0[6 * i + c];
+    for (int k = 0; k < m * dd; ++k) y[6 + k] = dw0 ? dw0[(size_t)i * m * dd + k] : 0.0;
+    // dy/dtau: base [v, -grad Phi_host]; per source j the mass block [dv, -gj - H dx] and the
+    // radius block [rv, -rj - H rx]
+    auto f = [&](double tau, const double* yy, double* kk) {
+      const double t = dir * tau;
+      double g[3], H[9];
+      total_eval(hp, nullptr, yy, 0, t, nullptr, g, H);
+      for (int c = 0; c < 3; ++c) { kk[c] = dir * yy[3 + c]; kk[3 + c] = -dir * g[c]; }
+      for (int j = 0; j < m; ++j) {
+        double gj[3], rj[3];
+        pert_terms(pert, j, 0, t, yy, gj, rj);
+        for (int b = 0; b < (with_radius ? 2 : 1); ++b) {
+          const double* q = yy + 6 + (size_t)j * dd + 6 * b;
+          double* o = kk + 6 + (size_t)j * dd + 6 * b;
+          const double* s3 = b ? rj : gj;
+          for (int r = 0; r < 3; ++r) {
+            const double hq = H[3 * r] * q[0] + H[3 * r + 1] * q[1] + H[3 * r + 2] * q[2];
+            o[r] = dir * q[3 + r];
+            o[3 + r] = -dir * (s3[r] + hq);
+          }
+        }
+      }
+    };
+    Dp5Opts o{rtol, atol, dtmin, dtmax, dt0, max_steps, nullptr, 0};
+    int32_t st[3];
+    bad += dp5_system(f, N, o, dir * ta, dir * t1, y, tau_s.data(), nts, ysl.data(), st);
+    for (int js = 0; js < nts; ++js) {
+      const double* row = ysl.data() + (size_t)N * js;
+      if (ys) for (int c = 0; c < 6; ++c) ys[((size_t)i * nts + js) * 6 + c] = row[c];
+      if (dys)
+        for (int k = 0; k < m * dd; ++k) dys[((size_t)i * nts + js) * m * dd + k] = row[6 + k];
+    }
+    if (w_out) for (int c = 0; c < 6; ++c) w_out[6 * i + c] = y[c];
+    if (dw_out) for (int k = 0; k < m * dd; ++k) dw_out[(size_t)i * m * dd + k] = y[6 + k];
+    if (stats) for (int c = 0; c < 3; ++c) stats[3 * i + c] = st[c];
+  }
+  return bad;
 }
 
 // Discrete adjoint of the kick-drift-kick leapfrog (no reference counterpart: the reference

@@ -55,6 +55,8 @@ SIGNATURES = {
     "ssc_dopri5": [_vp, _vp, _vp, _vp, _i64, _d, _d, _vp, _i32, _vp, _vp, _vp, _vp, _vp],
     "ssc_dopri5_dense": [_vp, _vp, _vp, _vp, _i64, _d, _d, _vp, _vp, _vp, _i32, _vp],
     "ssc_perturb_leapfrog": [_vp, _vp, _vp, _vp, _vp, _i64, _d, _d, _i32, _i32, _vp, _vp],
+    "ssc_perturb_dopri5": [_vp, _vp, _vp, _vp, _vp, _i64, _d, _d, _vp, _i32, _i32, _vp, _vp, _vp,
+                           _vp, _vp, _vp],
     "ssc_mlp_forward": [_vp, _vp, _i64, _vp, _vp, _i64],
     "ssc_mlp_loss_backward": [_vp, _vp, _vp, _vp, ctypes.c_float, _i64, _vp, _vp, _vp, _vp, _i64],
     "ssc_mlp_backward": [_vp, _vp, _vp, _i64, _vp, _vp, _i64],

@@ -4,13 +4,13 @@ A field is a class with ``term(t, coords, args)``.  ``integrate_field`` lowers t
 whose right-hand side is known to the gfx950 kernels:
 
 * ``hamiltonian_field(pot)``            -> the orbit integrator (``[v, -grad Phi]``);
-* ``MassRadiusPerturbation_OTF(pertgen)`` -> the response kernel ``ssc_perturb_leapfrog``:
-  base orbit in ``pertgen.potential_base`` plus first-order responses to each perturbing
-  subhalo of ``pertgen.potential_perturbation`` (mass block) and, when the generator carries a
-  structural potential, the radius block (coords[1] of shape (Nsh, 12));
+* ``MassRadiusPerturbation_OTF(pertgen)`` -> the response kernels: base orbit in
+  ``pertgen.potential_base`` plus first-order responses to each perturbing subhalo of
+  ``pertgen.potential_perturbation`` (mass block) and, when the generator carries a
+  structural potential, the radius block (coords[1] of shape (Nsh, 12)); ``solver=Dopri5()``
+  runs ``ssc_perturb_dopri5`` (adaptive, one step controller over the whole state),
+  ``solver="leapfrog"`` with ``dt0=`` runs ``ssc_perturb_leapfrog`` (fixed step);
 * ``CustomField(term)`` holds arbitrary Python and cannot be lowered: NotImplementedError.
-
-Fixed-step fields need ``solver="leapfrog"`` and ``dt0=``.
 """
 from __future__ import annotations
 
@@ -79,7 +79,9 @@ def integrate_field(w0=None, ts=None, dense=False, solver=None, field=None, args
                                          dtmin=dtmin, dtmax=dtmax, max_steps=max_steps, t0=t0v,
                                          t1=t1v, jump_ts=jump_ts, dt0=dt0)
     if isinstance(field, MassRadiusPerturbation_OTF):
-        return _integrate_perturbation(field.pertgen, w0, tsn, t0v, t1v, solver, dt0)
+        return _integrate_perturbation(field.pertgen, w0, tsn, t0v, t1v, solver, dt0,
+                                       dict(rtol=rtol, atol=atol, dtmin=dtmin, dtmax=dtmax,
+                                            max_steps=max_steps, jump_ts=jump_ts))
     if isinstance(field, CustomField):
         raise NotImplementedError("CustomField holds arbitrary Python and cannot be lowered to "
                                   "the gfx950 kernels")
@@ -122,10 +124,53 @@ def perturb_leapfrog(base, src, w0, dw0, start, t0, dt, nsteps, with_radius):
     return out, dout
 
 
-def _integrate_perturbation(pertgen, w0, tsn, t0v, t1v, solver, dt0):
-    if _solvers.resolve(solver, default="dopri5") != "leapfrog":
-        raise NotImplementedError("perturbation fields run with solver='leapfrog' and dt0=")
-    from ._grid import leapfrog_grid
+def perturb_dopri5(base, src, w0, dw0, t1, t0=0.0, t_start=None, ts=None, with_radius=False,
+                   rtol=1e-6, atol=1e-6, dtmin=0.05, dtmax=None, max_steps=10_000, dt0=None,
+                   throw=True):
+    """Run ssc_perturb_dopri5: base rows w0 (n, 6) and responses dw0 (n, m, 6|12) or None at
+    t_start (default t0), integrated to t1 with one adaptive step controller per particle over
+    its whole state; saved at ts (monotone in the direction of integration).  Returns
+    (w (n, 6), dw (n, m, D), ys (n, len(ts), 6) or None, dys (n, len(ts), m, D) or None,
+    stats (n, 3) int32 = accepted, rejected, status).  throw=True raises when a particle runs
+    out of steps (the reference's diffeqsolve default); otherwise its outputs are +inf."""
+    from .adaptive import _options
+    if src.kind == _lib.SRC_TRACK:
+        raise NotImplementedError("adaptive responses need subhalo positions at arbitrary "
+                                  "times; sources sampled on a fixed grid run with "
+                                  "solver='leapfrog'")
+    h = base._host_only()
+    w0 = _lib.f64(w0).reshape(-1, 6).contiguous()
+    n = w0.shape[0]
+    dev = w0.device
+    m = src.tensors()["gm"].numel()
+    dd = 12 if with_radius else 6
+    d0 = None if dw0 is None else _lib.f64(dw0).reshape(n, m, dd).contiguous()
+    tst = None if t_start is None else _lib.f64(t_start).reshape(-1).contiguous()
+    if tst is not None and tst.numel() != n:
+        raise ValueError("t_start needs one time per particle")
+    tsd = None if ts is None else _lib.f64(ts).reshape(-1).contiguous()
+    nts = 0 if tsd is None else tsd.numel()
+    out = torch.empty_like(w0)
+    dout = torch.empty(n, m, dd, dtype=torch.float64, device=dev)
+    ys = torch.empty(n, nts, 6, dtype=torch.float64, device=dev) if nts else None
+    dys = torch.empty(n, nts, m, dd, dtype=torch.float64, device=dev) if nts else None
+    stats = torch.empty(n, 3, dtype=torch.int32, device=dev)
+    opt = _options(rtol, atol, dtmin, dtmax, max_steps, None, dt0)
+    sd = src.desc()
+    _lib.call("ssc_perturb_dopri5", _lib.ctypes.byref(h), _lib.ctypes.byref(sd), _lib.ptr(w0),
+              _lib.ptr(d0), _lib.ptr(tst), n, float(t0), float(t1), _lib.ptr(tsd), nts,
+              int(with_radius), _lib.ctypes.byref(opt), _lib.ptr(out), _lib.ptr(dout),
+              _lib.ptr(ys), _lib.ptr(dys), _lib.ptr(stats))
+    if throw:
+        failed = int((stats[:, 2] != 0).sum().item())
+        if failed:
+            raise RuntimeError(f"Dopri5 responses: {failed} of {n} particles reached "
+                               f"max_steps={max_steps} before t1")
+    return out, dout, ys, dys, stats
+
+
+def _integrate_perturbation(pertgen, w0, tsn, t0v, t1v, solver, dt0, tol):
+    method = _solvers.resolve(solver, default="dopri5")
     base, src, radius = _pert_parts(pertgen)
     w, d = w0
     w = _lib.f64(w).reshape(1, 6)
@@ -135,6 +180,17 @@ def _integrate_perturbation(pertgen, w0, tsn, t0v, t1v, solver, dt0):
     with_radius = radius and dd == 12
     if not with_radius:
         d = d[..., :6].contiguous()
+    if method == "dopri5":
+        if tol.get("jump_ts") is not None:
+            raise NotImplementedError("jump_ts is not supported for perturbation fields")
+        tol = {k: v for k, v in tol.items() if k != "jump_ts"}
+        _, _, ys_w, ys_d, _ = perturb_dopri5(base, src, w, d, t1v, t0=t0v, ts=tsn,
+                                             with_radius=with_radius, dt0=dt0, **tol)
+        ys_d = ys_d[0]
+        if dd == 12 and not with_radius:  # radius block held at zero, as the fixed-step path
+            ys_d = torch.cat([ys_d, torch.zeros_like(ys_d)], dim=-1)
+        return Solution(_lib.f64(tsn), [ys_w[0], ys_d])
+    from ._grid import leapfrog_grid
     dt, nsteps, kidx = leapfrog_grid(tsn, t0v, t1v, dt0)
     if src.kind == _lib.SRC_TRACK:
         src = src.on_grid(t0v, dt, nsteps)

@@ -6,10 +6,9 @@ point); ``GenerateMassRadiusPerturbation`` integrates the progenitor's responses
 subhalo backwards from the observation time, maps them through the release Jacobian to the
 particles' response initial conditions, and ``compute_perturbation_OTF`` integrates every
 particle and its (mass, radius) responses from its release time to the observation time with
-the fused gfx950 response kernel.
-
-The perturbation fields run on the fixed-step leapfrog grid (``solver='leapfrog'``, ``dt0=``):
-the responses' kernel has no adaptive form, and a request for one raises.
+the fused gfx950 response kernels: adaptive Dormand-Prince 5(4) (``solver=Dopri5()``, one step
+controller over each particle's whole state, as the reference's Runge-Kutta path) or the
+fixed-step leapfrog (``solver='leapfrog'``, ``dt0=``).
 """
 from __future__ import annotations
 
@@ -18,14 +17,17 @@ import torch
 
 from . import _lib
 from . import solvers as _solvers
-from .fields import MassRadiusPerturbation_OTF, integrate_field, perturb_leapfrog
+from .fields import (MassRadiusPerturbation_OTF, integrate_field, perturb_dopri5,
+                     perturb_leapfrog)
 from .main import usys
 
 
-def _need_leapfrog(solver, dt0):
-    if _solvers.resolve(solver, default="dopri5") != "leapfrog" or dt0 is None:
-        raise NotImplementedError("the perturbation fields run with solver='leapfrog' and dt0= "
-                                  "(fixed-step response kernel)")
+def _method(solver, dt0):
+    """'dopri5' or 'leapfrog' (which needs dt0) for the response kernels."""
+    method = _solvers.resolve(solver, default="dopri5")
+    if method == "leapfrog" and dt0 is None:
+        raise ValueError("the fixed-step response kernel needs a step size: pass dt0=")
+    return method
 
 
 class BaseStreamModel:
@@ -95,7 +97,7 @@ class GenerateMassRadiusPerturbation:
     SubhaloLinePotentialCustom_dRadius_fromFunc) or None for the mass block alone."""
 
     def __init__(self, potential_base, potential_perturbation, potential_structural,
-                 BaseStreamModel=None, units=None, solver="leapfrog", dt0=None, **kwargs):
+                 BaseStreamModel=None, units=None, solver=None, dt0=None, **kwargs):
         self.units = units or usys
         self.potential_base = potential_base
         self.potential_perturbation = potential_perturbation
@@ -112,7 +114,8 @@ class GenerateMassRadiusPerturbation:
         bs = BaseStreamModel
         self.base_stream = bs
         dt0 = dt0 if dt0 is not None else bs.dt0
-        _need_leapfrog(solver, dt0)
+        solver = solver if solver is not None else bs.solver  # the base model's method
+        _method(solver, dt0)
         self.solver, self.dt0 = solver, dt0
         dev = bs.prog_loc_fwd.device
         self.field_wobs = [bs.prog_loc_fwd[-1],
@@ -149,21 +152,30 @@ class GenerateMassRadiusPerturbation:
         T = len(ts) - 1."""
         solver = self.solver if solver is None else solver
         dt0 = self.dt0 if dt0 is None else dt0
-        _need_leapfrog(solver, dt0)
-        from ._grid import release_grid
+        method = _method(solver, dt0)
         bs = self.base_stream
-        t0, dt, nsteps, kidx = release_grid(bs.ts, dt0)
         T = len(bs.ts) - 1
         base = self.potential_base
         src = self.potential_perturbation._kernel_sources()
         radius = self.potential_structural is not None
         dd = 12 if radius else 6
-        start = kidx[:T].astype(np.int32)
+        if method == "leapfrog":
+            from ._grid import release_grid
+            t0, dt, nsteps, kidx = release_grid(bs.ts, dt0)
+            start = kidx[:T].astype(np.int32)
         outs = []
         for w0, d0 in ((self.base_realspace_ICs_lead, self.perturbation_ICs_lead),
                        (self.base_realspace_ICs_trail, self.perturbation_ICs_trail)):
-            w, d = perturb_leapfrog(base, src, w0[:T].contiguous(),
-                                    d0[:T, :, :dd].contiguous(), start, t0, dt, nsteps, radius)
+            if method == "leapfrog":
+                w, d = perturb_leapfrog(base, src, w0[:T].contiguous(),
+                                        d0[:T, :, :dd].contiguous(), start, t0, dt, nsteps,
+                                        radius)
+            else:  # each particle from its release time ts[i] to ts[-1]
+                w, d, _, _, _ = perturb_dopri5(base, src, w0[:T].contiguous(),
+                                               d0[:T, :, :dd].contiguous(), float(bs.ts[-1]),
+                                               t_start=bs.ts[:T], with_radius=radius, rtol=rtol,
+                                               atol=atol, dtmin=dtmin, dtmax=dtmax,
+                                               max_steps=max_steps)
             if not radius:
                 full = torch.zeros(T, self.num_pert, 12, dtype=torch.float64, device=d.device)
                 full[:, :, :6] = d
