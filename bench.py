@@ -19,7 +19,11 @@ the full reverse chain), gd1 (a batch of 700 perturbed GD-1-like streams with ad
 one straight-line subhalo each: the workload of the reference's GPU example, reported as
 simulations/s and particle-steps/s), ensemble (the clumps potential and surrogate with 2000
 progenitors of 2-6 Gyr streams, 4e6 particles: loss + gradient with respect to the halo and
-every progenitor's initial row through the surrogate adjoint, checkpoints every 100 steps).
+every progenitor's initial row through the surrogate adjoint, checkpoints every 100 steps),
+derivs (one batch of the reference's generate_derivs loop with an adaptive solver: 10,000
+spray particles of a 4 Gyr stream and first-order mass and radius responses to 128
+straight-line Hernquist subhalos, rtol = atol = 1e-11, the progenitor's backward field solve
+and every particle's Dopri5 response solve under one controller per particle).
 
 Prints ONE JSON line on stdout; progress goes to stderr.
 """
@@ -317,6 +321,126 @@ def gd1_main(args):
     print(json.dumps(res), flush=True)
 
 
+def derivs_setup(n_arm, m):
+    import numpy as np
+    from streamsculptor_amd import workloads as W
+    from streamsculptor_amd.main import usys
+    from streamsculptor_amd.perturbative import BaseStreamModel
+    from streamsculptor_amd.potential import (HernquistPotential,
+                                              SubhaloLinePotentialCustom_dRadius_fromFunc,
+                                              SubhaloLinePotentialCustom_fromFunc)
+    from streamsculptor_amd.solvers import Dopri5
+    rng = np.random.default_rng(11)
+    base = W.host_potential()
+    ts = np.linspace(-4000.0, 0.0, n_arm + 1)
+    bs = BaseStreamModel(base, W.PROG_TODAY, ts, 1e4, 0, solver=Dopri5(), rtol=1e-10,
+                         atol=1e-10, dtmin=0.01)
+    prog = bs.prog_loc_fwd.cpu().numpy()
+    # impacts on the stream's path: a point of the progenitor orbit at a random time after the
+    # first quarter, offset by up to 5 scale radii, with a 150 km/s-like relative velocity
+    k = rng.integers(n_arm // 4, n_arm, m)
+    mass = 10 ** rng.uniform(6.0, 8.0, m)
+    rs = 1.05 * np.sqrt(mass / 1e8)
+    u = rng.normal(size=(m, 3))
+    u /= np.linalg.norm(u, axis=1, keepdims=True)
+    x0 = prog[k, :3] + rng.uniform(0.0, 5.0, m)[:, None] * rs[:, None] * u
+    v = prog[k, 3:] + rng.normal(scale=0.15, size=(m, 3))
+    tw = np.full(m, 150.0)
+    pert = SubhaloLinePotentialCustom_fromFunc(HernquistPotential, mass, rs, x0, v, ts[k], tw,
+                                               units=usys)
+    struct = SubhaloLinePotentialCustom_dRadius_fromFunc(HernquistPotential, mass, rs, x0, v,
+                                                         ts[k], tw, units=usys)
+    return dict(base=base, bs=bs, pert=pert, struct=struct, ts=ts, m=m,
+                mass=mass, rs=rs, x0=x0, v=v, tc=ts[k], tw=tw)
+
+
+DERIVS_TOL = dict(rtol=1e-11, atol=1e-11, dtmin=0.01, max_steps=100_000)
+
+
+def run_derivs(d, ev=None):
+    """One batch: the progenitor's backward field solve, response ICs, then every particle's
+    adaptive response solve (both arms; ``ev`` brackets the particle launches)."""
+    import torch
+    from streamsculptor_amd.fields import perturb_dopri5
+    from streamsculptor_amd.perturbative import GenerateMassRadiusPerturbation
+    gen = GenerateMassRadiusPerturbation(d["base"], d["pert"], d["struct"],
+                                         BaseStreamModel=d["bs"], **DERIVS_TOL)
+    T = len(d["ts"]) - 1
+    src = d["pert"]._kernel_sources()
+    w0 = torch.cat([gen.base_realspace_ICs_lead[:T], gen.base_realspace_ICs_trail[:T]])
+    d0 = torch.cat([gen.perturbation_ICs_lead[:T], gen.perturbation_ICs_trail[:T]])
+    tst = torch.cat([torch.as_tensor(d["ts"][:T])] * 2)
+    if ev:
+        ev[0].record()
+    out = perturb_dopri5(d["base"], src, w0, d0, float(d["ts"][-1]), t_start=tst,
+                         with_radius=True, **DERIVS_TOL)
+    if ev:
+        ev[1].record()
+    return w0, d0, tst, out
+
+
+def derivs_main(args):
+    import numpy as np
+    from streamsculptor_amd import flops
+    from streamsculptor_amd import workloads as W
+    n_arm, m = 5000, 128
+    d = derivs_setup(n_arm, m)
+    log(f"derivs: {2 * n_arm} particles x {m} Hernquist subhalos (mass + radius blocks), "
+        f"Dopri5 rtol = atol = 1e-11")
+    wall, kern, (w0, d0, tst, out) = timed(lambda ev: run_derivs(d, ev), args.steps,
+                                           args.warmup, "derivs")
+    st = out[4].cpu().numpy()
+    acc, rej, failed = int(st[:, 0].sum()), int(st[:, 1].sum()), int((st[:, 2] != 0).sum())
+    fl = flops.perturb_dopri5_step(2 * m)
+    achieved = fl * (acc + rej) / kern
+    res = {"metric": "particle-steps/s of adaptive first-order responses (Dopri5, one "
+                     "controller per particle over base row + responses)",
+           "value": acc / kern, "unit": "particle-steps/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": 1e3 * wall / args.steps,
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+           "data": "synthetic",
+           "config": {"workload": "generate_derivs batch", "particles": 2 * n_arm,
+                      "subhalos": m, "profile": "hernquist", "radius_block": True,
+                      "response_vectors": 2 * m, **DERIVS_TOL},
+           "batches_per_s": args.steps / wall, "accepted_steps": acc, "rejected_steps": rej,
+           "failed": failed,
+           "roofline": {"bound": "fp64-valu", "achieved": achieved / 1e12,
+                        "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s",
+                        "frac": achieved / PEAK_FP64, "traffic": None,
+                        "kernel": "perturb_dopri5_kernel<LINE_HERNQUIST, 1, 1>",
+                        "kernel_ms": kern * 1e3, "flop_per_step": fl}}
+    log(f"derivs: {acc / kern:.3e} particle-steps/s ({acc} accepted + {rej} rejected), "
+        f"kernel {kern * 1e3:.1f} ms, {achieved / 1e12:.2f} TFLOP/s, {failed} failed")
+    if not args.no_cpu:
+        import oracle as O
+        cores = min(len(os.sched_getaffinity(0)), 16)
+        h = O.host(W.host_components())
+        so = O.Sources(O.SRC_LINE_HERNQUIST, gm=W.G * d["mass"], rs=d["rs"], pos=d["x0"],
+                       vel=d["v"], t0=d["tc"], t_window=d["tw"])
+        n = w0.shape[0]
+        every = args.cpu_sample_every or 10
+        sel = np.arange(0, n, every)
+        w_np, d_np, t_np = w0.cpu().numpy()[sel], d0.cpu().numpy()[sel], tst.cpu().numpy()[sel]
+        tol = dict(DERIVS_TOL)
+        t0 = time.perf_counter()
+        wr, dwr, _, _, sto = O.perturb_dopri5(h, so, w_np, float(d["ts"][-1]), t_start=t_np,
+                                              dw0=d_np, with_radius=True, nthreads=cores,
+                                              **tol)
+        el = time.perf_counter() - t0
+        dw = out[1].cpu().numpy()[sel]
+        rel = np.linalg.norm((dw - dwr).reshape(len(sel), -1), axis=1) / np.maximum(
+            np.linalg.norm(dwr.reshape(len(sel), -1), axis=1), 1e-300)
+        res["cpu_baseline"] = {"value": float(sto[:, 0].sum()) / el, "unit": "particle-steps/s",
+                               "cores": cores, "kind": "port",
+                               "sample": f"every {every}th particle ({len(sel)} of {n}), "
+                                         f"oracle Dopri5 responses (OpenMP), {el:.1f} s"}
+        res["parity_sample_max_rel_err"] = float(rel.max())
+        res["parity_sample_same_steps"] = float((sto == st[sel]).all(axis=1).mean())
+        log(f"cpu baseline {float(sto[:, 0].sum()) / el:.3e} particle-steps/s on {cores} "
+            f"cores; sample parity {rel.max():.2e}")
+    print(json.dumps(res), flush=True)
+
+
 def ensemble_main(args):
     """Config: the clumps potential and surrogate with a differentiable ensemble of streams
     from many progenitors of different ages (loss + gradient through the surrogate adjoint)."""
@@ -432,7 +556,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="clumps",
                     choices=["clumps", "subhalos", "responses", "adjoint", "plain", "gd1",
-                             "ensemble"])
+                             "ensemble", "derivs"])
     ap.add_argument("--streams", type=int, default=2000, help="ensemble: progenitors")
     ap.add_argument("--releases", type=int, default=1000, help="ensemble: release steps each")
     ap.add_argument("--cpu-particles", type=int, default=16, help="ensemble: CPU sample size")
@@ -462,6 +586,9 @@ def main():
         return
     if args.workload == "ensemble":
         ensemble_main(args)
+        return
+    if args.workload == "derivs":
+        derivs_main(args)
         return
     wl = build_workload(args.workload, args.particles)
     ps = wl["psteps"]

@@ -14,8 +14,9 @@
 //
 // Layout: a particle is spread over a group of L lanes (a power of two, 8 <= L <= 256); lane l
 // owns the response vectors l, l + L, ... (G of them) and, for l < 6, base component l.  At each
-// stage the six base owners publish the stage row through LDS and every lane forms the host
-// gradient and Hessian there, so the base orbit is integrated once per group.  The stage values
+// stage the six base owners publish the stage row through LDS, lane c of the group forms host
+// component c's gradient and Hessian there and publishes them, and every lane sums the
+// components, so the base orbit and the host terms are formed once per group.  The stage values
 // and the error norm are block-synchronous: every lane of the block passes the same barriers,
 // and a group whose particle has finished idles until the block's last particle is done.
 #include <hip/hip_runtime.h>
@@ -92,21 +93,91 @@ __device__ __forceinline__ double err_of(double y, const double (&k)[7], double 
   return u * u;
 }
 
-// Quartic dense output through y, yn, h f0, h f7 and Shampine's midpoint value at s in (0, 1).
-__device__ __forceinline__ double dense_at(double y, const double (&k)[7], double yn, double h,
-                                           double s) {
-  const double ym =
-      y + h * (M1 * k[0] + M3 * k[2] + M4 * k[3] + M5 * k[4] + M6 * k[5] + M7 * k[6]);
-  const double F0 = h * k[0];
-  const double D1 = yn - y - F0, D2 = h * k[6] - F0, D3 = ym - y - 0.5 * F0;
+// Quartic through y (s = 0), yn (s = 1), slopes F0 = h f0 and F7 = h f7, and the midpoint
+// value ym, at s in (0, 1).
+__device__ __forceinline__ double quartic(double y, double yn, double F0, double F7, double ym,
+                                          double s) {
+  const double D1 = yn - y - F0, D2 = F7 - F0, D3 = ym - y - 0.5 * F0;
   const double a4 = 2.0 * D2 - 8.0 * D1 + 16.0 * D3;
   const double b3 = D2 - 2.0 * D1 - 2.0 * a4;
   const double c2 = D1 - a4 - b3;
   return y + s * (F0 + s * (c2 + s * (b3 + s * a4)));
 }
 
+// Dense output of a component with stored slopes k[0..6].
+__device__ __forceinline__ double dense_at(double y, const double (&k)[7], double yn, double h,
+                                           double s) {
+  const double ym =
+      y + h * (M1 * k[0] + M3 * k[2] + M4 * k[3] + M5 * k[4] + M6 * k[5] + M7 * k[6]);
+  return quartic(y, yn, h * k[0], h * k[6], ym, s);
+}
+
+// Position components of a response vector keep no slopes of their own: the slope of position
+// component c at stage j is dir V_j, V_j the stage value of velocity component c,
+// V_j = v0 + h sum_l a_jl kv_l.  A weighted sum h sum_j w_j kx_j is then
+// dir h (sum_j w_j v0 + h sum_l (sum_j w_j a_jl) kv_l), with the products of the tableau
+// formed at compile time.  Weight rows T: 0..6 the stage inputs (6 = b, the FSAL input),
+// 7 the starting-step probe (w = e_0), 8 the error weights b - bhat, 9 the midpoint weights.
+constexpr double butcher(int j, int l) {
+  constexpr double a[7][6] = {{0, 0, 0, 0, 0, 0},
+                              {A21, 0, 0, 0, 0, 0},
+                              {A31, A32, 0, 0, 0, 0},
+                              {A41, A42, A43, 0, 0, 0},
+                              {A51, A52, A53, A54, 0, 0},
+                              {A61, A62, A63, A64, A65, 0},
+                              {B1, 0, B3, B4, B5, B6}};
+  return a[j][l];
+}
+constexpr double wrow(int T, int j) {
+  constexpr double e[7] = {E1, 0, E3, E4, E5, E6, E7};
+  constexpr double mw[7] = {M1, 0, M3, M4, M5, M6, M7};
+  if (T == 7) return j == 0 ? 1.0 : 0.0;
+  if (T == 8) return e[j];
+  if (T == 9) return mw[j];
+  return j < 6 ? butcher(T, j) : 0.0;
+}
+constexpr double qcoef(int T, int l) {
+  double s = 0.0;
+  for (int j = 0; j < 7; ++j) s += wrow(T, j) * butcher(j, l);
+  return s;
+}
+constexpr double ccoef(int T) {
+  if (T == 8) return 0.0;  // b and bhat both sum to one
+  double s = 0.0;
+  for (int j = 0; j < 7; ++j) s += wrow(T, j);
+  return s;
+}
+// sum_{l <= L} qcoef(T, l) kv[l], zero coefficients skipped.
+template <int T, int L>
+__device__ __forceinline__ double qsum(const double (&kv)[7]) {
+  if constexpr (L < 0) {
+    return 0.0;
+  } else {
+    constexpr double q = qcoef(T, L);
+    if constexpr (q != 0.0)
+      return fma(q, kv[L], qsum<T, L - 1>(kv));
+    else
+      return qsum<T, L - 1>(kv);
+  }
+}
+// x0 + h sum_j w_j kx_j for weight row T, from the velocity component's v0 and slopes.
+template <int T>
+__device__ __forceinline__ double pos_sum(double x0, double v0, const double (&kv)[7], double h,
+                                          double dir) {
+  if constexpr (T == 0) {
+    return x0;
+  } else {
+    constexpr double c = ccoef(T);
+    return x0 + dir * h * (c * v0 + h * qsum<T, 5>(kv));
+  }
+}
+
+// One response vector per lane fits two waves per SIMD with a few spilled registers (1.5x
+// faster than one wave per SIMD without spills); two vectors per lane need one wave per SIMD.
 template <int SK, int RAD, int G>
-__global__ __launch_bounds__(PD_BLOCK) void perturb_dopri5_kernel(
+__global__ __launch_bounds__(PD_BLOCK) __attribute__((amdgpu_waves_per_eu(G == 1 ? 2 : 1,
+                                                                          G == 1 ? 2 : 1)))
+void perturb_dopri5_kernel(
     ssc_host_potential hp, ssc_sources pert, const double* __restrict__ w0,
     const double* __restrict__ dw0, const double* __restrict__ t_start, int64_t n, int m, int L,
     double t0, double t1, const double* __restrict__ ts, int nts, Dp5Ctl ctl,
@@ -115,6 +186,7 @@ __global__ __launch_bounds__(PD_BLOCK) void perturb_dopri5_kernel(
   extern __shared__ __align__(16) char lds_raw[];
   __shared__ double xv_sh[PD_GROUPS][2][6];
   __shared__ double part_sh[2][PD_WAVES];
+  __shared__ double hg_sh[PD_GROUPS][SSC_MAX_COMPONENTS][9];  // host component partials
   constexpr int DD = RAD ? 12 : 6;
   SourceLDS<SK> lds;
   lds.init(pert, lds_raw);  // barrier inside
@@ -144,7 +216,7 @@ __global__ __launch_bounds__(PD_BLOCK) void perturb_dopri5_kernel(
 
   // state: base component (owners) and response vectors, with the seven stage slopes
   double by = 0.0, byn = 0.0, bk[7];
-  double y[G][6], yn[G][6], kk[G][6][7];
+  double y[G][6], yn[G][6], kv[G][3][7];  // kv: slopes of the velocity components
 #pragma unroll
   for (int s = 0; s < 7; ++s) bk[s] = 0.0;
   if (valid && bc >= 0) by = w0[6 * i + bc];
@@ -157,7 +229,8 @@ __global__ __launch_bounds__(PD_BLOCK) void perturb_dopri5_kernel(
       y[q][c] = ok ? d[c] : 0.0;
       yn[q][c] = 0.0;
 #pragma unroll
-      for (int s = 0; s < 7; ++s) kk[q][c][s] = 0.0;
+      for (int s = 0; s < 7; ++s)
+        if (c < 3) kv[q][c][s] = 0.0;
     }
   }
 
@@ -171,7 +244,17 @@ __global__ __launch_bounds__(PD_BLOCK) void perturb_dopri5_kernel(
       if (!vh[q]) continue;
       double* o = dys + (row * m + vj[q]) * DD + voff[q];
 #pragma unroll
-      for (int c = 0; c < 6; ++c) o[c] = interp ? dense_at(y[q][c], kk[q][c], yn[q][c], h, s) : y[q][c];
+      for (int c = 0; c < 3; ++c) {
+        if (!interp) {
+          o[c] = y[q][c];
+          o[3 + c] = y[q][3 + c];
+          continue;
+        }
+        const double x0 = y[q][c], v0 = y[q][3 + c];
+        o[c] = quartic(x0, yn[q][c], dir * h * v0, dir * h * yn[q][3 + c],
+                       pos_sum<9>(x0, v0, kv[q][c], h, dir), s);
+        o[3 + c] = dense_at(v0, kv[q][c], yn[q][3 + c], h, s);
+      }
     }
   };
   auto fill_row = [&](int js, double val) {
@@ -194,8 +277,9 @@ __global__ __launch_bounds__(PD_BLOCK) void perturb_dopri5_kernel(
   bool live = valid && tau0 < tau1;
   int xb = 0, pb = 0;
 
-  // Stage S at tau_s = tau + c_S h: publish the base stage value, form the host gradient and
-  // Hessian at the stage position, then every owned component's slope.  One barrier.
+  // Stage S at tau_s = tau + c_S h: publish the base stage value; lane c < n_components of the
+  // group forms host component c's gradient and Hessian at the stage position and publishes
+  // them; every lane sums the components in order, then forms its slopes.  Two barriers.
   auto stage = [&](auto S_, double tau, double h) {
     constexpr int S = decltype(S_)::value;
     constexpr int O = stage_out<S>();
@@ -203,21 +287,41 @@ __global__ __launch_bounds__(PD_BLOCK) void perturb_dopri5_kernel(
     if (live && bc >= 0) xv_sh[gi][xb][bc] = bval;
     if (S == 6) byn = bval;
     __syncthreads();
+    const double* xv = xv_sh[gi][xb];
+    if (live && lane < hp.n_components) {
+      double phi = 0.0, gc[3] = {0.0, 0.0, 0.0}, Hc[9];
+#pragma unroll
+      for (int r = 0; r < 9; ++r) Hc[r] = 0.0;
+      comp_full(hp.c[lane], xv[0], xv[1], xv[2], phi, gc, Hc);
+      double* o = hg_sh[gi][lane];
+      o[0] = gc[0]; o[1] = gc[1]; o[2] = gc[2];
+      o[3] = Hc[0]; o[4] = Hc[1]; o[5] = Hc[2]; o[6] = Hc[4]; o[7] = Hc[5]; o[8] = Hc[8];
+    }
+    __syncthreads();
     if (live) {
-      const double* xv = xv_sh[gi][xb];
       const double px = xv[0], py = xv[1], pz = xv[2];
-      double phi, g[3], H[9];
-      host_full(hp, px, py, pz, phi, g, H);
+      double g[3] = {0.0, 0.0, 0.0}, H[9];
+#pragma unroll
+      for (int r = 0; r < 9; ++r) H[r] = 0.0;
+      for (int c = 0; c < hp.n_components; ++c) {
+        const double* o = hg_sh[gi][c];
+        g[0] += o[0]; g[1] += o[1]; g[2] += o[2];
+        H[0] += o[3]; H[1] += o[4]; H[2] += o[5]; H[4] += o[6]; H[5] += o[7]; H[8] += o[8];
+      }
+      H[3] = H[1]; H[6] = H[2]; H[7] = H[5];
       if (bc >= 0) bk[O] = bc < 3 ? dir * xv[3 + bc] : -dir * g[bc - 3];
       const double t = dir * (tau + stage_c<S>() * h);
 #pragma unroll
       for (int q = 0; q < G; ++q) {
         double yt[6];
 #pragma unroll
-        for (int c = 0; c < 6; ++c) {
-          yt[c] = stage_in<S>(y[q][c], kk[q][c], h);
-          if (S == 6) yn[q][c] = yt[c];
+        for (int c = 0; c < 3; ++c) {
+          yt[c] = pos_sum<S>(y[q][c], y[q][3 + c], kv[q][c], h, dir);
+          yt[3 + c] = stage_in<S>(y[q][3 + c], kv[q][c], h);
         }
+        if (S == 6)
+#pragma unroll
+          for (int c = 0; c < 6; ++c) yn[q][c] = yt[c];
         double fx = 0.0, fy = 0.0, fz = 0.0;
         double sx, sy, sz, gm, qs;
         if (vh[q] && lds.get(0, t, vj[q], sx, sy, sz, gm, qs)) {
@@ -240,12 +344,9 @@ __global__ __launch_bounds__(PD_BLOCK) void perturb_dopri5_kernel(
         const double hx = H[0] * yt[0] + H[1] * yt[1] + H[2] * yt[2];
         const double hy = H[3] * yt[0] + H[4] * yt[1] + H[5] * yt[2];
         const double hz = H[6] * yt[0] + H[7] * yt[1] + H[8] * yt[2];
-        kk[q][0][O] = dir * yt[3];
-        kk[q][1][O] = dir * yt[4];
-        kk[q][2][O] = dir * yt[5];
-        kk[q][3][O] = -dir * (fx + hx);
-        kk[q][4][O] = -dir * (fy + hy);
-        kk[q][5][O] = -dir * (fz + hz);
+        kv[q][0][O] = -dir * (fx + hx);
+        kv[q][1][O] = -dir * (fy + hy);
+        kv[q][2][O] = -dir * (fz + hz);
       }
     }
     xb ^= 1;
@@ -276,7 +377,8 @@ __global__ __launch_bounds__(PD_BLOCK) void perturb_dopri5_kernel(
 #pragma unroll
           for (int c = 0; c < 6; ++c) {
             const double sc = ctl.atol + fabs(y[q][c]) * ctl.rtol;
-            const double a = y[q][c] / sc, b = kk[q][c][0] / sc;
+            const double f0 = c < 3 ? dir * y[q][3 + c] : kv[q][c - 3][0];
+            const double a = y[q][c] / sc, b = f0 / sc;
             s0 = fma(a, a, s0);
             s1 = fma(b, b, s1);
           }
@@ -299,8 +401,11 @@ __global__ __launch_bounds__(PD_BLOCK) void perturb_dopri5_kernel(
           if (vh[q])
 #pragma unroll
             for (int c = 0; c < 6; ++c) {
-              const double u =
-                  (kk[q][c][1] - kk[q][c][0]) / (ctl.atol + fabs(y[q][c]) * ctl.rtol);
+              // position: f1 - f0 = dir (v0 + h0 kv0) - dir v0
+              const double df =
+                  c < 3 ? dir * stage_in<7>(y[q][3 + c], kv[q][c], h0) - dir * y[q][3 + c]
+                        : kv[q][c - 3][1] - kv[q][c - 3][0];
+              const double u = df / (ctl.atol + fabs(y[q][c]) * ctl.rtol);
               s2 = fma(u, u, s2);
             }
       }
@@ -328,7 +433,12 @@ __global__ __launch_bounds__(PD_BLOCK) void perturb_dopri5_kernel(
       for (int q = 0; q < G; ++q)
         if (vh[q])
 #pragma unroll
-          for (int c = 0; c < 6; ++c) ss += err_of(y[q][c], kk[q][c], yn[q][c], h, ctl);
+          for (int c = 0; c < 3; ++c) {
+            ss += err_of(y[q][3 + c], kv[q][c], yn[q][3 + c], h, ctl);
+            const double ex = dir * h * (h * qsum<8, 5>(kv[q][c]));
+            const double u = ex / (ctl.atol + ctl.rtol * fmax(fabs(y[q][c]), fabs(yn[q][c])));
+            ss = fma(u, u, ss);
+          }
     }
     const double e = sqrt(reduce(ss) / ntot);
     if (live) {
@@ -366,7 +476,7 @@ __global__ __launch_bounds__(PD_BLOCK) void perturb_dopri5_kernel(
 #pragma unroll
           for (int c = 0; c < 6; ++c) {
             y[q][c] = yn[q][c];
-            kk[q][c][0] = kk[q][c][6];
+            if (c < 3) kv[q][c][0] = kv[q][c][6];
           }
         t_new = tnext;
         ++nacc;

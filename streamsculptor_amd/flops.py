@@ -16,6 +16,11 @@ FP64 sqrt, division, rsqrt and log1p expand to multi-instruction sequences on CD
                   scaled error 3 = 62, x 6 components = 372; RMS norm 14 + controller 10
                   = 396, plus 6 right-hand sides of (host gradient + 18 per active source
                   + 6 for [dir v, -dir grad])
+  adaptive responses, per particle-step (accepted or rejected): the Dopri5 step of the base
+                  row (396) with 6 right-hand sides of host gradient + Hessian (57 + 96 + 6),
+                  and per 6-component response vector the same per-component step work
+                  (62 x 6 = 372) plus 6 right-hand sides of unit source 16 + H dx 15 +
+                  source add 3 = 34: 1,350 + 576 per vector
 """
 HOST_GRAD = 57
 KDK = 18
@@ -24,6 +29,7 @@ HOST_HESSIAN = 96
 RESPONSE_PER_SUBHALO = 52
 HERNQUIST_GRAD = 16
 DOPRI5_STEP = 396
+RESPONSE_RHS = 34
 
 
 def plain(n_sources: int = 0) -> int:
@@ -59,3 +65,12 @@ def dopri5_step(host_grad: int = HOST_GRAD, n_sources: int = 0) -> int:
     ``host_grad`` plus n active Plummer sources (3-component host: 774; with a Hernquist
     nucleus: 870; and one active subhalo: 978)."""
     return DOPRI5_STEP + 6 * (host_grad + PLUMMER_SOURCE * n_sources + 6)
+
+
+def perturb_dopri5_step(n_vectors: int, host_grad: int = HOST_GRAD) -> int:
+    """One adaptive step of a base row and n 6-component response vectors (mass and radius
+    blocks of n/2 subhalos, or mass blocks of n) under one controller (3-component host:
+    1,350 + 576 n; 256 vectors: 148,806)."""
+    per_comp = (DOPRI5_STEP - 24) // 6  # 62
+    return (DOPRI5_STEP + 6 * (host_grad + HOST_HESSIAN + 6) +
+            n_vectors * (6 * per_comp + 6 * RESPONSE_RHS))
