@@ -205,7 +205,7 @@ int ssc_perturb_leapfrog(const ssc_host_potential* host, const ssc_sources* pert
  * error norm is the RMS over all 6 + m D components (D = 12 with with_radius, else 6); step
  * control as ssc_dopri5 (opt->n_stops must be 0).
  *   pert            the m perturbing sources: STATIC, LINE or LINE_HERNQUIST, not grouped;
- *                   m <= 512 (256 with with_radius)
+ *                   m <= 1024 (512 with with_radius)
  *   w0[n][6], dw0[n][m][D] (NULL = zeros) at t_start[i] (device, NULL = t0 for all), run to t1
  *   w_out[n][6], dw_out[n][m][D]  final values (NULL allowed when saving)
  *   ts[nts], ys[n][nts][6], dys[n][nts][m][D]  saves (quartic dense output; NaN outside an

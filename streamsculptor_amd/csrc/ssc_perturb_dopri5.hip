@@ -13,7 +13,8 @@
 // end-of-interval clipping, quartic dense output at save times).
 //
 // Layout: a particle is spread over a group of L lanes (a power of two, 8 <= L <= 256); lane l
-// owns the response vectors l, l + L, ... (G of them) and, for l < 6, base component l.  At each
+// owns the response vectors l, l + L, ... (G = 1, 2 or 4 of them) and, for l < 6, base
+// component l.  At each
 // stage the six base owners publish the stage row through LDS, lane c of the group forms host
 // component c's gradient and Hessian there and publishes them, and every lane sums the
 // components, so the base orbit and the host terms are formed once per group.  The stage values
@@ -173,7 +174,8 @@ __device__ __forceinline__ double pos_sum(double x0, double v0, const double (&k
 }
 
 // One response vector per lane fits two waves per SIMD with a few spilled registers (1.5x
-// faster than one wave per SIMD without spills); two vectors per lane need one wave per SIMD.
+// faster than one wave per SIMD without spills); two vectors per lane need one wave per SIMD,
+// four (513 to 1024 vectors) spill about a hundred registers at one wave per SIMD.
 template <int SK, int RAD, int G>
 __global__ __launch_bounds__(PD_BLOCK) __attribute__((amdgpu_waves_per_eu(G == 1 ? 2 : 1,
                                                                           G == 1 ? 2 : 1)))
@@ -551,14 +553,17 @@ int dispatch_g(const ssc_host_potential& hp, const ssc_sources& s, const double*
                const double* ts, int nts, const Dp5Ctl& ctl, double* w_out, double* dw_out,
                double* ys, double* dys, int32_t* stats, hipStream_t st) {
   const int nvec = RAD ? 2 * s.n : s.n;
-  const int G = nvec <= PD_BLOCK ? 1 : 2;
+  const int G = nvec <= PD_BLOCK ? 1 : (nvec <= 2 * PD_BLOCK ? 2 : 4);
   const int per = (nvec + G - 1) / G;
   int L = PD_MIN_L;
   while (L < per) L <<= 1;
   if (G == 1)
     return launch_pd<SK, RAD, 1>(hp, s, w0, dw0, t_start, n, L, t0, t1, ts, nts, ctl, w_out,
                                  dw_out, ys, dys, stats, st);
-  return launch_pd<SK, RAD, 2>(hp, s, w0, dw0, t_start, n, L, t0, t1, ts, nts, ctl, w_out,
+  if (G == 2)
+    return launch_pd<SK, RAD, 2>(hp, s, w0, dw0, t_start, n, L, t0, t1, ts, nts, ctl, w_out,
+                                 dw_out, ys, dys, stats, st);
+  return launch_pd<SK, RAD, 4>(hp, s, w0, dw0, t_start, n, L, t0, t1, ts, nts, ctl, w_out,
                                dw_out, ys, dys, stats, st);
 }
 
@@ -590,9 +595,9 @@ extern "C" int ssc_perturb_dopri5(const ssc_host_potential* host, const ssc_sour
   if (s.group_size > 0)
     return fail(SSC_E_ARG, "ssc_perturb_dopri5: grouped sources are not supported");
   const int nvec = with_radius ? 2 * s.n : s.n;
-  if (nvec > 2 * PD_BLOCK)
-    return fail(SSC_E_LIMIT, "ssc_perturb_dopri5: at most 512 response vectors per particle "
-                             "(512 subhalos, or 256 with the radius block)");
+  if (nvec > 4 * PD_BLOCK)
+    return fail(SSC_E_LIMIT, "ssc_perturb_dopri5: at most 1024 response vectors per particle "
+                             "(1024 subhalos, or 512 with the radius block)");
   if (n == 0) return SSC_OK;
   if (!w0) return fail(SSC_E_ARG, "ssc_perturb_dopri5: w0 is NULL");
   if (nts > 0 && (!ts || !ys || !dys))

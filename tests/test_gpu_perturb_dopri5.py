@@ -4,7 +4,8 @@ with a Runge-Kutta solver, perturbative.py:24-135).
 
 Kernel vs oracle on the same inputs: final base rows and responses <= 1e-6 relative (the
 adaptive bar), saved rows likewise, step counts equal; the three lane layouts are covered
-(8 lanes per particle; 128 lanes over two waves; two response vectors per lane at 256 lanes).
+(8 lanes per particle; 128 lanes over two waves; two and four response vectors per lane at
+256 lanes).
 Pipeline: the mass block equals central differences of the nonlinear perturbed stream in the
 subhalo masses and the radius block the difference of the mass block in the radius, at tight
 tolerances (the reference's own test method, examples/tests.ipynb:2959,3182).  The subhalos of
@@ -53,6 +54,7 @@ def _sources(oracle, kind, rng, m, t_lo, t_hi):
     ("line", False, 3, 200, 600.0),        # 8 lanes per particle, 32 particles per block
     ("hernquist", True, 37, 40, -500.0),   # 74 vectors: 128 lanes over two waves, backwards
     ("static", True, 200, 8, 300.0),       # 400 vectors: two per lane at 256 lanes
+    ("line", True, 300, 4, -200.0),        # 600 vectors: four per lane at 256 lanes
 ])
 def test_kernel_matches_oracle(oracle, ssc, kind, radius, m, n, t1):
     from streamsculptor_amd import workloads as W
