@@ -1,0 +1,51 @@
+"""The batch driver of perturbation derivatives (reference generate_derivs.py:153-204 with the
+subhalos given): every saved batch equals GenerateMassRadiusPerturbation run directly on the
+same unit-mass Hernquist subhalos, in the reference's file layout."""
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+
+def test_batches_match_direct_computation(ssc, tmp_path):
+    from streamsculptor_amd import workloads as W
+    from streamsculptor_amd.derivs import generate_derivs_batches, load_derivs_batch
+    from streamsculptor_amd.perturbative import BaseStreamModel, GenerateMassRadiusPerturbation
+    from streamsculptor_amd.potential import (HernquistPotential,
+                                              SubhaloLinePotentialCustom_dRadius_fromFunc,
+                                              SubhaloLinePotentialCustom_fromFunc)
+    from streamsculptor_amd.solvers import Dopri5
+    rng = np.random.default_rng(5)
+    base = W.host_potential()
+    ts = np.linspace(-500.0, 0.0, 51)
+    tol = dict(rtol=1e-8, atol=1e-8, dtmin=0.05, max_steps=20_000)
+    bs = BaseStreamModel(base, W.PROG_TODAY, ts, 1e4, 3, solver=Dopri5(), **tol)
+    prog = bs.prog_loc_fwd.cpu().numpy()
+    batches = []
+    for _ in range(2):
+        k = rng.integers(10, 50, 3)
+        rs = 1.05 * np.sqrt(10 ** rng.uniform(6, 8, 3) / 1e8)
+        batches.append(dict(r_s=rs, x0=prog[k, :3] + rng.normal(scale=0.3, size=(3, 3)),
+                            v=prog[k, 3:] + rng.normal(scale=0.1, size=(3, 3)), t0=ts[k],
+                            ImpactFrameParams={"tImpact": ts[k]}))
+    print("[derivs] driver", flush=True)
+    names = generate_derivs_batches(base, bs, batches, str(tmp_path), save_iter_start=4,
+                                    units=ssc.usys, **tol)
+    assert [n.rsplit("/", 1)[1] for n in names] == ["4.npy", "5.npy"]
+    for i, b in enumerate(batches):
+        d = load_derivs_batch(str(tmp_path), 4 + i)
+        args = (HernquistPotential, np.ones(3), b["r_s"], b["x0"], b["v"], b["t0"],
+                np.full(3, 150.0))
+        gen = GenerateMassRadiusPerturbation(
+            base, SubhaloLinePotentialCustom_fromFunc(*args, units=ssc.usys),
+            SubhaloLinePotentialCustom_dRadius_fromFunc(*args, units=ssc.usys),
+            BaseStreamModel=bs, units=ssc.usys, **tol)
+        ref = gen.compute_perturbation_OTF(**tol)
+        for arm in range(2):
+            for k in range(2):
+                np.testing.assert_array_equal(d["pert_out"][arm][k], ref[arm][k].cpu().numpy())
+        assert d["pert_out"][0][1].shape == (len(ts) - 1, 3, 12)
+        np.testing.assert_array_equal(d["r_s_root"], b["r_s"])
+        np.testing.assert_array_equal(d["ImpactFrameParams"]["tImpact"], b["t0"])
+        assert np.abs(d["pert_out"][0][1]).max() > 0
