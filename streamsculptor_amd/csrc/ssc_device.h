@@ -43,11 +43,12 @@ __device__ __forceinline__ double rsq(double v) {
 // permutations, then the half-row and row mirrors), which need no trip through the LDS unit;
 // the 32- and 64-lane steps use lane shuffles.  Every lane of the wave must be active.
 // ------------------------------------------------------------------------------------------
-// Lanes per orbit for the grouped kernels: enough to give the launch about six waves per SIMD
-// (6 x 1024 SIMDs x 64 lanes), at most 16, and at least four sources per lane.
+// Lanes per orbit for the grouped kernels: enough to give the launch about three waves per SIMD
+// (3 x 1024 SIMDs x 64 lanes), at most 16, and at least four sources per lane.  (1e5 orbits in
+// 64 live subhalos: two lanes per orbit 18.4 ms, four 19.0 ms, eight 25.9 ms per stream.)
 inline int group_lanes(int64_t n, int nsrc) {
   if (nsrc < 16 || n <= 0) return 1;
-  const double want = 393216.0 / (double)n;
+  const double want = 196608.0 / (double)n;
   int g = 1;
   while (g < want && g < 16) g *= 2;
   while (g > 1 && nsrc / g < 4) g /= 2;
