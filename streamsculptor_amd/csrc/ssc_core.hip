@@ -381,6 +381,89 @@ __global__ __launch_bounds__(WAVE) void leapfrog_wave_kernel(
   }
 }
 
+// Orbits in the analytic host alone, for batches too small to fill the chip, where the step's
+// latency sets the rate: the host components are spread over the waves of a block (wave c forms
+// component c's gradient for the block's 64 orbits, one per lane; the kinds differ, so lanes of
+// one wave could not share the work without running every kind's code), the waves exchange
+// their gradient parts through LDS and each accumulates them in component order with the
+// operations of host_grad (so the orbits are bit-identical to the one-lane kernels') and
+// advances the same orbits.  One barrier per force evaluation (parts double-buffered).
+__global__ __launch_bounds__(WAVE * SSC_MAX_COMPONENTS) void leapfrog_comp_kernel(
+    ssc_host_potential hp, const double* __restrict__ w0, const int32_t* __restrict__ start,
+    int64_t n, double t0, double dt, int nsteps, double* __restrict__ w_out,
+    const int32_t* __restrict__ save_idx, int nsave, double* __restrict__ ys) {
+  __shared__ double part[2][SSC_MAX_COMPONENTS][WAVE][3];  // f, zm, zadd of each component
+  __shared__ int kmin_sh;
+  const int l = threadIdx.x & (WAVE - 1), c = threadIdx.x / WAVE;
+  const int64_t i = (int64_t)blockIdx.x * WAVE + l;
+  const bool valid = i < n;
+  const bool writer = valid && c == 0;
+  double x = 1.0, y = 0.0, z = 0.0, vx = 0.0, vy = 0.0, vz = 0.0;
+  int s = nsteps;
+  if (valid) {
+    const double* w = w0 + 6 * i;
+    x = w[0]; y = w[1]; z = w[2]; vx = w[3]; vy = w[4]; vz = w[5];
+    const int sp = start ? start[i] : 0;
+    s = sp < 0 ? 0 : (sp > nsteps ? nsteps : sp);
+  }
+  if (threadIdx.x == 0) kmin_sh = INT_MAX;
+  __syncthreads();
+  atomicMin(&kmin_sh, s);
+  __syncthreads();
+  const int kmin = kmin_sh;
+  const int nc = hp.n_components;
+  int buf = 0;
+  // block-uniform: every thread calls it the same number of times
+  auto accel = [&](double& ax, double& ay, double& az) {
+    double f, zm;
+    bool zadd;
+    comp_grad_parts(hp.c[c], x, y, z, f, zm, zadd);
+    part[buf][c][l][0] = f; part[buf][c][l][1] = zm; part[buf][c][l][2] = zadd ? 1.0 : 0.0;
+    __syncthreads();
+    double gx = 0.0, gy = 0.0, gz = 0.0;
+    for (int q = 0; q < nc; ++q)
+      comp_grad_add(part[buf][q][l][0], part[buf][q][l][1], part[buf][q][l][2] != 0.0, x, y, z,
+                    gx, gy, gz);
+    ax = -gx; ay = -gy; az = -gz;
+    buf ^= 1;
+  };
+  int next = 0;
+  while (next < nsave && save_idx[next] <= kmin) {
+    if (writer) {
+      double* o = ys + ((int64_t)next * n + i) * 6;
+      o[0] = x; o[1] = y; o[2] = z; o[3] = vx; o[4] = vy; o[5] = vz;
+    }
+    ++next;
+  }
+  const double h = 0.5 * dt;
+  double ax = 0.0, ay = 0.0, az = 0.0;
+  if (kmin < nsteps) accel(ax, ay, az);
+  // A particle that has not started keeps its initial row; the force evaluated there is the
+  // initial force it needs when it starts.
+  for (int k = kmin; k < nsteps; ++k) {
+    const bool act = k >= s;
+    if (act) {
+      vx = vx + h * ax; vy = vy + h * ay; vz = vz + h * az;
+      x = x + dt * vx; y = y + dt * vy; z = z + dt * vz;
+    }
+    accel(ax, ay, az);
+    if (act) {
+      vx = vx + h * ax; vy = vy + h * ay; vz = vz + h * az;
+    }
+    while (next < nsave && save_idx[next] == k + 1) {
+      if (writer) {
+        double* o = ys + ((int64_t)next * n + i) * 6;
+        o[0] = x; o[1] = y; o[2] = z; o[3] = vx; o[4] = vy; o[5] = vz;
+      }
+      ++next;
+    }
+  }
+  if (writer) {
+    double* o = w_out + 6 * i;
+    o[0] = x; o[1] = y; o[2] = z; o[3] = vx; o[4] = vy; o[5] = vz;
+  }
+}
+
 static int set_lds(const void* fn, size_t lds) {
   if (lds > 160 * 1024) return fail(SSC_E_LIMIT, "ssc_leapfrog: source table exceeds LDS");
   if (lds > 60 * 1024) {
@@ -392,7 +475,8 @@ static int set_lds(const void* fn, size_t lds) {
 
 // Launch shape: one wave per orbit for a handful of orbits in a large source table; two
 // particles per lane once the batch alone fills the chip; G lanes per orbit for mid-sized
-// batches with sources; one particle per lane otherwise.
+// batches with sources; host components over waves for up to 65536 orbits in the host alone;
+// one particle per lane otherwise.
 template <int SK>
 static int launch_leapfrog(const ssc_host_potential& hp, const ssc_sources& src, const double* w0,
                            const int32_t* start, int64_t n, double t0, double dt, int nsteps,
@@ -412,6 +496,13 @@ static int launch_leapfrog(const ssc_host_potential& hp, const ssc_sources& src,
     const int64_t blocks = (n + 2 * LF_BLOCK - 1) / (2 * LF_BLOCK);
     hipLaunchKernelGGL((leapfrog_kernel<SK, 2>), dim3((unsigned)blocks), dim3(LF_BLOCK), lds, st,
                        hp, src, w0, start, n, t0, dt, nsteps, w_out, save_idx, nsave, ys);
+    return check_launch("ssc_leapfrog");
+  }
+  if (SK == SSC_SRC_NONE && hp.n_components >= 2 && n <= 65536) {
+    const int64_t blocks = (n + WAVE - 1) / WAVE;
+    hipLaunchKernelGGL(leapfrog_comp_kernel, dim3((unsigned)blocks),
+                       dim3(WAVE * hp.n_components), 0, st, hp, w0, start, n, t0, dt, nsteps,
+                       w_out, save_idx, nsave, ys);
     return check_launch("ssc_leapfrog");
   }
   const int g = SK == SSC_SRC_NONE ? 1 : group_lanes(n, src.n);

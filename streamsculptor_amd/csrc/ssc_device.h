@@ -78,9 +78,17 @@ __device__ __forceinline__ double group_sum(double v) {
 // streamsculptor/potential.py:66-84 (MN, NFW), :124-138 (Plummer, Hernquist), differentiated
 // in closed form.
 // ------------------------------------------------------------------------------------------
-__device__ __forceinline__ void comp_grad(const ssc_component& c, double x, double y, double z,
-                                          double& gx, double& gy, double& gz) {
+// Gradient of one host component at (x, y, z) in parts: grad = (f x, f y, zm z), or
+// (f x, f y, zm) when zadd (the Miyamoto-Nagai z component, formed whole).  comp_grad_add
+// accumulates the parts with the
+// same operations everywhere (an FMA per coordinate, an add for the Miyamoto-Nagai z term), so
+// a sum formed on one lane and a sum of parts gathered from several waves are bit-identical.
+__device__ __forceinline__ void comp_grad_parts(const ssc_component& c, double x, double y,
+                                                double z, double& f, double& zm, bool& zadd) {
   const double gm = c.p[0];
+  f = 0.0;
+  zm = 0.0;
+  zadd = false;
   switch (c.kind) {
     case SSC_COMP_MIYAMOTO_NAGAI: {
       const double a = c.p[1], b = c.p[2];
@@ -88,10 +96,9 @@ __device__ __forceinline__ void comp_grad(const ssc_component& c, double x, doub
       const double S = a + zeta;
       const double D2 = x * x + y * y + S * S;
       const double iD = rsq(D2);
-      const double f = gm * iD * iD * iD;
-      gx += f * x;
-      gy += f * y;
-      gz += f * z * S / zeta;
+      f = gm * iD * iD * iD;
+      zm = f * z * S / zeta;
+      zadd = true;
       break;
     }
     case SSC_COMP_NFW: {
@@ -100,10 +107,8 @@ __device__ __forceinline__ void comp_grad(const ssc_component& c, double x, doub
       const double ir = rsq(r2);
       const double r = r2 * ir;
       const double L = log1p(r / rs);
-      const double f = gm * ir * ir * (L * ir - 1.0 / (r + rs));
-      gx += f * x;
-      gy += f * y;
-      gz += f * z;
+      f = gm * ir * ir * (L * ir - 1.0 / (r + rs));
+      zm = f;
       break;
     }
     case SSC_COMP_HERNQUIST: {
@@ -112,25 +117,36 @@ __device__ __forceinline__ void comp_grad(const ssc_component& c, double x, doub
       const double ir = rsq(r2);
       const double r = r2 * ir;
       const double rc = r + cc;
-      const double f = gm * ir / (rc * rc);
-      gx += f * x;
-      gy += f * y;
-      gz += f * z;
+      f = gm * ir / (rc * rc);
+      zm = f;
       break;
     }
     case SSC_COMP_PLUMMER: {
       const double rs = c.p[1];
       const double s2 = x * x + y * y + z * z + rs * rs;
       const double is = rsq(s2);
-      const double f = gm * is * is * is;
-      gx += f * x;
-      gy += f * y;
-      gz += f * z;
+      f = gm * is * is * is;
+      zm = f;
       break;
     }
     default:
       break;
   }
+}
+
+__device__ __forceinline__ void comp_grad_add(double f, double zm, bool zadd, double x, double y,
+                                              double z, double& gx, double& gy, double& gz) {
+  gx = fma(f, x, gx);
+  gy = fma(f, y, gy);
+  gz = zadd ? gz + zm : fma(zm, z, gz);
+}
+
+__device__ __forceinline__ void comp_grad(const ssc_component& c, double x, double y, double z,
+                                          double& gx, double& gy, double& gz) {
+  double f, zm;
+  bool zadd;
+  comp_grad_parts(c, x, y, z, f, zm, zadd);
+  comp_grad_add(f, zm, zadd, x, y, z, gx, gy, gz);
 }
 
 __device__ __forceinline__ void host_grad(const ssc_host_potential& hp, double x, double y,

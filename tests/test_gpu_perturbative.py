@@ -118,7 +118,10 @@ def test_pipeline_matches_nonlinear_differences(ssc, profile):
 
 def test_stream_helpers_assign_potentials(ssc):
     """gen_stream_vmapped_with_pert_fixed_prog keeps the progenitor in the base potential;
-    with a subhalo that is never active the perturbed helpers reduce to the base stream."""
+    with a subhalo that is never active the perturbed helpers reduce to the base stream (to
+    rounding: orbits in the host alone run on the kernel that spreads the host components over
+    waves, orbits with sources on the one-lane kernels, whose compiler shares subexpressions
+    between components)."""
     from streamsculptor_amd import workloads as W
     from streamsculptor_amd.potential import SubhaloLinePotential
     from streamsculptor_amd.streamhelpers import (gen_stream_ics_pert,
@@ -132,9 +135,11 @@ def test_stream_helpers_assign_potentials(ssc):
     l0, t0_ = base.gen_stream_vmapped(ts=ts, prog_w0=W.PROG_TODAY, Msat=1e4, seed_num=3, **kw)
     for fn in (gen_stream_vmapped_with_pert, gen_stream_vmapped_with_pert_fixed_prog):
         l1, t1 = fn(base, idle, ts, W.PROG_TODAY, 1e4, 3, **kw)
-        np.testing.assert_array_equal(l1.cpu().numpy(), l0.cpu().numpy())
-        np.testing.assert_array_equal(t1.cpu().numpy(), t0_.cpu().numpy())
+        for a, b in ((l1, l0), (t1, t0_)):
+            b = b.cpu().numpy()
+            np.testing.assert_allclose(a.cpu().numpy(), b, rtol=0, atol=1e-12 * np.abs(b).max())
     ics = gen_stream_ics_pert(base, idle, ts, W.PROG_TODAY, 1e4, 3, **kw)
     ics0 = base.gen_stream_ics(ts=ts, prog_w0=W.PROG_TODAY, Msat=1e4, seed_num=3, **kw)
     for a, b in zip(ics, ics0):
-        np.testing.assert_array_equal(a.cpu().numpy(), b.cpu().numpy())
+        b = b.cpu().numpy()
+        np.testing.assert_allclose(a.cpu().numpy(), b, rtol=0, atol=1e-12 * np.abs(b).max())
