@@ -30,16 +30,20 @@ template <int SK, int G>
 __device__ __forceinline__ void adj_apply(const ssc_host_potential& hp, const SourceLDS<SK>& lds,
                                           int k, double t, int rl, double x, double y, double z,
                                           const double* W, double* lx, double* gh) {
-  double phi, g[3], H[9];
-  host_full(hp, x, y, z, phi, g, H);
+  // host Hessian accumulated over the components (as host_full), each component's own
+  // gradient and parameter derivative taken from the same evaluation for its partials
+  double H[9];
+#pragma unroll
+  for (int r = 0; r < 9; ++r) H[r] = 0.0;
   for (int c = 0; c < hp.n_components; ++c) {
-    double gc[3] = {0.0, 0.0, 0.0};
-    comp_grad(hp.c[c], x, y, z, gc[0], gc[1], gc[2]);
-    double dp[3];
-    comp_grad_dp1(hp.c[c], x, y, z, dp);
+    double phic = 0.0, gc[3] = {0.0, 0.0, 0.0}, dp[3];
+    comp_full(hp.c[c], x, y, z, phic, gc, H, dp);
     gh[2 * c] -= (gc[0] * W[0] + gc[1] * W[1] + gc[2] * W[2]) / hp.c[c].p[0];
     gh[2 * c + 1] -= dp[0] * W[0] + dp[1] * W[1] + dp[2] * W[2];
   }
+  H[3] = H[1];
+  H[6] = H[2];
+  H[7] = H[5];
   double hw[3];
 #pragma unroll
   for (int r = 0; r < 3; ++r) hw[r] = H[3 * r] * W[0] + H[3 * r + 1] * W[1] + H[3 * r + 2] * W[2];

@@ -169,8 +169,11 @@ __device__ __forceinline__ void sph_hess(double x, double y, double z, double d1
 }
 
 // Potential, gradient and Hessian of one component (H upper triangle in H[0,1,2,4,5,8]).
+// Potential, gradient and Hessian (upper triangle) of one component, accumulated; with dp, also
+// d(grad Phi_c)/d(p1) (as comp_grad_dp1) from the same intermediate terms.
 __device__ __forceinline__ void comp_full(const ssc_component& c, double x, double y, double z,
-                                          double& phi, double* g, double* H) {
+                                          double& phi, double* g, double* H,
+                                          double* dp = nullptr) {
   const double gm = c.p[0];
   switch (c.kind) {
     case SSC_COMP_MIYAMOTO_NAGAI: {
@@ -194,6 +197,11 @@ __device__ __forceinline__ void comp_full(const ssc_component& c, double x, doub
       H[4] += k * y * y + gm * iD3;
       H[5] += k * y * dz;
       H[8] += k * dz * dz + gm * ddz * iD3;
+      if (dp) {
+        dp[0] = -3.0 * gm * x * S * iD5;
+        dp[1] = -3.0 * gm * y * S * iD5;
+        dp[2] = gm * (z / zeta) * iD3 - 3.0 * gm * (z * S / zeta) * S * iD5;
+      }
       break;
     }
     case SSC_COMP_NFW: {
@@ -201,6 +209,10 @@ __device__ __forceinline__ void comp_full(const ssc_component& c, double x, doub
       const double r = sqrt(x * x + y * y + z * z);
       const double L = log1p(r / rs);
       const double rp = r + rs;
+      if (dp) {
+        const double f = gm * (-1.0 / (r * rs * rp) + 1.0 / (r * rp * rp)) / r;
+        dp[0] = f * x; dp[1] = f * y; dp[2] = f * z;
+      }
       phi -= gm * L / r;
       const double d1 = gm * (L / (r * r) - 1.0 / (r * rp));
       const double d2 = gm * (1.0 / (r * r * rp) - 2.0 * L / (r * r * r) +
@@ -216,6 +228,10 @@ __device__ __forceinline__ void comp_full(const ssc_component& c, double x, doub
       const double cc = c.p[1], soft = c.p[2];
       const double r = sqrt(x * x + y * y + z * z + soft);
       const double rc = r + cc;
+      if (dp) {
+        const double f = -2.0 * gm / (r * rc * rc * rc);
+        dp[0] = f * x; dp[1] = f * y; dp[2] = f * z;
+      }
       phi -= gm / rc;
       const double d1 = gm / (rc * rc);
       const double d2 = -2.0 * gm / (rc * rc * rc);
@@ -231,6 +247,10 @@ __device__ __forceinline__ void comp_full(const ssc_component& c, double x, doub
       const double s2 = x * x + y * y + z * z + rs * rs;
       const double s = sqrt(s2);
       const double i3 = 1.0 / (s2 * s);
+      if (dp) {
+        const double f = -3.0 * gm * rs / (s2 * s2 * s);
+        dp[0] = f * x; dp[1] = f * y; dp[2] = f * z;
+      }
       phi -= gm / s;
       const double f = gm * i3;
       g[0] += f * x;
@@ -246,6 +266,7 @@ __device__ __forceinline__ void comp_full(const ssc_component& c, double x, doub
       break;
     }
     default:
+      if (dp) dp[0] = dp[1] = dp[2] = 0.0;
       break;
   }
 }
