@@ -233,41 +233,58 @@ __global__ __launch_bounds__(AJ_BLOCK) void adjoint_segment_kernel(
 //   part[chunk][r][j][4] = (dL/dS_jk (3), dL/d(Gm_j)) summed over the chunk's particles
 constexpr int SR_CHUNK = 2048;
 
+// The chunk's particle rows are staged 64 at a time through LDS with one coalesced load (lane l
+// reads particle l of the tile), and every lane then reads each particle as an LDS broadcast:
+// a wave-uniform load per particle straight from memory left the loop waiting on one round trip
+// per particle.
 template <int SK>
 __global__ __launch_bounds__(64) void source_adjoint_kernel(
     ssc_sources src, const double* __restrict__ X, const double* __restrict__ Wb, int64_t n,
     int k0, int rows, double t0, double dt, double* __restrict__ part) {
+  __shared__ double tile[64][6];  // x, y, z, W0, W1, W2 of 64 particles
   const int r = blockIdx.y, chunk = blockIdx.x;
   const int k = k0 + r;
   const double t = t0 + k * dt;
   const int64_t ib = (int64_t)chunk * SR_CHUNK, ie = min(n, ib + SR_CHUNK);
-  for (int j = threadIdx.x; j < src.n; j += 64) {
-    double sx, sy, sz;
-    if (!source_position(src, j, k, t, sx, sy, sz)) {
-      double* o = part + (((int64_t)chunk * rows + r) * src.n + j) * 4;
-      o[0] = o[1] = o[2] = o[3] = 0.0;
-      continue;
-    }
-    const double gm = src.gm[j], q2 = src.rs[j] * src.rs[j];
+  const int nj = (src.n + 63) / 64;  // block-uniform trip count (barriers inside)
+  for (int jj = 0; jj < nj; ++jj) {
+    const int j = jj * 64 + threadIdx.x;
+    double sx = 0.0, sy = 0.0, sz = 0.0;
+    const bool on = j < src.n && source_position(src, j, k, t, sx, sy, sz);
+    const double gm = j < src.n ? src.gm[j] : 0.0;
+    const double q2 = j < src.n ? src.rs[j] * src.rs[j] : 1.0;
     double gx = 0.0, gy = 0.0, gz = 0.0, gg = 0.0;
-    for (int64_t i = ib; i < ie; ++i) {
-      const double* w = Wb + ((int64_t)r * n + i) * 3;
-      const double W0 = w[0], W1 = w[1], W2 = w[2];
-      if (W0 == 0.0 && W1 == 0.0 && W2 == 0.0) continue;  // uniform across the wave
-      const double* xp = X + ((int64_t)r * n + i) * 3;
-      const double dx = xp[0] - sx, dy = xp[1] - sy, dz = xp[2] - sz;
-      const double s2 = fma(dx, dx, fma(dy, dy, fma(dz, dz, q2)));
-      const double is = rsq(s2);
-      const double i3 = is * is * is;
-      const double dW = dx * W0 + dy * W1 + dz * W2;
-      const double c5 = 3.0 * i3 * is * is * dW;
-      gx += gm * (i3 * W0 - c5 * dx);
-      gy += gm * (i3 * W1 - c5 * dy);
-      gz += gm * (i3 * W2 - c5 * dz);
-      gg -= i3 * dW;
+    for (int64_t i0 = ib; i0 < ie; i0 += 64) {
+      const int64_t i = i0 + threadIdx.x;
+      __syncthreads();  // the previous tile has been read
+      if (i < ie) {
+        const double* xp = X + ((int64_t)r * n + i) * 3;
+        const double* w = Wb + ((int64_t)r * n + i) * 3;
+        tile[threadIdx.x][0] = xp[0]; tile[threadIdx.x][1] = xp[1]; tile[threadIdx.x][2] = xp[2];
+        tile[threadIdx.x][3] = w[0]; tile[threadIdx.x][4] = w[1]; tile[threadIdx.x][5] = w[2];
+      }
+      __syncthreads();
+      const int cnt = (int)min((int64_t)64, ie - i0);
+      if (!on) continue;
+      for (int p = 0; p < cnt; ++p) {
+        const double W0 = tile[p][3], W1 = tile[p][4], W2 = tile[p][5];
+        if (W0 == 0.0 && W1 == 0.0 && W2 == 0.0) continue;  // uniform across the wave
+        const double dx = tile[p][0] - sx, dy = tile[p][1] - sy, dz = tile[p][2] - sz;
+        const double s2 = fma(dx, dx, fma(dy, dy, fma(dz, dz, q2)));
+        const double is = rsq(s2);
+        const double i3 = is * is * is;
+        const double dW = dx * W0 + dy * W1 + dz * W2;
+        const double c5 = 3.0 * i3 * is * is * dW;
+        gx += gm * (i3 * W0 - c5 * dx);
+        gy += gm * (i3 * W1 - c5 * dy);
+        gz += gm * (i3 * W2 - c5 * dz);
+        gg -= i3 * dW;
+      }
     }
-    double* o = part + (((int64_t)chunk * rows + r) * src.n + j) * 4;
-    o[0] = gx; o[1] = gy; o[2] = gz; o[3] = gg;
+    if (j < src.n) {
+      double* o = part + (((int64_t)chunk * rows + r) * src.n + j) * 4;
+      o[0] = gx; o[1] = gy; o[2] = gz; o[3] = gg;
+    }
   }
 }
 
