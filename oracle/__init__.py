@@ -27,6 +27,22 @@ What pins each function (independently of this file) -- see tests/test_oracle_*.
   not in the reference tree) is restated from its documentation: its step sequence is
   parity-unpinned; its invariants (dtmin floor, max_steps failure, tolerance convergence)
   are tested.
+* Subhalo sums (STATIC, TRACK, LINE Plummer; LINE_HERNQUIST): superposition of single-source
+  sums, the Plummer and Hernquist components at the source position, time-window semantics.
+* First-order responses (leapfrog): central differences in eps of the oracle leapfrog with the
+  subhalo potential scaled by eps (the reference's own test method, tests.ipynb:2959,3182);
+  radius block = difference of the mass block in the scale radius; linearity.
+* Adaptive responses (perturb_dopri5): inactive sources make it take exactly the steps of the
+  orbit oracle with tolerances divided by sqrt(6 / (6 + m D)) (the RMS norm over the whole
+  state); at tight tolerances the mass block = central differences of the nonlinear Dopri5
+  flow, the radius block = difference of the mass block in the scale radius; saved rows =
+  integrations ending at the save times.
+* Adjoint: every output against central differences of the forward loss.
+* MLP: forward/backward against torch CPU autograd in float64, BF16 rounding against torch's
+  bfloat16; the input VJP against torch autograd and central differences.
+* Surrogate leapfrog and adjoint (oracle/surrogate.py): with every cell falling back they equal
+  the plain oracle leapfrog and adjoint; with a smooth float64 network the adjoint equals
+  central differences of the forward map.
 """
 from __future__ import annotations
 
