@@ -3,7 +3,7 @@
 CPU FP64 restatement of the reference's stream-particle integration path (C++ in
 ``oracle/ssc_oracle.cpp``, wrapped here with ctypes + numpy).  Only ``tests/``,
 ``__graft_entry__.smoke()`` and ``bench.py``'s ``cpu_baseline`` leg may import this package, and
-only as the checker.  The product package (``streamsculptor_amd``) never imports it.
+only as the reference results are compared against.  The product package (``streamsculptor_amd``) never imports it.
 
 What pins each function (independently of this file) -- see tests/test_oracle_*.py:
 
@@ -96,7 +96,7 @@ _fast = None
 
 
 def lib(fast: bool = False):
-    """The checker build, or (fast=True) the -O3 x86-64-v3 build used to time the CPU
+    """The -O2 build, or (fast=True) the -O3 x86-64-v3 build used to time the CPU
     baseline."""
     global _lib, _fast
     if fast:

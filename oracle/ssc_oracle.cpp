@@ -1,6 +1,7 @@
 // ORACLE -- test infrastructure only.  This is a CPU FP64 restatement of the reference
 // algorithms on the stream-particle integration path, used by tests/, by smoke() and by the
-// cpu_baseline leg of bench.py as the CHECKER.  No product code may link, import or call it.
+// cpu_baseline leg of bench.py, as the reference results are compared against.  No product
+// code may link, import or call it.
 //
 // Each function cites the reference file:line it restates (paths relative to the reference
 // repository root).  Pins (what checks each part independently of this file) are listed in the
