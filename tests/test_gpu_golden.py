@@ -1,6 +1,7 @@
 """The HIP path against the committed golden vectors (tests/golden/oracle_small.npz): no oracle
-is run, the fixture is what the results are compared against.  Tolerances as in the other parity tests: fixed-step rows
-1e-9 relative, adaptive 1e-6, release 1e-12, MLP outputs 2e-3 (BF16 operands)."""
+is run, the fixture is what the results are compared against.  Tolerances as in the other
+parity tests: fixed-step rows 1e-9 relative, adaptive 1e-6, release 1e-12, MLP outputs 2e-3
+(BF16 operands)."""
 import os
 
 import numpy as np
