@@ -83,52 +83,61 @@ __device__ __forceinline__ double group_sum(double v) {
 // accumulates the parts with the
 // same operations everywhere (an FMA per coordinate, an add for the Miyamoto-Nagai z term), so
 // a sum formed on one lane and a sum of parts gathered from several waves are bit-identical.
+__device__ __forceinline__ void parts_mn(const ssc_component& c, double x, double y, double z,
+                                         double& f, double& zm) {
+  const double gm = c.p[0], a = c.p[1], b = c.p[2];
+  const double zeta = sqrt(z * z + b * b);
+  const double S = a + zeta;
+  const double D2 = x * x + y * y + S * S;
+  const double iD = rsq(D2);
+  f = gm * iD * iD * iD;
+  zm = f * z * S / zeta;
+}
+__device__ __forceinline__ double parts_nfw(const ssc_component& c, double x, double y,
+                                            double z) {
+  const double gm = c.p[0], rs = c.p[1];
+  const double r2 = x * x + y * y + z * z;
+  const double ir = rsq(r2);
+  const double r = r2 * ir;
+  const double L = log1p(r / rs);
+  return gm * ir * ir * (L * ir - 1.0 / (r + rs));
+}
+__device__ __forceinline__ double parts_hernquist(const ssc_component& c, double x, double y,
+                                                  double z) {
+  const double gm = c.p[0], cc = c.p[1], soft = c.p[2];
+  const double r2 = x * x + y * y + z * z + soft;
+  const double ir = rsq(r2);
+  const double r = r2 * ir;
+  const double rc = r + cc;
+  return gm * ir / (rc * rc);
+}
+__device__ __forceinline__ double parts_plummer(const ssc_component& c, double x, double y,
+                                                double z) {
+  const double gm = c.p[0], rs = c.p[1];
+  const double s2 = x * x + y * y + z * z + rs * rs;
+  const double is = rsq(s2);
+  return gm * is * is * is;
+}
+
 __device__ __forceinline__ void comp_grad_parts(const ssc_component& c, double x, double y,
                                                 double z, double& f, double& zm, bool& zadd) {
-  const double gm = c.p[0];
   f = 0.0;
   zm = 0.0;
   zadd = false;
   switch (c.kind) {
-    case SSC_COMP_MIYAMOTO_NAGAI: {
-      const double a = c.p[1], b = c.p[2];
-      const double zeta = sqrt(z * z + b * b);
-      const double S = a + zeta;
-      const double D2 = x * x + y * y + S * S;
-      const double iD = rsq(D2);
-      f = gm * iD * iD * iD;
-      zm = f * z * S / zeta;
+    case SSC_COMP_MIYAMOTO_NAGAI:
+      parts_mn(c, x, y, z, f, zm);
       zadd = true;
       break;
-    }
-    case SSC_COMP_NFW: {
-      const double rs = c.p[1];
-      const double r2 = x * x + y * y + z * z;
-      const double ir = rsq(r2);
-      const double r = r2 * ir;
-      const double L = log1p(r / rs);
-      f = gm * ir * ir * (L * ir - 1.0 / (r + rs));
-      zm = f;
+    case SSC_COMP_NFW:
+      f = zm = parts_nfw(c, x, y, z);
       break;
-    }
-    case SSC_COMP_HERNQUIST: {
-      const double cc = c.p[1], soft = c.p[2];
-      const double r2 = x * x + y * y + z * z + soft;
-      const double ir = rsq(r2);
-      const double r = r2 * ir;
-      const double rc = r + cc;
-      f = gm * ir / (rc * rc);
-      zm = f;
+    case SSC_COMP_HERNQUIST:
+      f = zm = parts_hernquist(c, x, y, z);
       break;
-    }
-    case SSC_COMP_PLUMMER: {
-      const double rs = c.p[1];
-      const double s2 = x * x + y * y + z * z + rs * rs;
-      const double is = rsq(s2);
-      f = gm * is * is * is;
-      zm = f;
+    case SSC_COMP_PLUMMER:
+      f = zm = parts_plummer(c, x, y, z);
       break;
-    }
     default:
       break;
   }
@@ -155,6 +164,43 @@ __device__ __forceinline__ void host_grad(const ssc_host_potential& hp, double x
   gy = 0.0;
   gz = 0.0;
   for (int c = 0; c < hp.n_components; ++c) comp_grad(hp.c[c], x, y, z, gx, gy, gz);
+}
+
+// Host signatures with straight-line gradient code.  The generic gradient walks the components
+// with a branch on each kind, so a wave finishes one component's dependent chain before it
+// issues the next; for the two hosts used throughout (MN + Hernquist + NFW, and the Milky Way
+// model MN + Hernquist + Hernquist + NFW) the kinds are known at compile time and the compiler
+// interleaves the independent chains.  The parts are accumulated as comp_grad_add does.
+constexpr int HS_GENERIC = 0, HS_MN_HQ_NFW = 1, HS_MN_HQ_HQ_NFW = 2;
+
+inline int host_signature(const ssc_host_potential& hp) {
+  const int n = hp.n_components;
+  if (n == 3 && hp.c[0].kind == SSC_COMP_MIYAMOTO_NAGAI && hp.c[1].kind == SSC_COMP_HERNQUIST &&
+      hp.c[2].kind == SSC_COMP_NFW)
+    return HS_MN_HQ_NFW;
+  if (n == 4 && hp.c[0].kind == SSC_COMP_MIYAMOTO_NAGAI && hp.c[1].kind == SSC_COMP_HERNQUIST &&
+      hp.c[2].kind == SSC_COMP_HERNQUIST && hp.c[3].kind == SSC_COMP_NFW)
+    return HS_MN_HQ_HQ_NFW;
+  return HS_GENERIC;
+}
+
+template <int HS>
+__device__ __forceinline__ void host_grad_sig(const ssc_host_potential& hp, double x, double y,
+                                              double z, double& gx, double& gy, double& gz) {
+  if constexpr (HS == HS_GENERIC) {
+    host_grad(hp, x, y, z, gx, gy, gz);
+  } else {
+    double fm, zm;
+    parts_mn(hp.c[0], x, y, z, fm, zm);
+    const double f1 = parts_hernquist(hp.c[1], x, y, z);
+    const double f2 = HS == HS_MN_HQ_HQ_NFW ? parts_hernquist(hp.c[2], x, y, z) : 0.0;
+    const double fn = parts_nfw(hp.c[HS == HS_MN_HQ_HQ_NFW ? 3 : 2], x, y, z);
+    gx = gy = gz = 0.0;
+    comp_grad_add(fm, zm, true, x, y, z, gx, gy, gz);
+    comp_grad_add(f1, f1, false, x, y, z, gx, gy, gz);
+    if (HS == HS_MN_HQ_HQ_NFW) comp_grad_add(f2, f2, false, x, y, z, gx, gy, gz);
+    comp_grad_add(fn, fn, false, x, y, z, gx, gy, gz);
+  }
 }
 
 // Spherical Hessian accumulation: H += (phi'/r) I + (phi'' - phi'/r) x x^T / r^2.

@@ -83,12 +83,12 @@ struct GroupSources {
 };
 
 // dy/dtau at tau (t = dir * tau) for the orbit ODE [v, -grad Phi].
-template <int SK, bool GR>
+template <int SK, bool GR, int HS>
 __device__ __forceinline__ void rhs(const ssc_host_potential& hp, const SourceLDS<SK>& lds,
                                     const GroupSources<SK>& grp, double dir, double tau,
                                     const double* y, double* k) {
   double gx, gy, gz;
-  host_grad(hp, y[0], y[1], y[2], gx, gy, gz);
+  host_grad_sig<HS>(hp, y[0], y[1], y[2], gx, gy, gz);
   if (GR)
     grp.add(dir * tau, y[0], y[1], y[2], gx, gy, gz);
   else
@@ -119,7 +119,7 @@ __device__ __forceinline__ double clip_stops(const double* __restrict__ stops, i
   return best;
 }
 
-template <int SK, bool GR>
+template <int SK, bool GR, int HS>
 __global__ __launch_bounds__(DP_BLOCK) void dopri5_kernel(
     ssc_host_potential hp, ssc_sources src, const double* __restrict__ w0,
     const double* __restrict__ t_start, int64_t n, double t0, double t1,
@@ -168,7 +168,7 @@ __global__ __launch_bounds__(DP_BLOCK) void dopri5_kernel(
   int nacc = 0, nrej = 0, nsteps = 0;
   double tprev = tau0;
   if (tau0 < tau1) {
-    rhs<SK, GR>(hp, lds, grp, dir, tau0, y, f0);
+    rhs<SK, GR, HS>(hp, lds, grp, dir, tau0, y, f0);
     double tnext;
     {
       double sc[6], u[6];
@@ -186,7 +186,7 @@ __global__ __launch_bounds__(DP_BLOCK) void dopri5_kernel(
         double y1[6], f1[6];
 #pragma unroll
         for (int c = 0; c < 6; ++c) y1[c] = y[c] + h0 * f0[c];
-        rhs<SK, GR>(hp, lds, grp, dir, tau0 + h0, y1, f1);
+        rhs<SK, GR, HS>(hp, lds, grp, dir, tau0 + h0, y1, f1);
 #pragma unroll
         for (int c = 0; c < 6; ++c) u[c] = (f1[c] - f0[c]) / sc[c];
         const double d2 = rms6(u) / h0;
@@ -203,25 +203,25 @@ __global__ __launch_bounds__(DP_BLOCK) void dopri5_kernel(
       double k2[6], k3[6], k4[6], k5[6], k6[6], k7[6], yt[6], yn[6];
 #pragma unroll
       for (int c = 0; c < 6; ++c) yt[c] = y[c] + h * (A21 * f0[c]);
-      rhs<SK, GR>(hp, lds, grp, dir, tprev + C2 * h, yt, k2);
+      rhs<SK, GR, HS>(hp, lds, grp, dir, tprev + C2 * h, yt, k2);
 #pragma unroll
       for (int c = 0; c < 6; ++c) yt[c] = y[c] + h * (A31 * f0[c] + A32 * k2[c]);
-      rhs<SK, GR>(hp, lds, grp, dir, tprev + C3 * h, yt, k3);
+      rhs<SK, GR, HS>(hp, lds, grp, dir, tprev + C3 * h, yt, k3);
 #pragma unroll
       for (int c = 0; c < 6; ++c) yt[c] = y[c] + h * (A41 * f0[c] + A42 * k2[c] + A43 * k3[c]);
-      rhs<SK, GR>(hp, lds, grp, dir, tprev + C4 * h, yt, k4);
+      rhs<SK, GR, HS>(hp, lds, grp, dir, tprev + C4 * h, yt, k4);
 #pragma unroll
       for (int c = 0; c < 6; ++c)
         yt[c] = y[c] + h * (A51 * f0[c] + A52 * k2[c] + A53 * k3[c] + A54 * k4[c]);
-      rhs<SK, GR>(hp, lds, grp, dir, tprev + C5 * h, yt, k5);
+      rhs<SK, GR, HS>(hp, lds, grp, dir, tprev + C5 * h, yt, k5);
 #pragma unroll
       for (int c = 0; c < 6; ++c)
         yt[c] = y[c] + h * (A61 * f0[c] + A62 * k2[c] + A63 * k3[c] + A64 * k4[c] + A65 * k5[c]);
-      rhs<SK, GR>(hp, lds, grp, dir, tnext, yt, k6);
+      rhs<SK, GR, HS>(hp, lds, grp, dir, tnext, yt, k6);
 #pragma unroll
       for (int c = 0; c < 6; ++c)
         yn[c] = y[c] + h * (B1 * f0[c] + B3 * k3[c] + B4 * k4[c] + B5 * k5[c] + B6 * k6[c]);
-      rhs<SK, GR>(hp, lds, grp, dir, tnext, yn, k7);
+      rhs<SK, GR, HS>(hp, lds, grp, dir, tnext, yn, k7);
       double u[6];
 #pragma unroll
       for (int c = 0; c < 6; ++c) {
@@ -344,16 +344,16 @@ int set_lds_dp(const void* fn, size_t lds) {
   return SSC_OK;
 }
 
-template <int SK, bool GR>
+template <int SK, bool GR, int HS>
 int launch_dopri5(const ssc_host_potential& hp, const ssc_sources& src, const double* w0,
                   const double* t_start, int64_t n, double t0, double t1, const double* ts,
                   int nts, const double* stops, const Dp5Ctl& ctl, double* w_out, double* ys,
                   int32_t* stats, double* dense, int dense_cap, hipStream_t st) {
   const size_t lds = GR ? 0 : SourceLDS<SK>::bytes(src);
-  int rc = set_lds_dp((const void*)dopri5_kernel<SK, GR>, lds);
+  int rc = set_lds_dp((const void*)dopri5_kernel<SK, GR, HS>, lds);
   if (rc) return rc;
   const int64_t blocks = (n + DP_BLOCK - 1) / DP_BLOCK;
-  hipLaunchKernelGGL((dopri5_kernel<SK, GR>), dim3((unsigned)blocks), dim3(DP_BLOCK), lds, st, hp,
+  hipLaunchKernelGGL((dopri5_kernel<SK, GR, HS>), dim3((unsigned)blocks), dim3(DP_BLOCK), lds, st, hp,
                      src, w0, t_start, n, t0, t1, ts, nts, stops, ctl, w_out, ys, stats, dense,
                      dense_cap);
   return check_launch("ssc_dopri5");
@@ -401,9 +401,14 @@ static int dopri5_entry(const ssc_host_potential* host, const ssc_sources* src, 
   Dp5Ctl ctl{opt->rtol, opt->atol, opt->dtmin, opt->dtmax, opt->dt0, opt->max_steps, opt->n_stops};
   hipStream_t st = (hipStream_t)stream;
   const bool grouped = s.kind != SSC_SRC_NONE && s.group_size > 0;
+  const int hs = host_signature(*host);
+#define SSC_DP3(K, G, H)                                                                          \
+  launch_dopri5<K, G, H>(*host, s, w0, t_start, n, t0, t1, ts, nts, opt->stops, ctl, w_out, ys,   \
+                         stats, dense, dense_cap, st)
 #define SSC_DP(K, G)                                                                              \
-  launch_dopri5<K, G>(*host, s, w0, t_start, n, t0, t1, ts, nts, opt->stops, ctl, w_out, ys,      \
-                      stats, dense, dense_cap, st)
+  (hs == HS_MN_HQ_NFW ? SSC_DP3(K, G, HS_MN_HQ_NFW)                                              \
+                      : (hs == HS_MN_HQ_HQ_NFW ? SSC_DP3(K, G, HS_MN_HQ_HQ_NFW)                  \
+                                               : SSC_DP3(K, G, HS_GENERIC)))
   switch (s.kind) {
     case SSC_SRC_NONE: return SSC_DP(SSC_SRC_NONE, false);
     case SSC_SRC_STATIC: return grouped ? SSC_DP(SSC_SRC_STATIC, true) : SSC_DP(SSC_SRC_STATIC, false);
@@ -413,6 +418,7 @@ static int dopri5_entry(const ssc_host_potential* host, const ssc_sources* src, 
     default: return fail(SSC_E_ARG, "ssc_dopri5: bad source kind");
   }
 #undef SSC_DP
+#undef SSC_DP3
 }
 
 extern "C" int ssc_dopri5(const ssc_host_potential* host, const ssc_sources* src,
