@@ -192,7 +192,7 @@ def cpu_baseline(wl, w0, start, out, sample_every):
     sts = start[::sample_every]
     nsteps = len(wl["ts"]) - 1
     reps, el = 0, 0.0
-    while reps == 0 or (el < CPU_SECONDS and reps < 50):
+    while reps == 0 or (el < CPU_SECONDS and reps < 1000):
         t0 = time.perf_counter()
         if wl["name"] == "responses":
             ref, _ = O.perturb_leapfrog(h, src, w0s, float(wl["ts"][0]), W.DT, nsteps, start=sts,

@@ -124,13 +124,13 @@ __global__ __launch_bounds__(256) void eval_kernel(ssc_host_potential hp, ssc_so
 constexpr int LF_BLOCK = 256;
 
 // Acceleration of P particles at grid row k / time t.
-template <int SK, int P>
+template <int SK, int P, int HS>
 __device__ __forceinline__ void accel_P(const ssc_host_potential& hp, const SourceLDS<SK>& lds,
                                         int k, double t, const double* x, const double* y,
                                         const double* z, double* ax, double* ay, double* az) {
   double gx[P], gy[P], gz[P];
 #pragma unroll
-  for (int p = 0; p < P; ++p) host_grad(hp, x[p], y[p], z[p], gx[p], gy[p], gz[p]);
+  for (int p = 0; p < P; ++p) host_grad_sig<HS>(hp, x[p], y[p], z[p], gx[p], gy[p], gz[p]);
   lds.template add_grad<P>(k, t, x, y, z, gx, gy, gz, 0, 1);
 #pragma unroll
   for (int p = 0; p < P; ++p) {
@@ -140,7 +140,7 @@ __device__ __forceinline__ void accel_P(const ssc_host_potential& hp, const Sour
   }
 }
 
-template <int SK, int P>
+template <int SK, int P, int HS>
 __global__ __launch_bounds__(LF_BLOCK) void leapfrog_kernel(
     ssc_host_potential hp, ssc_sources src, const double* __restrict__ w0,
     const int32_t* __restrict__ start, int64_t n, double t0, double dt, int nsteps,
@@ -188,7 +188,7 @@ __global__ __launch_bounds__(LF_BLOCK) void leapfrog_kernel(
   const double h = 0.5 * dt;
   if (kmin < nsteps) {
     lds.stage(src, kmin, nsteps + 1);
-    accel_P<SK, P>(hp, lds, kmin, t0 + kmin * dt, x, y, z, ax, ay, az);
+    accel_P<SK, P, HS>(hp, lds, kmin, t0 + kmin * dt, x, y, z, ax, ay, az);
   }
   // A particle that has not started keeps its initial row; the force evaluated there at
   // t_{k+1} is exactly the initial force it needs when it starts at k + 1.
@@ -200,7 +200,7 @@ __global__ __launch_bounds__(LF_BLOCK) void leapfrog_kernel(
         vx[p] = vx[p] + h * ax[p]; vy[p] = vy[p] + h * ay[p]; vz[p] = vz[p] + h * az[p];
         x[p] = x[p] + dt * vx[p]; y[p] = y[p] + dt * vy[p]; z[p] = z[p] + dt * vz[p];
       }
-    accel_P<SK, P>(hp, lds, k + 1, t0 + (k + 1) * dt, x, y, z, ax, ay, az);
+    accel_P<SK, P, HS>(hp, lds, k + 1, t0 + (k + 1) * dt, x, y, z, ax, ay, az);
 #pragma unroll
     for (int p = 0; p < P; ++p)
       if (k >= s[p]) {
@@ -229,7 +229,7 @@ __global__ __launch_bounds__(LF_BLOCK) void leapfrog_kernel(
 // are 1.5 waves per SIMD), G consecutive lanes split the orbit's source sum (lane r takes
 // sources r, r + G, ...), combine it with a butterfly over the group and all carry the same
 // orbit state.  Block-uniform grid walk and TRACK staging as in leapfrog_kernel.
-template <int SK, int G>
+template <int SK, int G, int HS>
 __device__ __forceinline__ void accel_group(const ssc_host_potential& hp, const SourceLDS<SK>& lds,
                                             int k, double t, int r, double x, double y, double z,
                                             double& ax, double& ay, double& az) {
@@ -239,13 +239,13 @@ __device__ __forceinline__ void accel_group(const ssc_host_potential& hp, const 
   gy = group_sum<G>(gy);
   gz = group_sum<G>(gz);
   double hx, hy, hz;
-  host_grad(hp, x, y, z, hx, hy, hz);
+  host_grad_sig<HS>(hp, x, y, z, hx, hy, hz);
   ax = -(hx + gx);
   ay = -(hy + gy);
   az = -(hz + gz);
 }
 
-template <int SK, int G>
+template <int SK, int G, int HS>
 __global__ __launch_bounds__(LF_BLOCK) void leapfrog_group_kernel(
     ssc_host_potential hp, ssc_sources src, const double* __restrict__ w0,
     const int32_t* __restrict__ start, int64_t n, double t0, double dt, int nsteps,
@@ -283,7 +283,7 @@ __global__ __launch_bounds__(LF_BLOCK) void leapfrog_group_kernel(
   double ax = 0.0, ay = 0.0, az = 0.0;
   if (kmin < nsteps) {
     lds.stage(src, kmin, nsteps + 1);
-    accel_group<SK, G>(hp, lds, kmin, t0 + kmin * dt, r, x, y, z, ax, ay, az);
+    accel_group<SK, G, HS>(hp, lds, kmin, t0 + kmin * dt, r, x, y, z, ax, ay, az);
   }
   for (int k = kmin; k < nsteps; ++k) {
     lds.stage(src, k, nsteps + 1);
@@ -292,7 +292,7 @@ __global__ __launch_bounds__(LF_BLOCK) void leapfrog_group_kernel(
       vx = vx + h * ax; vy = vy + h * ay; vz = vz + h * az;
       x = x + dt * vx; y = y + dt * vy; z = z + dt * vz;
     }
-    accel_group<SK, G>(hp, lds, k + 1, t0 + (k + 1) * dt, r, x, y, z, ax, ay, az);
+    accel_group<SK, G, HS>(hp, lds, k + 1, t0 + (k + 1) * dt, r, x, y, z, ax, ay, az);
     if (act) {
       vx = vx + h * ax; vy = vy + h * ay; vz = vz + h * az;
     }
@@ -315,7 +315,7 @@ __global__ __launch_bounds__(LF_BLOCK) void leapfrog_group_kernel(
 // the orbit state is replicated in every lane.
 constexpr int WAVE = 64;
 
-template <int SK>
+template <int SK, int HS>
 __device__ __forceinline__ void accel_wave(const ssc_host_potential& hp, const SourceLDS<SK>& lds,
                                            int k, double t, double x, double y, double z,
                                            double& ax, double& ay, double& az) {
@@ -325,13 +325,13 @@ __device__ __forceinline__ void accel_wave(const ssc_host_potential& hp, const S
   gy = group_sum<WAVE>(gy);
   gz = group_sum<WAVE>(gz);
   double hx, hy, hz;
-  host_grad(hp, x, y, z, hx, hy, hz);
+  host_grad_sig<HS>(hp, x, y, z, hx, hy, hz);
   ax = -(hx + gx);
   ay = -(hy + gy);
   az = -(hz + gz);
 }
 
-template <int SK>
+template <int SK, int HS>
 __global__ __launch_bounds__(WAVE) void leapfrog_wave_kernel(
     ssc_host_potential hp, ssc_sources src, const double* __restrict__ w0,
     const int32_t* __restrict__ start, int64_t n, double t0, double dt, int nsteps,
@@ -359,13 +359,13 @@ __global__ __launch_bounds__(WAVE) void leapfrog_wave_kernel(
   double ax = 0, ay = 0, az = 0;
   if (s < nsteps) {
     lds.stage(src, s, nsteps + 1);
-    accel_wave<SK>(hp, lds, s, t0 + s * dt, x, y, z, ax, ay, az);
+    accel_wave<SK, HS>(hp, lds, s, t0 + s * dt, x, y, z, ax, ay, az);
   }
   for (int k = s; k < nsteps; ++k) {
     lds.stage(src, k, nsteps + 1);
     vx = vx + h * ax; vy = vy + h * ay; vz = vz + h * az;
     x = x + dt * vx; y = y + dt * vy; z = z + dt * vz;
-    accel_wave<SK>(hp, lds, k + 1, t0 + (k + 1) * dt, x, y, z, ax, ay, az);
+    accel_wave<SK, HS>(hp, lds, k + 1, t0 + (k + 1) * dt, x, y, z, ax, ay, az);
     vx = vx + h * ax; vy = vy + h * ay; vz = vz + h * az;
     while (next < nsave && save_idx[next] == k + 1) {
       if (lead) {
@@ -475,30 +475,31 @@ static int set_lds(const void* fn, size_t lds) {
 
 // Launch shape: one wave per orbit for a handful of orbits in a large source table; two
 // particles per lane once the batch alone fills the chip; G lanes per orbit for mid-sized
-// batches with sources; host components over waves for up to 65536 orbits in the host alone;
-// one particle per lane otherwise.
-template <int SK>
+// batches with sources; for up to 65536 orbits in a host of no known signature, host components
+// over waves; one particle per lane otherwise (with straight-line host code for the known
+// hosts, which is faster still).
+template <int SK, int HS>
 static int launch_leapfrog(const ssc_host_potential& hp, const ssc_sources& src, const double* w0,
                            const int32_t* start, int64_t n, double t0, double dt, int nsteps,
                            double* w_out, const int32_t* save_idx, int nsave, double* ys,
                            hipStream_t st) {
   const size_t lds = SourceLDS<SK>::bytes(src);
   if (SK != SSC_SRC_NONE && src.n >= 64 && n <= 8192) {
-    int rc = set_lds((const void*)leapfrog_wave_kernel<SK>, lds);
+    int rc = set_lds((const void*)leapfrog_wave_kernel<SK, HS>, lds);
     if (rc) return rc;
-    hipLaunchKernelGGL(leapfrog_wave_kernel<SK>, dim3((unsigned)n), dim3(WAVE), lds, st, hp, src,
+    hipLaunchKernelGGL((leapfrog_wave_kernel<SK, HS>), dim3((unsigned)n), dim3(WAVE), lds, st, hp, src,
                        w0, start, n, t0, dt, nsteps, w_out, save_idx, nsave, ys);
     return check_launch("ssc_leapfrog");
   }
   if (SK != SSC_SRC_NONE && src.n >= 16 && n >= 393216) {
-    int rc = set_lds((const void*)leapfrog_kernel<SK, 2>, lds);
+    int rc = set_lds((const void*)leapfrog_kernel<SK, 2, HS>, lds);
     if (rc) return rc;
     const int64_t blocks = (n + 2 * LF_BLOCK - 1) / (2 * LF_BLOCK);
-    hipLaunchKernelGGL((leapfrog_kernel<SK, 2>), dim3((unsigned)blocks), dim3(LF_BLOCK), lds, st,
+    hipLaunchKernelGGL((leapfrog_kernel<SK, 2, HS>), dim3((unsigned)blocks), dim3(LF_BLOCK), lds, st,
                        hp, src, w0, start, n, t0, dt, nsteps, w_out, save_idx, nsave, ys);
     return check_launch("ssc_leapfrog");
   }
-  if (SK == SSC_SRC_NONE && hp.n_components >= 2 && n <= 65536) {
+  if (SK == SSC_SRC_NONE && HS == HS_GENERIC && hp.n_components >= 2 && n <= 65536) {
     const int64_t blocks = (n + WAVE - 1) / WAVE;
     hipLaunchKernelGGL(leapfrog_comp_kernel, dim3((unsigned)blocks),
                        dim3(WAVE * hp.n_components), 0, st, hp, w0, start, n, t0, dt, nsteps,
@@ -508,10 +509,10 @@ static int launch_leapfrog(const ssc_host_potential& hp, const ssc_sources& src,
   const int g = SK == SSC_SRC_NONE ? 1 : group_lanes(n, src.n);
 #define SSC_GROUP(GV)                                                                              \
   if (g == GV) {                                                                                   \
-    int rc = set_lds((const void*)leapfrog_group_kernel<SK, GV>, lds);                            \
+    int rc = set_lds((const void*)leapfrog_group_kernel<SK, GV, HS>, lds);                            \
     if (rc) return rc;                                                                             \
     const int64_t blocks = (n + LF_BLOCK / GV - 1) / (LF_BLOCK / GV);                              \
-    hipLaunchKernelGGL((leapfrog_group_kernel<SK, GV>), dim3((unsigned)blocks), dim3(LF_BLOCK),   \
+    hipLaunchKernelGGL((leapfrog_group_kernel<SK, GV, HS>), dim3((unsigned)blocks), dim3(LF_BLOCK),   \
                        lds, st, hp, src, w0, start, n, t0, dt, nsteps, w_out, save_idx, nsave, ys); \
     return check_launch("ssc_leapfrog");                                                          \
   }
@@ -520,10 +521,10 @@ static int launch_leapfrog(const ssc_host_potential& hp, const ssc_sources& src,
   SSC_GROUP(8)
   SSC_GROUP(16)
 #undef SSC_GROUP
-  int rc = set_lds((const void*)leapfrog_kernel<SK, 1>, lds);
+  int rc = set_lds((const void*)leapfrog_kernel<SK, 1, HS>, lds);
   if (rc) return rc;
   const int64_t blocks = (n + LF_BLOCK - 1) / LF_BLOCK;
-  hipLaunchKernelGGL((leapfrog_kernel<SK, 1>), dim3((unsigned)blocks), dim3(LF_BLOCK), lds, st, hp,
+  hipLaunchKernelGGL((leapfrog_kernel<SK, 1, HS>), dim3((unsigned)blocks), dim3(LF_BLOCK), lds, st, hp,
                      src, w0, start, n, t0, dt, nsteps, w_out, save_idx, nsave, ys);
   return check_launch("ssc_leapfrog");
 }
@@ -573,7 +574,11 @@ int ssc_leapfrog(const ssc_host_potential* host, const ssc_sources* src, const d
   ssc_sources s = src ? *src : empty_sources();
   if (s.n == 0) s.kind = SSC_SRC_NONE;
   hipStream_t st = (hipStream_t)stream;
-#define SSC_LF(K) launch_leapfrog<K>(*host, s, w0, start_step, n, t0, dt, nsteps, w_out, save_idx, nsave, ys, st)
+  const int hs = host_signature(*host);
+#define SSC_LF3(K, H) launch_leapfrog<K, H>(*host, s, w0, start_step, n, t0, dt, nsteps, w_out, save_idx, nsave, ys, st)
+#define SSC_LF(K)                                                                                 \
+  (hs == HS_MN_HQ_NFW ? SSC_LF3(K, HS_MN_HQ_NFW)                                                 \
+                      : (hs == HS_MN_HQ_HQ_NFW ? SSC_LF3(K, HS_MN_HQ_HQ_NFW) : SSC_LF3(K, HS_GENERIC)))
   switch (s.kind) {
     case SSC_SRC_NONE: return SSC_LF(SSC_SRC_NONE);
     case SSC_SRC_STATIC: return SSC_LF(SSC_SRC_STATIC);
@@ -583,6 +588,7 @@ int ssc_leapfrog(const ssc_host_potential* host, const ssc_sources* src, const d
     default: return fail(SSC_E_ARG, "ssc_leapfrog: bad source kind");
   }
 #undef SSC_LF
+#undef SSC_LF3
 }
 
 }  // extern "C"
