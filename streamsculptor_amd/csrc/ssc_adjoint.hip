@@ -26,7 +26,19 @@ constexpr int AJ_MAXC = 2 * SSC_MAX_COMPONENTS;
 // Adjoint contribution at one grid point: lx -= H W, host-parameter gradients += (da/dp)^T W.
 // With G lanes per orbit, lane r takes sources r, r + G, ... and the group sums the source part
 // of H W; the host part is computed by every lane of the group.
-template <int SK, int G>
+// One host component's share: its Hessian accumulated into H, its own gradient and parameter
+// derivative (from the same evaluation) into the partials gh[2c], gh[2c + 1].  K >= 0 fixes
+// the component's kind at compile time.
+template <int K>
+__device__ __forceinline__ void adj_comp(const ssc_component& cp, int c, double x, double y,
+                                         double z, const double* W, double* H, double* gh) {
+  double phic = 0.0, gc[3] = {0.0, 0.0, 0.0}, dp[3];
+  comp_full_as<K>(cp, x, y, z, phic, gc, H, dp);
+  gh[2 * c] -= (gc[0] * W[0] + gc[1] * W[1] + gc[2] * W[2]) / cp.p[0];
+  gh[2 * c + 1] -= dp[0] * W[0] + dp[1] * W[1] + dp[2] * W[2];
+}
+
+template <int SK, int G, int HS>
 __device__ __forceinline__ void adj_apply(const ssc_host_potential& hp, const SourceLDS<SK>& lds,
                                           int k, double t, int rl, double x, double y, double z,
                                           const double* W, double* lx, double* gh) {
@@ -35,11 +47,13 @@ __device__ __forceinline__ void adj_apply(const ssc_host_potential& hp, const So
   double H[9];
 #pragma unroll
   for (int r = 0; r < 9; ++r) H[r] = 0.0;
-  for (int c = 0; c < hp.n_components; ++c) {
-    double phic = 0.0, gc[3] = {0.0, 0.0, 0.0}, dp[3];
-    comp_full(hp.c[c], x, y, z, phic, gc, H, dp);
-    gh[2 * c] -= (gc[0] * W[0] + gc[1] * W[1] + gc[2] * W[2]) / hp.c[c].p[0];
-    gh[2 * c + 1] -= dp[0] * W[0] + dp[1] * W[1] + dp[2] * W[2];
+  if constexpr (HS == HS_GENERIC) {
+    for (int c = 0; c < hp.n_components; ++c) adj_comp<-1>(hp.c[c], c, x, y, z, W, H, gh);
+  } else {
+    adj_comp<hs_kind(HS, 0)>(hp.c[0], 0, x, y, z, W, H, gh);
+    adj_comp<hs_kind(HS, 1)>(hp.c[1], 1, x, y, z, W, H, gh);
+    adj_comp<hs_kind(HS, 2)>(hp.c[2], 2, x, y, z, W, H, gh);
+    if constexpr (hs_components(HS) > 3) adj_comp<hs_kind(HS, 3)>(hp.c[3], 3, x, y, z, W, H, gh);
   }
   H[3] = H[1];
   H[6] = H[2];
@@ -71,14 +85,14 @@ __device__ __forceinline__ void adj_apply(const ssc_host_potential& hp, const So
 
 // Total acceleration of one orbit whose G lanes split the source sum (all lanes of the group
 // must be active: they always are, since the group shares one orbit).
-template <int SK, int G>
+template <int SK, int G, int HS>
 __device__ __forceinline__ void accel_split(const ssc_host_potential& hp, const SourceLDS<SK>& lds,
                                             int k, double t, int rl, double x, double y, double z,
                                             double& ax, double& ay, double& az) {
   double sx = 0.0, sy = 0.0, sz = 0.0;
   lds.template add_grad<1>(k, t, &x, &y, &z, &sx, &sy, &sz, rl, G);
   double gx, gy, gz;
-  host_grad(hp, x, y, z, gx, gy, gz);
+  host_grad_sig<HS>(hp, x, y, z, gx, gy, gz);
   ax = -(gx + group_sum<G>(sx));
   ay = -(gy + group_sum<G>(sy));
   az = -(gz + group_sum<G>(sz));
@@ -90,7 +104,7 @@ __device__ __forceinline__ void accel_split(const ssc_host_potential& hp, const 
 //   lam:  in/out adjoint of each particle's state (at k1 on entry, at max(k0, start) on exit)
 //   X, Wb: [k1 - k0 + 1][n][3] recorded positions and weights (zero where inactive)
 //   gpart: [gridDim.x][2*n_comp] per-block host-parameter partials (accumulated)
-template <int SK, int G>
+template <int SK, int G, int HS>
 __global__ __launch_bounds__(AJ_BLOCK) void adjoint_segment_kernel(
     ssc_host_potential hp, ssc_sources src, const double* __restrict__ ckpt,
     const int32_t* __restrict__ start, int64_t n, double t0, double dt, int nsteps, int k0, int k1,
@@ -138,7 +152,7 @@ __global__ __launch_bounds__(AJ_BLOCK) void adjoint_segment_kernel(
       o[0] = x; o[1] = y; o[2] = z;
     }
     lds.stage(src, kmin, nsteps + 1);
-    accel_split<SK, G>(hp, lds, kmin, t0 + kmin * dt, rl, x, y, z, ax, ay, az);
+    accel_split<SK, G, HS>(hp, lds, kmin, t0 + kmin * dt, rl, x, y, z, ax, ay, az);
     for (int k = kmin; k < k1; ++k) {
       lds.stage(src, k, nsteps + 1);
       const bool mv = act && k >= kb;
@@ -146,7 +160,7 @@ __global__ __launch_bounds__(AJ_BLOCK) void adjoint_segment_kernel(
         vx += h * ax; vy += h * ay; vz += h * az;
         x += dt * vx; y += dt * vy; z += dt * vz;
       }
-      accel_split<SK, G>(hp, lds, k + 1, t0 + (k + 1) * dt, rl, x, y, z, ax, ay, az);
+      accel_split<SK, G, HS>(hp, lds, k + 1, t0 + (k + 1) * dt, rl, x, y, z, ax, ay, az);
       if (mv) {
         vx += h * ax; vy += h * ay; vz += h * az;
         if (lead) {
@@ -174,7 +188,7 @@ __global__ __launch_bounds__(AJ_BLOCK) void adjoint_segment_kernel(
       if (act) {
         const double* xp = X + ((int64_t)(k1 - k0) * n + i) * 3;
         const double W[3] = {h * lv[0], h * lv[1], h * lv[2]};
-        adj_apply<SK, G>(hp, lds, k1, t0 + k1 * dt, rl, xp[0], xp[1], xp[2], W, lx, gh);
+        adj_apply<SK, G, HS>(hp, lds, k1, t0 + k1 * dt, rl, xp[0], xp[1], xp[2], W, lx, gh);
         if (lead) {
           double* wb = Wb + ((int64_t)(k1 - k0) * n + i) * 3;
           wb[0] = W[0]; wb[1] = W[1]; wb[2] = W[2];
@@ -192,7 +206,7 @@ __global__ __launch_bounds__(AJ_BLOCK) void adjoint_segment_kernel(
 #pragma unroll
         for (int c = 0; c < 3; ++c) W[c] = h * lh[c] + (k > s ? h * lv[c] : 0.0);
         const double* xp = X + ((int64_t)(k - k0) * n + i) * 3;
-        adj_apply<SK, G>(hp, lds, k, t0 + k * dt, rl, xp[0], xp[1], xp[2], W, lx, gh);
+        adj_apply<SK, G, HS>(hp, lds, k, t0 + k * dt, rl, xp[0], xp[1], xp[2], W, lx, gh);
         if (lead) {
           double* wb = Wb + ((int64_t)(k - k0) * n + i) * 3;
           wb[0] = W[0]; wb[1] = W[1]; wb[2] = W[2];
@@ -365,7 +379,7 @@ static int64_t adj_bytes(int64_t n, int nsteps, int E, int m, AdjWs* w, char* ba
   return off;
 }
 
-template <int SK>
+template <int SK, int HS>
 static int adjoint_backward(const ssc_host_potential& hp, const ssc_sources& src,
                             const int32_t* start, int64_t n, double t0, double dt, int nsteps,
                             int E, const double* lam_T, const double* forcing, double* lam_0,
@@ -379,11 +393,11 @@ static int adjoint_backward(const ssc_host_potential& hp, const ssc_sources& src
   const size_t lds = SourceLDS<SK>::bytes(src);
   auto seg_kernel = [&](void) -> const void* {
     switch (G) {
-      case 2: return (const void*)adjoint_segment_kernel<SK, 2>;
-      case 4: return (const void*)adjoint_segment_kernel<SK, 4>;
-      case 8: return (const void*)adjoint_segment_kernel<SK, 8>;
-      case 16: return (const void*)adjoint_segment_kernel<SK, 16>;
-      default: return (const void*)adjoint_segment_kernel<SK, 1>;
+      case 2: return (const void*)adjoint_segment_kernel<SK, 2, HS>;
+      case 4: return (const void*)adjoint_segment_kernel<SK, 4, HS>;
+      case 8: return (const void*)adjoint_segment_kernel<SK, 8, HS>;
+      case 16: return (const void*)adjoint_segment_kernel<SK, 16, HS>;
+      default: return (const void*)adjoint_segment_kernel<SK, 1, HS>;
     }
   };
   if (lds > 60 * 1024) {
@@ -402,7 +416,7 @@ static int adjoint_backward(const ssc_host_potential& hp, const ssc_sources& src
     const int k0 = (int)(c * E), k1 = (int)std::min<int64_t>((c + 1) * E, nsteps);
     const int rows = k1 - k0 + 1;
 #define SSC_SEG(GV)                                                                                \
-  hipLaunchKernelGGL((adjoint_segment_kernel<SK, GV>), dim3(nblk), dim3(AJ_BLOCK), lds, st, hp, src, \
+  hipLaunchKernelGGL((adjoint_segment_kernel<SK, GV, HS>), dim3(nblk), dim3(AJ_BLOCK), lds, st, hp, src, \
                      w.ckpt + c * n * 6, start, n, t0, dt, nsteps, k0, k1, lam_T, forcing, w.lam,    \
                      w.X, w.Wb, w.gpart)
     switch (G) {
@@ -481,16 +495,19 @@ int ssc_adjoint_backward(const ssc_host_potential* host, const ssc_sources* src,
   AdjWs w;
   adj_bytes(n, nsteps, checkpoint_every, m, &w, (char*)workspace);
   hipStream_t st = (hipStream_t)stream;
+  const int hs = host_signature(*host);
+#define SSC_AB3(K, H)                                                                             \
+  adjoint_backward<K, H>(*host, s, start_step, n, t0, dt, nsteps, checkpoint_every, lam_T, forcing, \
+                         lam_0, g_host, g_gm, g_pos, w, st)
+#define SSC_AB(K)                                                                            \
+  (hs == HS_MN_HQ_NFW ? SSC_AB3(K, HS_MN_HQ_NFW)                                             \
+                      : (hs == HS_MN_HQ_HQ_NFW ? SSC_AB3(K, HS_MN_HQ_HQ_NFW) : SSC_AB3(K, HS_GENERIC)))
   switch (s.kind) {
-    case SSC_SRC_NONE:
-      return adjoint_backward<SSC_SRC_NONE>(*host, s, start_step, n, t0, dt, nsteps, checkpoint_every,
-                                            lam_T, forcing, lam_0, g_host, g_gm, g_pos, w, st);
-    case SSC_SRC_STATIC:
-      return adjoint_backward<SSC_SRC_STATIC>(*host, s, start_step, n, t0, dt, nsteps, checkpoint_every,
-                                              lam_T, forcing, lam_0, g_host, g_gm, g_pos, w, st);
-    case SSC_SRC_TRACK:
-      return adjoint_backward<SSC_SRC_TRACK>(*host, s, start_step, n, t0, dt, nsteps, checkpoint_every,
-                                             lam_T, forcing, lam_0, g_host, g_gm, g_pos, w, st);
+    case SSC_SRC_NONE: return SSC_AB(SSC_SRC_NONE);
+    case SSC_SRC_STATIC: return SSC_AB(SSC_SRC_STATIC);
+    case SSC_SRC_TRACK: return SSC_AB(SSC_SRC_TRACK);
+#undef SSC_AB
+#undef SSC_AB3
     default:
       return fail(SSC_E_ARG, "ssc_adjoint_backward: LINE sources (Plummer or Hernquist) are not "
                              "differentiated");

@@ -329,6 +329,50 @@ __device__ __forceinline__ void host_full(const ssc_host_potential& hp, double x
   H[7] = H[5];
 }
 
+// comp_full with the component's kind fixed at compile time (K >= 0: the kind switch folds
+// away; K < 0: the kind is read from c as in comp_full).
+template <int K>
+__device__ __forceinline__ void comp_full_as(const ssc_component& c, double x, double y, double z,
+                                             double& phi, double* g, double* H,
+                                             double* dp = nullptr) {
+  if constexpr (K < 0) {
+    comp_full(c, x, y, z, phi, g, H, dp);
+  } else {
+    ssc_component k = c;
+    k.kind = K;
+    comp_full(k, x, y, z, phi, g, H, dp);
+  }
+}
+
+// Number of components and kind of component C of a known host signature.
+constexpr int hs_components(int HS) { return HS == HS_MN_HQ_HQ_NFW ? 4 : 3; }
+constexpr int hs_kind(int HS, int C) {
+  return C == 0 ? SSC_COMP_MIYAMOTO_NAGAI
+                : (C == hs_components(HS) - 1 ? SSC_COMP_NFW : SSC_COMP_HERNQUIST);
+}
+
+// host_full as straight-line code for a known signature (same accumulation order).
+template <int HS>
+__device__ __forceinline__ void host_full_sig(const ssc_host_potential& hp, double x, double y,
+                                              double z, double& phi, double* g, double* H) {
+  if constexpr (HS == HS_GENERIC) {
+    host_full(hp, x, y, z, phi, g, H);
+  } else {
+    phi = 0.0;
+    g[0] = g[1] = g[2] = 0.0;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) H[i] = 0.0;
+    comp_full_as<hs_kind(HS, 0)>(hp.c[0], x, y, z, phi, g, H);
+    comp_full_as<hs_kind(HS, 1)>(hp.c[1], x, y, z, phi, g, H);
+    comp_full_as<hs_kind(HS, 2)>(hp.c[2], x, y, z, phi, g, H);
+    if constexpr (hs_components(HS) > 3)
+      comp_full_as<hs_kind(HS, 3)>(hp.c[3], x, y, z, phi, g, H);
+    H[3] = H[1];
+    H[6] = H[2];
+    H[7] = H[5];
+  }
+}
+
 // d(grad Phi_c)/d(p1) for one component (MN a, NFW r_s, Hernquist c, Plummer r_s).
 __device__ __forceinline__ void comp_grad_dp1(const ssc_component& c, double x, double y, double z,
                                               double* out) {

@@ -26,13 +26,14 @@ namespace ssc {
 constexpr int PT_BLOCK = 256;
 
 // host gradient and Hessian (upper triangle filled into all 9) at one point
+template <int HS>
 __device__ __forceinline__ void host_grad_hess(const ssc_host_potential& hp, double x, double y,
                                                double z, double* g, double* H) {
   double phi;
-  host_full(hp, x, y, z, phi, g, H);
+  host_full_sig<HS>(hp, x, y, z, phi, g, H);
 }
 
-template <int SK, int G, int RAD>
+template <int SK, int G, int RAD, int HS>
 __global__ __launch_bounds__(PT_BLOCK) void perturb_kernel(
     ssc_host_potential hp, ssc_sources pert, const double* __restrict__ w0,
     const double* __restrict__ dw0, const int32_t* __restrict__ start, int64_t n, int m, int L,
@@ -85,7 +86,7 @@ __global__ __launch_bounds__(PT_BLOCK) void perturb_kernel(
   auto apply = [&](int k, double c, bool move) {
     const double t = t0 + k * dt;
     double g[3], H[9];
-    host_grad_hess(hp, x, y, z, g, H);
+    host_grad_hess<HS>(hp, x, y, z, g, H);
     if (move) {
       vx -= c * g[0]; vy -= c * g[1]; vz -= c * g[2];
     }
@@ -185,7 +186,7 @@ __global__ __launch_bounds__(PT_BLOCK) void perturb_kernel(
   }
 }
 
-template <int SK, int G, int RAD>
+template <int SK, int G, int RAD, int HS>
 static int launch_perturb(const ssc_host_potential& hp, const ssc_sources& pert, const double* w0,
                           const double* dw0, const int32_t* start, int64_t n, double t0, double dt,
                           int nsteps, double* w_out, double* dw_out, hipStream_t st) {
@@ -197,12 +198,12 @@ static int launch_perturb(const ssc_host_potential& hp, const ssc_sources& pert,
   const int ppb = PT_BLOCK / Lp;
   const size_t lds = SourceLDS<SK>::bytes(pert);
   if (lds > 60 * 1024) {
-    hipError_t e = hipFuncSetAttribute((const void*)perturb_kernel<SK, G, RAD>,
+    hipError_t e = hipFuncSetAttribute((const void*)perturb_kernel<SK, G, RAD, HS>,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return fail(SSC_E_HIP, std::string("hipFuncSetAttribute: ") + hipGetErrorString(e));
   }
   const int64_t blocks = (n + ppb - 1) / ppb;
-  hipLaunchKernelGGL((perturb_kernel<SK, G, RAD>), dim3((unsigned)blocks), dim3(PT_BLOCK), lds, st,
+  hipLaunchKernelGGL((perturb_kernel<SK, G, RAD, HS>), dim3((unsigned)blocks), dim3(PT_BLOCK), lds, st,
                      hp, pert, w0, dw0, start, n, m, Lp, t0, dt, nsteps, w_out, dw_out);
   return check_launch("ssc_perturb_leapfrog");
 }
@@ -227,7 +228,12 @@ extern "C" int ssc_perturb_leapfrog(const ssc_host_potential* host, const ssc_so
   if (!w0 || !w_out || !dw_out) return fail(SSC_E_ARG, "ssc_perturb_leapfrog: NULL array");
   hipStream_t st = (hipStream_t)stream;
   const ssc_sources s = *pert;
-#define SSC_PT(K, R) launch_perturb<K, 8, R>(*host, s, w0, dw0, start_step, n, t0, dt, nsteps, w_out, dw_out, st)
+  const int hs = host_signature(*host);
+#define SSC_PT3(K, R, H) launch_perturb<K, 8, R, H>(*host, s, w0, dw0, start_step, n, t0, dt, nsteps, w_out, dw_out, st)
+#define SSC_PT(K, R)                                                                 \
+  (hs == HS_MN_HQ_NFW ? SSC_PT3(K, R, HS_MN_HQ_NFW)                                  \
+                      : (hs == HS_MN_HQ_HQ_NFW ? SSC_PT3(K, R, HS_MN_HQ_HQ_NFW)      \
+                                               : SSC_PT3(K, R, HS_GENERIC)))
   if (with_radius) {
     switch (s.kind) {
       case SSC_SRC_STATIC: return SSC_PT(SSC_SRC_STATIC, 1);
@@ -244,5 +250,6 @@ extern "C" int ssc_perturb_leapfrog(const ssc_host_potential* host, const ssc_so
     }
   }
 #undef SSC_PT
+#undef SSC_PT3
   return fail(SSC_E_ARG, "ssc_perturb_leapfrog: bad source kind");
 }
