@@ -83,6 +83,38 @@ def test_no_fallback_matches_oracle_integrator(oracle, ssc):
     assert np.median(err) < 1e-7 and err.max() < 1e-5
 
 
+def test_many_blocks_with_starts_match_oracle_surrogate(oracle, ssc):
+    """Several 192-particle blocks (the last one partial), staggered start steps and the trained
+    residual map (network and exact-sum cells mixed): a sample spread over every block follows
+    the oracle's surrogate integrator (oracle/surrogate.py, BF16-emulating MLP, exact Plummer
+    sum in flagged cells and outside the grid)."""
+    import oracle.surrogate as OS
+    from streamsculptor_amd import workloads as W
+    host, clumps, pot, sur, pos, m, rs = _setup(ssc, iters=200)
+    h = oracle.host(W.host_components())
+    src = oracle.Sources(oracle.SRC_STATIC, gm=G * m, rs=rs, pos=pos)
+    sc = sur.scaling
+    om = OS.Model(h, src, sur.mlp.params.cpu().numpy(), sc.center, sc.half, sc.g_scale,
+                  np.asarray(sur.grid_lo), np.asarray(sur.grid_cell), np.asarray(sur.grid_n),
+                  sur.flags.cpu().numpy())
+    rng = np.random.default_rng(11)
+    n, nsteps, dt = 3000, 150, 1.0
+    w0 = bound_orbits(rng, n, rmin=6.0, rmax=14.0)
+    start = rng.integers(0, 100, n).astype(np.int32)
+    out, cnt = sur.leapfrog(w0, 0.0, dt, nsteps, start=start)
+    sel = np.unique(np.concatenate([np.linspace(0, n - 1, 40).astype(np.int64),
+                                    np.arange(n - 8, n)]))
+    ref, nfb = OS.leapfrog(om, w0[sel], dt, nsteps, start=start[sel], emulate=True)
+    o = out.cpu().numpy()[sel]
+    err = np.linalg.norm(o - ref, axis=1) / np.linalg.norm(ref, axis=1)
+    psteps = int((nsteps - start).sum())
+    print(f"{n} particles, {len(sel)} sampled: median {np.median(err):.2e} max {err.max():.2e}; "
+          f"GPU fallback {int(cnt.item())} of {psteps} particle-steps, oracle sample {nfb}")
+    assert 0 < nfb < int((nsteps - start[sel]).sum())  # both kinds of cell are visited
+    assert 0 < int(cnt.item()) < psteps
+    assert np.median(err) < 1e-7 and err.max() < 1e-5
+
+
 def test_trained_network_beats_host_only(ssc):
     """Median final-position error against the plain method: surrogate << host force only."""
     host, clumps, pot, sur, *_ = _setup(ssc, iters=1500)
