@@ -27,14 +27,14 @@ struct d3 {
   double x, y, z;
 };
 
-// 1/sqrt(v) for positive finite v: the hardware estimate refined by one Halley-type step
-// y (1 + e/2 + 3e^2/8), e = 1 - v y^2, which triples the estimate's correct bits (full FP64).
+// 1/sqrt(v) for positive finite v: the hardware estimate refined by one Newton step
+// y (1 + e/2), e = 1 - v y^2, which doubles the estimate's correct bits.
 // Unlike the library rsqrt it skips the zero/inf/denormal class handling, which no squared
 // radius on this path needs.
 __device__ __forceinline__ double rsq(double v) {
   const double y = __builtin_amdgcn_rsq(v);
   const double e = fma(-v * y, y, 1.0);
-  return fma(y * e, fma(e, 0.375, 0.5), y);
+  return fma(0.5 * y, e, y);
 }
 
 // ------------------------------------------------------------------------------------------
