@@ -445,6 +445,20 @@ __device__ __forceinline__ double gm_inv_s3(double gm, double s2) {
   return gm * y * y2 * t;
 }
 
+// plummer_grad with G m given as (c1, c2) = (-1.5 G m, 2.5 G m), so that the product by G m
+// folds into the correction of gm_inv_s3: G m / s^3 = y^3 fma(s^2 y^2, c1, c2).
+__device__ __forceinline__ void plummer_grad_k(double dx, double dy, double dz, double rs2,
+                                               double c1, double c2, double& gx, double& gy,
+                                               double& gz) {
+  const double s2 = fma(dx, dx, fma(dy, dy, fma(dz, dz, rs2)));
+  const double y = __builtin_amdgcn_rsq(s2);
+  const double y2 = y * y;
+  const double f = y * y2 * fma(s2 * y2, c1, c2);
+  gx += f * dx;
+  gy += f * dy;
+  gz += f * dz;
+}
+
 __device__ __forceinline__ void plummer_grad(double dx, double dy, double dz, double gm,
                                              double rs2, double& gx, double& gy, double& gz) {
   const double s2 = fma(dx, dx, fma(dy, dy, fma(dz, dz, rs2)));

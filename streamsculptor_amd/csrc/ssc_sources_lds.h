@@ -1,5 +1,5 @@
 // LDS-resident Plummer source tables shared by the leapfrog, response and adjoint kernels.
-//   STATIC: rec[j] = (x, y, z, G*m), rs2[j]           (loaded once)
+//   STATIC: geo[j] = (x, y, z, r_s^2), kc[j] = (-1.5 G*m, 2.5 G*m), gmv[j]   (loaded once)
 //   TRACK:  rows kb..kb+chunk of the trajectory table, rec[r][j] = (x, y, z, G*m)
 //   LINE:   (x0, v, G*m, r_s^2, t0, t_window) per source, position formed per step
 //   LINE_HERNQUIST: the same with r_s in place of r_s^2 and the Hernquist term
@@ -36,19 +36,34 @@ struct SourceLDS<SSC_SRC_NONE> {
   }
 };
 
+// STATIC tables keep the geometry and G m in the forms the pair loop consumes:
+// geo[j] = (x, y, z, r_s^2), kc[j] = (-1.5 G m, 2.5 G m) for plummer_grad_k, gmv[j] = G m.
+struct alignas(16) SrcGeo {
+  double x, y, z, rs2;
+};
+struct alignas(16) SrcK {
+  double c1, c2;
+};
+
 template <>
 struct SourceLDS<SSC_SRC_STATIC> {
-  SrcRec* rec;
-  double* rs2;
+  SrcGeo* geo;
+  SrcK* kc;
+  double* gmv;
   int n;
-  static size_t bytes(const ssc_sources& s) { return (size_t)s.n * (sizeof(SrcRec) + 8); }
+  static size_t bytes(const ssc_sources& s) {
+    return (size_t)s.n * (sizeof(SrcGeo) + sizeof(SrcK) + 8);
+  }
   __device__ void init(const ssc_sources& s, char* lds) {
     n = s.n;
-    rec = reinterpret_cast<SrcRec*>(lds);
-    rs2 = reinterpret_cast<double*>(lds + (size_t)n * sizeof(SrcRec));
+    geo = reinterpret_cast<SrcGeo*>(lds);
+    kc = reinterpret_cast<SrcK*>(lds + (size_t)n * sizeof(SrcGeo));
+    gmv = reinterpret_cast<double*>(lds + (size_t)n * (sizeof(SrcGeo) + sizeof(SrcK)));
     for (int j = threadIdx.x; j < n; j += blockDim.x) {
-      rec[j] = SrcRec{s.pos[3 * j], s.pos[3 * j + 1], s.pos[3 * j + 2], s.gm[j]};
-      rs2[j] = s.rs[j] * s.rs[j];
+      const double gm = s.gm[j];
+      geo[j] = SrcGeo{s.pos[3 * j], s.pos[3 * j + 1], s.pos[3 * j + 2], s.rs[j] * s.rs[j]};
+      kc[j] = SrcK{-1.5 * gm, 2.5 * gm};
+      gmv[j] = gm;
     }
     __syncthreads();
   }
@@ -58,16 +73,17 @@ struct SourceLDS<SSC_SRC_STATIC> {
   __device__ void add_grad(int, double, const double* x, const double* y, const double* z,
                            double* gx, double* gy, double* gz, int j0, int js) const {
     for (int j = j0; j < n; j += js) {
-      const SrcRec r = rec[j];
-      const double q = rs2[j];
+      const SrcGeo g = geo[j];
+      const SrcK c = kc[j];
 #pragma unroll
-      for (int p = 0; p < P; ++p) plummer_grad(x[p] - r.x, y[p] - r.y, z[p] - r.z, r.gm, q, gx[p], gy[p], gz[p]);
+      for (int p = 0; p < P; ++p)
+        plummer_grad_k(x[p] - g.x, y[p] - g.y, z[p] - g.z, g.rs2, c.c1, c.c2, gx[p], gy[p], gz[p]);
     }
   }
   __device__ bool get(int, double, int j, double& sx, double& sy, double& sz, double& gm,
                       double& q) const {
-    const SrcRec r = rec[j];
-    sx = r.x; sy = r.y; sz = r.z; gm = r.gm; q = rs2[j];
+    const SrcGeo g = geo[j];
+    sx = g.x; sy = g.y; sz = g.z; gm = gmv[j]; q = g.rs2;
     return true;
   }
 };
