@@ -1,6 +1,6 @@
 // LDS-resident Plummer source tables shared by the leapfrog, response and adjoint kernels.
 //   STATIC: geo[j] = (x, y, z, r_s^2), kc[j] = (-1.5 G*m, 2.5 G*m), gmv[j]   (loaded once)
-//   TRACK:  rows kb..kb+chunk of the trajectory table, rec[r][j] = (x, y, z, G*m)
+//   TRACK:  rows kb..kb+chunk of the trajectory table, rec[r][j] = (x, y, z, r_s^2), kc[j], gmv[j]
 //   LINE:   (x0, v, G*m, r_s^2, t0, t_window) per source, position formed per step
 //   LINE_HERNQUIST: the same with r_s in place of r_s^2 and the Hernquist term
 // add_grad(j0, js) sums sources j0, j0 + js, ... so a wave can also split one orbit's sum;
@@ -97,20 +97,28 @@ __host__ __device__ inline int track_chunk(int n) {
 
 template <>
 struct SourceLDS<SSC_SRC_TRACK> {
-  SrcRec* rec;  // [chunk + 1][n]
-  double* rs2;  // [n]
+  SrcGeo* rec;  // [chunk + 1][n]: (x, y, z, r_s^2)
+  SrcK* kc;     // [n]: (-1.5 G m, 2.5 G m)
+  double* gmv;  // [n]: G m
   int n;
   int chunk;
   int kb;       // grid row held in rec[0]
   static size_t bytes(const ssc_sources& s) {
-    return (size_t)(track_chunk(s.n) + 1) * s.n * sizeof(SrcRec) + (size_t)s.n * 8;
+    return (size_t)(track_chunk(s.n) + 1) * s.n * sizeof(SrcGeo) +
+           (size_t)s.n * (sizeof(SrcK) + 8);
   }
   __device__ void init(const ssc_sources& s, char* lds) {
     n = s.n;
     chunk = track_chunk(n);
-    rec = reinterpret_cast<SrcRec*>(lds);
-    rs2 = reinterpret_cast<double*>(lds + (size_t)(chunk + 1) * n * sizeof(SrcRec));
-    for (int j = threadIdx.x; j < n; j += blockDim.x) rs2[j] = s.rs[j] * s.rs[j];
+    rec = reinterpret_cast<SrcGeo*>(lds);
+    kc = reinterpret_cast<SrcK*>(lds + (size_t)(chunk + 1) * n * sizeof(SrcGeo));
+    gmv = reinterpret_cast<double*>(lds + (size_t)(chunk + 1) * n * sizeof(SrcGeo) +
+                                    (size_t)n * sizeof(SrcK));
+    for (int j = threadIdx.x; j < n; j += blockDim.x) {
+      const double gm = s.gm[j];
+      kc[j] = SrcK{-1.5 * gm, 2.5 * gm};
+      gmv[j] = gm;
+    }
     kb = INT_MIN;
   }
   // Make rows k and k+1 resident (block-uniform k; contains barriers).
@@ -123,7 +131,8 @@ struct SourceLDS<SSC_SRC_TRACK> {
     for (int e = threadIdx.x; e < tot; e += blockDim.x) {
       const int r = e / n, j = e - r * n;
       const double* p = s.pos + ((int64_t)(k + r) * n + j) * 3;
-      rec[r * n + j] = SrcRec{p[0], p[1], p[2], s.gm[j]};
+      const double rs = s.rs[j];
+      rec[r * n + j] = SrcGeo{p[0], p[1], p[2], rs * rs};
     }
     __syncthreads();
   }
@@ -135,18 +144,19 @@ struct SourceLDS<SSC_SRC_TRACK> {
   template <int P>
   __device__ void add_grad(int k, double, const double* x, const double* y, const double* z,
                            double* gx, double* gy, double* gz, int j0, int js) const {
-    const SrcRec* row = rec + (k - kb) * n;
+    const SrcGeo* row = rec + (k - kb) * n;
     for (int j = j0; j < n; j += js) {
-      const SrcRec r = row[j];
-      const double q = rs2[j];
+      const SrcGeo g = row[j];
+      const SrcK c = kc[j];
 #pragma unroll
-      for (int p = 0; p < P; ++p) plummer_grad(x[p] - r.x, y[p] - r.y, z[p] - r.z, r.gm, q, gx[p], gy[p], gz[p]);
+      for (int p = 0; p < P; ++p)
+        plummer_grad_k(x[p] - g.x, y[p] - g.y, z[p] - g.z, g.rs2, c.c1, c.c2, gx[p], gy[p], gz[p]);
     }
   }
   __device__ bool get(int k, double, int j, double& sx, double& sy, double& sz, double& gm,
                       double& q) const {
-    const SrcRec r = rec[(k - kb) * n + j];
-    sx = r.x; sy = r.y; sz = r.z; gm = r.gm; q = rs2[j];
+    const SrcGeo g = rec[(k - kb) * n + j];
+    sx = g.x; sy = g.y; sz = g.z; gm = gmv[j]; q = g.rs2;
     return true;
   }
 };
