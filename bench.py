@@ -83,9 +83,14 @@ def build_workload(name, particles=None):
 
 
 def spray(wl):
+    """Spray rows of the workload.  The perturbed streams (live subhalos, responses) release in
+    the smooth host as the reference's perturbed-stream helpers do (streamhelpers.py:57-66,
+    perturbative.py:253-278); the clumpy halo releases in the whole potential, clumps included
+    (gen_stream_vmapped, main.py:287-306)."""
     from streamsculptor_amd import workloads as W
     from streamsculptor_amd.streams import spray_particles
-    return spray_particles(wl["pot"], wl["pot"], wl["ts"], wl["prog_w0"], W.MSAT, 0,
+    rel = wl["host"] if wl["name"] in ("subhalos", "responses") else wl["pot"]
+    return spray_particles(wl["pot"], rel, wl["ts"], wl["prog_w0"], W.MSAT, 0,
                            wl["per_step"], W.DT)
 
 
@@ -171,6 +176,24 @@ def train_surrogate(wl, iters):
     return ClumpSurrogate.train(wl["host"], wl["clumps"], P, iters=iters, log=log)
 
 
+def host_cpu():
+    """(threads, description) of the host cores the CPU baseline runs on: every CPU this
+    process may run on, capped at 16 (the host share of one GPU on a shared node), with nproc
+    and the CPU model reported beside it."""
+    avail = len(os.sched_getaffinity(0))
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    threads = min(avail, 16)
+    return threads, {"nproc": os.cpu_count(), "affinity": avail, "cpu_model": model}
+
+
 def cpu_baseline(wl, w0, start, out, sample_every):
     """The oracle (-O3 x86-64-v3 + OpenMP build of the same algorithm) on a strided sample of
     the same particles, repeated until about CPU_SECONDS of CPU work; also returns the sample's
@@ -178,7 +201,7 @@ def cpu_baseline(wl, w0, start, out, sample_every):
     import numpy as np
     import oracle as O
     from streamsculptor_amd import workloads as W
-    cores = min(len(os.sched_getaffinity(0)), 16)
+    cores, cpu = host_cpu()
     h = O.host(W.host_components())
     src = None
     if wl["name"] == "clumps":
@@ -206,7 +229,7 @@ def cpu_baseline(wl, w0, start, out, sample_every):
     g = out[::sample_every].cpu().numpy()
     err = float((np.linalg.norm(g - ref, axis=1) / np.linalg.norm(ref, axis=1)).max())
     every = f"every {sample_every}th particle" if sample_every > 1 else "every particle"
-    return dict(value=reps * psteps / el, unit="particle-steps/s", cores=cores, kind="port",
+    return dict(value=reps * psteps / el, unit="particle-steps/s", cores=cores, **cpu, kind="port",
                 sample=f"{every} of the same stream ({len(w0s)} particles, {psteps} "
                        f"particle-steps) x {reps} passes, {el:.1f} s, oracle OpenMP"), err
 
@@ -278,7 +301,7 @@ def gd1_main(args):
         f"particle kernel {kern * 1e3:.1f} ms, {failed} failed orbits")
     if not args.no_cpu:
         import oracle as O
-        cores = min(len(os.sched_getaffinity(0)), 16)
+        cores, cpu = host_cpu()
         h = O.host(W.gala_mw_components())
         bt = g["batch"]
         ic_np = ic.cpu().numpy()
@@ -290,7 +313,8 @@ def gd1_main(args):
             src = O.Sources(O.SRC_LINE, gm=W.G * bt.m[b], rs=bt.a[b], pos=bt.subhalo_x0[b],
                             vel=bt.subhalo_v[b], t0=bt.subhalo_t0[b], t_window=bt.t_window[b])
             lo, tr, sto = O.stream_dopri5(h, ic_np[b], g["ts"], 1e4, 532, src=src,
-                                          nthreads=cores, fast=True, rtol=1e-6, atol=1e-6,
+                                          release_host=h, nthreads=cores, fast=True,
+                                          rtol=1e-6, atol=1e-6,
                                           dtmin=0.05, max_steps=10_000)
             psteps += int(sto[:, 0].sum())
             sto = sto.reshape(T, 2, 3)
@@ -301,7 +325,7 @@ def gd1_main(args):
         el = time.perf_counter() - t0
         errs = np.concatenate(errs)
         res["cpu_baseline"] = {"value": len(sample) / el, "unit": "simulations/s",
-                               "cores": cores, "kind": "port",
+                               "cores": cores, **cpu, "kind": "port",
                                "particle_steps_per_s": psteps / el,
                                "sample": f"{len(sample)} of the {B} simulations, oracle Dopri5 "
                                          f"(OpenMP), {el:.1f} s"}
@@ -413,7 +437,7 @@ def derivs_main(args):
         f"kernel {kern * 1e3:.1f} ms, {achieved / 1e12:.2f} TFLOP/s, {failed} failed")
     if not args.no_cpu:
         import oracle as O
-        cores = min(len(os.sched_getaffinity(0)), 16)
+        cores, cpu = host_cpu()
         h = O.host(W.host_components())
         so = O.Sources(O.SRC_LINE_HERNQUIST, gm=W.G * d["mass"], rs=d["rs"], pos=d["x0"],
                        vel=d["v"], t0=d["tc"], t_window=d["tw"])
@@ -431,7 +455,7 @@ def derivs_main(args):
         rel = np.linalg.norm((dw - dwr).reshape(len(sel), -1), axis=1) / np.maximum(
             np.linalg.norm(dwr.reshape(len(sel), -1), axis=1), 1e-300)
         res["cpu_baseline"] = {"value": float(sto[:, 0].sum()) / el, "unit": "particle-steps/s",
-                               "cores": cores, "kind": "port",
+                               "cores": cores, **cpu, "kind": "port",
                                "sample": f"every {every}th particle ({len(sel)} of {n}), "
                                          f"oracle Dopri5 responses (OpenMP), {el:.1f} s"}
         res["parity_sample_max_rel_err"] = float(rel.max())
@@ -540,7 +564,7 @@ def ensemble_main(args):
             el = time.perf_counter() - tc
         psteps = int((ens.T - start).sum())
         res["cpu_baseline"] = {"value": psteps / el, "unit": "particle-steps/s", "cores": 1,
-                               "kind": "port",
+                               **host_cpu()[1], "kind": "port",
                                "sample": f"{len(sel)} particles of the ensemble (every "
                                          f"{ens.n // args.cpu_particles}th), forward + adjoint "
                                          f"through the oracle surrogate (numpy, BF16 emulation), "

@@ -173,11 +173,17 @@ int ssc_dopri5_dense(const ssc_host_potential* host, const ssc_sources* src, con
  * prog_w[i][6] with satellite mass G*Msat = g_msat[i] and release index rel_idx[i]
  * (NULL = i; the index seeds the per-release keys):
  *   pos_close/pos_far/vel_close/vel_far [nrel][3]  (close = leading, far = trailing).
+ * The tidal radius uses d2Phi/dr2 of host + src (src NULL or NONE: the host alone), as the
+ * reference takes the Hessian of the whole potential (main.py:77-85,230-237); release i is at
+ * time t_rel[i] (device; needed by LINE sources) and source-table row step_rel[i] (device;
+ * needed by TRACK sources, a row outside the table gives NaN rows).  Both may be NULL
+ * otherwise.
  * kvals: NULL selects the reference defaults {2.0, 0.3, 0, 0, 0.4, 0.4, 0.5, 0.5}, else 8
  * host doubles.  Offsets are drawn with the reference's key scheme (PRNGKey(seed) ->
  * randint(5, 0, 1000) -> PRNGKey(i*r_k) -> normal) on a Threefry-2x32-20 counter PRNG.
  * g_msat is G*Msat per release (device). */
-int ssc_release(const ssc_host_potential* host, const double* prog_w, const int32_t* rel_idx,
+int ssc_release(const ssc_host_potential* host, const ssc_sources* src, const double* prog_w,
+                const int32_t* rel_idx, const double* t_rel, const int32_t* step_rel,
                 const double* g_msat, int32_t nrel, int64_t seed, const double* kvals,
                 double* pos_close, double* pos_far, double* vel_close, double* vel_far,
                 void* stream);
@@ -218,8 +224,10 @@ int ssc_perturb_dopri5(const ssc_host_potential* host, const ssc_sources* pert,
                        const ssc_dopri5_options* opt, double* w_out, double* dw_out, double* ys,
                        double* dys, int32_t* stats, void* stream);
 
-/* Vector-Jacobian product of the release model: given dL/d(pos_close, pos_far, vel_close,
- * vel_far) [nrel][3] each, writes dL/d(progenitor row) g_prog[nrel][6] and dL/d(G m_c, p1_c) of
+/* Vector-Jacobian product of the release model in the host potential alone (no point sources:
+ * the release of the reference's perturbed streams, streamhelpers.py:57-66): given
+ * dL/d(pos_close, pos_far, vel_close, vel_far) [nrel][3] each, writes dL/d(progenitor row)
+ * g_prog[nrel][6] and dL/d(G m_c, p1_c) of
  * the host components into g_host[n_components][2] (summed over releases).  scratch:
  * [nrel][2 * n_components] doubles.  Extends the reference's release-model Jacobian
  * (perturbative.py:282-296) with the host parameters. */

@@ -121,7 +121,7 @@ def _bind(L):
     L.orc_release_normals.argtypes = [i64, i64, vp]
     L.orc_eval.argtypes = [vp, vp, vp, i64, d, i32, vp, vp, vp]
     L.orc_leapfrog.argtypes = [vp, vp, vp, vp, i64, d, d, i32, vp, vp, ctypes.c_int]
-    L.orc_release.argtypes = [vp, vp, vp, vp, i32, i64, vp, vp, vp, vp, vp]
+    L.orc_release.argtypes = [vp, vp, vp, vp, vp, vp, i32, i64, vp, vp, vp, vp, vp]
     L.orc_release_jacobian.argtypes = [vp, vp, vp, i32, i64, vp, vp]
     L.orc_adjoint_leapfrog.argtypes = [vp, vp, vp, vp, i64, d, d, i32, vp, vp, vp, vp, vp, vp,
                                         ctypes.c_int]
@@ -299,7 +299,10 @@ def stream_dopri5(h, prog_w0, ts, msat, seed, src=None, kvals=None, release_host
     T = len(ts) - 1
     _, prog, _ = dopri5(h, prog_w0, float(ts.max()), t0=float(ts.min()), ts=ts, src=src,
                         nthreads=nthreads, fast=fast, **tol)
-    pc, pf, vc, vf = release(release_host or h, prog[0], ts, msat, seed, kvals)
+    if release_host is None:  # release in the whole potential (gen_stream_vmapped)
+        pc, pf, vc, vf = release(h, prog[0], ts, msat, seed, kvals, src=src)
+    else:                     # release in a base potential (gen_stream_vmapped_with_pert)
+        pc, pf, vc, vf = release(release_host, prog[0], ts, msat, seed, kvals)
     w0 = np.empty((2 * T, 6))
     w0[0::2, :3], w0[0::2, 3:] = pc[:T], vc[:T]
     w0[1::2, :3], w0[1::2, 3:] = pf[:T], vf[:T]
@@ -334,15 +337,22 @@ def dp5_step(h_pot, t, h, y, s=(), src=None):
     return yn, err, dense[:s.size]
 
 
-def release(h, prog_w, ts, msat, seed, kvals=None):
+def release(h, prog_w, ts, msat, seed, kvals=None, src=None, steps=None):
+    """Release model at every progenitor row (restates gen_stream_ics' vmapped release_model,
+    main.py:287-306) in host + sources: the tidal radius uses the Hessian of the whole potential
+    (main.py:77-85).  Release i is at time ts[i]; TRACK sources need the grid row steps[i]."""
     prog_w = np.ascontiguousarray(prog_w, dtype=np.float64)
     nrel = prog_w.shape[0]
     gms = np.ascontiguousarray(G * np.broadcast_to(np.asarray(msat, dtype=np.float64), (nrel,)))
-    ts = np.ascontiguousarray(ts, dtype=np.float64)
+    ts = np.ascontiguousarray(np.broadcast_to(np.asarray(ts, dtype=np.float64), (nrel,)))
+    st = None if steps is None else np.ascontiguousarray(steps, dtype=np.int32)
     kv = None if kvals is None else np.ascontiguousarray(kvals, dtype=np.float64)
     outs = [np.zeros((nrel, 3)) for _ in range(4)]
-    lib().orc_release(ctypes.addressof(h), prog_w.ctypes.data, ts.ctypes.data, gms.ctypes.data,
-                      nrel, seed, _p(kv), *[o.ctypes.data for o in outs])
+    src, sp = _src_ptr(src)
+    rc = lib().orc_release(ctypes.addressof(h), sp, prog_w.ctypes.data, ts.ctypes.data, _p(st),
+                           gms.ctypes.data, nrel, seed, _p(kv), *[o.ctypes.data for o in outs])
+    if rc < 0:
+        raise ValueError("release: TRACK sources need steps")
     return tuple(outs)  # pos_close, pos_far, vel_close, vel_far
 
 
@@ -369,7 +379,10 @@ def stream_leapfrog(h, prog_w0, ts, msat, seed, dt, src=None, kvals=None, nthrea
     nsteps = int(kidx[-1])
     _, prog = leapfrog(h, prog_w0, ts[0], dt, nsteps, src=src, traj=True)
     prog_at = prog[kidx, 0, :]
-    pc, pf, vc, vf = release(release_host or h, prog_at, ts, msat, seed, kvals)
+    if release_host is None:  # release in the whole potential (gen_stream_vmapped)
+        pc, pf, vc, vf = release(h, prog_at, ts, msat, seed, kvals, src=src, steps=kidx)
+    else:                     # release in a base potential (gen_stream_vmapped_with_pert)
+        pc, pf, vc, vf = release(release_host, prog_at, ts, msat, seed, kvals)
     w0 = np.empty((2 * T, 6))
     w0[0::2, :3], w0[0::2, 3:] = pc[:T], vc[:T]
     w0[1::2, :3], w0[1::2, 3:] = pf[:T], vf[:T]

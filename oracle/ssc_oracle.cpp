@@ -362,15 +362,20 @@ double prng_normal(int64_t key_seed) {
 // taken with duals (reference release_func_jacobian, perturbative.py:282-296).
 // in[6] = progenitor (x, v); hostp = host component parameters as scalars.
 // ---------------------------------------------------------------------------------------
+// H_extra (nullable): Hessian of the potential's point sources at the progenitor, added to the
+// host Hessian (the reference takes d2phidr2 of the whole potential, main.py:77-85).
 template <typename T>
 void release_T(const ssc_host_potential* hp, const T* hostp /*[n_comp*4]*/, const T in[6],
-               T g_msat, const double z[4], const double kv[8], T out[12]) {
+               T g_msat, const double z[4], const double kv[8], T out[12],
+               const double* H_extra = nullptr) {
   T x[3] = {in[0], in[1], in[2]}, v[3] = {in[3], in[4], in[5]};
   // Hessian of the host at x.
   T g[3], H[9];
   for (int i = 0; i < 3; ++i) g[i] = T(0.0);
   for (int i = 0; i < 9; ++i) H[i] = T(0.0);
   for (int c = 0; c < hp->n_components; ++c) comp_eval(hp->c[c].kind, hostp + 4 * c, x, (T*)nullptr, g, H);
+  if (H_extra)
+    for (int i = 0; i < 9; ++i) H[i] = H[i] + T(H_extra[i]);
   T rad = sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2]);
   T rhat[3] = {x[0] / rad, x[1] / rad, x[2] / rad};
   // d2phidr2 = rhat^T H rhat (rhat held fixed), main.py:77-85
@@ -564,9 +569,16 @@ int dp5_orbit(const ssc_host_potential* hp, const ssc_sources* src, const Dp5Opt
     const double md = d1 > d2 ? d1 : d2;
     const double h1 = md <= 1e-15 ? std::max(1e-6, h0 * 1e-3) : std::pow(0.01 / md, 1.0 / 5.0);
     double dt0 = o.dt0 > 0 ? o.dt0 : std::min(100.0 * h0, h1);
+    // PIDController.init clamps the first step to [dtmin, dtmax] and a step raised to dtmin is
+    // accepted whatever its error
+    bool at_dtmin = false;
+    if (o.dtmax > 0) dt0 = std::min(dt0, o.dtmax);
+    if (o.dtmin > 0) {
+      at_dtmin = dt0 <= o.dtmin;
+      dt0 = std::max(dt0, o.dtmin);
+    }
     double tnext = clip_stops(o, tau0, tau0 + dt0);
     if (tnext > tau1) tnext = tau1;
-    bool at_dtmin = false;
     while (tprev < tau1 && nsteps < o.max_steps) {
       const double h = tnext - tprev;
       Dp5Step r;
@@ -693,9 +705,14 @@ int dp5_system(F&& f, int N, const Dp5Opts& o, double tau0, double tau1, std::ve
       const double h1 = md <= 1e-15 ? std::max(1e-6, h0 * 1e-3) : std::pow(0.01 / md, 1.0 / 5.0);
       dt0 = std::min(100.0 * h0, h1);
     }
+    bool at_dtmin = false;  // first step clamped to [dtmin, dtmax] as in dp5_orbit
+    if (o.dtmax > 0) dt0 = std::min(dt0, o.dtmax);
+    if (o.dtmin > 0) {
+      at_dtmin = dt0 <= o.dtmin;
+      dt0 = std::max(dt0, o.dtmin);
+    }
     double tnext = tau0 + dt0;
     if (tnext > tau1) tnext = tau1;
-    bool at_dtmin = false;
     while (tprev < tau1 && nsteps < o.max_steps) {
       const double h = tnext - tprev;
       for (int c = 0; c < N; ++c) yt[c] = y[c] + h * (A21 * f0[c]);
@@ -857,20 +874,29 @@ int orc_leapfrog(const ssc_host_potential* hp, const ssc_sources* src, const dou
   return 0;
 }
 
-// Spray release conditions (reference gen_stream_ics, main.py:287-306).
-int orc_release(const ssc_host_potential* hp, const double* prog_w, const double* ts,
-                const double* g_msat, int32_t nrel, int64_t seed, const double* kvals,
-                double* pos_close, double* pos_far, double* vel_close, double* vel_far) {
+// Spray release conditions (reference gen_stream_ics, main.py:287-306) in host + sources:
+// release i happens at time ts[i] (LINE sources) and grid row steps[i] (TRACK sources; NULL
+// when there are none).
+int orc_release(const ssc_host_potential* hp, const ssc_sources* src, const double* prog_w,
+                const double* ts, const int32_t* steps, const double* g_msat, int32_t nrel,
+                int64_t seed, const double* kvals, double* pos_close, double* pos_far,
+                double* vel_close, double* vel_far) {
   const double* kv = kvals ? kvals : kDefaultK;
   std::vector<double> hostp(4 * hp->n_components);
   for (int c = 0; c < hp->n_components; ++c)
     for (int k = 0; k < 4; ++k) hostp[4 * c + k] = hp->c[c].p[k];
-  (void)ts;
+  const bool with_src = src && src->kind != SSC_SRC_NONE;
+  if (with_src && src->kind == SSC_SRC_TRACK && !steps) return -1;
+  if (with_src && (src->kind == SSC_SRC_LINE || src->kind == SSC_SRC_LINE_HERNQUIST) && !ts)
+    return -1;
 #pragma omp parallel for schedule(static)
   for (int i = 0; i < nrel; ++i) {
-    double z[4], out[12];
+    double z[4], out[12], Hs[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, gs[3] = {0, 0, 0};
     release_normals(seed, i, z);
-    release_T<double>(hp, hostp.data(), prog_w + 6 * i, g_msat[i], z, kv, out);
+    if (with_src)
+      sources_eval(src, prog_w + 6 * i, steps ? steps[i] : 0, ts ? ts[i] : 0.0, nullptr, gs, Hs);
+    release_T<double>(hp, hostp.data(), prog_w + 6 * i, g_msat[i], z, kv, out,
+                      with_src ? Hs : nullptr);
     for (int k = 0; k < 3; ++k) {
       pos_close[3 * i + k] = out[k];
       pos_far[3 * i + k] = out[3 + k];

@@ -72,7 +72,8 @@ SIGNATURES = {
     "ssc_adjoint_forward": [_vp, _vp, _vp, _vp, _i64, _d, _d, _i32, _i32, _vp, _vp, _i64],
     "ssc_adjoint_backward": [_vp, _vp, _vp, _i64, _d, _d, _i32, _i32, _vp, _vp, _vp, _vp, _vp, _vp,
                              _vp, _i64],
-    "ssc_release": [_vp, _vp, _vp, _vp, _i32, ctypes.c_int64, _vp, _vp, _vp, _vp, _vp, _vp],
+    "ssc_release": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _i32, ctypes.c_int64, _vp, _vp, _vp, _vp,
+                    _vp],
     "ssc_surrogate_adjoint_forward": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _i64, _d, _d, _i32, _i32,
                                       _vp, _vp, _vp, _i64],
     "ssc_surrogate_adjoint_backward": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _i64, _d, _i32, _i32,

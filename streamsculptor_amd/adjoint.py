@@ -69,8 +69,13 @@ class LeapfrogAdjoint:
 
 
 def release_vjp(pot, prog_w, rel_idx, g_msat, seed, g_pc, g_pf, g_vc, g_vf, kval_arr=1.0):
-    """dL/d(progenitor rows) (nrel, 6) and dL/d(host G m, p1) (n_comp, 2) of the release model."""
-    h = pot._host_only()
+    """dL/d(progenitor rows) (nrel, 6) and dL/d(host G m, p1) (n_comp, 2) of the release model
+    in the analytic potential ``pot`` (the release of the perturbed streams, reference
+    streamhelpers.py:57-66, which these gradients follow)."""
+    h, src = pot._descriptor()
+    if src is not None:
+        raise NotImplementedError("the release-model VJP covers analytic potentials only: "
+                                  "release in the smooth base potential")
     prog_w = _lib.f64(prog_w).contiguous()
     nrel = prog_w.shape[0]
     dev = prog_w.device
@@ -162,8 +167,8 @@ class StreamModel:
         dev = prog_rows.device
         table = torch.empty(nseed, 2, T + 1, 6, dtype=torch.float64, device=dev)
         for s in range(nseed):
-            pc, pf, vc, vf = host._release_along(prog_rows, T + 1, self.msat, self.seed + s,
-                                                 self.kval_arr)
+            pc, pf, vc, vf = host._release_along(prog_rows, self.t0 + self.dt * np.arange(T + 1),
+                                                 self.msat, self.seed + s, self.kval_arr)
             table[s, 0, :, :3], table[s, 0, :, 3:] = pc, vc
             table[s, 1, :, :3], table[s, 1, :, 3:] = pf, vf
         sel_s, sel_a, rel = self.layout()

@@ -170,6 +170,7 @@ __global__ __launch_bounds__(DP_BLOCK) void dopri5_kernel(
   if (tau0 < tau1) {
     rhs<SK, GR, HS>(hp, lds, grp, dir, tau0, y, f0);
     double tnext;
+    bool at_dtmin = false;
     {
       double sc[6], u[6];
 #pragma unroll
@@ -194,10 +195,16 @@ __global__ __launch_bounds__(DP_BLOCK) void dopri5_kernel(
         const double h1 = md <= 1e-15 ? fmax(1e-6, h0 * 1e-3) : pow(0.01 / md, 0.2);
         dt0 = fmin(100.0 * h0, h1);
       }
+      // the controller's init clamps the first step to [dtmin, dtmax] like every later one:
+      // a first step raised to dtmin is accepted whatever its error
+      if (ctl.dtmax > 0.0) dt0 = fmin(dt0, ctl.dtmax);
+      if (ctl.dtmin > 0.0) {
+        at_dtmin = dt0 <= ctl.dtmin;
+        dt0 = fmax(dt0, ctl.dtmin);
+      }
       tnext = clip_stops(stops, ctl.n_stops, dir, tau0, tau0 + dt0);
       if (tnext > tau1) tnext = tau1;
     }
-    bool at_dtmin = false;
     while (tprev < tau1 && nsteps < ctl.max_steps) {
       const double h = tnext - tprev;
       double k2[6], k3[6], k4[6], k5[6], k6[6], k7[6], yt[6], yn[6];

@@ -416,6 +416,13 @@ void perturb_dopri5_kernel(
       const double h1 = md <= 1e-15 ? fmax(1e-6, h0 * 1e-3) : pow(0.01 / md, 0.2);
       dt0 = fmin(100.0 * h0, h1);
     }
+    // the first step is clamped to [dtmin, dtmax] like every later one (a first step raised
+    // to dtmin is accepted whatever its error)
+    if (ctl.dtmax > 0.0) dt0 = fmin(dt0, ctl.dtmax);
+    if (ctl.dtmin > 0.0) {
+      at_dtmin = dt0 <= ctl.dtmin;
+      dt0 = fmax(dt0, ctl.dtmin);
+    }
     tnext = tau0 + dt0;
     if (tnext > tau1) tnext = tau1;
   }

@@ -89,9 +89,14 @@ struct KVals {
   double k[8];
 };
 
-__global__ __launch_bounds__(256) void release_kernel(ssc_host_potential hp,
+// Release i happens at time t_rel[i] (LINE sources) and source-table row step_rel[i] (TRACK
+// sources); the point sources add their Hessian to the host's, since the reference takes
+// d2phidr2 of the whole potential.  A TRACK row outside the table gives NaN rows.
+__global__ __launch_bounds__(256) void release_kernel(ssc_host_potential hp, ssc_sources src,
                                                       const double* __restrict__ prog_w,
                                                       const int32_t* __restrict__ rel_idx,
+                                                      const double* __restrict__ t_rel,
+                                                      const int32_t* __restrict__ step_rel,
                                                       const double* __restrict__ g_msat, int nrel,
                                                       int64_t seed, KVals kv,
                                                       double* __restrict__ pos_close,
@@ -104,6 +109,24 @@ __global__ __launch_bounds__(256) void release_kernel(ssc_host_potential hp,
   const double x[3] = {w[0], w[1], w[2]}, v[3] = {w[3], w[4], w[5]};
   double phi, g[3], H[9];
   host_full(hp, x[0], x[1], x[2], phi, g, H);
+  if (src.kind != SSC_SRC_NONE) {
+    const int step = step_rel ? step_rel[i] : 0;
+    const double t = t_rel ? t_rel[i] : 0.0;
+    if (src.kind == SSC_SRC_TRACK && (step < 0 || step >= src.n_track)) {
+#pragma unroll
+      for (int k = 0; k < 9; ++k) H[k] = __builtin_nan("");
+    } else {
+      for (int j = 0; j < src.n; ++j) {
+        double sx, sy, sz;
+        if (!source_position(src, j, step, t, sx, sy, sz)) continue;
+        const double rs = src.rs[j];
+        if (src.kind == SSC_SRC_LINE_HERNQUIST)
+          hernquist_full(x[0] - sx, x[1] - sy, x[2] - sz, src.gm[j], rs, phi, g, H);
+        else
+          plummer_full(x[0] - sx, x[1] - sy, x[2] - sz, src.gm[j], rs * rs, phi, g, H);
+      }
+    }
+  }
   const double rad = sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2]);
   const double rh[3] = {x[0] / rad, x[1] / rad, x[2] / rad};
   // d2phi/dr2 = rhat^T H rhat  (reference main.py:77-85)
@@ -315,21 +338,31 @@ __global__ __launch_bounds__(128) void release_vjp_kernel(
 
 using namespace ssc;
 
-extern "C" int ssc_release(const ssc_host_potential* host, const double* prog_w, const int32_t* rel_idx,
-                           const double* g_msat, int32_t nrel, int64_t seed, const double* kvals,
-                           double* pos_close, double* pos_far, double* vel_close, double* vel_far,
-                           void* stream) {
+extern "C" int ssc_release(const ssc_host_potential* host, const ssc_sources* src,
+                           const double* prog_w, const int32_t* rel_idx, const double* t_rel,
+                           const int32_t* step_rel, const double* g_msat, int32_t nrel,
+                           int64_t seed, const double* kvals, double* pos_close, double* pos_far,
+                           double* vel_close, double* vel_far, void* stream) {
   int rc = validate_host(host);
+  if (rc) return rc;
+  rc = validate_sources(src, 0);
   if (rc) return rc;
   if (nrel < 0) return fail(SSC_E_ARG, "ssc_release: nrel < 0");
   if (nrel == 0) return SSC_OK;
   if (!prog_w || !g_msat || !pos_close || !pos_far || !vel_close || !vel_far)
     return fail(SSC_E_ARG, "ssc_release: NULL array argument");
+  const ssc_sources none = {};
+  const bool with_src = src && src->kind != SSC_SRC_NONE && src->n > 0;
+  if (with_src && src->kind == SSC_SRC_TRACK && !step_rel)
+    return fail(SSC_E_ARG, "ssc_release: TRACK sources need step_rel");
+  if (with_src && (src->kind == SSC_SRC_LINE || src->kind == SSC_SRC_LINE_HERNQUIST) && !t_rel)
+    return fail(SSC_E_ARG, "ssc_release: LINE sources need t_rel");
   KVals kv;
   const double def[8] = {2.0, 0.3, 0.0, 0.0, 0.4, 0.4, 0.5, 0.5};
   for (int k = 0; k < 8; ++k) kv.k[k] = kvals ? kvals[k] : def[k];
   hipLaunchKernelGGL(release_kernel, dim3((nrel + 255) / 256), dim3(256), 0, (hipStream_t)stream,
-                     *host, prog_w, rel_idx, g_msat, nrel, seed, kv, pos_close, pos_far, vel_close, vel_far);
+                     *host, with_src ? *src : none, prog_w, rel_idx, t_rel, step_rel, g_msat,
+                     nrel, seed, kv, pos_close, pos_far, vel_close, vel_far);
   return check_launch("ssc_release");
 }
 

@@ -372,11 +372,16 @@ class Potential:
         n = w.shape[0]
         idx = _lib.i32(np.broadcast_to(np.asarray(_as_float(i), dtype=np.int64), (n,)).copy())
         gms = self._G * _lib.f64(np.broadcast_to(np.asarray(_as_float(Msat), dtype=np.float64), (n,)).copy())
-        outs = self._release(w, idx, gms, seed_num, kval_arr)
+        tt = None if t is None else np.broadcast_to(np.asarray(_as_float(t), dtype=np.float64), (n,))
+        outs = self._release(w, idx, gms, seed_num, kval_arr, t_rel=tt)
         return tuple(o.reshape(*shape, 3) for o in outs)
 
-    def _release(self, prog_w, rel_idx, g_msat, seed_num, kval_arr):
-        h = self._host_only()
+    def _release(self, prog_w, rel_idx, g_msat, seed_num, kval_arr, t_rel=None):
+        """Release model of rows prog_w (n, 6) at times t_rel (n,) in this whole potential: its
+        point sources enter the tidal radius through d2phidr2 (reference main.py:77-85,
+        230-237).  t_rel is needed when the potential has point sources (LINE: their positions;
+        TRACK: the table row of each release)."""
+        h, src = self._descriptor()
         n = prog_w.shape[0]
         kv = np.asarray(_as_float(kval_arr), dtype=np.float64)
         kvp = None
@@ -384,9 +389,21 @@ class Potential:
             kv = np.ascontiguousarray(np.ones(8) * kv)
             kvp = kv.ctypes.data_as(_lib.ctypes.c_void_p)
         dev = prog_w.device
+        sd, t_dev, step_dev = None, None, None
+        if src is not None:
+            if t_rel is None:
+                raise ValueError("the release in a potential with point sources needs the "
+                                 "release times")
+            tr = np.asarray(_as_float(t_rel), dtype=np.float64).reshape(-1)
+            if src.kind == _lib.SRC_TRACK:
+                step_dev = _lib.i32(np.array([src.step_of(t) for t in tr], dtype=np.int32))
+            t_dev = _lib.f64(tr)
+            sd = src.desc()
         outs = [torch.empty(n, 3, dtype=torch.float64, device=dev) for _ in range(4)]
-        _lib.call("ssc_release", _lib.ctypes.byref(h), _lib.ptr(prog_w.contiguous()),
-                  _lib.ptr(rel_idx), _lib.ptr(g_msat.contiguous()), n, int(seed_num), kvp,
+        _lib.call("ssc_release", _lib.ctypes.byref(h),
+                  None if sd is None else _lib.ctypes.byref(sd), _lib.ptr(prog_w.contiguous()),
+                  _lib.ptr(rel_idx), _lib.ptr(t_dev), _lib.ptr(step_dev),
+                  _lib.ptr(g_msat.contiguous()), n, int(seed_num), kvp,
                   *[_lib.ptr(o) for o in outs])
         return outs  # pos_close (lead), pos_far (trail), vel_close, vel_far
 
@@ -398,12 +415,15 @@ class Potential:
         ts_t = _lib.f64(ts).reshape(-1)
         ws = self.integrate_orbit(w0=prog_w0, ts=ts_t, solver=solver, rtol=rtol, atol=atol,
                                   dtmin=dtmin, dtmax=dtmax, max_steps=max_steps, dt0=dt0).ys
-        return self._release_along(ws, len(ts_t), Msat, seed_num, kval_arr)
+        return self._release_along(ws, ts_t.cpu().numpy(), Msat, seed_num, kval_arr)
 
-    def _release_along(self, ws, nts, Msat, seed_num, kval_arr):
+    def _release_along(self, ws, ts, Msat, seed_num, kval_arr):
+        """Release at every row of the progenitor orbit ws (T+1, 6) saved at times ts (T+1,)."""
+        ts = np.asarray(_as_float(ts), dtype=np.float64).reshape(-1)
+        nts = ts.size
         gms = self._G * _lib.f64(np.broadcast_to(np.asarray(_as_float(Msat), dtype=np.float64), (nts,)).copy())
         idx = torch.arange(nts, dtype=torch.int32, device=ws.device)
-        return self._release(ws.contiguous(), idx, gms, seed_num, kval_arr)
+        return self._release(ws.contiguous(), idx, gms, seed_num, kval_arr, t_rel=ts)
 
     def gen_stream_vmapped(self, ts=None, prog_w0=None, Msat=None, seed_num=None, solver=None,
                            kval_arr=1.0, rtol=1e-7, atol=1e-7, dtmin=0.3, dtmax=None,

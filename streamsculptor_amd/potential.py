@@ -170,6 +170,42 @@ class SubhaloLinePotentialCustom_fromFunc(Potential):
         return _per_source(self, xyz, t, "phi")
 
 
+class SubhaloLinePotential_Custom(Potential):
+    """Straight-line subhalos sharing one profile instance (reference
+    streamsculptor/potential.py:908-956): ``pot`` is a ``PlummerPotential`` or
+    ``HernquistPotential`` (soft = 0) with a single (m, r_s) -- the reference's lax.cond needs
+    the profile's value to have the shape (1,) of its inactive branch -- and subhalo j is that
+    profile at x - (x0_j + v_j (t - t0_j)) while |t - t0_j| < t_window_j.  Other profiles raise
+    NotImplementedError (no kernel form)."""
+
+    def __init__(self, pot, subhalo_x0, subhalo_v, subhalo_t0, t_window, units=None):
+        super().__init__(units, {"pot": pot, "subhalo_x0": subhalo_x0, "subhalo_v": subhalo_v,
+                                 "subhalo_t0": subhalo_t0, "t_window": t_window})
+        self._kind = _profile_kind(type(pot))
+        if self._kind == _lib.SRC_LINE_HERNQUIST and np.any(_arr(getattr(pot, "soft", 0.0)) != 0):
+            raise NotImplementedError("softened Hernquist subhalos have no kernel form")
+
+    def _kernel_components(self):
+        return []
+
+    def _kernel_sources(self):
+        m = np.atleast_1d(_arr(self.pot.m)).reshape(-1)
+        r_s = np.atleast_1d(_arr(self.pot.r_s)).reshape(-1)
+        if m.size != 1 or r_s.size != 1:
+            raise ValueError("SubhaloLinePotential_Custom takes a profile with one mass and one "
+                             "scale radius, shared by every subhalo")
+        n = _arr(self.subhalo_x0).reshape(-1, 3).shape[0]
+        return SourceSet(self._kind, gm=np.full(n, self.pot._G * m[0]), rs=np.full(n, r_s[0]),
+                         pos=_arr(self.subhalo_x0).reshape(n, 3),
+                         vel=_arr(self.subhalo_v).reshape(n, 3),
+                         t0=np.broadcast_to(np.atleast_1d(_arr(self.subhalo_t0)), (n,)).copy(),
+                         t_window=np.broadcast_to(np.atleast_1d(_arr(self.t_window)), (n,)).copy())
+
+    def potential_per_SH(self, xyz, t):
+        """Per-subhalo potential at xyz (reference streamsculptor/potential.py:939-956)."""
+        return _per_source(self, xyz, t, "phi")
+
+
 class SubhaloLinePotential_dRadius(Potential):
     """d Phi / d a of the straight-line Plummer subhalos of a SubhaloLinePotential (reference
     streamsculptor/potential.py:852-904): the structural potential whose gradient drives the
@@ -313,7 +349,8 @@ class _SourceOnly(Potential):
 
 __all__ = ["MiyamotoNagaiDisk", "NFWPotential", "PlummerPotential", "HernquistPotential",
            "Potential_Combine", "GalaMilkyWayPotential", "CustomPotential",
-           "SubhaloLinePotential", "SubhaloLinePotentialCustom_fromFunc",
+           "SubhaloLinePotential", "SubhaloLinePotential_Custom",
+           "SubhaloLinePotentialCustom_fromFunc",
            "SubhaloLinePotential_dRadius", "SubhaloLinePotentialCustom_dRadius_fromFunc",
            "PlummerClumps",
            "LiveSubhaloPotential", "usys"]

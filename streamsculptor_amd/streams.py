@@ -31,7 +31,7 @@ def stream_ics(prog_pot, release_pot, ts, prog_w0, Msat, seed_num, solver=None, 
         ws = prog_pot.integrate_orbit(w0=prog_w0, ts=ts_np, solver=solver, dt0=dt0,
                                       **adaptive_kw).ys
         grid = None
-    ics = release_pot._release_along(ws, len(ts_np), Msat, seed_num, kval_arr)
+    ics = release_pot._release_along(ws, ts_np, Msat, seed_num, kval_arr)
     return ics, grid, ts_np
 
 
@@ -101,7 +101,7 @@ def spray_particles(prog_pot, release_pot, ts, prog_w0, Msat, seed_num, per_step
     # table[s, arm, i, 6]
     table = torch.empty(nseed, 2, T + 1, 6, dtype=torch.float64, device=dev)
     for s in range(nseed):
-        pc, pf, vc, vf = release_pot._release_along(ws, T + 1, Msat, seed_num + s, kval_arr)
+        pc, pf, vc, vf = release_pot._release_along(ws, ts_np, Msat, seed_num + s, kval_arr)
         table[s, 0, :, :3], table[s, 0, :, 3:] = pc, vc
         table[s, 1, :, :3], table[s, 1, :, 3:] = pf, vf
     lay = np.array([even, odd], dtype=np.int64)            # (parity, per_step, [seed, arm])

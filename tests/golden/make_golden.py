@@ -60,6 +60,34 @@ REFERENCE_KATS = {
                   -0.01770436],
         "note": "Dopri8 at rtol = atol = 1e-7, dtmin = 0.3; about 1e-5 from converged",
         "source": "streamsculptor/examples/StreamSubhaloExample.ipynb:35"},
+    "spray_patch_w_impact": {
+        "potential": "GalaMilkyWayPotential (disk, bulge, nucleus, halo)",
+        "prog_w0": [-7.23164146, -7.96692572, -10.81840286, 0.19182623, -0.20351324,
+                    -0.01770436],
+        "ts": "linspace(-3500, 0, 5000)", "Msat": "linspace(1e4, 0, 5000)", "seed": 583,
+        "kvals": [2.0, 0.3, 0.0, 0.0, 0.5, 0.5, 0.5, 0.5],
+        "stream": "gen_stream_scan, Dopri5 rtol = atol = 1e-7, dtmin = 0.3",
+        "patch": "lead and trail rows with -5 < y < -4, mean row",
+        "t_impact": -1000.0,
+        "value": [-18.84511332, 4.23873108, -17.32878089, -0.04416233, -0.15074649,
+                  -0.07956093],
+        "note": "prog_w0 is the printed initial condition (StreamSubhaloExample.ipynb:35); the "
+                "kvals are the vintage kept in the comment at streamsculptor/main.py:214, with "
+                "which the notebook was run (the current defaults miss by 1.5e-3); the mean "
+                "row is integrated back 1000 Myr with Dopri8 at rtol = atol = 1e-7",
+        "source": "streamsculptor/examples/StreamSubhaloExample.ipynb:80,89-90 (stream), "
+                  ":156,186-190 (patch, back integration), :179 (printed value)"},
+    "subhalo_line_potentials": {
+        "xyz": [1.0, 2.0, 3.0], "t": -850.0,
+        "subhalo": "m = 1e7, r_s = 0.2, x0 = w_impact[:3], v = w_impact[3:] + (0.02, 0, 0.02), "
+                   "t0 = -1000, t_window = 250 (w_impact = spray_patch_w_impact.value)",
+        "plummer": -1.053815931966235e-06,
+        "hernquist": -1.048913077256255e-06,
+        "note": "plummer: SubhaloLinePotential; hernquist: SubhaloLinePotential_Custom with "
+                "HernquistPotential(m, r_s); the subhalo row follows from the 8-digit printed "
+                "w_impact, so both values hold to about 1e-7 relative",
+        "source": "streamsculptor/examples/StreamSubhaloExample.ipynb:193-204,237 (subhalo), "
+                  ":228-229 (printed values)"},
     "threefry2x32_20": {
         "vectors": [[[0, 0], [0, 0], [0x6B200159, 0x99BA4EFE]],
                     [[0xFFFFFFFF, 0xFFFFFFFF], [0xFFFFFFFF, 0xFFFFFFFF],

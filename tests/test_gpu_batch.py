@@ -105,24 +105,34 @@ def test_gd1_batch_statistics_vs_oracle(oracle, ssc):
     st = st.cpu().numpy()
     assert st[..., 2].max() == 0
     h = oracle.host(W.gala_mw_components())
-    errs, flips, n = [], 0, 0
+    errs, same, n = [], [], 0
     for b in (0, 11, 23):
         src = oracle.Sources(oracle.SRC_LINE, gm=G * bt.m[b], rs=bt.a[b], pos=bt.subhalo_x0[b],
                              vel=bt.subhalo_v[b], t0=bt.subhalo_t0[b], t_window=bt.t_window[b])
         ic_o, _, _ = oracle.dopri5(h, W.GD1_TODAY, -W.GD1_AGE, t0=0.0, src=src)
         assert rel_err_rows(ic[b:b + 1], ic_o) < 1e-6
+        # released in the smooth base potential (reference streamhelpers.py:57-66)
         lo, tr, sto = oracle.stream_dopri5(h, ic[b].cpu().numpy(), ts, 1e4, 532, src=src,
-                                           rtol=1e-6, atol=1e-6, dtmin=0.05)
+                                           release_host=h, rtol=1e-6, atol=1e-6, dtmin=0.05)
         sto = sto.reshape(-1, 2, 3)
-        flips += int(np.sum(np.any(sto[:, :, :2] != st[b, :, :, :2], axis=2)))
+        same.append(np.all(sto[:, :, :2] == st[b, :, :, :2], axis=2).T.reshape(-1))
         n += sto.shape[0] * 2
         for a_, r_ in ((lead[b], lo), (trail[b], tr)):
             a_ = a_.cpu().numpy()
             errs.append(np.linalg.norm(a_ - r_, axis=1) / np.linalg.norm(r_, axis=1))
     errs = np.concatenate(errs)
-    print(f"[gd1] median {np.median(errs):.2e} p99.9 {np.quantile(errs, 0.999):.2e} "
-          f"max {errs.max():.2e}; {flips} of {n} orbits took another step sequence", flush=True)
-    assert np.median(errs) < 1e-9
-    assert np.quantile(errs, 0.999) < 1e-5
+    same = np.concatenate(same)
+    flips = int((~same).sum())
+    e_same = errs[same]
+    e_flip = errs[~same]
+    print(f"[gd1] same step sequence: {same.sum()} orbits, median {np.median(e_same):.2e} "
+          f"max {e_same.max():.2e}; another sequence: {flips} of {n} orbits, max "
+          f"{e_flip.max() if flips else 0.0:.2e}", flush=True)
+    # orbits on the oracle's step sequence: the adaptive parity bar
+    assert np.median(e_same) < 1e-9
+    assert e_same.max() <= 1e-6
+    # orbits whose error estimate fell within rounding of the acceptance threshold take another,
+    # equally valid sequence: few, and off by no more than the truncation error of a step that
+    # straddles the subhalo's window (~1e-3 at rtol 1e-6)
     assert flips <= max(3, n // 1000)
-    assert errs.max() < 1e-2
+    assert flips == 0 or e_flip.max() < 1e-2
