@@ -71,16 +71,20 @@ typedef struct ssc_host_potential {
  *   LINE_HERNQUIST: LINE geometry with Hernquist profiles, Phi = -G m / (r + rs) (reference
  *           SubhaloLinePotentialCustom_fromFunc with func=HernquistPotential,
  *           streamsculptor/potential.py:1161-1213).  Evaluation, leapfrog and Dopri5 only: the
- *           response and adjoint kernels differentiate Plummer sources.                      */
+ *           response and adjoint kernels differentiate Plummer sources.
+ *   TRACK_HERNQUIST: TRACK geometry with Hernquist profiles: a Hernquist sphere whose centre
+ *           follows a track sampled on the integration grid (reference ProgenitorPotential and
+ *           TimeDepTranslatingPotential, streamsculptor/potential.py:140-153,448-462, with a
+ *           Hernquist profile).  Evaluation, release and leapfrog only.                     */
 enum { SSC_SRC_NONE = 0, SSC_SRC_STATIC = 1, SSC_SRC_TRACK = 2, SSC_SRC_LINE = 3,
-       SSC_SRC_LINE_HERNQUIST = 4 };
+       SSC_SRC_LINE_HERNQUIST = 4, SSC_SRC_TRACK_HERNQUIST = 5 };
 
 #define SSC_MAX_SOURCES 1024
 
 typedef struct ssc_sources {
   int32_t kind;
   int32_t n;            /* number of sources, 0..SSC_MAX_SOURCES (ungrouped) */
-  int32_t n_track;      /* TRACK: rows in pos */
+  int32_t n_track;      /* TRACK, TRACK_HERNQUIST: rows in pos */
   int32_t group_size;   /* 0: every orbit sees all n sources.  > 0: the sources form
                            consecutive groups of group_size and orbit i sees only group
                            group[i] -- a batch of independent simulations in one call
@@ -88,7 +92,7 @@ typedef struct ssc_sources {
                            then exceed SSC_MAX_SOURCES) */
   const double* gm;     /* [n] G*m (device) */
   const double* rs;     /* [n] scale radius of the profile (device) */
-  const double* pos;    /* STATIC/LINE*: [n][3]; TRACK: [n_track][n][3] (device) */
+  const double* pos;    /* STATIC/LINE*: [n][3]; TRACK*: [n_track][n][3] (device) */
   const double* vel;    /* LINE*: [n][3] (device) */
   const double* t0;     /* LINE*: [n] (device) */
   const double* t_window; /* LINE*: [n] (device) */
@@ -154,6 +158,18 @@ int ssc_dopri5(const ssc_host_potential* host, const ssc_sources* src, const dou
                const double* t_start, int64_t n, double t0, double t1, const double* ts,
                int32_t nts, const ssc_dopri5_options* opt, double* w_out, double* ys,
                int32_t* stats, void* stream);
+
+/* ssc_dopri5 with a time interval and save times of its own for every orbit: orbit i runs from
+ * t_start[i] to t_end[i] (device, both required; forwards or backwards) and saves at
+ * ts[i][0..nts) (device, monotone in its direction of integration) into ys[n][nts][6] (NaN
+ * outside the orbit's interval).  Replaces integrate_orbit_batch_vmapped / _batch_scan with a
+ * per-orbit (N, M) ts array (reference streamsculptor/main.py:166-202: each orbit integrates
+ * from min(ts[i]) to max(ts[i]) unless t0/t1 are given).  opt->n_stops applies to every
+ * orbit; other semantics as ssc_dopri5. */
+int ssc_dopri5_orbits(const ssc_host_potential* host, const ssc_sources* src, const double* w0,
+                      const double* t_start, const double* t_end, int64_t n, const double* ts,
+                      int32_t nts, const ssc_dopri5_options* opt, double* w_out, double* ys,
+                      int32_t* stats, void* stream);
 
 /* ssc_dopri5 with dense output instead of save times (reference integrate_orbit(dense=True),
  * SaveAt(dense=True), streamsculptor/main.py:126,141, and gen_stream_vmapped_dense,

@@ -59,7 +59,7 @@ _FAST_PATH = os.path.join(_HERE, "libssc_oracle_fast.so")
 G = 4.498502151469553e-12  # kpc^3 / (Msun Myr^2)
 
 COMP_MN, COMP_NFW, COMP_HERNQUIST, COMP_PLUMMER = 1, 2, 3, 4
-SRC_NONE, SRC_STATIC, SRC_TRACK, SRC_LINE, SRC_LINE_HERNQUIST = 0, 1, 2, 3, 4
+SRC_NONE, SRC_STATIC, SRC_TRACK, SRC_LINE, SRC_LINE_HERNQUIST, SRC_TRACK_HERNQUIST = 0, 1, 2, 3, 4, 5
 MAX_COMPONENTS = 8
 
 
@@ -174,7 +174,7 @@ class Sources:
             rs = np.ascontiguousarray(rs, dtype=np.float64)
             pos = np.ascontiguousarray(pos, dtype=np.float64)
             s.n = gm.shape[0]
-            s.n_track = pos.shape[0] if kind == SRC_TRACK else 0
+            s.n_track = pos.shape[0] if kind in (SRC_TRACK, SRC_TRACK_HERNQUIST) else 0
             self.arrays.update(gm=gm, rs=rs, pos=pos)
             s.gm, s.rs, s.pos = gm.ctypes.data, rs.ctypes.data, pos.ctypes.data
             if kind in (SRC_LINE, SRC_LINE_HERNQUIST):

@@ -212,7 +212,7 @@ bool source_pos(const ssc_sources* s, int j, int step, double t, double out[3]) 
     for (int i = 0; i < 3; ++i) out[i] = s->pos[3 * j + i];
     return true;
   }
-  if (s->kind == SSC_SRC_TRACK) {
+  if (s->kind == SSC_SRC_TRACK || s->kind == SSC_SRC_TRACK_HERNQUIST) {
     const double* row = s->pos + ((int64_t)step * s->n + j) * 3;
     for (int i = 0; i < 3; ++i) out[i] = row[i];
     return true;
@@ -228,7 +228,9 @@ bool source_pos(const ssc_sources* s, int j, int step, double t, double out[3]) 
 
 // Direct N x M sum over Plummer spheres (reference SubhaloLinePotential.potential,
 // streamsculptor/potential.py:813-830) or, for LINE_HERNQUIST, Hernquist spheres (reference
-// SubhaloLinePotentialCustom_fromFunc with func=HernquistPotential, potential.py:1161-1190).
+// SubhaloLinePotentialCustom_fromFunc with func=HernquistPotential, potential.py:1161-1190);
+// TRACK_HERNQUIST: Hernquist spheres centred on a sampled track (reference ProgenitorPotential /
+// TimeDepTranslatingPotential with a Hernquist profile, potential.py:140-153,448-462).
 void sources_eval(const ssc_sources* s, const double x[3], int step, double t,
                   double* phi, double g[3], double H[9]) {
   if (!s || s->kind == SSC_SRC_NONE) return;
@@ -236,7 +238,7 @@ void sources_eval(const ssc_sources* s, const double x[3], int step, double t,
     double sp[3];
     if (!source_pos(s, j, step, t, sp)) continue;
     double d[3] = {x[0] - sp[0], x[1] - sp[1], x[2] - sp[2]};
-    if (s->kind == SSC_SRC_LINE_HERNQUIST) {
+    if (s->kind == SSC_SRC_LINE_HERNQUIST || s->kind == SSC_SRC_TRACK_HERNQUIST) {
       const double p[4] = {s->gm[j], s->rs[j], 0.0, 0.0};
       hernquist_eval<double>(p, d, phi, g, H);
     } else {
@@ -886,7 +888,8 @@ int orc_release(const ssc_host_potential* hp, const ssc_sources* src, const doub
   for (int c = 0; c < hp->n_components; ++c)
     for (int k = 0; k < 4; ++k) hostp[4 * c + k] = hp->c[c].p[k];
   const bool with_src = src && src->kind != SSC_SRC_NONE;
-  if (with_src && src->kind == SSC_SRC_TRACK && !steps) return -1;
+  if (with_src && (src->kind == SSC_SRC_TRACK || src->kind == SSC_SRC_TRACK_HERNQUIST) && !steps)
+    return -1;
   if (with_src && (src->kind == SSC_SRC_LINE || src->kind == SSC_SRC_LINE_HERNQUIST) && !ts)
     return -1;
 #pragma omp parallel for schedule(static)

@@ -16,7 +16,8 @@ LIB_PATH = os.path.join(HERE, "libssc_mi355x.so")
 G_USYS = 4.498502151469553e-12  # G in kpc^3 / (Msun Myr^2)
 
 COMP_MIYAMOTO_NAGAI, COMP_NFW, COMP_HERNQUIST, COMP_PLUMMER = 1, 2, 3, 4
-SRC_NONE, SRC_STATIC, SRC_TRACK, SRC_LINE, SRC_LINE_HERNQUIST = 0, 1, 2, 3, 4
+SRC_NONE, SRC_STATIC, SRC_TRACK, SRC_LINE, SRC_LINE_HERNQUIST, SRC_TRACK_HERNQUIST = 0, 1, 2, 3, 4, 5
+TRACK_KINDS = (SRC_TRACK, SRC_TRACK_HERNQUIST)
 MAX_COMPONENTS = 8
 MAX_SOURCES = 1024
 
@@ -54,6 +55,7 @@ SIGNATURES = {
     "ssc_leapfrog": [_vp, _vp, _vp, _vp, _i64, _d, _d, _i32, _vp, _vp, _i32, _vp, _vp],
     "ssc_dopri5": [_vp, _vp, _vp, _vp, _i64, _d, _d, _vp, _i32, _vp, _vp, _vp, _vp, _vp],
     "ssc_dopri5_dense": [_vp, _vp, _vp, _vp, _i64, _d, _d, _vp, _vp, _vp, _i32, _vp],
+    "ssc_dopri5_orbits": [_vp, _vp, _vp, _vp, _vp, _i64, _vp, _i32, _vp, _vp, _vp, _vp],
     "ssc_perturb_leapfrog": [_vp, _vp, _vp, _vp, _vp, _i64, _d, _d, _i32, _i32, _vp, _vp],
     "ssc_perturb_dopri5": [_vp, _vp, _vp, _vp, _vp, _i64, _d, _d, _vp, _i32, _i32, _vp, _vp, _vp,
                            _vp, _vp, _vp],

@@ -30,7 +30,7 @@ def _options(rtol, atol, dtmin, dtmax, max_steps, stops_t, dt0=None):
 
 def _sources(pot):
     h, src = pot._descriptor()
-    if src is not None and src.kind == _lib.SRC_TRACK:
+    if src is not None and src.kind in _lib.TRACK_KINDS:
         raise NotImplementedError("adaptive steps need source positions at arbitrary times; "
                                   "sources sampled on a fixed grid run with solver='leapfrog'")
     return h, src
@@ -65,6 +65,39 @@ def run(h, sd, w0, t1, t0=0.0, t_start=None, ts=None, rtol=1e-7, atol=1e-7, dtmi
         if failed:
             raise RuntimeError(f"Dopri5: {failed} of {n} orbits reached max_steps={max_steps} "
                                "before t1")
+    return out, ys, stats
+
+
+def run_orbits(h, sd, w0, t_start, t_end, ts, rtol=1e-7, atol=1e-7, dtmin=0.3, dtmax=None,
+               max_steps=10_000, jump_ts=None, dt0=None, throw=False):
+    """ssc_dopri5_orbits: orbit i runs from t_start[i] to t_end[i] and saves at its own row of
+    ts (n, m), monotone in its direction of integration.  Returns (w_final (n, 6),
+    ys (n, m, 6), stats (n, 3))."""
+    w0 = _lib.f64(w0).reshape(-1, 6).contiguous()
+    n = w0.shape[0]
+    dev = w0.device
+    tst = _lib.f64(t_start).reshape(-1).contiguous()
+    ten = _lib.f64(t_end).reshape(-1).contiguous()
+    tsd = _lib.f64(ts).reshape(n, -1).contiguous()
+    if tst.numel() != n or ten.numel() != n:
+        raise ValueError("t_start and t_end need one time per orbit")
+    nts = tsd.shape[1]
+    stops = None
+    if jump_ts is not None:
+        stops = _lib.f64(np.sort(np.asarray(jump_ts, dtype=np.float64).reshape(-1)))
+    out = torch.empty_like(w0)
+    ys = torch.empty(n, nts, 6, dtype=torch.float64, device=dev)
+    stats = torch.empty(n, 3, dtype=torch.int32, device=dev)
+    opt = _options(rtol, atol, dtmin, dtmax, max_steps, stops, dt0)
+    _lib.call("ssc_dopri5_orbits", _lib.ctypes.byref(h),
+              None if sd is None else _lib.ctypes.byref(sd), _lib.ptr(w0), _lib.ptr(tst),
+              _lib.ptr(ten), n, _lib.ptr(tsd) if nts else None, nts, _lib.ctypes.byref(opt),
+              _lib.ptr(out), _lib.ptr(ys) if nts else None, _lib.ptr(stats))
+    if throw:
+        failed = int((stats[:, 2] != 0).sum().item())
+        if failed:
+            raise RuntimeError(f"Dopri5: {failed} of {n} orbits reached max_steps={max_steps} "
+                               "before their end time")
     return out, ys, stats
 
 
@@ -200,6 +233,23 @@ def integrate_orbit(pot, w, tsn, t0v, t1v, rtol=1e-7, atol=1e-7, dtmin=0.3, dtma
         ys = ys[:, inv]
     ts_t = _lib.f64(tsn)
     return Solution(ts_t, ys if batched else ys[0])
+
+
+def integrate_orbits_own_ts(pot, w, ts2, t0s, t1s, rtol=1e-7, atol=1e-7, dtmin=0.3,
+                            dtmax=None, max_steps=10_000, jump_ts=None, throw=False, dt0=None):
+    """Solution for integrate_orbit_batch_vmapped with a per-orbit ts (N, M) and solver=Dopri5
+    (reference main.py:186-202): orbit i from t0s[i] to t1s[i], saved at ts2[i]."""
+    from .main import Solution
+    h, src = _sources(pot)
+    sd = src.desc() if src is not None else None
+    direction = np.where(t1s >= t0s, 1.0, -1.0)[:, None]
+    order = np.argsort(direction * ts2, axis=1, kind="stable")
+    srt = np.take_along_axis(ts2, order, axis=1)
+    _, ys, _ = run_orbits(h, sd, w, t0s, t1s, srt, rtol=rtol, atol=atol, dtmin=dtmin, dtmax=dtmax,
+                          max_steps=max_steps, jump_ts=jump_ts, dt0=dt0, throw=throw)
+    inv = torch.as_tensor(np.argsort(order, axis=1), device=ys.device)
+    ys = torch.take_along_dim(ys, inv[:, :, None], dim=1)
+    return Solution(_lib.f64(ts2), ys)
 
 
 def integrate_orbit_dense(pot, w, t0v, t1v, rtol=1e-7, atol=1e-7, dtmin=0.3, dtmax=None,

@@ -302,10 +302,12 @@ __global__ __launch_bounds__(64) void source_adjoint_kernel(
   }
 }
 
-// Sum the chunk partials: g_pos[row][j][3] += ..., g_gm[j] += ... (one thread per (row, j)).
+// Sum the chunk partials of each (row, source) into row_sums[row][j][4] (one thread per
+// (row, j)); TRACK sources own one position row per grid row, so their g_pos[k0 + row][j] is
+// added here directly.
 __global__ void source_adjoint_reduce(const double* __restrict__ part, int nchunk, int rows, int m,
                                       int k0, int track, double* __restrict__ g_pos,
-                                      double* __restrict__ gm_rows) {
+                                      double* __restrict__ row_sums) {
   const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= (int64_t)rows * m) return;
   const int r = (int)(e / m), j = (int)(e - (int64_t)r * m);
@@ -314,20 +316,31 @@ __global__ void source_adjoint_reduce(const double* __restrict__ part, int nchun
     const double* p = part + (((int64_t)c * rows + r) * m + j) * 4;
     a[0] += p[0]; a[1] += p[1]; a[2] += p[2]; a[3] += p[3];
   }
-  if (g_pos) {
-    double* o = g_pos + ((track ? (int64_t)(k0 + r) * m : 0) + j) * 3;
+  if (g_pos && track) {
+    double* o = g_pos + ((int64_t)(k0 + r) * m + j) * 3;
     o[0] += a[0]; o[1] += a[1]; o[2] += a[2];
   }
-  gm_rows[(int64_t)r * m + j] = a[3];
+  double* s = row_sums + ((int64_t)r * m + j) * 4;
+  s[0] = a[0]; s[1] = a[1]; s[2] = a[2]; s[3] = a[3];
 }
 
-__global__ void gm_rows_reduce(const double* __restrict__ gm_rows, int rows, int m,
-                               double* __restrict__ g_gm) {
+// Sum a segment's rows per source (one thread per source, the only writer of its outputs):
+// g_gm[j] always, and g_pos[j] for STATIC sources, whose one position every row shares.
+__global__ void rows_reduce(const double* __restrict__ row_sums, int rows, int m, int track,
+                            double* __restrict__ g_pos, double* __restrict__ g_gm) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= m) return;
-  double a = 0.0;
-  for (int r = 0; r < rows; ++r) a += gm_rows[(int64_t)r * m + j];
-  g_gm[j] += a;
+  double a[4] = {0, 0, 0, 0};
+  for (int r = 0; r < rows; ++r) {
+    const double* s = row_sums + ((int64_t)r * m + j) * 4;
+    a[0] += s[0]; a[1] += s[1]; a[2] += s[2]; a[3] += s[3];
+  }
+  if (g_gm) g_gm[j] += a[3];
+  if (g_pos && !track) {
+    g_pos[3 * j] += a[0];
+    g_pos[3 * j + 1] += a[1];
+    g_pos[3 * j + 2] += a[2];
+  }
 }
 
 
@@ -369,7 +382,7 @@ static int64_t adj_bytes(int64_t n, int nsteps, int E, int m, AdjWs* w, char* ba
   char* c4 = take(rows * n * 3 * 8);
   char* c5 = take(nblk * AJ_MAXC * 8);
   char* c6 = take(nchunk * rows * (m > 0 ? m : 1) * 4 * 8);
-  char* c7 = take(rows * (m > 0 ? m : 1) * 8);
+  char* c7 = take(rows * (m > 0 ? m : 1) * 4 * 8);
   char* c8 = take((nseg + 1) * 4);
   if (w) {
     w->ckpt = (double*)c1; w->lam = (double*)c2; w->X = (double*)c3; w->Wb = (double*)c4;
@@ -433,9 +446,8 @@ static int adjoint_backward(const ssc_host_potential& hp, const ssc_sources& src
       hipLaunchKernelGGL(source_adjoint_reduce, dim3((unsigned)((rows * m + 255) / 256)), dim3(256),
                          0, st, w.part, (int)nchunk, rows, m, k0, SK == SSC_SRC_TRACK ? 1 : 0,
                          g_pos, w.gm_rows);
-      if (g_gm)
-        hipLaunchKernelGGL(gm_rows_reduce, dim3((m + 255) / 256), dim3(256), 0, st, w.gm_rows, rows,
-                           m, g_gm);
+      hipLaunchKernelGGL(rows_reduce, dim3((m + 255) / 256), dim3(256), 0, st, w.gm_rows, rows, m,
+                         SK == SSC_SRC_TRACK ? 1 : 0, g_pos, g_gm);
     }
   }
   hipMemcpyAsync(lam_0, w.lam, n * 6 * 8, hipMemcpyDeviceToDevice, st);
@@ -509,7 +521,7 @@ int ssc_adjoint_backward(const ssc_host_potential* host, const ssc_sources* src,
 #undef SSC_AB
 #undef SSC_AB3
     default:
-      return fail(SSC_E_ARG, "ssc_adjoint_backward: LINE sources (Plummer or Hernquist) are not "
+      return fail(SSC_E_ARG, "ssc_adjoint_backward: LINE and Hernquist sources are not "
                              "differentiated");
   }
 }

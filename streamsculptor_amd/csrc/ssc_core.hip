@@ -59,13 +59,14 @@ int validate_sources(const ssc_sources* src, int32_t nsteps_needed, bool allow_g
     return fail(SSC_E_LIMIT, "sources: n out of range [0, " + std::to_string(SSC_MAX_SOURCES) + "]");
   }
   if (!src->gm || !src->rs || !src->pos) return fail(SSC_E_ARG, "sources: NULL gm/rs/pos");
-  if (src->kind == SSC_SRC_TRACK && src->n_track < nsteps_needed + 1)
+  if ((src->kind == SSC_SRC_TRACK || src->kind == SSC_SRC_TRACK_HERNQUIST) &&
+      src->n_track < nsteps_needed + 1)
     return fail(SSC_E_ARG, "sources: TRACK table has " + std::to_string(src->n_track) +
                                " rows, the integration needs " + std::to_string(nsteps_needed + 1));
   if ((src->kind == SSC_SRC_LINE || src->kind == SSC_SRC_LINE_HERNQUIST) &&
       (!src->vel || !src->t0 || !src->t_window))
     return fail(SSC_E_ARG, "sources: LINE needs vel, t0 and t_window");
-  if (src->kind < SSC_SRC_NONE || src->kind > SSC_SRC_LINE_HERNQUIST)
+  if (src->kind < SSC_SRC_NONE || src->kind > SSC_SRC_TRACK_HERNQUIST)
     return fail(SSC_E_ARG, "sources: unknown kind " + std::to_string(src->kind));
   return SSC_OK;
 }
@@ -94,7 +95,7 @@ __global__ __launch_bounds__(256) void eval_kernel(ssc_host_potential hp, ssc_so
       double sx, sy, sz;
       if (!source_position(src, j, step, t, sx, sy, sz)) continue;
       const double rs = src.rs[j];
-      if (src.kind == SSC_SRC_LINE_HERNQUIST)
+      if (src.kind == SSC_SRC_LINE_HERNQUIST || src.kind == SSC_SRC_TRACK_HERNQUIST)
         hernquist_full(x - sx, y - sy, z - sz, src.gm[j], rs, phi, g, H);
       else
         plummer_full(x - sx, y - sy, z - sz, src.gm[j], rs * rs, phi, g, H);
@@ -585,6 +586,7 @@ int ssc_leapfrog(const ssc_host_potential* host, const ssc_sources* src, const d
     case SSC_SRC_TRACK: return SSC_LF(SSC_SRC_TRACK);
     case SSC_SRC_LINE: return SSC_LF(SSC_SRC_LINE);
     case SSC_SRC_LINE_HERNQUIST: return SSC_LF(SSC_SRC_LINE_HERNQUIST);
+    case SSC_SRC_TRACK_HERNQUIST: return SSC_LF(SSC_SRC_TRACK_HERNQUIST);
     default: return fail(SSC_E_ARG, "ssc_leapfrog: bad source kind");
   }
 #undef SSC_LF

@@ -536,7 +536,7 @@ __device__ __forceinline__ bool source_position(const ssc_sources& s, int j, int
     sz = s.pos[3 * j + 2];
     return true;
   }
-  if (s.kind == SSC_SRC_TRACK) {
+  if (s.kind == SSC_SRC_TRACK || s.kind == SSC_SRC_TRACK_HERNQUIST) {
     const double* row = s.pos + ((int64_t)step * s.n + j) * 3;
     sx = row[0];
     sy = row[1];

@@ -122,10 +122,11 @@ __device__ __forceinline__ double clip_stops(const double* __restrict__ stops, i
 template <int SK, bool GR, int HS>
 __global__ __launch_bounds__(DP_BLOCK) void dopri5_kernel(
     ssc_host_potential hp, ssc_sources src, const double* __restrict__ w0,
-    const double* __restrict__ t_start, int64_t n, double t0, double t1,
-    const double* __restrict__ ts, int nts, const double* __restrict__ stops, Dp5Ctl ctl,
-    double* __restrict__ w_out, double* __restrict__ ys, int32_t* __restrict__ stats,
-    double* __restrict__ dense, int dense_cap) {
+    const double* __restrict__ t_start, const double* __restrict__ t_end, int64_t n, double t0,
+    double t1, const double* __restrict__ ts, int nts, int64_t ts_stride,
+    const double* __restrict__ stops, Dp5Ctl ctl, double* __restrict__ w_out,
+    double* __restrict__ ys, int32_t* __restrict__ stats, double* __restrict__ dense,
+    int dense_cap) {
   extern __shared__ __align__(16) char lds_raw[];
   SourceLDS<SK> lds;
   if (!GR) lds.init(src, lds_raw);  // the only barrier: every lane reaches it
@@ -146,9 +147,13 @@ __global__ __launch_bounds__(DP_BLOCK) void dopri5_kernel(
     return;
   }
 
+  // orbit i runs from t_start[i] to t_end[i] (or the shared t0 / t1) and saves at its own row
+  // of ts when ts_stride = nts (the shared times when ts_stride = 0)
   const double ta = t_start ? t_start[i] : t0;
-  const double dir = t1 >= ta ? 1.0 : -1.0;
-  const double tau0 = dir * ta, tau1 = dir * t1;
+  const double tb = t_end ? t_end[i] : t1;
+  const double dir = tb >= ta ? 1.0 : -1.0;
+  const double tau0 = dir * ta, tau1 = dir * tb;
+  const double* __restrict__ tsi = ts ? ts + ts_stride * i : nullptr;
   double y[6], f0[6];
 #pragma unroll
   for (int c = 0; c < 6; ++c) {
@@ -158,10 +163,10 @@ __global__ __launch_bounds__(DP_BLOCK) void dopri5_kernel(
   double* __restrict__ yrow = ys ? ys + (int64_t)6 * nts * i : nullptr;
   double* __restrict__ drow = dense ? dense + (int64_t)dense_cap * DP_REC * i : nullptr;
   int js = 0;
-  for (; js < nts && dir * ts[js] < tau0; ++js)
+  for (; js < nts && dir * tsi[js] < tau0; ++js)
 #pragma unroll
     for (int c = 0; c < 6; ++c) yrow[6 * js + c] = __builtin_nan("");
-  for (; js < nts && dir * ts[js] == tau0; ++js)
+  for (; js < nts && dir * tsi[js] == tau0; ++js)
 #pragma unroll
     for (int c = 0; c < 6; ++c) yrow[6 * js + c] = y[c];
 
@@ -260,7 +265,7 @@ __global__ __launch_bounds__(DP_BLOCK) void dopri5_kernel(
         }
       }
       if (keep) {
-        if (yrow && js < nts && dir * ts[js] <= tnext) {
+        if (yrow && js < nts && dir * tsi[js] <= tnext) {
           // dense output on (tprev, tnext]
           double F0[6], a4[6], b3[6], c2[6];
 #pragma unroll
@@ -274,8 +279,8 @@ __global__ __launch_bounds__(DP_BLOCK) void dopri5_kernel(
             b3[c] = D2 - 2.0 * D1 - 2.0 * a4[c];
             c2[c] = D1 - a4[c] - b3[c];
           }
-          for (; js < nts && dir * ts[js] <= tnext; ++js) {
-            const double tj = dir * ts[js];
+          for (; js < nts && dir * tsi[js] <= tnext; ++js) {
+            const double tj = dir * tsi[js];
             if (tj == tnext) {
 #pragma unroll
               for (int c = 0; c < 6; ++c) yrow[6 * js + c] = yn[c];
@@ -353,16 +358,17 @@ int set_lds_dp(const void* fn, size_t lds) {
 
 template <int SK, bool GR, int HS>
 int launch_dopri5(const ssc_host_potential& hp, const ssc_sources& src, const double* w0,
-                  const double* t_start, int64_t n, double t0, double t1, const double* ts,
-                  int nts, const double* stops, const Dp5Ctl& ctl, double* w_out, double* ys,
-                  int32_t* stats, double* dense, int dense_cap, hipStream_t st) {
+                  const double* t_start, const double* t_end, int64_t n, double t0, double t1,
+                  const double* ts, int nts, int64_t ts_stride, const double* stops,
+                  const Dp5Ctl& ctl, double* w_out, double* ys, int32_t* stats, double* dense,
+                  int dense_cap, hipStream_t st) {
   const size_t lds = GR ? 0 : SourceLDS<SK>::bytes(src);
   int rc = set_lds_dp((const void*)dopri5_kernel<SK, GR, HS>, lds);
   if (rc) return rc;
   const int64_t blocks = (n + DP_BLOCK - 1) / DP_BLOCK;
   hipLaunchKernelGGL((dopri5_kernel<SK, GR, HS>), dim3((unsigned)blocks), dim3(DP_BLOCK), lds, st, hp,
-                     src, w0, t_start, n, t0, t1, ts, nts, stops, ctl, w_out, ys, stats, dense,
-                     dense_cap);
+                     src, w0, t_start, t_end, n, t0, t1, ts, nts, ts_stride, stops, ctl, w_out,
+                     ys, stats, dense, dense_cap);
   return check_launch("ssc_dopri5");
 }
 
@@ -372,8 +378,9 @@ int launch_dopri5(const ssc_host_potential& hp, const ssc_sources& src, const do
 using namespace ssc;
 
 static int dopri5_entry(const ssc_host_potential* host, const ssc_sources* src, const double* w0,
-                        const double* t_start, int64_t n, double t0, double t1, const double* ts,
-                        int32_t nts, const ssc_dopri5_options* opt, double* w_out, double* ys,
+                        const double* t_start, const double* t_end, int64_t n, double t0,
+                        double t1, const double* ts, int32_t nts, bool per_orbit_ts,
+                        const ssc_dopri5_options* opt, double* w_out, double* ys,
                         int32_t* stats, double* dense, int32_t dense_cap, void* stream) {
   int rc = validate_host(host);
   if (rc) return rc;
@@ -401,7 +408,7 @@ static int dopri5_entry(const ssc_host_potential* host, const ssc_sources* src, 
     s = ssc_sources{};
   }
   if (s.n == 0) s.kind = SSC_SRC_NONE;
-  if (s.kind == SSC_SRC_TRACK)
+  if (s.kind == SSC_SRC_TRACK || s.kind == SSC_SRC_TRACK_HERNQUIST)
     return fail(SSC_E_ARG, "ssc_dopri5: TRACK sources are sampled on a fixed grid; adaptive steps "
                            "need STATIC or LINE sources");
   if (nts == 0) ys = nullptr;
@@ -410,8 +417,9 @@ static int dopri5_entry(const ssc_host_potential* host, const ssc_sources* src, 
   const bool grouped = s.kind != SSC_SRC_NONE && s.group_size > 0;
   const int hs = host_signature(*host);
 #define SSC_DP3(K, G, H)                                                                          \
-  launch_dopri5<K, G, H>(*host, s, w0, t_start, n, t0, t1, ts, nts, opt->stops, ctl, w_out, ys,   \
-                         stats, dense, dense_cap, st)
+  launch_dopri5<K, G, H>(*host, s, w0, t_start, t_end, n, t0, t1, ts, nts,                       \
+                         per_orbit_ts ? (int64_t)nts : 0, opt->stops, ctl, w_out, ys, stats,     \
+                         dense, dense_cap, st)
 #define SSC_DP(K, G)                                                                              \
   (hs == HS_MN_HQ_NFW ? SSC_DP3(K, G, HS_MN_HQ_NFW)                                              \
                       : (hs == HS_MN_HQ_HQ_NFW ? SSC_DP3(K, G, HS_MN_HQ_HQ_NFW)                  \
@@ -433,8 +441,19 @@ extern "C" int ssc_dopri5(const ssc_host_potential* host, const ssc_sources* src
                           double t1, const double* ts, int32_t nts,
                           const ssc_dopri5_options* opt, double* w_out, double* ys,
                           int32_t* stats, void* stream) {
-  return dopri5_entry(host, src, w0, t_start, n, t0, t1, ts, nts, opt, w_out, ys, stats, nullptr,
-                      0, stream);
+  return dopri5_entry(host, src, w0, t_start, nullptr, n, t0, t1, ts, nts, false, opt, w_out, ys,
+                      stats, nullptr, 0, stream);
+}
+
+extern "C" int ssc_dopri5_orbits(const ssc_host_potential* host, const ssc_sources* src,
+                                 const double* w0, const double* t_start, const double* t_end,
+                                 int64_t n, const double* ts, int32_t nts,
+                                 const ssc_dopri5_options* opt, double* w_out, double* ys,
+                                 int32_t* stats, void* stream) {
+  if (n > 0 && (!t_start || !t_end))
+    return fail(SSC_E_ARG, "ssc_dopri5_orbits: t_start and t_end are required");
+  return dopri5_entry(host, src, w0, t_start, t_end, n, 0.0, 0.0, ts, nts, true, opt, w_out, ys,
+                      stats, nullptr, 0, stream);
 }
 
 extern "C" int ssc_dopri5_dense(const ssc_host_potential* host, const ssc_sources* src,
@@ -442,6 +461,6 @@ extern "C" int ssc_dopri5_dense(const ssc_host_potential* host, const ssc_source
                                 double t1, const ssc_dopri5_options* opt, double* w_out,
                                 double* dense, int32_t dense_cap, int32_t* stats, void* stream) {
   if (!dense || !stats) return fail(SSC_E_ARG, "ssc_dopri5_dense: dense and stats are required");
-  return dopri5_entry(host, src, w0, t_start, n, t0, t1, nullptr, 0, opt, w_out, nullptr, stats,
-                      dense, dense_cap, stream);
+  return dopri5_entry(host, src, w0, t_start, nullptr, n, t0, t1, nullptr, 0, false, opt, w_out,
+                      nullptr, stats, dense, dense_cap, stream);
 }

@@ -623,6 +623,7 @@ extern "C" int ssc_perturb_dopri5(const ssc_host_potential* host, const ssc_sour
     case SSC_SRC_LINE_HERNQUIST:
       return with_radius ? SSC_PD(SSC_SRC_LINE_HERNQUIST, 1) : SSC_PD(SSC_SRC_LINE_HERNQUIST, 0);
     case SSC_SRC_TRACK:
+    case SSC_SRC_TRACK_HERNQUIST:
       return fail(SSC_E_ARG, "ssc_perturb_dopri5: TRACK sources are sampled on a fixed grid; "
                              "adaptive steps need STATIC or LINE sources");
     default: return fail(SSC_E_ARG, "ssc_perturb_dopri5: bad source kind");

@@ -4,6 +4,8 @@
 // PRNG (Salmon et al., SC'11) for the Gaussian offsets.
 #include <hip/hip_runtime.h>
 
+#include <string>
+
 #include "ssc_device.h"
 
 namespace ssc {
@@ -112,7 +114,8 @@ __global__ __launch_bounds__(256) void release_kernel(ssc_host_potential hp, ssc
   if (src.kind != SSC_SRC_NONE) {
     const int step = step_rel ? step_rel[i] : 0;
     const double t = t_rel ? t_rel[i] : 0.0;
-    if (src.kind == SSC_SRC_TRACK && (step < 0 || step >= src.n_track)) {
+    if ((src.kind == SSC_SRC_TRACK || src.kind == SSC_SRC_TRACK_HERNQUIST) &&
+        (step < 0 || step >= src.n_track)) {
 #pragma unroll
       for (int k = 0; k < 9; ++k) H[k] = __builtin_nan("");
     } else {
@@ -120,7 +123,7 @@ __global__ __launch_bounds__(256) void release_kernel(ssc_host_potential hp, ssc
         double sx, sy, sz;
         if (!source_position(src, j, step, t, sx, sy, sz)) continue;
         const double rs = src.rs[j];
-        if (src.kind == SSC_SRC_LINE_HERNQUIST)
+        if (src.kind == SSC_SRC_LINE_HERNQUIST || src.kind == SSC_SRC_TRACK_HERNQUIST)
           hernquist_full(x[0] - sx, x[1] - sy, x[2] - sz, src.gm[j], rs, phi, g, H);
         else
           plummer_full(x[0] - sx, x[1] - sy, x[2] - sz, src.gm[j], rs * rs, phi, g, H);
@@ -353,7 +356,8 @@ extern "C" int ssc_release(const ssc_host_potential* host, const ssc_sources* sr
     return fail(SSC_E_ARG, "ssc_release: NULL array argument");
   const ssc_sources none = {};
   const bool with_src = src && src->kind != SSC_SRC_NONE && src->n > 0;
-  if (with_src && src->kind == SSC_SRC_TRACK && !step_rel)
+  if (with_src && (src->kind == SSC_SRC_TRACK || src->kind == SSC_SRC_TRACK_HERNQUIST) &&
+      !step_rel)
     return fail(SSC_E_ARG, "ssc_release: TRACK sources need step_rel");
   if (with_src && (src->kind == SSC_SRC_LINE || src->kind == SSC_SRC_LINE_HERNQUIST) && !t_rel)
     return fail(SSC_E_ARG, "ssc_release: LINE sources need t_rel");
@@ -366,18 +370,15 @@ extern "C" int ssc_release(const ssc_host_potential* host, const ssc_sources* sr
   return check_launch("ssc_release");
 }
 
-extern "C" int ssc_release_vjp(const ssc_host_potential* host, const double* prog_w,
-                               const int32_t* rel_idx, const double* g_msat, int32_t nrel,
-                               int64_t seed, const double* kvals, const double* g_pos_close,
-                               const double* g_pos_far, const double* g_vel_close,
-                               const double* g_vel_far, double* g_prog, double* g_host,
-                               double* scratch, void* stream) {
-  int rc = validate_host(host);
-  if (rc) return rc;
-  if (nrel <= 0) return fail(SSC_E_ARG, "ssc_release_vjp: nrel must be positive");
-  if (!prog_w || !g_msat || !g_pos_close || !g_pos_far || !g_vel_close || !g_vel_far || !g_prog ||
-      !g_host || !scratch)
-    return fail(SSC_E_ARG, "ssc_release_vjp: NULL array argument");
+// The release VJP proper: per-release partials of the host parameters go to `scratch`
+// ([nrel][2 n_components] doubles, every entry written before it is read), then one block per
+// parameter sums them into g_host.
+static int release_vjp_launch(const char* name, const ssc_host_potential* host,
+                              const double* prog_w, const int32_t* rel_idx, const double* g_msat,
+                              int32_t nrel, int64_t seed, const double* kvals,
+                              const double* g_pos_close, const double* g_pos_far,
+                              const double* g_vel_close, const double* g_vel_far, double* g_prog,
+                              double* g_host, double* scratch, void* stream) {
   KVals kv;
   const double def[8] = {2.0, 0.3, 0.0, 0.0, 0.4, 0.4, 0.5, 0.5};
   for (int k = 0; k < 8; ++k) kv.k[k] = kvals ? kvals[k] : def[k];
@@ -388,5 +389,35 @@ extern "C" int ssc_release_vjp(const ssc_host_potential* host, const double* pro
   if (host->n_components > 0)
     hipLaunchKernelGGL(column_sum_kernel<256>, dim3(2 * host->n_components), dim3(256), 0, st,
                        scratch, (int64_t)nrel, 2 * host->n_components, g_host);
-  return check_launch("ssc_release_vjp");
+  return check_launch(name);
 }
+
+static int release_vjp_check(const char* name, const ssc_host_potential* host,
+                             const double* prog_w, const double* g_msat, int32_t nrel,
+                             const double* g_pos_close, const double* g_pos_far,
+                             const double* g_vel_close, const double* g_vel_far,
+                             const double* g_prog, const double* g_host) {
+  int rc = validate_host(host);
+  if (rc) return rc;
+  if (nrel <= 0) return fail(SSC_E_ARG, std::string(name) + ": nrel must be positive");
+  if (!prog_w || !g_msat || !g_pos_close || !g_pos_far || !g_vel_close || !g_vel_far || !g_prog ||
+      !g_host)
+    return fail(SSC_E_ARG, std::string(name) + ": NULL array argument");
+  return SSC_OK;
+}
+
+extern "C" int ssc_release_vjp(const ssc_host_potential* host, const double* prog_w,
+                               const int32_t* rel_idx, const double* g_msat, int32_t nrel,
+                               int64_t seed, const double* kvals, const double* g_pos_close,
+                               const double* g_pos_far, const double* g_vel_close,
+                               const double* g_vel_far, double* g_prog, double* g_host,
+                               double* scratch, void* stream) {
+  int rc = release_vjp_check("ssc_release_vjp", host, prog_w, g_msat, nrel, g_pos_close,
+                             g_pos_far, g_vel_close, g_vel_far, g_prog, g_host);
+  if (rc) return rc;
+  if (!scratch) return fail(SSC_E_ARG, "ssc_release_vjp: NULL scratch");
+  return release_vjp_launch("ssc_release_vjp", host, prog_w, rel_idx, g_msat, nrel, seed, kvals,
+                            g_pos_close, g_pos_far, g_vel_close, g_vel_far, g_prog, g_host,
+                            scratch, stream);
+}
+

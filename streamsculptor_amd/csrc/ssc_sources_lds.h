@@ -3,6 +3,7 @@
 //   TRACK:  rows kb..kb+chunk of the trajectory table, rec[r][j] = (x, y, z, r_s^2), kc[j], gmv[j]
 //   LINE:   (x0, v, G*m, r_s^2, t0, t_window) per source, position formed per step
 //   LINE_HERNQUIST: the same with r_s in place of r_s^2 and the Hernquist term
+//   TRACK_HERNQUIST: TRACK rows with r_s in place of r_s^2 and the Hernquist term
 // add_grad(j0, js) sums sources j0, j0 + js, ... so a wave can also split one orbit's sum;
 // get(k, t, j, ...) returns one source's position/G*m/r_s^2 and whether it is active.
 #pragma once
@@ -95,9 +96,11 @@ __host__ __device__ inline int track_chunk(int n) {
   return c < 1 ? 1 : (c > 16 ? 16 : c);
 }
 
-template <>
-struct SourceLDS<SSC_SRC_TRACK> {
-  SrcGeo* rec;  // [chunk + 1][n]: (x, y, z, r_s^2)
+// TRACK tables with Plummer (HQ = false: rec q = r_s^2) or Hernquist (HQ = true: q = r_s)
+// profiles.
+template <bool HQ>
+struct TrackLDS {
+  SrcGeo* rec;  // [chunk + 1][n]: (x, y, z, q)
   SrcK* kc;     // [n]: (-1.5 G m, 2.5 G m)
   double* gmv;  // [n]: G m
   int n;
@@ -132,7 +135,7 @@ struct SourceLDS<SSC_SRC_TRACK> {
       const int r = e / n, j = e - r * n;
       const double* p = s.pos + ((int64_t)(k + r) * n + j) * 3;
       const double rs = s.rs[j];
-      rec[r * n + j] = SrcGeo{p[0], p[1], p[2], rs * rs};
+      rec[r * n + j] = SrcGeo{p[0], p[1], p[2], HQ ? rs : rs * rs};
     }
     __syncthreads();
   }
@@ -149,8 +152,13 @@ struct SourceLDS<SSC_SRC_TRACK> {
       const SrcGeo g = row[j];
       const SrcK c = kc[j];
 #pragma unroll
-      for (int p = 0; p < P; ++p)
-        plummer_grad_k(x[p] - g.x, y[p] - g.y, z[p] - g.z, g.rs2, c.c1, c.c2, gx[p], gy[p], gz[p]);
+      for (int p = 0; p < P; ++p) {
+        if (HQ)
+          hernquist_grad(x[p] - g.x, y[p] - g.y, z[p] - g.z, gmv[j], g.rs2, gx[p], gy[p], gz[p]);
+        else
+          plummer_grad_k(x[p] - g.x, y[p] - g.y, z[p] - g.z, g.rs2, c.c1, c.c2, gx[p], gy[p],
+                         gz[p]);
+      }
     }
   }
   __device__ bool get(int k, double, int j, double& sx, double& sy, double& sz, double& gm,
@@ -160,6 +168,12 @@ struct SourceLDS<SSC_SRC_TRACK> {
     return true;
   }
 };
+
+template <>
+struct SourceLDS<SSC_SRC_TRACK> : TrackLDS<false> {};
+
+template <>
+struct SourceLDS<SSC_SRC_TRACK_HERNQUIST> : TrackLDS<true> {};
 
 struct alignas(16) LineRec {
   double x0, y0, z0, gm, vx, vy, vz, q, t0, tw;  // q: r_s^2 (Plummer) or r_s (Hernquist)

@@ -46,9 +46,16 @@ def save_derivs_batch(path, index, pert_out, r_s_root, impact_frame_params=None)
     return fname + ".npy"
 
 
-def load_derivs_batch(path, index):
-    """Read a batch written by ``save_derivs_batch`` (a file this code wrote)."""
-    return np.load(os.path.join(path, f"{int(index)}.npy"), allow_pickle=True).item()
+def load_derivs_batch(path, index, trusted=False):
+    """Read a batch written by ``save_derivs_batch``.  The reference's layout is a pickled dict,
+    and unpickling runs code stored in the file: pass ``trusted=True`` only for files this code
+    (or another trusted writer) produced; otherwise the call raises."""
+    fname = os.path.join(path, f"{int(index)}.npy")
+    if not trusted:
+        raise ValueError(f"{fname} holds a pickled dict (the reference's batch layout); loading "
+                         "it runs code stored in the file -- pass trusted=True for a file you "
+                         "wrote yourself")
+    return np.load(fname, allow_pickle=True).item()
 
 
 def generate_derivs_batches(potential_base, BaseStreamModel, batches, path, func=HernquistPotential,

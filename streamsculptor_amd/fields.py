@@ -70,14 +70,32 @@ def _times(ts, t0, t1, backwards_int):
 def integrate_field(w0=None, ts=None, dense=False, solver=None, field=None, args=None, rtol=1e-7,
                     atol=1e-7, dtmin=0.05, dtmax=None, max_steps=1_000, jump_ts=None,
                     backwards_int=False, t0=0.0, t1=0.0, step_ts=None, dt0=None):
-    """Integrate ``w0`` along ``field`` and save at ``ts`` (reference fields.py:34-99)."""
+    """Integrate ``w0`` along ``field`` and save at ``ts`` (reference fields.py:34-99).
+
+    ``step_ts`` (times at which the controller must end a step, reference fields.py:54) are step
+    ends of the adaptive orbit solve, like ``jump_ts``; the adaptive perturbation solve takes
+    neither, and the fixed-step solver ends a step at every grid time, so there step_ts must
+    lie on the grid.  ``args`` is what the reference hands to ``field.term``; the fields with a
+    kernel form take no arguments."""
     if dense:
         raise NotImplementedError("dense=True is not available")
     tsn, t0v, t1v = _times(ts, t0, t1, backwards_int)
+    method = _solvers.resolve(solver)
+    if step_ts is not None and method == "leapfrog":
+        from ._grid import leapfrog_grid
+        leapfrog_grid(np.sort(np.asarray(_as_float(step_ts), dtype=np.float64).reshape(-1))
+                      [::1 if t1v >= t0v else -1], t0v, t1v, dt0)  # raises when off the grid
+        step_ts = None
     if isinstance(field, hamiltonian_field):
+        stops = jump_ts
+        if step_ts is not None:
+            stops = np.concatenate([np.asarray(_as_float(x), dtype=np.float64).reshape(-1)
+                                    for x in (jump_ts, step_ts) if x is not None])
         return field.pot.integrate_orbit(w0=w0, ts=tsn, solver=solver, rtol=rtol, atol=atol,
                                          dtmin=dtmin, dtmax=dtmax, max_steps=max_steps, t0=t0v,
-                                         t1=t1v, jump_ts=jump_ts, dt0=dt0)
+                                         t1=t1v, jump_ts=stops, dt0=dt0)
+    if step_ts is not None:
+        raise NotImplementedError("step_ts is not supported by the adaptive perturbation solve")
     if isinstance(field, MassRadiusPerturbation_OTF):
         return _integrate_perturbation(field.pertgen, w0, tsn, t0v, t1v, solver, dt0,
                                        dict(rtol=rtol, atol=atol, dtmin=dtmin, dtmax=dtmax,

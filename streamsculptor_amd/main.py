@@ -93,7 +93,7 @@ class SourceSet:
         s.n = t["gm"].numel()
         if s.n > _lib.MAX_SOURCES:
             raise ValueError(f"at most {_lib.MAX_SOURCES} sources per potential")
-        s.n_track = t["pos"].shape[0] if self.kind == _lib.SRC_TRACK else 0
+        s.n_track = t["pos"].shape[0] if self.kind in _lib.TRACK_KINDS else 0
         for k in ("gm", "rs", "pos", "vel", "t0", "t_window"):
             setattr(s, k, None if t[k] is None else t[k].data_ptr())
         return s
@@ -139,9 +139,21 @@ class SourceSet:
         return SourceSet(self.kind, gm=t["gm"], rs=t["rs"], pos=t["pos"][idx].contiguous(),
                          grid=(float(t0), float(dt)))
 
+    def rows_for(self, times):
+        """(source set, table row of each time) for evaluations at several times: TRACK tables
+        give their rows, other kinds (None)."""
+        if self.kind not in _lib.TRACK_KINDS:
+            return self, None
+        return self, np.array([self.step_of(t) for t in np.asarray(times).reshape(-1)],
+                              dtype=np.int32)
+
+    def at_time(self, t):
+        """(source set, table row) for an evaluation at time t."""
+        return self, self.step_of(t)
+
     def step_of(self, t):
         """Grid row of time t for TRACK sources."""
-        if self.kind != _lib.SRC_TRACK:
+        if self.kind not in _lib.TRACK_KINDS:
             return 0
         g0, gdt = self.grid
         kf = (float(t) - g0) / gdt
@@ -149,6 +161,44 @@ class SourceSet:
         if abs(kf - k) > 1e-9 * max(1.0, abs(kf)) or k < 0 or k >= self.tensors()["pos"].shape[0]:
             raise ValueError(f"time {t} is not a row of the source track (t0={g0}, dt={gdt})")
         return k
+
+
+class SampledTrack(SourceSet):
+    """TRACK sources (Plummer or Hernquist profiles) whose positions come from a function of
+    time, ``centre(times) -> (len(times), n, 3)``: the table is sampled on the integration grid
+    when the grid is known (``on_grid``), or at the times of an evaluation."""
+
+    def __init__(self, kind, gm, rs, centre):
+        super().__init__(kind, gm=gm, rs=rs, pos=None)
+        self.centre = centre
+
+    def _table(self, times):
+        t = self.tensors()
+        pos = _lib.f64(self.centre(np.asarray(times, dtype=np.float64).reshape(-1)))
+        return SourceSet(self.kind, gm=t["gm"], rs=t["rs"], pos=pos.reshape(len(times), -1, 3))
+
+    def tensors(self):
+        if self._dev is None:
+            self._dev = {k: (None if v is None else _lib.f64(v)) for k, v in self.host.items()}
+        return self._dev
+
+    def desc(self):
+        raise RuntimeError("a sampled track has no table until it is sampled (on_grid/at_time)")
+
+    def on_grid(self, t0, dt, nsteps):
+        src = self._table(t0 + dt * np.arange(nsteps + 1))
+        src.grid = (float(t0), float(dt))
+        return src
+
+    def at_time(self, t):
+        return self._table([float(_as_float(t))]), 0
+
+    def rows_for(self, times):
+        times = np.asarray(times, dtype=np.float64).reshape(-1)
+        return self._table(times), np.arange(times.size, dtype=np.int32)
+
+    def step_of(self, t):
+        raise RuntimeError("a sampled track is evaluated through at_time / rows_for")
 
 
 def _merge_sources(sets):
@@ -220,8 +270,10 @@ class Potential:
         phi = torch.empty(n, dtype=torch.float64, device=dev) if want_phi else None
         grad = torch.empty(n, 3, dtype=torch.float64, device=dev) if want_grad else None
         hess = torch.empty(n, 3, 3, dtype=torch.float64, device=dev) if want_hess else None
+        step = 0
+        if src is not None:
+            src, step = src.at_time(t)
         sd = src.desc() if src is not None else None
-        step = src.step_of(t) if src is not None else 0
         _lib.call("ssc_eval", _lib.ctypes.byref(h), None if sd is None else _lib.ctypes.byref(sd),
                   _lib.ptr(x), n, float(_as_float(t)), step, _lib.ptr(phi), _lib.ptr(grad),
                   _lib.ptr(hess))
@@ -302,7 +354,7 @@ class Potential:
         si = None if save_idx is None else _lib.i32(save_idx)
         nsave = 0 if si is None else si.numel()
         ys = torch.empty(nsave, n, 6, dtype=torch.float64, device=dev) if nsave else None
-        if src is not None and src.kind == _lib.SRC_TRACK:
+        if src is not None and src.kind in _lib.TRACK_KINDS:
             src = src.on_grid(t0, dt, nsteps)
         sd = src.desc() if src is not None else None
         _lib.call("ssc_leapfrog", _lib.ctypes.byref(h),
@@ -350,14 +402,62 @@ class Potential:
     def integrate_orbit_batch_vmapped(self, w0=None, ts=None, dense=False, solver=None, rtol=1e-7,
                                       atol=1e-7, dtmin=0.3, dtmax=None, max_steps=10_000, t0=None,
                                       t1=None, steps=False, jump_ts=None, dt0=None):
-        """Batch of orbits (reference main.py:186-202); ts shared (1-D)."""
-        ts_a = np.asarray(_as_float(ts))
-        if ts_a.ndim != 1:
-            raise NotImplementedError("per-orbit ts arrays are not supported; pass a shared 1-D ts")
-        return self.integrate_orbit(w0=_lib.f64(w0).reshape(-1, 6), ts=ts, dense=dense,
-                                    solver=solver, rtol=rtol, atol=atol, dtmin=dtmin, dtmax=dtmax,
-                                    max_steps=max_steps, t0=t0, t1=t1, steps=steps,
-                                    jump_ts=jump_ts, dt0=dt0)
+        """Batch of orbits (reference main.py:186-202).  ts is shared (1-D) or per orbit
+        (N, M): orbit i then runs from min(ts[i]) to max(ts[i]) -- or from t0 to t1 when given --
+        and is saved at ts[i]; ys has shape (N, M, 6)."""
+        ts_a = np.asarray(_as_float(ts), dtype=np.float64)
+        if ts_a.ndim == 1:
+            return self.integrate_orbit(w0=_lib.f64(w0).reshape(-1, 6), ts=ts, dense=dense,
+                                        solver=solver, rtol=rtol, atol=atol, dtmin=dtmin,
+                                        dtmax=dtmax, max_steps=max_steps, t0=t0, t1=t1,
+                                        steps=steps, jump_ts=jump_ts, dt0=dt0)
+        w = _lib.f64(w0).reshape(-1, 6)
+        if ts_a.ndim != 2 or ts_a.shape[0] != w.shape[0]:
+            raise ValueError("ts must be 1-D or have one row per orbit, shape (N, M)")
+        if dense or steps:
+            raise NotImplementedError("dense=True and steps=True take a shared 1-D ts")
+        n = w.shape[0]
+        t0s = ts_a.min(1) if t0 is None else np.full(n, float(_as_float(t0)))
+        t1s = ts_a.max(1) if t1 is None else np.full(n, float(_as_float(t1)))
+        if _solvers.resolve(solver) == "leapfrog":
+            return self._leapfrog_own_ts(w, ts_a, t0s, t1s, dt0)
+        from . import adaptive
+        return adaptive.integrate_orbits_own_ts(self, w, ts_a, t0s, t1s, rtol=rtol, atol=atol,
+                                                dtmin=dtmin, dtmax=dtmax, max_steps=max_steps,
+                                                jump_ts=jump_ts, dt0=dt0)
+
+    def _leapfrog_own_ts(self, w, ts2, t0s, t1s, dt0):
+        """Fixed-step orbits with their own intervals and save times on one grid: every orbit
+        starts at its own grid row and the kernel records the union of all save rows, from
+        which each orbit's rows are gathered (rows after an orbit's end time are not used)."""
+        from ._grid import _round_checked
+        if dt0 is None:
+            raise ValueError("the leapfrog solver needs a step size: pass dt0=")
+        nz = np.sign(t1s - t0s)
+        nz = nz[nz != 0]
+        if nz.size and np.any(nz != nz[0]):
+            raise NotImplementedError("per-orbit intervals must all run in one direction")
+        d = float(nz[0]) if nz.size else 1.0
+        dt = abs(float(dt0)) * d
+        tg0 = float((d * t0s).min() * d)
+        k_start = _round_checked((t0s - tg0) / dt, "orbit start times")
+        k_end = _round_checked((t1s - tg0) / dt, "orbit end times")
+        k_ts = _round_checked((ts2 - tg0) / dt, "save times ts")
+        if np.any(k_ts < k_start[:, None]) or np.any(k_ts > k_end[:, None]):
+            raise ValueError("save times must lie within each orbit's [t0, t1]")
+        nsteps = int(k_end.max())
+        rows = np.unique(k_ts)
+        order = np.argsort(k_start, kind="stable")  # the kernel walks blocks from their earliest start
+        _, ys = self._leapfrog(w[torch.as_tensor(order, device=w.device)], tg0, dt, nsteps,
+                               start=k_start[order].astype(np.int32),
+                               save_idx=rows.astype(np.int32))
+        inv = np.empty_like(order)
+        inv[order] = np.arange(order.size)
+        pos = np.searchsorted(rows, k_ts)                      # (N, M) rows of ys
+        dev = ys.device
+        out = ys[torch.as_tensor(pos, device=dev),
+                 torch.as_tensor(inv, device=dev)[:, None]]    # (N, M, 6)
+        return Solution(_lib.f64(ts2), out)
 
     integrate_orbit_batch_scan = integrate_orbit_batch_vmapped
 
@@ -395,8 +495,9 @@ class Potential:
                 raise ValueError("the release in a potential with point sources needs the "
                                  "release times")
             tr = np.asarray(_as_float(t_rel), dtype=np.float64).reshape(-1)
-            if src.kind == _lib.SRC_TRACK:
-                step_dev = _lib.i32(np.array([src.step_of(t) for t in tr], dtype=np.int32))
+            src, steps = src.rows_for(tr)
+            if steps is not None:
+                step_dev = _lib.i32(steps)
             t_dev = _lib.f64(tr)
             sd = src.desc()
         outs = [torch.empty(n, 3, dtype=torch.float64, device=dev) for _ in range(4)]

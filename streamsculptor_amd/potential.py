@@ -10,7 +10,7 @@ import numpy as np
 import torch
 
 from . import _lib
-from .main import Potential, SourceSet, usys, _as_float
+from .main import Potential, SampledTrack, SourceSet, usys, _as_float
 
 
 def _arr(x):
@@ -314,6 +314,91 @@ class LiveSubhaloPotential(Potential):
         return _per_source(self, xyz, t, "phi")
 
 
+def _track_profile(cls):
+    """Source kind of a profile centred on a moving track."""
+    if cls is PlummerPotential:
+        return _lib.SRC_TRACK
+    if cls is HernquistPotential:
+        return _lib.SRC_TRACK_HERNQUIST
+    raise NotImplementedError(
+        f"a {getattr(cls, '__name__', cls)!r} centred on a moving track has no kernel form; "
+        "use PlummerPotential or HernquistPotential")
+
+
+def _positions_of(track, times):
+    """Positions (len(times), 3) of a centre given as an interpolant with ``evaluate(t)`` (a
+    Dopri5 DenseSolution, or any object returning rows (..., >= 3) for an array of times) or as
+    a function of time (vectorised over times, or called once per time)."""
+    def _np(v):
+        return v.detach().cpu().numpy() if isinstance(v, torch.Tensor) else np.asarray(v)
+
+    times = np.asarray(times, dtype=np.float64).reshape(-1)
+    if hasattr(track, "evaluate"):
+        v = _np(track.evaluate(_lib.f64(times)))
+        return np.asarray(v, dtype=np.float64).reshape(times.size, -1)[:, :3]
+    try:
+        v = np.asarray(_np(track(times)), dtype=np.float64)
+        if v.shape == (times.size, 3):
+            return v
+    except Exception:  # noqa: BLE001 -- a scalar-only function: fall back to one call per time
+        pass
+    return np.stack([np.asarray(_np(track(float(t))), dtype=np.float64).reshape(-1)[:3]
+                     for t in times])
+
+
+class ProgenitorPotential(Potential):
+    """A Plummer or Hernquist sphere (``prog_pot`` = PlummerPotential or HernquistPotential,
+    with m and r_s) centred on a moving track: Phi(x - c(t)), c(t) = interp_func.evaluate(t)[:3]
+    (reference streamsculptor/potential.py:140-153, where interp_func is a dense diffrax
+    solution of the progenitor orbit; here a DenseSolution of ``integrate_orbit(dense=True)``,
+    or any object with evaluate(t)).  The centre is sampled on the integration grid (fixed-step
+    leapfrog, evaluation, release); adaptive solvers raise."""
+
+    def __init__(self, m, r_s, interp_func, prog_pot, units=None):
+        super().__init__(units, {"m": m, "r_s": r_s, "interp_func": interp_func,
+                                 "prog_pot": prog_pot})
+        self._kind = _track_profile(prog_pot)
+
+    def _kernel_components(self):
+        return []
+
+    def _kernel_sources(self):
+        gm = np.atleast_1d(self._G * _arr(self.m)).reshape(-1)[:1]
+        rs = np.atleast_1d(_arr(self.r_s)).reshape(-1)[:1]
+        interp = self.interp_func
+        return SampledTrack(self._kind, gm, rs,
+                            lambda times: _positions_of(interp, times)[:, None, :])
+
+
+class TimeDepTranslatingPotential(Potential):
+    """A potential translating with a track: Phi_pot(x - c(t)) with c = center_spl(t)
+    (reference streamsculptor/potential.py:448-462).  ``pot`` is a PlummerPotential or
+    HernquistPotential, or a Potential_Combine of spheres of one of these profiles; other
+    profiles have no kernel form and raise.  The centre is sampled on the integration grid
+    (fixed-step leapfrog, evaluation, release); adaptive solvers raise."""
+
+    def __init__(self, pot, center_spl, units=None):
+        super().__init__(units, {"pot": pot, "center_spl": center_spl})
+        parts = pot.potential_list if isinstance(pot, Potential_Combine) else [pot]
+        kinds = {_track_profile(type(p)) for p in parts}
+        if len(kinds) != 1:
+            raise NotImplementedError("the translating spheres must share one profile")
+        if any(np.any(_arr(getattr(p, "soft", 0.0)) != 0) for p in parts):
+            raise NotImplementedError("softened Hernquist spheres have no kernel form")
+        self._kind = kinds.pop()
+        self._parts = parts
+
+    def _kernel_components(self):
+        return []
+
+    def _kernel_sources(self):
+        gm = np.array([p._G * float(_arr(p.m)) for p in self._parts])
+        rs = np.array([float(_arr(p.r_s)) for p in self._parts])
+        n, fn = len(self._parts), self.center_spl
+        return SampledTrack(self._kind, gm, rs,
+                            lambda times: np.repeat(_positions_of(fn, times)[:, None, :], n, 1))
+
+
 def _per_source(pot, xyz, t, what):
     """Per-source potential (M,) or gradient (M, 3) at one point, one kernel call per source
     set row (used by the perturbation helpers)."""
@@ -324,8 +409,8 @@ def _per_source(pot, xyz, t, what):
     for j in range(n):
         sub = SourceSet(src.kind, gm=tens["gm"][j:j + 1],
                         rs=tens["rs"][j:j + 1],
-                        pos=(tens["pos"][:, j:j + 1, :].contiguous() if src.kind == _lib.SRC_TRACK
-                             else tens["pos"][j:j + 1]),
+                        pos=(tens["pos"][:, j:j + 1, :].contiguous()
+                             if src.kind in _lib.TRACK_KINDS else tens["pos"][j:j + 1]),
                         vel=None if tens["vel"] is None else tens["vel"][j:j + 1],
                         t0=None if tens["t0"] is None else tens["t0"][j:j + 1],
                         t_window=None if tens["t_window"] is None else tens["t_window"][j:j + 1],
@@ -352,5 +437,5 @@ __all__ = ["MiyamotoNagaiDisk", "NFWPotential", "PlummerPotential", "HernquistPo
            "SubhaloLinePotential", "SubhaloLinePotential_Custom",
            "SubhaloLinePotentialCustom_fromFunc",
            "SubhaloLinePotential_dRadius", "SubhaloLinePotentialCustom_dRadius_fromFunc",
-           "PlummerClumps",
+           "PlummerClumps", "ProgenitorPotential", "TimeDepTranslatingPotential",
            "LiveSubhaloPotential", "usys"]

@@ -168,14 +168,23 @@ class ClumpSurrogate:
         self.grid_n = (n, n, n)
         flat, _ = self.cell_index(xva)
         worst = torch.full((n ** 3,), -1.0, dtype=torch.float64, device=xva.device)
-        worst = worst.scatter_reduce(0, flat, res, reduce="amax")
-        flags = ((worst < 0) | (worst > self.threshold)).to(torch.uint8)
-        self.flags = flags.reshape(n, n, n).contiguous()
-        fb = self.fallback_mask(xva)
+        self.cell_worst = worst.scatter_reduce(0, flat, res, reduce="amax")
+        self._xva = xva
         self.report.update(val_residual_median=float(res.median().item()),
                            val_residual_p99=float(torch.quantile(res.float(), 0.99).item()),
-                           val_residual_max=float(res.max().item()),
-                           flagged_cells=int(flags.sum().item()), cells=n ** 3,
+                           val_residual_max=float(res.max().item()))
+        self.set_threshold(self.threshold)
+
+    def set_threshold(self, threshold):
+        """Re-flag the cells for another residual threshold (cells without validation samples
+        stay flagged)."""
+        self.threshold = float(threshold)
+        worst = self.cell_worst
+        flags = ((worst < 0) | (worst > self.threshold)).to(torch.uint8)
+        self.flags = flags.reshape(self.grid_n).contiguous()
+        fb = self.fallback_mask(self._xva)
+        self.report.update(flagged_cells=int(flags.sum().item()), cells=int(worst.numel()),
+                           threshold=self.threshold,
                            val_fallback_fraction=float(fb.double().mean().item()))
 
 

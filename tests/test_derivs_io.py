@@ -2,6 +2,7 @@
 with pert_out = [[lead, dlead], [trail, dtrail]], r_s_root and ImpactFrameParams, written as
 <path>/<index>.npy and read back unchanged (CPU only)."""
 import numpy as np
+import pytest
 import torch
 
 from streamsculptor_amd.derivs import load_derivs_batch, save_derivs_batch
@@ -16,7 +17,9 @@ def test_round_trip(tmp_path):
     ifp = {"bImpact": rng.uniform(size=nsh), "tImpact": -rng.uniform(size=nsh) * 1000}
     name = save_derivs_batch(str(tmp_path), 12, pert_out, rs, ifp)
     assert name.endswith("/12.npy")
-    d = load_derivs_batch(str(tmp_path), 12)
+    with pytest.raises(ValueError):
+        load_derivs_batch(str(tmp_path), 12)  # pickled files are refused unless trusted
+    d = load_derivs_batch(str(tmp_path), 12, trusted=True)
     assert set(d) == {"pert_out", "r_s_root", "ImpactFrameParams"}
     for arm in range(2):
         for k in range(2):
@@ -26,4 +29,4 @@ def test_round_trip(tmp_path):
     np.testing.assert_array_equal(d["ImpactFrameParams"]["tImpact"], ifp["tImpact"])
     # no impact parameters
     save_derivs_batch(str(tmp_path), 13, pert_out, rs)
-    assert load_derivs_batch(str(tmp_path), 13)["ImpactFrameParams"] is None
+    assert load_derivs_batch(str(tmp_path), 13, trusted=True)["ImpactFrameParams"] is None

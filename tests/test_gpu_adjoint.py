@@ -19,7 +19,7 @@ def _nrel(a, b):
     return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
 
 
-@pytest.mark.parametrize("kind", ["track", "none"])
+@pytest.mark.parametrize("kind", ["track", "static", "none"])
 def test_adjoint_kernel_vs_oracle(oracle, ssc, kind):
     from streamsculptor_amd import workloads as W
     from streamsculptor_amd.adjoint import LeapfrogAdjoint
@@ -38,6 +38,12 @@ def test_adjoint_kernel_vs_oracle(oracle, ssc, kind):
         pos = np.ascontiguousarray(tr[:, :, :3])
         src = SourceSet(_lib.SRC_TRACK, gm=G * m * 30, rs=rs, pos=pos, grid=(t0, dt))
         so = oracle.Sources(oracle.SRC_TRACK, gm=G * m * 30, rs=rs, pos=pos)
+    elif kind == "static":
+        # every grid row adds to the same source position adjoint
+        _, m, rs = W.subhalo_ics(n=16, seed=2)
+        pos = rng.normal(scale=8.0, size=(16, 3))
+        src = SourceSet(_lib.SRC_STATIC, gm=G * m * 30, rs=rs, pos=pos)
+        so = oracle.Sources(oracle.SRC_STATIC, gm=G * m * 30, rs=rs, pos=pos)
     lamT = rng.normal(size=(n, 6))
     F = rng.normal(scale=1e-3, size=(nsteps + 1, n, 6))
     for i, s in enumerate(start):  # rows before a particle's start carry no forcing
@@ -49,7 +55,7 @@ def test_adjoint_kernel_vs_oracle(oracle, ssc, kind):
     fwd = oracle.leapfrog(h, w0, t0, dt, nsteps, start=start, src=so)
     assert _nrel(wT, fwd) < 1e-12
     e = {"lam0": _nrel(out["lam0"], ref["lam0"]), "g_host": _nrel(out["g_host"], ref["g_host"])}
-    if kind == "track":
+    if kind != "none":
         e["g_gm"] = _nrel(out["g_gm"], ref["g_gm"])
         e["g_pos"] = _nrel(out["g_pos"], ref["g_pos"])
     print(f"adjoint[{kind}] " + " ".join(f"{k} {v:.2e}" for k, v in e.items()))

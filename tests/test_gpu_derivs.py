@@ -34,7 +34,7 @@ def test_batches_match_direct_computation(ssc, tmp_path):
                                     units=ssc.usys, **tol)
     assert [n.rsplit("/", 1)[1] for n in names] == ["4.npy", "5.npy"]
     for i, b in enumerate(batches):
-        d = load_derivs_batch(str(tmp_path), 4 + i)
+        d = load_derivs_batch(str(tmp_path), 4 + i, trusted=True)
         args = (HernquistPotential, np.ones(3), b["r_s"], b["x0"], b["v"], b["t0"],
                 np.full(3, 150.0))
         gen = GenerateMassRadiusPerturbation(
