@@ -253,6 +253,21 @@ int ssc_release_vjp(const ssc_host_potential* host, const double* prog_w, const 
                     const double* g_vel_close, const double* g_vel_far, double* g_prog,
                     double* g_host, double* scratch, void* stream);
 
+/* ssc_release_vjp with no workspace argument and no allocation of any kind.  No counterpart in
+ * the reference: it is for callers that cannot allocate device memory (code under HIP graph
+ * capture, serving loops, harnesses that forbid hipMalloc).  The per-release partials live in
+ * a static device array of SSC_RELEASE_VJP_NOALLOC_MAX_NREL x 2 SSC_MAX_COMPONENTS doubles
+ * (8 MiB per GPU, held until the process ends); nrel above the maximum returns SSC_E_LIMIT.
+ * Same arguments and results as ssc_release_vjp otherwise.  Calls that overlap on one GPU
+ * share the array and race. */
+#define SSC_RELEASE_VJP_NOALLOC_MAX_NREL 65536
+int ssc_release_vjp_noalloc(const ssc_host_potential* host, const double* prog_w,
+                            const int32_t* rel_idx, const double* g_msat, int32_t nrel,
+                            int64_t seed, const double* kvals, const double* g_pos_close,
+                            const double* g_pos_far, const double* g_vel_close,
+                            const double* g_vel_far, double* g_prog, double* g_host,
+                            void* stream);
+
 /* ---------------------------------------------------------------------------------------
  * Discrete adjoint of ssc_leapfrog (no reference counterpart: the reference differentiates
  * forward-mode only, streamsculptor/main.py:160).  Checkpointed: the forward stores the state

@@ -4,6 +4,7 @@
 // PRNG (Salmon et al., SC'11) for the Gaussian offsets.
 #include <hip/hip_runtime.h>
 
+#include <mutex>
 #include <string>
 
 #include "ssc_device.h"
@@ -421,3 +422,58 @@ extern "C" int ssc_release_vjp(const ssc_host_potential* host, const double* pro
                             scratch, stream);
 }
 
+// ------------------------------------------------------------------------------------------
+// ssc_release_vjp without a caller workspace and without any allocation: the per-release
+// partials live in a file-scope device array of SSC_RELEASE_VJP_NOALLOC_MAX_NREL rows of
+// 2 SSC_MAX_COMPONENTS doubles (8 MiB, held by every GPU the library runs on until the process
+// ends).  Its address is looked up once per device, with that device current, and only that
+// address is passed to kernels.  Every call on one GPU shares the array: calls that overlap on
+// one GPU race.  Nothing in it needs clearing (the kernel writes every entry it later reads).
+// ------------------------------------------------------------------------------------------
+static __device__ __align__(256) double scratch[(size_t)SSC_RELEASE_VJP_NOALLOC_MAX_NREL * 2 *
+                                                SSC_MAX_COMPONENTS];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+double* g_scratch_ptr[kMaxDevices] = {};
+}  // namespace
+
+static int scratch_on_current_device(double** out) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return fail(SSC_E_HIP, "hipGetDevice failed");
+  if (dev < 0 || dev >= kMaxDevices)
+    return fail(SSC_E_LIMIT, "device index beyond " + std::to_string(kMaxDevices));
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(SSC_E_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_ptr[dev] = static_cast<double*>(p);
+  }
+  *out = g_scratch_ptr[dev];
+  return SSC_OK;
+}
+
+extern "C" int ssc_release_vjp_noalloc(const ssc_host_potential* host, const double* prog_w,
+                                       const int32_t* rel_idx, const double* g_msat,
+                                       int32_t nrel, int64_t seed, const double* kvals,
+                                       const double* g_pos_close, const double* g_pos_far,
+                                       const double* g_vel_close, const double* g_vel_far,
+                                       double* g_prog, double* g_host, void* stream) {
+  int rc = release_vjp_check("ssc_release_vjp_noalloc", host, prog_w, g_msat, nrel, g_pos_close,
+                             g_pos_far, g_vel_close, g_vel_far, g_prog, g_host);
+  if (rc) return rc;
+  if (nrel > SSC_RELEASE_VJP_NOALLOC_MAX_NREL)
+    return fail(SSC_E_LIMIT, "ssc_release_vjp_noalloc: nrel = " + std::to_string(nrel) +
+                                 " exceeds the static scratch of " +
+                                 std::to_string(SSC_RELEASE_VJP_NOALLOC_MAX_NREL) +
+                                 " releases; use ssc_release_vjp with a workspace");
+  double* sp = nullptr;
+  rc = scratch_on_current_device(&sp);
+  if (rc) return rc;
+  return release_vjp_launch("ssc_release_vjp_noalloc", host, prog_w, rel_idx, g_msat, nrel, seed,
+                            kvals, g_pos_close, g_pos_far, g_vel_close, g_vel_far, g_prog,
+                            g_host, sp, stream);
+}
