@@ -109,12 +109,13 @@ def run_plain(wl, ev=None):
     return w0, start, out
 
 
-def adjoint_setup(wl):
+def adjoint_setup(wl, checkpoint_every=100):
     import numpy as np
     from streamsculptor_amd import workloads as W
     from streamsculptor_amd.adjoint import StreamModel
     sw0, m, rs = W.subhalo_ics()
-    model = StreamModel(wl["ts"], sw0, rs, W.MSAT, wl["per_step"], seed=0, checkpoint_every=100)
+    model = StreamModel(wl["ts"], sw0, rs, W.MSAT, wl["per_step"], seed=0,
+                        checkpoint_every=checkpoint_every)
     prog_w0 = wl["prog_w0"].cpu().numpy()
     halo = np.array([W.HOST["m_halo"], W.HOST["rs_halo"]])
     st = model.forward(halo[0] * 1.05, halo[1] * 0.97, m, prog_w0)
@@ -639,6 +640,14 @@ def main():
                             "flop_per_particle_step": fl_a},
                "loss": float(L.item()), "grad_halo": gh.tolist(),
                "grad_prog_w0": gw.tolist()}
+        # the same loss and gradient with every particle row kept (nothing re-run backwards)
+        del ad
+        ad1 = adjoint_setup(wl, checkpoint_every=1)
+        wall_1, _, (L1, gh1, _, _) = timed(lambda ev: run_adjoint(wl, ad1, ev), args.steps,
+                                           args.warmup, "adjoint, every row stored")
+        res["stored_rows"] = {"checkpoint_every": 1, "ms_per_step": 1e3 * wall_1 / args.steps,
+                              "value": ps * args.steps / wall_1, "loss": float(L1.item()),
+                              "grad_halo": gh1.tolist()}
         print(json.dumps(res), flush=True)
         return
 

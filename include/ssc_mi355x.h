@@ -213,11 +213,20 @@ int ssc_release(const ssc_host_potential* host, const ssc_sources* src, const do
  *   w0[n][6]        base rows at grid index start_step[i]
  *   dw0[n][m][D]    initial responses (NULL = zeros), D = 6 (mass block: d(x,v)/d eps_j) or
  *                   12 with with_radius (adds d^2(x,v)/(d a_j d eps_j), a_j = Plummer radius)
- *   w_out[n][6], dw_out[n][m][D] at grid index nsteps. */
+ *   w_out[n][6], dw_out[n][m][D] at grid index nsteps (must not alias w0, dw0 when a
+ *                   workspace is given)
+ *   workspace       NULL, or device memory of workspace_bytes (ssc_perturb_workspace_bytes
+ *                   gives a size of at most 2 GiB): the base orbits are then advanced once per
+ *                   particle through segments of grid rows, recording position and host Hessian
+ *                   per row for the responses; without it every lane that holds responses of a
+ *                   particle recomputes them (about twice the arithmetic; the source forces
+ *                   are summed in another order, so results agree to the last bits). */
+int64_t ssc_perturb_workspace_bytes(int64_t n, int32_t nsteps);
 int ssc_perturb_leapfrog(const ssc_host_potential* host, const ssc_sources* pert,
                          const double* w0, const double* dw0, const int32_t* start_step,
                          int64_t n, double t0, double dt, int32_t nsteps, int32_t with_radius,
-                         double* w_out, double* dw_out, void* stream);
+                         double* w_out, double* dw_out, void* workspace, int64_t workspace_bytes,
+                         void* stream);
 
 /* Base orbits plus first-order responses integrated with adaptive Dormand-Prince 5(4): the
  * reference's Runge-Kutta path of GenerateMassRadiusPerturbation (the progenitor's backward
@@ -272,7 +281,9 @@ int ssc_release_vjp_noalloc(const ssc_host_potential* host, const double* prog_w
  * Discrete adjoint of ssc_leapfrog (no reference counterpart: the reference differentiates
  * forward-mode only, streamsculptor/main.py:160).  Checkpointed: the forward stores the state
  * every checkpoint_every grid steps in the caller's workspace; the backward re-runs each
- * segment and sweeps it in reverse.  Sources: NONE, STATIC or TRACK.
+ * segment and sweeps it in reverse.  checkpoint_every = 1 stores every row (48 B per orbit per
+ * grid row: 19.2 GB for 1e5 orbits x 4000 steps) and the backward re-runs nothing.  Sources:
+ * NONE, STATIC or TRACK.
  * --------------------------------------------------------------------------------------- */
 int64_t ssc_adjoint_workspace_bytes(int64_t n, int32_t nsteps, int32_t checkpoint_every,
                                     int32_t n_src);
@@ -282,6 +293,14 @@ int ssc_adjoint_forward(const ssc_host_potential* host, const ssc_sources* src, 
                         const int32_t* start_step, int64_t n, double t0, double dt, int32_t nsteps,
                         int32_t checkpoint_every, double* w_out, void* workspace,
                         int64_t workspace_bytes, void* stream);
+
+/* The checkpoints ssc_adjoint_forward would store, taken from orbits the caller already
+ * integrated with every grid row saved: rows[nsteps + 1][n][6] (device; ssc_leapfrog's ys with
+ * save_idx = 0..nsteps, bit-identical to what ssc_adjoint_forward computes); n_src as for
+ * ssc_adjoint_workspace_bytes.  ssc_adjoint_backward may follow without a forward pass. */
+int ssc_adjoint_checkpoints_from_rows(const double* rows, int64_t n, int32_t nsteps,
+                                      int32_t checkpoint_every, int32_t n_src, void* workspace,
+                                      int64_t workspace_bytes, void* stream);
 
 /* Backward pass after ssc_adjoint_forward with the same arguments and workspace.
  *   lam_T[n][6]    dL/d(final row)
@@ -297,6 +316,22 @@ int ssc_adjoint_backward(const ssc_host_potential* host, const ssc_sources* src,
                          int32_t nsteps, int32_t checkpoint_every, const double* lam_T,
                          const double* forcing, double* lam_0, double* g_host, double* g_gm,
                          double* g_pos, void* workspace, int64_t workspace_bytes, void* stream);
+
+/* The backward pass in pieces: the grid rows [k_lo, k_hi) only, from k_hi down (k_hi = nsteps
+ * includes the final row).  Call with k_hi = nsteps first, then each next call's k_hi = the
+ * previous k_lo, down to k_lo = 0 (the adjoint state between calls stays in the workspace);
+ * the whole range in one call is ssc_adjoint_backward.  With checkpoint_every > 1, k_lo and
+ * k_hi must be multiples of it (or k_hi = nsteps).  Each call writes its range's share: g_host
+ * and g_gm (the sums over the range), g_pos (zero outside the range's rows), and lam_0, whose
+ * rows are final for the orbits that start at a row >= k_lo.  Forcing rows outside the range
+ * are not read.  Lets a caller start work that depends on the late rows' adjoints (a release
+ * model, the source orbits) while the early rows are still being swept. */
+int ssc_adjoint_backward_range(const ssc_host_potential* host, const ssc_sources* src,
+                               const int32_t* start_step, int64_t n, double t0, double dt,
+                               int32_t nsteps, int32_t checkpoint_every, int32_t k_lo,
+                               int32_t k_hi, const double* lam_T, const double* forcing,
+                               double* lam_0, double* g_host, double* g_gm, double* g_pos,
+                               void* workspace, int64_t workspace_bytes, void* stream);
 
 /* ---------------------------------------------------------------------------------------
  * Clump-force MLP (no counterpart in the reference, which has no network): 3 -> 256 -> 256 ->

@@ -56,7 +56,8 @@ SIGNATURES = {
     "ssc_dopri5": [_vp, _vp, _vp, _vp, _i64, _d, _d, _vp, _i32, _vp, _vp, _vp, _vp, _vp],
     "ssc_dopri5_dense": [_vp, _vp, _vp, _vp, _i64, _d, _d, _vp, _vp, _vp, _i32, _vp],
     "ssc_dopri5_orbits": [_vp, _vp, _vp, _vp, _vp, _i64, _vp, _i32, _vp, _vp, _vp, _vp],
-    "ssc_perturb_leapfrog": [_vp, _vp, _vp, _vp, _vp, _i64, _d, _d, _i32, _i32, _vp, _vp],
+    "ssc_perturb_leapfrog": [_vp, _vp, _vp, _vp, _vp, _i64, _d, _d, _i32, _i32, _vp, _vp, _vp,
+                             _i64],
     "ssc_perturb_dopri5": [_vp, _vp, _vp, _vp, _vp, _i64, _d, _d, _vp, _i32, _i32, _vp, _vp, _vp,
                            _vp, _vp, _vp],
     "ssc_mlp_forward": [_vp, _vp, _i64, _vp, _vp, _i64],
@@ -74,8 +75,11 @@ SIGNATURES = {
     "ssc_release_vjp_noalloc": [_vp, _vp, _vp, _vp, _i32, ctypes.c_int64, _vp, _vp, _vp, _vp, _vp,
                                 _vp, _vp],
     "ssc_adjoint_forward": [_vp, _vp, _vp, _vp, _i64, _d, _d, _i32, _i32, _vp, _vp, _i64],
+    "ssc_adjoint_checkpoints_from_rows": [_vp, _i64, _i32, _i32, _i32, _vp, _i64],
     "ssc_adjoint_backward": [_vp, _vp, _vp, _i64, _d, _d, _i32, _i32, _vp, _vp, _vp, _vp, _vp, _vp,
                              _vp, _i64],
+    "ssc_adjoint_backward_range": [_vp, _vp, _vp, _i64, _d, _d, _i32, _i32, _i32, _i32, _vp, _vp,
+                                   _vp, _vp, _vp, _vp, _vp, _i64],
     "ssc_release": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _i32, ctypes.c_int64, _vp, _vp, _vp, _vp,
                     _vp],
     "ssc_surrogate_adjoint_forward": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _i64, _d, _d, _i32, _i32,
@@ -110,6 +114,8 @@ def load():
         L.ssc_surrogate_adjoint_workspace_bytes.restype = ctypes.c_int64
         L.ssc_surrogate_adjoint_workspace_bytes.argtypes = [ctypes.c_int64, ctypes.c_int32,
                                                             ctypes.c_int32]
+        L.ssc_perturb_workspace_bytes.restype = ctypes.c_int64
+        L.ssc_perturb_workspace_bytes.argtypes = [ctypes.c_int64, ctypes.c_int32]
         L.ssc_adjoint_workspace_bytes.restype = ctypes.c_int64
         L.ssc_adjoint_workspace_bytes.argtypes = [ctypes.c_int64, ctypes.c_int32, ctypes.c_int32,
                                                   ctypes.c_int32]

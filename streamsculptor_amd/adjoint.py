@@ -49,7 +49,19 @@ class LeapfrogAdjoint:
                   _lib.ptr(self.w_out), _lib.ptr(self.ws), self.ws.numel())
         return self.w_out
 
-    def backward(self, lam_T, forcing=None, want_sources=True):
+    def forward_from_rows(self, rows):
+        """Take the checkpoints from orbits already integrated with every grid row saved,
+        rows (nsteps + 1, n, 6) (a leapfrog with save_idx = 0..nsteps: the same arithmetic as
+        ``forward``), instead of integrating again."""
+        rows = _lib.f64(rows).reshape(self.nsteps + 1, self.n, 6).contiguous()
+        _lib.call("ssc_adjoint_checkpoints_from_rows", _lib.ptr(rows), self.n, self.nsteps, self.E,
+                  self.m, _lib.ptr(self.ws), self.ws.numel())
+        self.w_out = rows[-1].clone()
+        return self.w_out
+
+    def backward(self, lam_T, forcing=None, want_sources=True, rows=None):
+        """Adjoint sweep; rows = (k_lo, k_hi) sweeps only those grid rows (called from the last
+        range down, see ssc_adjoint_backward_range), returning that range's share."""
         lam_T = _lib.f64(lam_T).reshape(self.n, 6).contiguous()
         F = None if forcing is None else _lib.f64(forcing).contiguous()
         dev = lam_T.device
@@ -59,12 +71,13 @@ class LeapfrogAdjoint:
         gg = gp = None
         if want_sources and self.m:
             gg = torch.zeros(self.m, dtype=torch.float64, device=dev)
-            rows = self.nsteps + 1 if self.src.kind == _lib.SRC_TRACK else 1
-            gp = torch.zeros(rows, self.m, 3, dtype=torch.float64, device=dev)
-        _lib.call("ssc_adjoint_backward", _lib.ctypes.byref(self.h), self._srcp(),
-                  _lib.ptr(self.start), self.n, self.t0, self.dt, self.nsteps, self.E,
-                  _lib.ptr(lam_T), _lib.ptr(F), _lib.ptr(lam0), _lib.ptr(gh), _lib.ptr(gg),
-                  _lib.ptr(gp), _lib.ptr(self.ws), self.ws.numel())
+            prow = self.nsteps + 1 if self.src.kind == _lib.SRC_TRACK else 1
+            gp = torch.zeros(prow, self.m, 3, dtype=torch.float64, device=dev)
+        k_lo, k_hi = (0, self.nsteps) if rows is None else rows
+        _lib.call("ssc_adjoint_backward_range", _lib.ctypes.byref(self.h), self._srcp(),
+                  _lib.ptr(self.start), self.n, self.t0, self.dt, self.nsteps, self.E, int(k_lo),
+                  int(k_hi), _lib.ptr(lam_T), _lib.ptr(F), _lib.ptr(lam0), _lib.ptr(gh),
+                  _lib.ptr(gg), _lib.ptr(gp), _lib.ptr(self.ws), self.ws.numel())
         return dict(lam0=lam0, g_host=gh[:nc], g_gm=gg, g_pos=gp)
 
 
@@ -99,10 +112,16 @@ def release_vjp(pot, prog_w, rel_idx, g_msat, seed, g_pc, g_pf, g_vc, g_vf, kval
 # ------------------------------------------------------------------------------------------
 class StreamModel:
     """Fixed inputs of a perturbed spray stream: host components other than the halo, the
-    subhalo initial rows and radii, the release grid and the spray layout."""
+    subhalo initial rows and radii, the release grid and the spray layout.
+
+    ``checkpoint_every`` (default 100) keeps a checkpoint every that many steps and re-runs each
+    segment in the backward pass; 1 keeps every particle row of the forward pass instead (48 B
+    per particle per step, 19.2 GB for 1e5 particles over 4000 steps) and re-runs nothing.
+    ``pipeline`` row ranges of the backward pass overlap the particles' sweep with the release
+    model and the progenitor and subhalo adjoints of the ranges already swept."""
 
     def __init__(self, ts, sub_w0, sub_rs, msat, per_step, seed=0, dt=1.0, halo_index=2,
-                 base_components=None, checkpoint_every=100, kval_arr=1.0):
+                 base_components=None, checkpoint_every=100, kval_arr=1.0, pipeline=8):
         from . import workloads as W
         self.ts = np.asarray(ts, dtype=np.float64)
         self.t0 = float(self.ts[0])
@@ -114,6 +133,7 @@ class StreamModel:
         self.per_step = int(per_step)
         self.seed = int(seed)
         self.E = int(checkpoint_every)
+        self.pipeline = max(1, int(pipeline))
         self.kval_arr = kval_arr
         self.base = base_components or W.host_components()
         self.halo_index = halo_index
@@ -178,52 +198,91 @@ class StreamModel:
         padj = LeapfrogAdjoint(total, w0, self.t0, self.dt, T, start=start,
                                checkpoint_every=self.E, src=src)
         wT = padj.forward()
+        # the progenitor's and the subhalos' rows are all saved already: they are the
+        # checkpoints (every row stored)
+        pa = LeapfrogAdjoint(total, _lib.f64(prog_w0).reshape(1, 6), self.t0, self.dt, T,
+                             checkpoint_every=1, src=src)
+        pa.forward_from_rows(pys)
+        sa = LeapfrogAdjoint(host, self.sub_w0, self.t0, self.dt, T, checkpoint_every=1)
+        sa.forward_from_rows(sub_traj)
         return dict(host=host, src=src, total=total, prog_rows=prog_rows, wT=wT, padj=padj,
-                    idx=(si, ai, ri), nseed=nseed, prog_w0=_lib.f64(prog_w0).reshape(1, 6))
+                    pa=pa, sa=sa, idx=(si, ai, ri), nseed=nseed,
+                    prog_w0=_lib.f64(prog_w0).reshape(1, 6))
+
+    def _ranges(self):
+        """Row ranges (k_lo, k_hi) of the pipelined backward pass, from the last."""
+        T = self.nsteps
+        J = min(self.pipeline, T)
+        q = 1 if self.E == 1 else self.E
+        cuts = sorted({0, T} | {min(T, (T * j // J) // q * q) for j in range(1, J)})
+        return [(cuts[i], cuts[i + 1]) for i in range(len(cuts) - 1)][::-1]
 
     def backward(self, st, lam_T):
         """Gradients of L with dL/d(final rows) = lam_T: (dL/dm_halo, dL/drs_halo, dL/dm_j,
-        dL/dprog_w0)."""
+        dL/dprog_w0).
+
+        The particles' sweep runs on the current stream one row range at a time; after each
+        range, a second stream takes the release-model VJP of the releases in that range (their
+        adjoints are final once the sweep has passed their release rows) and the progenitor and
+        subhalo adjoints over the same rows, while the current stream sweeps the next range."""
         T = self.nsteps
-        host, src, total = st["host"], st["src"], st["total"]
+        host, src = st["host"], st["src"]
         hi = self.halo_index
-        # particles
-        A = st["padj"].backward(lam_T)
-        g_host = A["g_host"].clone()
-        g_gm = A["g_gm"].clone()
-        g_pos = A["g_pos"].clone()
-        # release model: scatter the particle IC adjoints back to (seed, arm, release)
-        si, ai, ri = st["idx"]
         dev = lam_T.device
-        g_table = torch.zeros(st["nseed"], 2, T + 1, 6, dtype=torch.float64, device=dev)
-        flat = (si * 2 + ai) * (T + 1) + ri
-        g_table.view(-1, 6).index_add_(0, flat, A["lam0"])
-        g_prog_rows = torch.zeros(T + 1, 6, dtype=torch.float64, device=dev)
-        rel_idx = np.arange(T + 1, dtype=np.int32)
-        gms = torch.full((T + 1,), G * self.msat, dtype=torch.float64, device=dev)
-        for s in range(st["nseed"]):
-            gp, gh = release_vjp(host, st["prog_rows"], rel_idx, gms, self.seed + s,
-                                 g_table[s, 0, :, :3], g_table[s, 1, :, :3],
-                                 g_table[s, 0, :, 3:], g_table[s, 1, :, 3:], self.kval_arr)
-            g_prog_rows += gp
-            g_host += gh
-        # progenitor orbit (in host + subhalos), forced by the release adjoints
-        pa = LeapfrogAdjoint(total, st["prog_w0"], self.t0, self.dt, T, checkpoint_every=self.E,
-                             src=src)
-        pa.forward()
-        C = pa.backward(torch.zeros(1, 6, dtype=torch.float64, device=dev),
-                        forcing=g_prog_rows.reshape(T + 1, 1, 6))
-        g_host += C["g_host"]
-        g_gm += C["g_gm"]
-        g_pos += C["g_pos"]
-        # subhalo orbits (in the host), forced by the source-position adjoints
         M = self.sub_w0.shape[0]
+        nc = max(host._descriptor()[0].n_components, 1)
+        si, ai, ri = st["idx"]
+        flat = (si * 2 + ai) * (T + 1) + ri
+        ps = self.per_step
+        # accumulators (current stream)
+        g_host = torch.zeros(nc, 2, dtype=torch.float64, device=dev)
+        g_gm = torch.zeros(M, dtype=torch.float64, device=dev)
+        g_table = torch.zeros(st["nseed"], 2, T + 1, 6, dtype=torch.float64, device=dev)
+        g_prog_rows = torch.zeros(T + 1, 1, 6, dtype=torch.float64, device=dev)
         F = torch.zeros(T + 1, M, 6, dtype=torch.float64, device=dev)
-        F[:, :, :3] = g_pos
-        sa = LeapfrogAdjoint(host, self.sub_w0, self.t0, self.dt, T, checkpoint_every=self.E)
-        sa.forward()
-        D = sa.backward(torch.zeros(M, 6, dtype=torch.float64, device=dev), forcing=F)
-        g_host += D["g_host"]
+        side_host = torch.zeros_like(g_host)
+        side_gm = torch.zeros_like(g_gm)
+        gms = torch.full((T + 1,), G * self.msat, dtype=torch.float64, device=dev)
+        rel_all = torch.arange(T + 1, dtype=torch.int32, device=dev)
+        z1 = torch.zeros(1, 6, dtype=torch.float64, device=dev)
+        zM = torch.zeros(M, 6, dtype=torch.float64, device=dev)
+        main = torch.cuda.current_stream(dev)
+        side = torch.cuda.Stream(device=dev)
+        side.wait_stream(main)
+        keep = []
+        C = None
+        for lo, hi_ in self._ranges():
+            A = st["padj"].backward(lam_T, rows=(lo, hi_))
+            g_host += A["g_host"]
+            g_gm += A["g_gm"]
+            done = torch.cuda.Event()
+            done.record(main)
+            keep.append(A)
+            top = hi_ + 1 if hi_ == T else hi_
+            with torch.cuda.stream(side):
+                side.wait_event(done)
+                # releases at rows [lo, hi): the particles in that slice (sorted by release row)
+                a, b = lo * ps, min(hi_, T) * ps
+                g_table.view(-1, 6).index_add_(0, flat[a:b], A["lam0"][a:b])
+                for s in range(st["nseed"]):
+                    gp, gh = release_vjp(host, st["prog_rows"][lo:hi_], rel_all[lo:hi_],
+                                         gms[lo:hi_], self.seed + s, g_table[s, 0, lo:hi_, :3],
+                                         g_table[s, 1, lo:hi_, :3], g_table[s, 0, lo:hi_, 3:],
+                                         g_table[s, 1, lo:hi_, 3:], self.kval_arr)
+                    g_prog_rows[lo:hi_, 0] += gp
+                    side_host += gh
+                # progenitor orbit (in host + subhalos) over the same rows
+                C = st["pa"].backward(z1, forcing=g_prog_rows, rows=(lo, hi_))
+                side_host += C["g_host"]
+                side_gm += C["g_gm"]
+                # subhalo orbits (in the host), forced by the source-position adjoints
+                F[lo:top, :, :3] = A["g_pos"][lo:top] + C["g_pos"][lo:top]
+                D = st["sa"].backward(zM, forcing=F, rows=(lo, hi_))
+                side_host += D["g_host"]
+                keep.append((C, D))
+        main.wait_stream(side)
+        g_host += side_host
+        g_gm += side_gm
         return (G * g_host[hi, 0], g_host[hi, 1], G * g_gm, C["lam0"].reshape(6))
 
 

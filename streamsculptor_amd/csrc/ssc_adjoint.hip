@@ -2,7 +2,8 @@
 //
 // Forward: ssc_leapfrog with checkpoint rows every E grid steps (FP64, stored in the caller's
 // workspace).  Backward, segment by segment from the last: each lane re-runs its orbit from the
-// segment's checkpoint storing the positions in HBM, then sweeps back through the segment:
+// segment's checkpoint storing the positions in HBM (E = 1: every row is a checkpoint, the
+// forward's rows are read in place and nothing is re-run), then sweeps back through the segment:
 //   lv_half = lv + dt lx;  lv = lv_half + F_v(k);  lx += F_x(k)
 //   W_k = h lv_half + h lv (interior; only h lv_half at the start row, h lv at the last row)
 //   lx -= H(x_k) W_k   (H: Hessian of host + sources),  dL/dtheta_host += (da/dtheta)^T W_k
@@ -22,6 +23,8 @@ namespace ssc {
 
 constexpr int AJ_BLOCK = 256;
 constexpr int AJ_MAXC = 2 * SSC_MAX_COMPONENTS;
+// Segment length of the backward sweep when every row is stored (checkpoint_every = 1).
+constexpr int AJ_STORED_SEG = 100;
 
 // Adjoint contribution at one grid point: lx -= H W, host-parameter gradients += (da/dp)^T W.
 // With G lanes per orbit, lane r takes sources r, r + G, ... and the group sums the source part
@@ -102,14 +105,16 @@ __device__ __forceinline__ void accel_split(const ssc_host_potential& hp, const 
 // particle (lane rl of the group takes sources rl, rl + G, ...; lane 0 writes the records).
 //   ckpt: state at row k0 (or the initial row for particles starting later)
 //   lam:  in/out adjoint of each particle's state (at k1 on entry, at max(k0, start) on exit)
-//   X, Wb: [k1 - k0 + 1][n][3] recorded positions and weights (zero where inactive)
+//   X:    [k1 - k0 + 1][n][xs] positions of rows k0..k1; xs = 3: written here by re-running the
+//         segment from ckpt; xs = 6: the stored forward rows (x, v), read only
+//   Wb:   [k1 - k0 + 1][n][3] recorded weights (zero where inactive)
 //   gpart: [gridDim.x][2*n_comp] per-block host-parameter partials (accumulated)
 template <int SK, int G, int HS>
 __global__ __launch_bounds__(AJ_BLOCK) void adjoint_segment_kernel(
     ssc_host_potential hp, ssc_sources src, const double* __restrict__ ckpt,
     const int32_t* __restrict__ start, int64_t n, double t0, double dt, int nsteps, int k0, int k1,
     const double* __restrict__ lam_T, const double* __restrict__ forcing, double* __restrict__ lam,
-    double* __restrict__ X, double* __restrict__ Wb, double* __restrict__ gpart) {
+    double* __restrict__ X, int xs, double* __restrict__ Wb, double* __restrict__ gpart) {
   extern __shared__ __align__(16) char lds_raw[];
   __shared__ int kmin_sh;
   __shared__ double red[AJ_BLOCK / 64][AJ_MAXC];
@@ -144,7 +149,7 @@ __global__ __launch_bounds__(AJ_BLOCK) void adjoint_segment_kernel(
   double gh[AJ_MAXC];
 #pragma unroll
   for (int c = 0; c < AJ_MAXC; ++c) gh[c] = 0.0;
-  if (kmin < k1) {
+  if (kmin < k1 && xs == 3) {
     // ---- recompute the segment, recording positions
     double ax = 0.0, ay = 0.0, az = 0.0;
     if (act && lead) {
@@ -169,6 +174,8 @@ __global__ __launch_bounds__(AJ_BLOCK) void adjoint_segment_kernel(
         }
       }
     }
+  }
+  if (kmin < k1) {
     // ---- reverse sweep
     double lx[3] = {0, 0, 0}, lv[3] = {0, 0, 0};
     if (valid) {
@@ -186,7 +193,7 @@ __global__ __launch_bounds__(AJ_BLOCK) void adjoint_segment_kernel(
       }
       lds.stage_row(src, k1, nsteps + 1);
       if (act) {
-        const double* xp = X + ((int64_t)(k1 - k0) * n + i) * 3;
+        const double* xp = X + ((int64_t)(k1 - k0) * n + i) * xs;
         const double W[3] = {h * lv[0], h * lv[1], h * lv[2]};
         adj_apply<SK, G, HS>(hp, lds, k1, t0 + k1 * dt, rl, xp[0], xp[1], xp[2], W, lx, gh);
         if (lead) {
@@ -205,7 +212,7 @@ __global__ __launch_bounds__(AJ_BLOCK) void adjoint_segment_kernel(
         for (int c = 0; c < 3; ++c) { lv[c] = lh[c] + F(k, 3 + c); lx[c] += F(k, c); }
 #pragma unroll
         for (int c = 0; c < 3; ++c) W[c] = h * lh[c] + (k > s ? h * lv[c] : 0.0);
-        const double* xp = X + ((int64_t)(k - k0) * n + i) * 3;
+        const double* xp = X + ((int64_t)(k - k0) * n + i) * xs;
         adj_apply<SK, G, HS>(hp, lds, k, t0 + k * dt, rl, xp[0], xp[1], xp[2], W, lx, gh);
         if (lead) {
           double* wb = Wb + ((int64_t)(k - k0) * n + i) * 3;
@@ -253,7 +260,7 @@ constexpr int SR_CHUNK = 2048;
 // per particle.
 template <int SK>
 __global__ __launch_bounds__(64) void source_adjoint_kernel(
-    ssc_sources src, const double* __restrict__ X, const double* __restrict__ Wb, int64_t n,
+    ssc_sources src, const double* __restrict__ X, int xs, const double* __restrict__ Wb, int64_t n,
     int k0, int rows, double t0, double dt, double* __restrict__ part) {
   __shared__ double tile[64][6];  // x, y, z, W0, W1, W2 of 64 particles
   const int r = blockIdx.y, chunk = blockIdx.x;
@@ -272,7 +279,7 @@ __global__ __launch_bounds__(64) void source_adjoint_kernel(
       const int64_t i = i0 + threadIdx.x;
       __syncthreads();  // the previous tile has been read
       if (i < ie) {
-        const double* xp = X + ((int64_t)r * n + i) * 3;
+        const double* xp = X + ((int64_t)r * n + i) * xs;
         const double* w = Wb + ((int64_t)r * n + i) * 3;
         tile[threadIdx.x][0] = xp[0]; tile[threadIdx.x][1] = xp[1]; tile[threadIdx.x][2] = xp[2];
         tile[threadIdx.x][3] = w[0]; tile[threadIdx.x][4] = w[1]; tile[threadIdx.x][5] = w[2];
@@ -349,6 +356,17 @@ __global__ void ckpt_index_kernel(int32_t* __restrict__ idx, int64_t nseg, int E
   if (c <= nseg) idx[c] = (int32_t)min((int64_t)nsteps, c * E);
 }
 
+// checkpoint c = rows[min(c E, nsteps)] of orbits saved at every grid row
+__global__ void ckpt_from_rows_kernel(const double* __restrict__ rows, int64_t n, int64_t nseg,
+                                      int E, int nsteps, double* __restrict__ ckpt) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t per = n * 6;
+  if (e >= (nseg + 1) * per) return;
+  const int64_t c = e / per, rem = e - c * per;
+  const int64_t k = c * E < (int64_t)nsteps ? c * E : (int64_t)nsteps;
+  ckpt[e] = rows[k * per + rem];
+}
+
 __global__ void zero_kernel(double* __restrict__ p, int64_t n) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = 0.0;
@@ -364,12 +382,15 @@ struct AdjWs {
 
 static int64_t nseg_of(int nsteps, int E) { return nsteps == 0 ? 0 : (nsteps + E - 1) / E; }
 
+// rows per backward segment: E, or AJ_STORED_SEG when every row is stored (E = 1)
+static int seg_len(int E) { return E == 1 ? AJ_STORED_SEG : E; }
+
 static int64_t adj_bytes(int64_t n, int nsteps, int E, int m, AdjWs* w, char* base) {
   const int64_t nseg = nseg_of(nsteps, E);
   // segment-kernel blocks: G lanes per particle (the same choice adjoint_backward makes)
   const int64_t nblk = (n * group_lanes(n, m) + AJ_BLOCK - 1) / AJ_BLOCK;
   const int64_t nchunk = (n + SR_CHUNK - 1) / SR_CHUNK;
-  const int64_t rows = (int64_t)E + 1;
+  const int64_t rows = (int64_t)seg_len(E) + 1;
   int64_t off = 0;
   auto take = [&](int64_t bytes) {
     char* r = base ? base + off : nullptr;
@@ -378,7 +399,7 @@ static int64_t adj_bytes(int64_t n, int nsteps, int E, int m, AdjWs* w, char* ba
   };
   char* c1 = take((nseg + 1) * n * 6 * 8);
   char* c2 = take(n * 6 * 8);
-  char* c3 = take(rows * n * 3 * 8);
+  char* c3 = take(E == 1 ? 0 : rows * n * 3 * 8);  // stored rows are read from the checkpoints
   char* c4 = take(rows * n * 3 * 8);
   char* c5 = take(nblk * AJ_MAXC * 8);
   char* c6 = take(nchunk * rows * (m > 0 ? m : 1) * 4 * 8);
@@ -392,14 +413,16 @@ static int64_t adj_bytes(int64_t n, int nsteps, int E, int m, AdjWs* w, char* ba
   return off;
 }
 
+// Segments covering rows [k_lo, k_hi), from the last (see ssc_adjoint_backward_range).
 template <int SK, int HS>
 static int adjoint_backward(const ssc_host_potential& hp, const ssc_sources& src,
                             const int32_t* start, int64_t n, double t0, double dt, int nsteps,
-                            int E, const double* lam_T, const double* forcing, double* lam_0,
-                            double* g_host, double* g_gm, double* g_pos, AdjWs& w,
+                            int E, int k_lo, int k_hi, const double* lam_T, const double* forcing,
+                            double* lam_0, double* g_host, double* g_gm, double* g_pos, AdjWs& w,
                             hipStream_t st) {
   const int m = SK == SSC_SRC_NONE ? 0 : src.n;
-  const int64_t nseg = nseg_of(nsteps, E);
+  const bool stored = E == 1;
+  const int L = seg_len(E);
   const int G = SK == SSC_SRC_NONE ? 1 : group_lanes(n, m);
   const unsigned nblk = (unsigned)((n * G + AJ_BLOCK - 1) / AJ_BLOCK);
   const int64_t nchunk = (n + SR_CHUNK - 1) / SR_CHUNK;
@@ -425,13 +448,17 @@ static int adjoint_backward(const ssc_host_potential& hp, const ssc_sources& src
     const int64_t np = (SK == SSC_SRC_TRACK ? (int64_t)(nsteps + 1) : 1) * m * 3;
     hipLaunchKernelGGL(zero_kernel, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, st, g_pos, np);
   }
-  for (int64_t c = nseg - 1; c >= 0; --c) {
-    const int k0 = (int)(c * E), k1 = (int)std::min<int64_t>((c + 1) * E, nsteps);
+  // stored rows: segments of L rows ending at k_hi; checkpoints: the E-row segments in range
+  for (int k1 = k_hi; k1 > k_lo;) {
+    const int k0 = stored ? std::max(k_lo, k1 - L) : (k1 - 1) / E * E;
     const int rows = k1 - k0 + 1;
+    const double* ck = w.ckpt + (int64_t)(stored ? k0 : k0 / E) * n * 6;
+    double* X = stored ? (double*)ck : w.X;
+    const int xs = stored ? 6 : 3;
 #define SSC_SEG(GV)                                                                                \
   hipLaunchKernelGGL((adjoint_segment_kernel<SK, GV, HS>), dim3(nblk), dim3(AJ_BLOCK), lds, st, hp, src, \
-                     w.ckpt + c * n * 6, start, n, t0, dt, nsteps, k0, k1, lam_T, forcing, w.lam,    \
-                     w.X, w.Wb, w.gpart)
+                     ck, start, n, t0, dt, nsteps, k0, k1, lam_T, forcing, w.lam, X, xs, w.Wb,      \
+                     w.gpart)
     switch (G) {
       case 2: SSC_SEG(2); break;
       case 4: SSC_SEG(4); break;
@@ -442,13 +469,14 @@ static int adjoint_backward(const ssc_host_potential& hp, const ssc_sources& src
 #undef SSC_SEG
     if (m > 0 && (g_gm || g_pos)) {
       hipLaunchKernelGGL(source_adjoint_kernel<SK>, dim3((unsigned)nchunk, (unsigned)rows), dim3(64),
-                         0, st, src, w.X, w.Wb, n, k0, rows, t0, dt, w.part);
+                         0, st, src, X, xs, w.Wb, n, k0, rows, t0, dt, w.part);
       hipLaunchKernelGGL(source_adjoint_reduce, dim3((unsigned)((rows * m + 255) / 256)), dim3(256),
                          0, st, w.part, (int)nchunk, rows, m, k0, SK == SSC_SRC_TRACK ? 1 : 0,
                          g_pos, w.gm_rows);
       hipLaunchKernelGGL(rows_reduce, dim3((m + 255) / 256), dim3(256), 0, st, w.gm_rows, rows, m,
                          SK == SSC_SRC_TRACK ? 1 : 0, g_pos, g_gm);
     }
+    k1 = k0;
   }
   hipMemcpyAsync(lam_0, w.lam, n * 6 * 8, hipMemcpyDeviceToDevice, st);
   if (hp.n_components > 0)
@@ -487,17 +515,40 @@ int ssc_adjoint_forward(const ssc_host_potential* host, const ssc_sources* src, 
                       (int32_t)(nseg + 1), w.ckpt, stream);
 }
 
-int ssc_adjoint_backward(const ssc_host_potential* host, const ssc_sources* src,
-                         const int32_t* start_step, int64_t n, double t0, double dt,
-                         int32_t nsteps, int32_t checkpoint_every, const double* lam_T,
-                         const double* forcing, double* lam_0, double* g_host, double* g_gm,
-                         double* g_pos, void* workspace, int64_t workspace_bytes, void* stream) {
+int ssc_adjoint_checkpoints_from_rows(const double* rows, int64_t n, int32_t nsteps,
+                                      int32_t checkpoint_every, int32_t n_src, void* workspace,
+                                      int64_t workspace_bytes, void* stream) {
+  if (n <= 0 || nsteps < 0 || checkpoint_every <= 0 || n_src < 0 || !rows || !workspace)
+    return fail(SSC_E_ARG, "ssc_adjoint_checkpoints_from_rows: bad argument");
+  if (workspace_bytes < adj_bytes(n, nsteps, checkpoint_every, n_src, nullptr, nullptr))
+    return fail(SSC_E_ARG, "ssc_adjoint_checkpoints_from_rows: workspace too small");
+  AdjWs w;
+  adj_bytes(n, nsteps, checkpoint_every, n_src, &w, (char*)workspace);
+  const int64_t nseg = nseg_of(nsteps, checkpoint_every);
+  const int64_t tot = (nseg + 1) * n * 6;
+  hipLaunchKernelGGL(ckpt_from_rows_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0,
+                     (hipStream_t)stream, rows, n, nseg, (int)checkpoint_every, (int)nsteps,
+                     w.ckpt);
+  return check_launch("ssc_adjoint_checkpoints_from_rows");
+}
+
+int ssc_adjoint_backward_range(const ssc_host_potential* host, const ssc_sources* src,
+                               const int32_t* start_step, int64_t n, double t0, double dt,
+                               int32_t nsteps, int32_t checkpoint_every, int32_t k_lo,
+                               int32_t k_hi, const double* lam_T, const double* forcing,
+                               double* lam_0, double* g_host, double* g_gm, double* g_pos,
+                               void* workspace, int64_t workspace_bytes, void* stream) {
   int rc = validate_host(host);
   if (rc) return rc;
   rc = validate_sources(src, nsteps);
   if (rc) return rc;
   if (n <= 0 || nsteps < 1 || checkpoint_every <= 0 || !lam_T || !lam_0 || !g_host || !workspace)
     return fail(SSC_E_ARG, "ssc_adjoint_backward: bad argument (needs n >= 1, nsteps >= 1)");
+  if (k_lo < 0 || k_hi > nsteps || k_lo >= k_hi ||
+      (checkpoint_every > 1 && (k_lo % checkpoint_every != 0 ||
+                                (k_hi != nsteps && k_hi % checkpoint_every != 0))))
+    return fail(SSC_E_ARG, "ssc_adjoint_backward_range: needs 0 <= k_lo < k_hi <= nsteps, both "
+                           "checkpoint rows (multiples of checkpoint_every, or k_hi = nsteps)");
   ssc_sources s;
   if (src) s = *src; else memset(&s, 0, sizeof(s));
   if (s.n == 0) s.kind = SSC_SRC_NONE;
@@ -509,8 +560,8 @@ int ssc_adjoint_backward(const ssc_host_potential* host, const ssc_sources* src,
   hipStream_t st = (hipStream_t)stream;
   const int hs = host_signature(*host);
 #define SSC_AB3(K, H)                                                                             \
-  adjoint_backward<K, H>(*host, s, start_step, n, t0, dt, nsteps, checkpoint_every, lam_T, forcing, \
-                         lam_0, g_host, g_gm, g_pos, w, st)
+  adjoint_backward<K, H>(*host, s, start_step, n, t0, dt, nsteps, checkpoint_every, k_lo, k_hi,   \
+                         lam_T, forcing, lam_0, g_host, g_gm, g_pos, w, st)
 #define SSC_AB(K)                                                                            \
   (hs == HS_MN_HQ_NFW ? SSC_AB3(K, HS_MN_HQ_NFW)                                             \
                       : (hs == HS_MN_HQ_HQ_NFW ? SSC_AB3(K, HS_MN_HQ_HQ_NFW) : SSC_AB3(K, HS_GENERIC)))
@@ -524,6 +575,16 @@ int ssc_adjoint_backward(const ssc_host_potential* host, const ssc_sources* src,
       return fail(SSC_E_ARG, "ssc_adjoint_backward: LINE and Hernquist sources are not "
                              "differentiated");
   }
+}
+
+int ssc_adjoint_backward(const ssc_host_potential* host, const ssc_sources* src,
+                         const int32_t* start_step, int64_t n, double t0, double dt,
+                         int32_t nsteps, int32_t checkpoint_every, const double* lam_T,
+                         const double* forcing, double* lam_0, double* g_host, double* g_gm,
+                         double* g_pos, void* workspace, int64_t workspace_bytes, void* stream) {
+  return ssc_adjoint_backward_range(host, src, start_step, n, t0, dt, nsteps, checkpoint_every, 0,
+                                    nsteps, lam_T, forcing, lam_0, g_host, g_gm, g_pos, workspace,
+                                    workspace_bytes, stream);
 }
 
 }  // extern "C"

@@ -122,8 +122,12 @@ def _pert_parts(pertgen):
     return base, src, radius
 
 
-def perturb_leapfrog(base, src, w0, dw0, start, t0, dt, nsteps, with_radius):
-    """Run ssc_perturb_leapfrog: base rows w0 (n, 6), responses dw0 (n, m, 6|12) or None."""
+def perturb_leapfrog(base, src, w0, dw0, start, t0, dt, nsteps, with_radius, split=True,
+                     segment_rows=None):
+    """Run ssc_perturb_leapfrog: base rows w0 (n, 6), responses dw0 (n, m, 6|12) or None.
+    split: advance the base orbits once per particle through segments of rows kept in a
+    workspace (otherwise every lane of a particle recomputes them); segment_rows: rows per
+    segment (default: the library's choice, at most 2 GiB of records)."""
     h = base._host_only()
     w0 = _lib.f64(w0).reshape(-1, 6).contiguous()
     n = w0.shape[0]
@@ -136,9 +140,14 @@ def perturb_leapfrog(base, src, w0, dw0, start, t0, dt, nsteps, with_radius):
     if src.kind == _lib.SRC_TRACK:
         src = src.on_grid(t0, dt, nsteps)
     sd = src.desc()
+    ws = None
+    if split and n > 0:
+        nbytes = (_lib.load().ssc_perturb_workspace_bytes(n, int(nsteps)) if segment_rows is None
+                  else int(segment_rows) * n * 9 * 8)
+        ws = torch.empty(max(nbytes, 1), dtype=torch.uint8, device=w0.device)
     _lib.call("ssc_perturb_leapfrog", _lib.ctypes.byref(h), _lib.ctypes.byref(sd), _lib.ptr(w0),
               _lib.ptr(d0), _lib.ptr(st), n, float(t0), float(dt), int(nsteps), int(with_radius),
-              _lib.ptr(out), _lib.ptr(dout))
+              _lib.ptr(out), _lib.ptr(dout), _lib.ptr(ws), 0 if ws is None else ws.numel())
     return out, dout
 
 

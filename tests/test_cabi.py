@@ -39,6 +39,7 @@ def test_library_exports_every_declared_symbol():
     for n in _declared():
         if n not in ("ssc_last_error", "ssc_version", "ssc_mlp_num_params",
                      "ssc_mlp_packed_bytes", "ssc_mlp_packed_vjp_bytes", "ssc_mlp_workspace_bytes",
+                     "ssc_perturb_workspace_bytes",
                      "ssc_adjoint_workspace_bytes", "ssc_surrogate_adjoint_workspace_bytes"):
             assert n in _lib.SIGNATURES, n
 

@@ -60,6 +60,45 @@ def test_adjoint_kernel_vs_oracle(oracle, ssc, kind):
         e["g_pos"] = _nrel(out["g_pos"], ref["g_pos"])
     print(f"adjoint[{kind}] " + " ".join(f"{k} {v:.2e}" for k, v in e.items()))
     assert all(v < 1e-9 for v in e.values()), e
+    # checkpoints taken from rows saved by a plain leapfrog: the same backward, bit for bit
+    from streamsculptor_amd.adjoint import _with_sources
+    total = host if src is None else _with_sources(host, src)
+    _, rows = total._leapfrog(w0, t0, dt, nsteps, start=start, save_idx=np.arange(nsteps + 1))
+    adj2 = LeapfrogAdjoint(host, w0, t0, dt, nsteps, start=start, checkpoint_every=37, src=src)
+    assert torch.equal(adj2.forward_from_rows(rows), wT)
+    out2 = adj2.backward(lamT, forcing=F)
+    assert torch.equal(out2["lam0"], out["lam0"])
+    for k in ("g_host", "g_gm", "g_pos"):  # block sums may be added in another order
+        if out[k] is not None:
+            assert _nrel(out2[k], out[k].cpu().numpy()) < 1e-13, k
+    # the backward in row ranges (checkpoint rows 111, 37): shares add up to the whole sweep
+    _check_ranges(adj, lamT, F, out, [(111, nsteps), (37, 111), (0, 37)])
+    # every row stored (checkpoint_every = 1): nothing re-run, same adjoint, any ranges
+    adj1 = LeapfrogAdjoint(host, w0, t0, dt, nsteps, start=start, checkpoint_every=1, src=src)
+    assert _nrel(adj1.forward(), fwd) < 1e-12
+    out1 = adj1.backward(lamT, forcing=F)
+    e1 = {k: _nrel(out1[k], ref[k]) for k in out if out[k] is not None}
+    print(f"adjoint[{kind}] stored rows " + " ".join(f"{k} {v:.2e}" for k, v in e1.items()))
+    assert all(v < 1e-9 for v in e1.values()), e1
+    _check_ranges(adj1, lamT, F, out1, [(97, nsteps), (96, 97), (10, 96), (0, 10)])
+
+
+def _check_ranges(adj, lamT, F, whole, ranges):
+    """Sweep ``ranges`` (from the last) and compare the summed shares with one whole sweep."""
+    acc = {}
+    for lo, hi in ranges:
+        part = adj.backward(lamT, forcing=F, rows=(lo, hi))
+        for k in ("g_host", "g_gm", "g_pos"):
+            if part[k] is not None:
+                acc[k] = part[k] + acc.get(k, 0.0)
+        if part["g_pos"] is not None and part["g_pos"].shape[0] > 1:
+            outside = torch.ones(part["g_pos"].shape[0], dtype=torch.bool)
+            outside[lo:hi + 1 if hi == adj.nsteps else hi] = False
+            assert not part["g_pos"][outside.to(part["g_pos"].device)].any()
+    # every orbit's adjoint is final after the last range (same segments: bit for bit)
+    assert torch.equal(part["lam0"], whole["lam0"])
+    for k, v in acc.items():
+        assert _nrel(v, whole[k].cpu().numpy()) < 1e-13, k
 
 
 def test_release_vjp_vs_oracle(oracle, ssc):

@@ -62,6 +62,17 @@ def test_perturb_parity(oracle, ssc, kind, radius):
     e1, e2 = _rel(w, wr), _rel(dw, dwr)
     print(f"perturb[{kind}, radius={radius}] base {e1:.2e} responses {e2:.2e}", flush=True)
     assert e1 <= 1e-9 and e2 <= 1e-9
+    # the split form (base orbits once per particle, through segments of rows) with segments
+    # of 37 rows, whose boundaries fall inside particles' lives: the same arithmetic, bit for
+    # bit; the form that recomputes the base orbit in every lane sums the source forces in
+    # another order: the same to the last bits
+    w2, dw2 = perturb_leapfrog(host, src, w0, dw0, start, t0, dt, nsteps, radius, segment_rows=37)
+    np.testing.assert_array_equal(w2.cpu().numpy(), w.cpu().numpy())
+    np.testing.assert_array_equal(dw2.cpu().numpy(), dw.cpu().numpy())
+    w3, dw3 = perturb_leapfrog(host, src, w0, dw0, start, t0, dt, nsteps, radius, split=False)
+    e3, e4 = _rel(w3, w.cpu().numpy()), _rel(dw3, dw.cpu().numpy())
+    print(f"perturb[{kind}] split vs per-lane base orbits: base {e3:.2e} responses {e4:.2e}")
+    assert e3 <= 1e-11 and e4 <= 1e-9
 
 
 def test_integrate_field_api(oracle, ssc):
