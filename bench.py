@@ -586,6 +586,8 @@ def main():
     ap.add_argument("--releases", type=int, default=1000, help="ensemble: release steps each")
     ap.add_argument("--cpu-particles", type=int, default=16, help="ensemble: CPU sample size")
     ap.add_argument("--no-exact", action="store_true", help="ensemble: skip the exact run")
+    ap.add_argument("--no-stored", action="store_true",
+                    help="adjoint: skip the run with every particle row stored")
     ap.add_argument("--sims", type=int, default=700, help="gd1: simulations per batch")
     ap.add_argument("--cpu-sims", type=int, default=600, help="gd1: simulations in the CPU sample")
     ap.add_argument("--particles", type=int, default=None)
@@ -640,6 +642,9 @@ def main():
                             "flop_per_particle_step": fl_a},
                "loss": float(L.item()), "grad_halo": gh.tolist(),
                "grad_prog_w0": gw.tolist()}
+        if args.no_stored:
+            print(json.dumps(res), flush=True)
+            return
         # the same loss and gradient with every particle row kept (nothing re-run backwards)
         del ad
         ad1 = adjoint_setup(wl, checkpoint_every=1)

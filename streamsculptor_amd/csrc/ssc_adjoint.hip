@@ -69,14 +69,16 @@ __device__ __forceinline__ void adj_apply(const ssc_host_potential& hp, const So
     for (int j = rl; j < lds.n; j += G) {
       double sx, sy, sz, gm, q2;
       if (!lds.get(k, t, j, sx, sy, sz, gm, q2)) continue;
+      // H_j W = G m (W / s^3 - 3 d (d.W) / s^5), accumulated by two FMAs per component
       const double dx = x - sx, dy = y - sy, dz = z - sz;
       const double s2 = fma(dx, dx, fma(dy, dy, fma(dz, dz, q2)));
       const double is = rsq(s2);
-      const double i3 = gm * is * is * is;
-      const double c5 = 3.0 * i3 * is * is * (dx * W[0] + dy * W[1] + dz * W[2]);
-      sw[0] += i3 * W[0] - c5 * dx;
-      sw[1] += i3 * W[1] - c5 * dy;
-      sw[2] += i3 * W[2] - c5 * dz;
+      const double is2 = is * is;
+      const double i3 = gm * (is2 * is);
+      const double c = (i3 * is2) * (-3.0 * fma(dx, W[0], fma(dy, W[1], dz * W[2])));
+      sw[0] = fma(i3, W[0], sw[0]); sw[0] = fma(c, dx, sw[0]);
+      sw[1] = fma(i3, W[1], sw[1]); sw[1] = fma(c, dy, sw[1]);
+      sw[2] = fma(i3, W[2], sw[2]); sw[2] = fma(c, dz, sw[2]);
     }
 #pragma unroll
     for (int c = 0; c < 3; ++c) hw[c] += group_sum<G>(sw[c]);
@@ -257,7 +259,10 @@ constexpr int SR_CHUNK = 2048;
 // The chunk's particle rows are staged 64 at a time through LDS with one coalesced load (lane l
 // reads particle l of the tile), and every lane then reads each particle as an LDS broadcast:
 // a wave-uniform load per particle straight from memory left the loop waiting on one round trip
-// per particle.
+// per particle.  The tile's particles with a non-zero weight (those that move at this row) form
+// a wave-uniform bit mask, and the pair loop visits only its set bits.  Per pair: 3 differences,
+// 3 FMAs for s^2, 1/s (hardware estimate + Newton step), s^-3 and s^-5, d.W, and 7 accumulating
+// FMAs; G m multiplies the sums once at the end.
 template <int SK>
 __global__ __launch_bounds__(64) void source_adjoint_kernel(
     ssc_sources src, const double* __restrict__ X, int xs, const double* __restrict__ Wb, int64_t n,
@@ -272,39 +277,41 @@ __global__ __launch_bounds__(64) void source_adjoint_kernel(
     const int j = jj * 64 + threadIdx.x;
     double sx = 0.0, sy = 0.0, sz = 0.0;
     const bool on = j < src.n && source_position(src, j, k, t, sx, sy, sz);
-    const double gm = j < src.n ? src.gm[j] : 0.0;
-    const double q2 = j < src.n ? src.rs[j] * src.rs[j] : 1.0;
+    const double q2 = j < src.n ? src.rs[j] * src.rs[j] : 1.0;  // finite pairs on idle lanes
     double gx = 0.0, gy = 0.0, gz = 0.0, gg = 0.0;
     for (int64_t i0 = ib; i0 < ie; i0 += 64) {
       const int64_t i = i0 + threadIdx.x;
       __syncthreads();  // the previous tile has been read
+      bool live = false;
       if (i < ie) {
         const double* xp = X + ((int64_t)r * n + i) * xs;
         const double* w = Wb + ((int64_t)r * n + i) * 3;
         tile[threadIdx.x][0] = xp[0]; tile[threadIdx.x][1] = xp[1]; tile[threadIdx.x][2] = xp[2];
         tile[threadIdx.x][3] = w[0]; tile[threadIdx.x][4] = w[1]; tile[threadIdx.x][5] = w[2];
+        live = w[0] != 0.0 || w[1] != 0.0 || w[2] != 0.0;
       }
+      const uint64_t mask = __ballot(live);
       __syncthreads();
-      const int cnt = (int)min((int64_t)64, ie - i0);
-      if (!on) continue;
-      for (int p = 0; p < cnt; ++p) {
+      for (uint64_t mk = mask; mk != 0; mk &= mk - 1) {
+        const int p = __builtin_ctzll(mk);
         const double W0 = tile[p][3], W1 = tile[p][4], W2 = tile[p][5];
-        if (W0 == 0.0 && W1 == 0.0 && W2 == 0.0) continue;  // uniform across the wave
         const double dx = tile[p][0] - sx, dy = tile[p][1] - sy, dz = tile[p][2] - sz;
         const double s2 = fma(dx, dx, fma(dy, dy, fma(dz, dz, q2)));
         const double is = rsq(s2);
-        const double i3 = is * is * is;
-        const double dW = dx * W0 + dy * W1 + dz * W2;
-        const double c5 = 3.0 * i3 * is * is * dW;
-        gx += gm * (i3 * W0 - c5 * dx);
-        gy += gm * (i3 * W1 - c5 * dy);
-        gz += gm * (i3 * W2 - c5 * dz);
-        gg -= i3 * dW;
+        const double is2 = is * is;
+        const double i3 = is2 * is;
+        const double dW = fma(dx, W0, fma(dy, W1, dz * W2));
+        const double c = (i3 * is2) * (-3.0 * dW);
+        gx = fma(i3, W0, gx); gx = fma(c, dx, gx);
+        gy = fma(i3, W1, gy); gy = fma(c, dy, gy);
+        gz = fma(i3, W2, gz); gz = fma(c, dz, gz);
+        gg = fma(-i3, dW, gg);
       }
     }
     if (j < src.n) {
+      const double gm = on ? src.gm[j] : 0.0;
       double* o = part + (((int64_t)chunk * rows + r) * src.n + j) * 4;
-      o[0] = gx; o[1] = gy; o[2] = gz; o[3] = gg;
+      o[0] = gm * gx; o[1] = gm * gy; o[2] = gm * gz; o[3] = on ? gg : 0.0;
     }
   }
 }

@@ -310,9 +310,14 @@ __global__ __launch_bounds__(128) void release_vjp_kernel(
     const double* __restrict__ gpc, const double* __restrict__ gpf,
     const double* __restrict__ gvc, const double* __restrict__ gvf, double* __restrict__ g_prog,
     double* __restrict__ g_host_rel) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nrel) return;
+  // one thread per (release, input direction): the 6 + 2 n_components dual-number columns of
+  // a release are independent evaluations, so they run side by side rather than in sequence
   const int nc = hp.n_components;
+  const int nin = 6 + 2 * nc;
+  const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t i = tid / nin;
+  const int dir = (int)(tid - i * nin);
+  if (i >= nrel) return;
   double zr[4];
   const int64_t ridx = rel_idx ? (int64_t)rel_idx[i] : (int64_t)i;
   for (int k = 0; k < 4; ++k) zr[k] = prng_normal(ridx * randint1000(seed, k));
@@ -323,8 +328,7 @@ __global__ __launch_bounds__(128) void release_vjp_kernel(
     gout[6 + a] = gvc[3 * (int64_t)i + a];
     gout[9 + a] = gvf[3 * (int64_t)i + a];
   }
-  const int nin = 6 + 2 * nc;
-  for (int dir = 0; dir < nin; ++dir) {
+  {
     D1 in[6], hostp[4 * SSC_MAX_COMPONENTS], out[12];
     for (int k = 0; k < 6; ++k) in[k] = D1{prog_w[6 * (int64_t)i + k], dir == k ? 1.0 : 0.0};
     for (int c = 0; c < nc; ++c)
@@ -384,7 +388,9 @@ static int release_vjp_launch(const char* name, const ssc_host_potential* host,
   const double def[8] = {2.0, 0.3, 0.0, 0.0, 0.4, 0.4, 0.5, 0.5};
   for (int k = 0; k < 8; ++k) kv.k[k] = kvals ? kvals[k] : def[k];
   hipStream_t st = (hipStream_t)stream;
-  hipLaunchKernelGGL(release_vjp_kernel, dim3((nrel + 127) / 128), dim3(128), 0, st, *host, prog_w,
+  const int64_t nthr = (int64_t)nrel * (6 + 2 * host->n_components);
+  hipLaunchKernelGGL(release_vjp_kernel, dim3((unsigned)((nthr + 127) / 128)), dim3(128), 0, st,
+                     *host, prog_w,
                      rel_idx, g_msat, nrel, seed, kv, g_pos_close, g_pos_far, g_vel_close,
                      g_vel_far, g_prog, scratch);
   if (host->n_components > 0)
