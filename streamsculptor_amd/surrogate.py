@@ -108,9 +108,14 @@ class ClumpSurrogate:
     # ------------------------------------------------------------------ training
     @classmethod
     def train(cls, host, clumps, positions, n_train=1 << 22, n_val=1 << 18, batch=8192,
-              iters=8000, lr=2e-3, lr_min=1e-5, seed=0, threshold=1e-3, grid_cells=64,
-              weight_cut=3.0, log=None):
-        """Train on ``positions`` (sampled along plain-method orbits, (N, 3) on the GPU)."""
+              iters=96000, lr=2e-3, lr_min=1e-5, seed=0, threshold=1e-3, grid_cells=64,
+              weight_cut=30.0, log=None):
+        """Train on ``positions`` (sampled along plain-method orbits, (N, 3) on the GPU).
+        Loss weights 1 / (1 + (|g| / g_ref)^2), g_ref = weight_cut x the median clump force:
+        the spikes near clumps (which the residual map sends to the exact sum anyway) do not
+        set the fit.  The defaults (96,000 iterations of 8,192 points, cosine learning rate
+        2e-3 -> 1e-5, weight_cut 30) came from a sweep of the final stream-position error
+        against the plain method on the clumpy-halo workload."""
         t_start = time.perf_counter()
         dev = _lib.device()
         g = torch.Generator(device=dev)
