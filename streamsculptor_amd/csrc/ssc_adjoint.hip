@@ -279,19 +279,26 @@ __global__ __launch_bounds__(64) void source_adjoint_kernel(
     const bool on = j < src.n && source_position(src, j, k, t, sx, sy, sz);
     const double q2 = j < src.n ? src.rs[j] * src.rs[j] : 1.0;  // finite pairs on idle lanes
     double gx = 0.0, gy = 0.0, gz = 0.0, gg = 0.0;
-    for (int64_t i0 = ib; i0 < ie; i0 += 64) {
+    // the next tile's row is loaded into registers while the current tile is processed
+    double nx[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    auto fetch = [&](int64_t i0) {
       const int64_t i = i0 + threadIdx.x;
-      __syncthreads();  // the previous tile has been read
-      bool live = false;
       if (i < ie) {
         const double* xp = X + ((int64_t)r * n + i) * xs;
         const double* w = Wb + ((int64_t)r * n + i) * 3;
-        tile[threadIdx.x][0] = xp[0]; tile[threadIdx.x][1] = xp[1]; tile[threadIdx.x][2] = xp[2];
-        tile[threadIdx.x][3] = w[0]; tile[threadIdx.x][4] = w[1]; tile[threadIdx.x][5] = w[2];
-        live = w[0] != 0.0 || w[1] != 0.0 || w[2] != 0.0;
+        nx[0] = xp[0]; nx[1] = xp[1]; nx[2] = xp[2]; nx[3] = w[0]; nx[4] = w[1]; nx[5] = w[2];
+      } else {
+        nx[3] = nx[4] = nx[5] = 0.0;
       }
-      const uint64_t mask = __ballot(live);
+    };
+    fetch(ib);
+    for (int64_t i0 = ib; i0 < ie; i0 += 64) {
+      __syncthreads();  // the previous tile has been read
+#pragma unroll
+      for (int c = 0; c < 6; ++c) tile[threadIdx.x][c] = nx[c];
+      const uint64_t mask = __ballot(nx[3] != 0.0 || nx[4] != 0.0 || nx[5] != 0.0);
       __syncthreads();
+      if (i0 + 64 < ie) fetch(i0 + 64);
       for (uint64_t mk = mask; mk != 0; mk &= mk - 1) {
         const int p = __builtin_ctzll(mk);
         const double W0 = tile[p][3], W1 = tile[p][4], W2 = tile[p][5];
