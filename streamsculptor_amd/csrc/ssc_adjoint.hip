@@ -148,9 +148,12 @@ __global__ __launch_bounds__(AJ_BLOCK) void adjoint_segment_kernel(
       double* wb = Wb + ((int64_t)r * n + i) * 3;
       wb[0] = wb[1] = wb[2] = 0.0;
     }
-  double gh[AJ_MAXC];
+  // host-parameter partials: 2 per component, as many as the host signature has components
+  // (a fixed count keeps them in registers indexed at compile time)
+  constexpr int NGH = HS == HS_GENERIC ? AJ_MAXC : 2 * hs_components(HS);
+  double gh[NGH];
 #pragma unroll
-  for (int c = 0; c < AJ_MAXC; ++c) gh[c] = 0.0;
+  for (int c = 0; c < NGH; ++c) gh[c] = 0.0;
   if (kmin < k1 && xs == 3) {
     // ---- recompute the segment, recording positions
     double ax = 0.0, ay = 0.0, az = 0.0;
@@ -238,7 +241,9 @@ __global__ __launch_bounds__(AJ_BLOCK) void adjoint_segment_kernel(
   // block reduction of the host-parameter gradients (wave shuffles, then LDS); every lane of a
   // group holds the same values, so only lane 0 of each group contributes
   const int nc2 = 2 * hp.n_components;
-  for (int c = 0; c < nc2; ++c) {
+#pragma unroll
+  for (int c = 0; c < NGH; ++c) {
+    if (c >= nc2) break;
     double v = rl == 0 ? gh[c] : 0.0;
 #pragma unroll
     for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
