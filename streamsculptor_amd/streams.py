@@ -35,12 +35,85 @@ def stream_ics(prog_pot, release_pot, ts, prog_w0, Msat, seed_num, solver=None, 
     return ics, grid, ts_np
 
 
+def _static_in_time(pot):
+    """True when the potential does not depend on time (no LINE or TRACK sources): a fixed-step
+    orbit in it can be restarted at any grid row with the same arithmetic."""
+    _, src = pot._descriptor()
+    return src is None or src.kind == _lib.SRC_STATIC
+
+
+def _leapfrog_stream_pipelined(prog_pot, release_pot, particle_pot, ts_np, prog_w0, Msat,
+                               seed_num, kval_arr, dt0, chunks):
+    """The fixed-step spray stream in ``chunks`` ranges of release rows: the progenitor orbit
+    advances one range at a time on the current stream, each range's releases follow it, and
+    the particles released there start integrating on a stream of their own while the
+    progenitor advances through the next range -- the single-orbit chain (one lane, latency
+    bound) overlaps the particles' integration instead of preceding all of it.  Every orbit
+    sees the arithmetic of the unchunked path (a restart at a grid row recomputes the
+    acceleration there from the same state, and each particle starts at its own row), so the
+    rows are the same bit for bit.  Returns the particle rows (2T, 6), lead/trail interleaved."""
+    t0, dt, nsteps, kidx = release_grid(ts_np, dt0)
+    T = len(ts_np) - 1
+    dev = _lib.device()
+    w = _lib.f64(prog_w0).reshape(1, 6)
+    cuts = sorted({0, T} | {T * j // chunks for j in range(1, chunks)})
+    from .main import _as_float
+    msat = np.broadcast_to(np.asarray(_as_float(Msat), dtype=np.float64), (T + 1,))
+    gms_all = release_pot._G * _lib.f64(np.ascontiguousarray(msat[:T]))
+    # index arrays on the device up front: no host-to-device copy inside the pipeline
+    start_all = _lib.i32(np.repeat(kidx[:T], 2).astype(np.int32))
+    save_all = [_lib.i32((kidx[cuts[c]:cuts[c + 1]] - kidx[cuts[c]]).astype(np.int32))
+                for c in range(len(cuts) - 1)]
+    main = torch.cuda.current_stream(dev)
+    sides = [torch.cuda.Stream(device=dev) for _ in range(min(3, len(cuts) - 2))]
+    out = torch.empty(2 * T, 6, dtype=torch.float64, device=dev)
+    keep = []
+    for c in range(len(cuts) - 1):
+        a, b = cuts[c], cuts[c + 1]
+        # progenitor rows kidx[a..b-1], and its state at the next range's first row
+        r0 = int(kidx[a])
+        r1 = int(kidx[b]) if b < T else int(kidx[T - 1])
+        w_next, ys = prog_pot._leapfrog(w, t0 + r0 * dt, dt, r1 - r0, save_idx=save_all[c])
+        ws = ys[:, 0, :].contiguous()
+        idx = torch.arange(a, b, dtype=torch.int32, device=dev)
+        pc, pf, vc, vf = release_pot._release(ws, idx, gms_all[a:b].contiguous(), seed_num,
+                                              kval_arr, t_rel=ts_np[a:b])
+        w0 = torch.empty(2 * (b - a), 6, dtype=torch.float64, device=dev)
+        w0[0::2, :3], w0[0::2, 3:] = pc, vc
+        w0[1::2, :3], w0[1::2, 3:] = pf, vf
+        start = start_all[2 * a:2 * b]
+        last = c == len(cuts) - 2
+        st = main if (last or not sides) else sides[c % len(sides)]
+        if st is not main:
+            st.wait_stream(main)
+        with torch.cuda.stream(st):
+            o, _ = particle_pot._leapfrog(w0, t0, dt, nsteps, start=start)
+            out[2 * a:2 * b] = o
+        keep.append((w0, o))
+        w = w_next
+    for s in sides:
+        main.wait_stream(s)
+    return out
+
+
 def gen_stream(prog_pot, release_pot, ts, prog_w0, Msat, seed_num, solver=None, kval_arr=1.0,
                rtol=1e-7, atol=1e-7, dtmin=0.3, dtmax=None, max_steps=10_000, throw=False,
-               dt0=None, particle_pot=None):
-    """Returns (lead (T, 6), trail (T, 6)) with T = len(ts) - 1."""
+               dt0=None, particle_pot=None, chunks=4):
+    """Returns (lead (T, 6), trail (T, 6)) with T = len(ts) - 1.  With the fixed-step solver in
+    time-independent potentials the progenitor orbit and the particles are pipelined over
+    ``chunks`` ranges of release rows (same rows bit for bit; chunks=1: one pass after the
+    other)."""
     particle_pot = particle_pot or prog_pot
     method = _solvers.resolve(solver)
+    if method == "leapfrog" and chunks > 1:
+        ts_np = np.asarray(ts.detach().cpu().numpy() if isinstance(ts, torch.Tensor) else ts,
+                           dtype=np.float64).reshape(-1)
+        T = len(ts_np) - 1
+        if (T >= 4 * chunks and _static_in_time(prog_pot) and _static_in_time(release_pot)
+                and _static_in_time(particle_pot)):
+            out = _leapfrog_stream_pipelined(prog_pot, release_pot, particle_pot, ts_np,
+                                             prog_w0, Msat, seed_num, kval_arr, dt0, chunks)
+            return out[0::2], out[1::2]
     adaptive_kw = dict(rtol=rtol, atol=atol, dtmin=dtmin, dtmax=dtmax, max_steps=max_steps)
     (pc, pf, vc, vf), grid, ts_np = stream_ics(prog_pot, release_pot, ts, prog_w0, Msat, seed_num,
                                                solver=solver, kval_arr=kval_arr, dt0=dt0,

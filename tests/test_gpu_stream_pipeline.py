@@ -1,0 +1,67 @@
+"""The fixed-step spray stream pipelined over ranges of release rows (the progenitor orbit
+advancing range by range while the particles of the ranges behind it integrate on other
+streams) gives the rows of the one-pass path bit for bit: the configs[0] stream (host only,
+4000 steps, two particles per step), a clumpy halo with a mass-loss history (Msat per release),
+and a time-dependent potential, which takes the one-pass path."""
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+
+def _both(pot, ts, prog_w0, msat, seed):
+    from streamsculptor_amd.streams import gen_stream
+    a = gen_stream(pot, pot, ts, prog_w0, msat, seed, solver="leapfrog", dt0=1.0, chunks=4)
+    b = gen_stream(pot, pot, ts, prog_w0, msat, seed, solver="leapfrog", dt0=1.0, chunks=1)
+    return a, b
+
+
+def test_pipelined_stream_equals_one_pass(oracle, ssc):
+    from streamsculptor_amd import workloads as W
+    from streamsculptor_amd.main import usys
+    from streamsculptor_amd.potential import PlummerClumps, Potential_Combine
+    host = W.host_potential()
+    ts = W.release_times()
+    back = host.integrate_orbit(w0=W.PROG_TODAY, ts=[0.0, ts[0]], t0=0.0, t1=float(ts[0]),
+                                solver="leapfrog", dt0=1.0).ys[-1]
+    (l4, t4), (l1, t1) = _both(host, ts, back, W.MSAT, 0)
+    assert torch.equal(l4, l1) and torch.equal(t4, t1)
+    # the public entry point takes the pipelined path and still matches the oracle stream
+    lead, trail = host.gen_stream_vmapped(ts=ts, prog_w0=back.cpu().numpy(), Msat=W.MSAT,
+                                          seed_num=0, solver="leapfrog", dt0=1.0)
+    assert torch.equal(lead, l1) and torch.equal(trail, t1)
+    h = oracle.host(W.host_components())
+    ref_lead, _ = oracle.stream_leapfrog(h, back.cpu().numpy(), ts, W.MSAT, 0, 1.0)
+    e = (np.linalg.norm(lead.cpu().numpy() - ref_lead, axis=1) /
+         np.linalg.norm(ref_lead, axis=1)).max()
+    assert e <= 1e-9
+    # static clumps, Msat falling along the stream
+    pos, m, rs = W.clump_table(96, seed=3)
+    pot = Potential_Combine([host, PlummerClumps(m, rs, pos, units=usys)], units=usys)
+    ts2 = np.linspace(-1000.0, 0.0, 1001)
+    msat = np.linspace(2e4, 1e3, ts2.size)
+    (l4, t4), (l1, t1) = _both(pot, ts2, back, msat, 3)
+    assert torch.equal(l4, l1) and torch.equal(t4, t1)
+
+
+def test_time_dependent_potential_takes_one_pass(ssc):
+    from streamsculptor_amd import workloads as W
+    from streamsculptor_amd.main import usys
+    from streamsculptor_amd.potential import (HernquistPotential, Potential_Combine,
+                                              TimeDepTranslatingPotential)
+    from streamsculptor_amd.streams import _static_in_time
+    host = W.host_potential()
+
+    def centre(t):
+        t = np.asarray(t, dtype=np.float64)
+        return np.stack([40.0 + 0.0 * t, -30.0 - 0.02 * t, 10.0 + 0.0 * t], axis=-1)
+
+    lmc = TimeDepTranslatingPotential(HernquistPotential(m=1e11, r_s=15.0, units=usys), centre,
+                                      units=usys)
+    pot = Potential_Combine([host, lmc], units=usys)
+    assert _static_in_time(host) and not _static_in_time(pot)
+    ts = np.linspace(-500.0, 0.0, 501)
+    back = np.array([11.0, 1.0, 0.5, 0.02, 0.17, 0.04])
+    (l4, t4), (l1, t1) = _both(pot, ts, back, 1e4, 1)
+    assert torch.equal(l4, l1) and torch.equal(t4, t1)
