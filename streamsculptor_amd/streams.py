@@ -42,67 +42,90 @@ def _static_in_time(pot):
     return src is None or src.kind == _lib.SRC_STATIC
 
 
+_SIDE = {}
+
+
+def _side_stream(dev):
+    """One second stream per device, kept for the process: a stream of its own keeps its
+    hardware queue, and creating one per call costs host time ahead of the first launch."""
+    key = torch.device(dev).index
+    if key not in _SIDE:
+        _SIDE[key] = torch.cuda.Stream(device=dev)
+    return _SIDE[key]
+
+
 def _leapfrog_stream_pipelined(prog_pot, release_pot, particle_pot, ts_np, prog_w0, Msat,
                                seed_num, kval_arr, dt0, chunks):
-    """The fixed-step spray stream in ``chunks`` ranges of release rows: the progenitor orbit
-    advances one range at a time on the current stream, each range's releases follow it, and
-    the particles released there start integrating on a stream of their own while the
-    progenitor advances through the next range -- the single-orbit chain (one lane, latency
-    bound) overlaps the particles' integration instead of preceding all of it.  Every orbit
-    sees the arithmetic of the unchunked path (a restart at a grid row recomputes the
-    acceleration there from the same state, and each particle starts at its own row), so the
-    rows are the same bit for bit.  Returns the particle rows (2T, 6), lead/trail interleaved."""
+    """The fixed-step spray stream in ``chunks`` windows of grid rows on two streams: the
+    current stream advances the progenitor orbit through a window and releases the particles
+    of that window; a second stream then advances every particle released so far through the
+    same window while the progenitor moves on through the next.  The single-orbit chain (one
+    lane, latency-bound) thus overlaps the particles' integration instead of preceding all of
+    it, and two streams keep clear of hardware-queue sharing, which serialises streams mapped
+    to one queue.  Every orbit takes the steps of the one-pass path (a restart at a grid row
+    recomputes the acceleration there from the same state; a particle starts at its own row):
+    in the host alone the rows are the same bit for bit; with sources the acceleration at a
+    window's first row comes from the kernel's prologue, whose FMA contraction can differ from
+    the step loop's in the last bit.  Returns the particle rows (2T, 6), lead/trail
+    interleaved, in release order."""
+    from .main import _as_float
     t0, dt, nsteps, kidx = release_grid(ts_np, dt0)
     T = len(ts_np) - 1
     dev = _lib.device()
     w = _lib.f64(prog_w0).reshape(1, 6)
+    # windows of release rows [cuts[c], cuts[c + 1]): grid rows [kidx[cuts[c]], kidx[cuts[c + 1]]]
     cuts = sorted({0, T} | {T * j // chunks for j in range(1, chunks)})
-    from .main import _as_float
     msat = np.broadcast_to(np.asarray(_as_float(Msat), dtype=np.float64), (T + 1,))
     gms_all = release_pot._G * _lib.f64(np.ascontiguousarray(msat[:T]))
-    # index arrays on the device up front: no host-to-device copy inside the pipeline
-    start_all = _lib.i32(np.repeat(kidx[:T], 2).astype(np.int32))
-    save_all = [_lib.i32((kidx[cuts[c]:cuts[c + 1]] - kidx[cuts[c]]).astype(np.int32))
-                for c in range(len(cuts) - 1)]
+    # every window's index arrays in one host-to-device copy, up front
+    rows_all = np.repeat(kidx[:T], 2).astype(np.int64)
+    parts, spans = [], []
+    for c in range(len(cuts) - 1):
+        a, b = cuts[c], cuts[c + 1]
+        g0 = int(kidx[a])
+        for arr in ((kidx[a:b] - g0), np.maximum(rows_all[:2 * b] - g0, 0)):
+            spans.append((sum(p.size for p in parts), arr.size))
+            parts.append(arr.astype(np.int32))
+    flat = _lib.i32(np.concatenate(parts))
+    save_all = [flat[o:o + k] for o, k in spans[0::2]]
+    start_all = [flat[o:o + k] for o, k in spans[1::2]]
+    idx_all = torch.arange(T, dtype=torch.int32, device=dev)
     main = torch.cuda.current_stream(dev)
-    sides = [torch.cuda.Stream(device=dev) for _ in range(min(3, len(cuts) - 2))]
-    out = torch.empty(2 * T, 6, dtype=torch.float64, device=dev)
+    side = _side_stream(dev)
+    state = None
     keep = []
     for c in range(len(cuts) - 1):
         a, b = cuts[c], cuts[c + 1]
-        # progenitor rows kidx[a..b-1], and its state at the next range's first row
-        r0 = int(kidx[a])
-        r1 = int(kidx[b]) if b < T else int(kidx[T - 1])
-        w_next, ys = prog_pot._leapfrog(w, t0 + r0 * dt, dt, r1 - r0, save_idx=save_all[c])
+        g0 = int(kidx[a])
+        g1 = int(kidx[b]) if b < T else nsteps
+        # progenitor rows kidx[a..b-1], and its state at the next window's first row g1
+        pn = (g1 if b < T else int(kidx[T - 1])) - g0
+        w_next, ys = prog_pot._leapfrog(w, t0 + g0 * dt, dt, pn, save_idx=save_all[c])
         ws = ys[:, 0, :].contiguous()
-        idx = torch.arange(a, b, dtype=torch.int32, device=dev)
-        pc, pf, vc, vf = release_pot._release(ws, idx, gms_all[a:b].contiguous(), seed_num,
-                                              kval_arr, t_rel=ts_np[a:b])
-        w0 = torch.empty(2 * (b - a), 6, dtype=torch.float64, device=dev)
-        w0[0::2, :3], w0[0::2, 3:] = pc, vc
-        w0[1::2, :3], w0[1::2, 3:] = pf, vf
-        start = start_all[2 * a:2 * b]
-        last = c == len(cuts) - 2
-        st = main if (last or not sides) else sides[c % len(sides)]
-        if st is not main:
-            st.wait_stream(main)
-        with torch.cuda.stream(st):
-            o, _ = particle_pot._leapfrog(w0, t0, dt, nsteps, start=start)
-            out[2 * a:2 * b] = o
-        keep.append((w0, o))
+        pc, pf, vc, vf = release_pot._release(ws, idx_all[a:b].contiguous(),
+                                              gms_all[a:b].contiguous(), seed_num, kval_arr,
+                                              t_rel=ts_np[a:b])
+        new = torch.empty(2 * (b - a), 6, dtype=torch.float64, device=dev)
+        new[0::2, :3], new[0::2, 3:] = pc, vc
+        new[1::2, :3], new[1::2, 3:] = pf, vf
+        side.wait_stream(main)
+        with torch.cuda.stream(side):
+            w_in = new if state is None else torch.cat([state, new], 0)
+            state, _ = particle_pot._leapfrog(w_in, t0 + g0 * dt, dt, g1 - g0,
+                                              start=start_all[c])
+        keep.append((new, w_in))
         w = w_next
-    for s in sides:
-        main.wait_stream(s)
-    return out
+    main.wait_stream(side)
+    return state
 
 
 def gen_stream(prog_pot, release_pot, ts, prog_w0, Msat, seed_num, solver=None, kval_arr=1.0,
                rtol=1e-7, atol=1e-7, dtmin=0.3, dtmax=None, max_steps=10_000, throw=False,
-               dt0=None, particle_pot=None, chunks=4):
+               dt0=None, particle_pot=None, chunks=8):
     """Returns (lead (T, 6), trail (T, 6)) with T = len(ts) - 1.  With the fixed-step solver in
     time-independent potentials the progenitor orbit and the particles are pipelined over
-    ``chunks`` ranges of release rows (same rows bit for bit; chunks=1: one pass after the
-    other)."""
+    ``chunks`` windows of grid rows (the one-pass rows, to the last bit with sources; chunks=1:
+    one pass after the other)."""
     particle_pot = particle_pot or prog_pot
     method = _solvers.resolve(solver)
     if method == "leapfrog" and chunks > 1:

@@ -1,8 +1,11 @@
-"""The fixed-step spray stream pipelined over ranges of release rows (the progenitor orbit
-advancing range by range while the particles of the ranges behind it integrate on other
-streams) gives the rows of the one-pass path bit for bit: the configs[0] stream (host only,
-4000 steps, two particles per step), a clumpy halo with a mass-loss history (Msat per release),
-and a time-dependent potential, which takes the one-pass path."""
+"""The fixed-step spray stream pipelined over windows of grid rows (the progenitor orbit
+advancing window by window while, on a second stream, every particle released so far advances
+through the window behind it) against the one-pass path: the configs[0] stream (host only,
+4000 steps, two particles per step) bit for bit in 4, 7 and the default 8 windows; a clumpy
+halo with a mass-loss history (Msat per release) to 1e-11 (a restart forms the acceleration at
+a window's first row in the kernel's prologue, whose FMA contraction can differ in the last bit
+from the step loop's when the source sum is split over lanes; grown along 1000 steps among the
+clumps: 1.5e-13 measured); a time-dependent potential takes the one-pass path."""
 import numpy as np
 import pytest
 import torch
@@ -10,10 +13,20 @@ import torch
 pytestmark = pytest.mark.gpu
 
 
-def _both(pot, ts, prog_w0, msat, seed):
+def _rel(a, b):
+    return ((a - b).norm(dim=1) / b.norm(dim=1)).max().item()
+
+
+def _both(pot, ts, prog_w0, msat, seed, exact=True):
     from streamsculptor_amd.streams import gen_stream
     a = gen_stream(pot, pot, ts, prog_w0, msat, seed, solver="leapfrog", dt0=1.0, chunks=4)
     b = gen_stream(pot, pot, ts, prog_w0, msat, seed, solver="leapfrog", dt0=1.0, chunks=1)
+    c = gen_stream(pot, pot, ts, prog_w0, msat, seed, solver="leapfrog", dt0=1.0, chunks=7)
+    e = max(_rel(x, y) for x, y in zip(a + c, b + b))
+    print(f"[pipeline] windows vs one pass: {e:.2e}", flush=True)
+    if exact:
+        assert all(torch.equal(x, y) for x, y in zip(a + c, b + b))
+    assert e <= 1e-11
     return a, b
 
 
@@ -26,7 +39,6 @@ def test_pipelined_stream_equals_one_pass(oracle, ssc):
     back = host.integrate_orbit(w0=W.PROG_TODAY, ts=[0.0, ts[0]], t0=0.0, t1=float(ts[0]),
                                 solver="leapfrog", dt0=1.0).ys[-1]
     (l4, t4), (l1, t1) = _both(host, ts, back, W.MSAT, 0)
-    assert torch.equal(l4, l1) and torch.equal(t4, t1)
     # the public entry point takes the pipelined path and still matches the oracle stream
     lead, trail = host.gen_stream_vmapped(ts=ts, prog_w0=back.cpu().numpy(), Msat=W.MSAT,
                                           seed_num=0, solver="leapfrog", dt0=1.0)
@@ -41,8 +53,7 @@ def test_pipelined_stream_equals_one_pass(oracle, ssc):
     pot = Potential_Combine([host, PlummerClumps(m, rs, pos, units=usys)], units=usys)
     ts2 = np.linspace(-1000.0, 0.0, 1001)
     msat = np.linspace(2e4, 1e3, ts2.size)
-    (l4, t4), (l1, t1) = _both(pot, ts2, back, msat, 3)
-    assert torch.equal(l4, l1) and torch.equal(t4, t1)
+    _both(pot, ts2, back, msat, 3, exact=False)
 
 
 def test_time_dependent_potential_takes_one_pass(ssc):
