@@ -97,7 +97,7 @@ __device__ __forceinline__ void accel_split(const ssc_host_potential& hp, const 
   double sx = 0.0, sy = 0.0, sz = 0.0;
   lds.template add_grad<1>(k, t, &x, &y, &z, &sx, &sy, &sz, rl, G);
   double gx, gy, gz;
-  host_grad_sig<HS>(hp, x, y, z, gx, gy, gz);
+  host_grad_sig<HS, lean_host(SK)>(hp, x, y, z, gx, gy, gz);
   ax = -(gx + group_sum<G>(sx));
   ay = -(gy + group_sum<G>(sy));
   az = -(gz + group_sum<G>(sz));
@@ -268,6 +268,24 @@ constexpr int SR_CHUNK = 2048;
 // a wave-uniform bit mask, and the pair loop visits only its set bits.  Per pair: 3 differences,
 // 3 FMAs for s^2, 1/s (hardware estimate + Newton step), s^-3 and s^-5, d.W, and 7 accumulating
 // FMAs; G m multiplies the sums once at the end.
+// One particle's term for one source: H_j W / (G m) and -(d.W) / s^3 added to (gx, gy, gz, gg).
+__device__ __forceinline__ void pair_adjoint(const double* t, double sx, double sy, double sz,
+                                             double q2, double& gx, double& gy, double& gz,
+                                             double& gg) {
+  const double W0 = t[3], W1 = t[4], W2 = t[5];
+  const double dx = t[0] - sx, dy = t[1] - sy, dz = t[2] - sz;
+  const double s2 = fma(dx, dx, fma(dy, dy, fma(dz, dz, q2)));
+  const double is = rsq(s2);
+  const double is2 = is * is;
+  const double i3 = is2 * is;
+  const double dW = fma(dx, W0, fma(dy, W1, dz * W2));
+  const double c = (i3 * is2) * (-3.0 * dW);
+  gx = fma(i3, W0, gx); gx = fma(c, dx, gx);
+  gy = fma(i3, W1, gy); gy = fma(c, dy, gy);
+  gz = fma(i3, W2, gz); gz = fma(c, dz, gz);
+  gg = fma(-i3, dW, gg);
+}
+
 template <int SK>
 __global__ __launch_bounds__(64) void source_adjoint_kernel(
     ssc_sources src, const double* __restrict__ X, int xs, const double* __restrict__ Wb, int64_t n,
@@ -283,7 +301,7 @@ __global__ __launch_bounds__(64) void source_adjoint_kernel(
     double sx = 0.0, sy = 0.0, sz = 0.0;
     const bool on = j < src.n && source_position(src, j, k, t, sx, sy, sz);
     const double q2 = j < src.n ? src.rs[j] * src.rs[j] : 1.0;  // finite pairs on idle lanes
-    double gx = 0.0, gy = 0.0, gz = 0.0, gg = 0.0;
+    double gx = 0.0, gy = 0.0, gz = 0.0, gg = 0.0, hx = 0.0, hy = 0.0, hz = 0.0, hg = 0.0;
     // the next tile's row is loaded into registers while the current tile is processed
     double nx[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     auto fetch = [&](int64_t i0) {
@@ -304,22 +322,20 @@ __global__ __launch_bounds__(64) void source_adjoint_kernel(
       const uint64_t mask = __ballot(nx[3] != 0.0 || nx[4] != 0.0 || nx[5] != 0.0);
       __syncthreads();
       if (i0 + 64 < ie) fetch(i0 + 64);
-      for (uint64_t mk = mask; mk != 0; mk &= mk - 1) {
+      // two particles per iteration into separate sums: their dependent chains interleave
+      uint64_t mk = mask;
+      while (mk != 0) {
         const int p = __builtin_ctzll(mk);
-        const double W0 = tile[p][3], W1 = tile[p][4], W2 = tile[p][5];
-        const double dx = tile[p][0] - sx, dy = tile[p][1] - sy, dz = tile[p][2] - sz;
-        const double s2 = fma(dx, dx, fma(dy, dy, fma(dz, dz, q2)));
-        const double is = rsq(s2);
-        const double is2 = is * is;
-        const double i3 = is2 * is;
-        const double dW = fma(dx, W0, fma(dy, W1, dz * W2));
-        const double c = (i3 * is2) * (-3.0 * dW);
-        gx = fma(i3, W0, gx); gx = fma(c, dx, gx);
-        gy = fma(i3, W1, gy); gy = fma(c, dy, gy);
-        gz = fma(i3, W2, gz); gz = fma(c, dz, gz);
-        gg = fma(-i3, dW, gg);
+        mk &= mk - 1;
+        pair_adjoint(tile[p], sx, sy, sz, q2, gx, gy, gz, gg);
+        if (mk != 0) {
+          const int p2 = __builtin_ctzll(mk);
+          mk &= mk - 1;
+          pair_adjoint(tile[p2], sx, sy, sz, q2, hx, hy, hz, hg);
+        }
       }
     }
+    gx += hx; gy += hy; gz += hz; gg += hg;
     if (j < src.n) {
       const double gm = on ? src.gm[j] : 0.0;
       double* o = part + (((int64_t)chunk * rows + r) * src.n + j) * 4;

@@ -131,7 +131,8 @@ __device__ __forceinline__ void accel_P(const ssc_host_potential& hp, const Sour
                                         const double* z, double* ax, double* ay, double* az) {
   double gx[P], gy[P], gz[P];
 #pragma unroll
-  for (int p = 0; p < P; ++p) host_grad_sig<HS>(hp, x[p], y[p], z[p], gx[p], gy[p], gz[p]);
+  for (int p = 0; p < P; ++p)
+    host_grad_sig<HS, lean_host(SK)>(hp, x[p], y[p], z[p], gx[p], gy[p], gz[p]);
   lds.template add_grad<P>(k, t, x, y, z, gx, gy, gz, 0, 1);
 #pragma unroll
   for (int p = 0; p < P; ++p) {
@@ -240,7 +241,7 @@ __device__ __forceinline__ void accel_group(const ssc_host_potential& hp, const 
   gy = group_sum<G>(gy);
   gz = group_sum<G>(gz);
   double hx, hy, hz;
-  host_grad_sig<HS>(hp, x, y, z, hx, hy, hz);
+  host_grad_sig<HS, lean_host(SK)>(hp, x, y, z, hx, hy, hz);
   ax = -(hx + gx);
   ay = -(hy + gy);
   az = -(hz + gz);
@@ -326,7 +327,7 @@ __device__ __forceinline__ void accel_wave(const ssc_host_potential& hp, const S
   gy = group_sum<WAVE>(gy);
   gz = group_sum<WAVE>(gz);
   double hx, hy, hz;
-  host_grad_sig<HS>(hp, x, y, z, hx, hy, hz);
+  host_grad_sig<HS, lean_host(SK)>(hp, x, y, z, hx, hy, hz);
   ax = -(hx + gx);
   ay = -(hy + gy);
   az = -(hz + gz);

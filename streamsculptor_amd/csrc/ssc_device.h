@@ -37,6 +37,57 @@ __device__ __forceinline__ double rsq(double v) {
   return fma(0.5 * y, e, y);
 }
 
+// 1/b for positive finite b: the hardware estimate refined by two Newton steps (about one unit
+// in the last place; five instructions against the eleven of an IEEE division).
+__device__ __forceinline__ double recip(double b) {
+  double r = __builtin_amdgcn_rcp(b);
+  r = fma(r, fma(-b, r, 1.0), r);
+  return fma(r, fma(-b, r, 1.0), r);
+}
+
+// log(1 + u) for u >= 0 (the NFW profile's log(1 + r / r_s)), within about two units in the
+// last place.  The library's log1p evaluates in double-double arithmetic, about 130
+// instructions a call on gfx950 -- half of a host-force evaluation -- and on a wave64 every
+// FP64 instruction holds the SIMD for four cycles however few lanes are active, so the
+// one-orbit chains (progenitors, subhalo orbits) pay for every one of them.  Method: t = 1 + u
+// rounded and its rounding error c = u - (t - 1) (exact for t <= 2^53); t = m 2^e with
+// m in [sqrt(1/2), sqrt(2)); log m = 2 atanh(s), s = (m - 1) / (m + 1), |s| <= 0.1716, by its
+// Taylor series 2 s (1 + z/3 + z^2/5 + ...), z = s^2, up to z^10 (the first omitted term is
+// below 2e-18 relative); log(1 + u) = e ln 2 + log m + c / t, with ln 2 split so that e ln2_hi
+// is exact.  About 37 instructions.
+__device__ __forceinline__ double log1p_nonneg(double u) {
+  const double t = 1.0 + u;
+  const double c = u - (t - 1.0);
+  int e = __builtin_amdgcn_frexp_exp(t);  // t = mant 2^e, mant in [1/2, 1)
+  double m = __builtin_amdgcn_frexp_mant(t);
+  if (m < 0.70710678118654752) {
+    m *= 2.0;
+    e -= 1;
+  }
+  const double num = m - 1.0;  // exact
+  const double den = m + 1.0;
+  double r = __builtin_amdgcn_rcp(den);
+  r = fma(r, fma(-den, r, 1.0), r);
+  double s = num * r;
+  s = fma(r, fma(-den, s, num), s);
+  const double z = s * s;
+  double p = 0.09523809523809523;  // 2/21
+  p = fma(p, z, 0.10526315789473684);
+  p = fma(p, z, 0.11764705882352941);
+  p = fma(p, z, 0.13333333333333333);
+  p = fma(p, z, 0.15384615384615385);
+  p = fma(p, z, 0.18181818181818182);
+  p = fma(p, z, 0.2222222222222222);
+  p = fma(p, z, 0.2857142857142857);
+  p = fma(p, z, 0.4);
+  p = fma(p, z, 0.6666666666666666);  // 2/3
+  const double lm = fma(s * z, p, 2.0 * s);
+  const double fe = (double)e;
+  const double ln2_hi = 0.6931467056274414;       // 21 significant bits
+  const double ln2_lo = 4.7493250390316726e-07;   // ln 2 - ln2_hi
+  return fma(fe, ln2_hi, fma(fe, ln2_lo, lm) + c * __builtin_amdgcn_rcp(t));
+}
+
 // ------------------------------------------------------------------------------------------
 // Sum of a double over aligned groups of G consecutive lanes (G = 1, 2, 4, ..., 64), the result
 // in every lane of the group.  Within a 16-lane row the exchanges are DPP moves (quad
@@ -83,25 +134,50 @@ __device__ __forceinline__ double group_sum(double v) {
 // accumulates the parts with the
 // same operations everywhere (an FMA per coordinate, an add for the Miyamoto-Nagai z term), so
 // a sum formed on one lane and a sum of parts gathered from several waves are bit-identical.
+// LEAN (the fixed-step kernels): the refined hardware reciprocal square root and reciprocal
+// (rsq, recip) where the library would expand an IEEE square root or division, and
+// log1p_nonneg -- about one unit in the last place each, and half the instructions of a host
+// force (241 -> 115 per step of the one-lane leapfrog with MN + Hernquist + NFW).  The
+// adaptive kernels keep the IEEE forms (LEAN = false): their step sequences follow the last
+// bits of every force, and parity with the reference's step sequence is stated there.
+template <bool LEAN = true>
 __device__ __forceinline__ void parts_mn(const ssc_component& c, double x, double y, double z,
                                          double& f, double& zm) {
   const double gm = c.p[0], a = c.p[1], b = c.p[2];
-  const double zeta = sqrt(z * z + b * b);
-  const double S = a + zeta;
-  const double D2 = x * x + y * y + S * S;
-  const double iD = rsq(D2);
-  f = gm * iD * iD * iD;
-  zm = f * z * S / zeta;
+  if constexpr (LEAN) {
+    const double v = z * z + b * b;
+    const double iz = rsq(v);  // 1 / zeta
+    const double zeta = v * iz;
+    const double S = a + zeta;
+    const double D2 = x * x + y * y + S * S;
+    const double iD = rsq(D2);
+    f = gm * iD * iD * iD;
+    zm = f * z * S * iz;
+  } else {
+    const double zeta = sqrt(z * z + b * b);
+    const double S = a + zeta;
+    const double D2 = x * x + y * y + S * S;
+    const double iD = rsq(D2);
+    f = gm * iD * iD * iD;
+    zm = f * z * S / zeta;
+  }
 }
+template <bool LEAN = true>
 __device__ __forceinline__ double parts_nfw(const ssc_component& c, double x, double y,
                                             double z) {
   const double gm = c.p[0], rs = c.p[1];
   const double r2 = x * x + y * y + z * z;
   const double ir = rsq(r2);
   const double r = r2 * ir;
-  const double L = log1p(r / rs);
-  return gm * ir * ir * (L * ir - 1.0 / (r + rs));
+  if constexpr (LEAN) {
+    const double L = log1p_nonneg(r * recip(rs));
+    return gm * ir * ir * (L * ir - recip(r + rs));
+  } else {
+    const double L = log1p(r / rs);
+    return gm * ir * ir * (L * ir - 1.0 / (r + rs));
+  }
 }
+template <bool LEAN = true>
 __device__ __forceinline__ double parts_hernquist(const ssc_component& c, double x, double y,
                                                   double z) {
   const double gm = c.p[0], cc = c.p[1], soft = c.p[2];
@@ -109,6 +185,7 @@ __device__ __forceinline__ double parts_hernquist(const ssc_component& c, double
   const double ir = rsq(r2);
   const double r = r2 * ir;
   const double rc = r + cc;
+  if constexpr (LEAN) return gm * ir * recip(rc * rc);
   return gm * ir / (rc * rc);
 }
 __device__ __forceinline__ double parts_plummer(const ssc_component& c, double x, double y,
@@ -119,6 +196,7 @@ __device__ __forceinline__ double parts_plummer(const ssc_component& c, double x
   return gm * is * is * is;
 }
 
+template <bool LEAN = true>
 __device__ __forceinline__ void comp_grad_parts(const ssc_component& c, double x, double y,
                                                 double z, double& f, double& zm, bool& zadd) {
   f = 0.0;
@@ -126,14 +204,14 @@ __device__ __forceinline__ void comp_grad_parts(const ssc_component& c, double x
   zadd = false;
   switch (c.kind) {
     case SSC_COMP_MIYAMOTO_NAGAI:
-      parts_mn(c, x, y, z, f, zm);
+      parts_mn<LEAN>(c, x, y, z, f, zm);
       zadd = true;
       break;
     case SSC_COMP_NFW:
-      f = zm = parts_nfw(c, x, y, z);
+      f = zm = parts_nfw<LEAN>(c, x, y, z);
       break;
     case SSC_COMP_HERNQUIST:
-      f = zm = parts_hernquist(c, x, y, z);
+      f = zm = parts_hernquist<LEAN>(c, x, y, z);
       break;
     case SSC_COMP_PLUMMER:
       f = zm = parts_plummer(c, x, y, z);
@@ -150,20 +228,22 @@ __device__ __forceinline__ void comp_grad_add(double f, double zm, bool zadd, do
   gz = zadd ? gz + zm : fma(zm, z, gz);
 }
 
+template <bool LEAN = true>
 __device__ __forceinline__ void comp_grad(const ssc_component& c, double x, double y, double z,
                                           double& gx, double& gy, double& gz) {
   double f, zm;
   bool zadd;
-  comp_grad_parts(c, x, y, z, f, zm, zadd);
+  comp_grad_parts<LEAN>(c, x, y, z, f, zm, zadd);
   comp_grad_add(f, zm, zadd, x, y, z, gx, gy, gz);
 }
 
+template <bool LEAN = true>
 __device__ __forceinline__ void host_grad(const ssc_host_potential& hp, double x, double y,
                                           double z, double& gx, double& gy, double& gz) {
   gx = 0.0;
   gy = 0.0;
   gz = 0.0;
-  for (int c = 0; c < hp.n_components; ++c) comp_grad(hp.c[c], x, y, z, gx, gy, gz);
+  for (int c = 0; c < hp.n_components; ++c) comp_grad<LEAN>(hp.c[c], x, y, z, gx, gy, gz);
 }
 
 // Host signatures with straight-line gradient code.  The generic gradient walks the components
@@ -184,17 +264,25 @@ inline int host_signature(const ssc_host_potential& hp) {
   return HS_GENERIC;
 }
 
-template <int HS>
+// The fixed-step kernels take the LEAN host forms except with STATIC sources (the clumpy
+// halo): there the 512-term source sum is nearly all of a step, so the host forms' cost does
+// not matter, and the IEEE forms keep the plain orbits -- and with them the surrogate's
+// training positions and its measured error -- as they were measured (that error moves with
+// the last bits of the training data: one training seed went from 2.8 to 19.4 pc when the
+// plain orbits changed only there).
+constexpr bool lean_host(int SK) { return SK != SSC_SRC_STATIC; }
+
+template <int HS, bool LEAN = true>
 __device__ __forceinline__ void host_grad_sig(const ssc_host_potential& hp, double x, double y,
                                               double z, double& gx, double& gy, double& gz) {
   if constexpr (HS == HS_GENERIC) {
-    host_grad(hp, x, y, z, gx, gy, gz);
+    host_grad<LEAN>(hp, x, y, z, gx, gy, gz);
   } else {
     double fm, zm;
-    parts_mn(hp.c[0], x, y, z, fm, zm);
-    const double f1 = parts_hernquist(hp.c[1], x, y, z);
-    const double f2 = HS == HS_MN_HQ_HQ_NFW ? parts_hernquist(hp.c[2], x, y, z) : 0.0;
-    const double fn = parts_nfw(hp.c[HS == HS_MN_HQ_HQ_NFW ? 3 : 2], x, y, z);
+    parts_mn<LEAN>(hp.c[0], x, y, z, fm, zm);
+    const double f1 = parts_hernquist<LEAN>(hp.c[1], x, y, z);
+    const double f2 = HS == HS_MN_HQ_HQ_NFW ? parts_hernquist<LEAN>(hp.c[2], x, y, z) : 0.0;
+    const double fn = parts_nfw<LEAN>(hp.c[HS == HS_MN_HQ_HQ_NFW ? 3 : 2], x, y, z);
     gx = gy = gz = 0.0;
     comp_grad_add(fm, zm, true, x, y, z, gx, gy, gz);
     comp_grad_add(f1, f1, false, x, y, z, gx, gy, gz);

@@ -88,7 +88,7 @@ __device__ __forceinline__ void rhs(const ssc_host_potential& hp, const SourceLD
                                     const GroupSources<SK>& grp, double dir, double tau,
                                     const double* y, double* k) {
   double gx, gy, gz;
-  host_grad_sig<HS>(hp, y[0], y[1], y[2], gx, gy, gz);
+  host_grad_sig<HS, false>(hp, y[0], y[1], y[2], gx, gy, gz);
   if (GR)
     grp.add(dir * tau, y[0], y[1], y[2], gx, gy, gz);
   else

@@ -134,7 +134,7 @@ __global__ __launch_bounds__(SG_BLOCK) void surrogate_leapfrog_kernel(
       if (threadIdx.x == 0) nfb = 0;
     }
     double hx, hy, hz;
-    host_grad(hp, x, y, z, hx, hy, hz);
+    host_grad<false>(hp, x, y, z, hx, hy, hz);
     ax = -(hx + gx);
     ay = -(hy + gy);
     az = -(hz + gz);
