@@ -32,11 +32,11 @@ constexpr int AJ_STORED_SEG = 100;
 // One host component's share: its Hessian accumulated into H, its own gradient and parameter
 // derivative (from the same evaluation) into the partials gh[2c], gh[2c + 1].  K >= 0 fixes
 // the component's kind at compile time.
-template <int K>
+template <int K, bool LEAN>
 __device__ __forceinline__ void adj_comp(const ssc_component& cp, int c, double x, double y,
                                          double z, const double* W, double* H, double* gh) {
   double phic = 0.0, gc[3] = {0.0, 0.0, 0.0}, dp[3];
-  comp_full_as<K>(cp, x, y, z, phic, gc, H, dp);
+  comp_full_as<K, LEAN>(cp, x, y, z, phic, gc, H, dp);
   gh[2 * c] -= (gc[0] * W[0] + gc[1] * W[1] + gc[2] * W[2]) / cp.p[0];
   gh[2 * c + 1] -= dp[0] * W[0] + dp[1] * W[1] + dp[2] * W[2];
 }
@@ -51,12 +51,13 @@ __device__ __forceinline__ void adj_apply(const ssc_host_potential& hp, const So
 #pragma unroll
   for (int r = 0; r < 9; ++r) H[r] = 0.0;
   if constexpr (HS == HS_GENERIC) {
-    for (int c = 0; c < hp.n_components; ++c) adj_comp<-1>(hp.c[c], c, x, y, z, W, H, gh);
+    for (int c = 0; c < hp.n_components; ++c)
+      adj_comp<-1, lean_host(SK)>(hp.c[c], c, x, y, z, W, H, gh);
   } else {
-    adj_comp<hs_kind(HS, 0)>(hp.c[0], 0, x, y, z, W, H, gh);
-    adj_comp<hs_kind(HS, 1)>(hp.c[1], 1, x, y, z, W, H, gh);
-    adj_comp<hs_kind(HS, 2)>(hp.c[2], 2, x, y, z, W, H, gh);
-    if constexpr (hs_components(HS) > 3) adj_comp<hs_kind(HS, 3)>(hp.c[3], 3, x, y, z, W, H, gh);
+    adj_comp<hs_kind(HS, 0), lean_host(SK)>(hp.c[0], 0, x, y, z, W, H, gh);
+    adj_comp<hs_kind(HS, 1), lean_host(SK)>(hp.c[1], 1, x, y, z, W, H, gh);
+    adj_comp<hs_kind(HS, 2), lean_host(SK)>(hp.c[2], 2, x, y, z, W, H, gh);
+    if constexpr (hs_components(HS) > 3) adj_comp<hs_kind(HS, 3), lean_host(SK)>(hp.c[3], 3, x, y, z, W, H, gh);
   }
   H[3] = H[1];
   H[6] = H[2];

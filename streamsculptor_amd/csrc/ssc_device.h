@@ -417,19 +417,133 @@ __device__ __forceinline__ void host_full(const ssc_host_potential& hp, double x
   H[7] = H[5];
 }
 
+// comp_full in the lean forms (see parts_mn): every division and square root of comp_full
+// becomes a product of the refined reciprocal square root of the squared radius and of one
+// refined reciprocal (rsq, recip), and log1p becomes log1p_nonneg.  Same outputs.
+__device__ __forceinline__ void comp_full_lean(const ssc_component& c, double x, double y,
+                                               double z, double& phi, double* g, double* H,
+                                               double* dp = nullptr) {
+  const double gm = c.p[0];
+  switch (c.kind) {
+    case SSC_COMP_MIYAMOTO_NAGAI: {
+      const double a = c.p[1], b = c.p[2];
+      const double v = z * z + b * b;
+      const double iz = rsq(v);  // 1 / zeta
+      const double zeta = v * iz;
+      const double S = a + zeta;
+      const double D2 = x * x + y * y + S * S;
+      const double iD = rsq(D2);
+      const double iD2 = iD * iD;
+      const double iD3 = iD2 * iD;
+      const double iD5 = iD3 * iD2;
+      const double dz = z * S * iz;
+      phi -= gm * iD;
+      g[0] += gm * x * iD3;
+      g[1] += gm * y * iD3;
+      g[2] += gm * dz * iD3;
+      const double k = -3.0 * gm * iD5;
+      const double iz2 = iz * iz;
+      const double ddz = S * b * b * (iz2 * iz) + z * z * iz2;
+      H[0] += k * x * x + gm * iD3;
+      H[1] += k * x * y;
+      H[2] += k * x * dz;
+      H[4] += k * y * y + gm * iD3;
+      H[5] += k * y * dz;
+      H[8] += k * dz * dz + gm * ddz * iD3;
+      if (dp) {
+        dp[0] = -3.0 * gm * x * S * iD5;
+        dp[1] = -3.0 * gm * y * S * iD5;
+        dp[2] = gm * (z * iz) * iD3 - 3.0 * gm * dz * S * iD5;
+      }
+      break;
+    }
+    case SSC_COMP_NFW: {
+      const double rs = c.p[1];
+      const double r2 = x * x + y * y + z * z;
+      const double ir = rsq(r2);
+      const double r = r2 * ir;
+      const double ir2 = ir * ir;
+      const double irs = recip(rs);
+      const double rp = r + rs;
+      const double irp = recip(rp);
+      const double L = log1p_nonneg(r * irs);
+      if (dp) {
+        const double f = gm * (ir * irp) * (irp - irs) * ir;
+        dp[0] = f * x; dp[1] = f * y; dp[2] = f * z;
+      }
+      phi -= gm * L * ir;
+      const double d1 = gm * (L * ir2 - ir * irp);
+      const double d2 = gm * (ir2 * irp - 2.0 * L * ir2 * ir + (2.0 * r + rs) * ir2 * irp * irp);
+      const double d1r = d1 * ir;
+      g[0] += d1r * x;
+      g[1] += d1r * y;
+      g[2] += d1r * z;
+      sph_hess(x, y, z, d1r, (d2 - d1r) * ir2, H);
+      break;
+    }
+    case SSC_COMP_HERNQUIST: {
+      const double cc = c.p[1], soft = c.p[2];
+      const double r2 = x * x + y * y + z * z + soft;
+      const double ir = rsq(r2);
+      const double r = r2 * ir;
+      const double irc = recip(r + cc);
+      const double irc2 = irc * irc;
+      if (dp) {
+        const double f = -2.0 * gm * ir * irc2 * irc;
+        dp[0] = f * x; dp[1] = f * y; dp[2] = f * z;
+      }
+      phi -= gm * irc;
+      const double d1 = gm * irc2;
+      const double d2 = -2.0 * gm * irc2 * irc;
+      const double d1r = d1 * ir;
+      g[0] += d1r * x;
+      g[1] += d1r * y;
+      g[2] += d1r * z;
+      sph_hess(x, y, z, d1r, (d2 - d1r) * (ir * ir), H);
+      break;
+    }
+    case SSC_COMP_PLUMMER: {
+      const double rs = c.p[1];
+      const double s2 = x * x + y * y + z * z + rs * rs;
+      const double is = rsq(s2);
+      const double is2 = is * is;
+      const double i3 = is2 * is;
+      if (dp) {
+        const double f = -3.0 * gm * rs * (i3 * is2);
+        dp[0] = f * x; dp[1] = f * y; dp[2] = f * z;
+      }
+      phi -= gm * is;
+      const double f = gm * i3;
+      g[0] += f * x;
+      g[1] += f * y;
+      g[2] += f * z;
+      const double k = -3.0 * f * is2;
+      H[0] += f + k * x * x;
+      H[1] += k * x * y;
+      H[2] += k * x * z;
+      H[4] += f + k * y * y;
+      H[5] += k * y * z;
+      H[8] += f + k * z * z;
+      break;
+    }
+    default:
+      if (dp) dp[0] = dp[1] = dp[2] = 0.0;
+      break;
+  }
+}
+
 // comp_full with the component's kind fixed at compile time (K >= 0: the kind switch folds
-// away; K < 0: the kind is read from c as in comp_full).
-template <int K>
+// away; K < 0: the kind is read from c as in comp_full); LEAN selects comp_full_lean.
+template <int K, bool LEAN = false>
 __device__ __forceinline__ void comp_full_as(const ssc_component& c, double x, double y, double z,
                                              double& phi, double* g, double* H,
                                              double* dp = nullptr) {
-  if constexpr (K < 0) {
-    comp_full(c, x, y, z, phi, g, H, dp);
-  } else {
-    ssc_component k = c;
-    k.kind = K;
+  ssc_component k = c;
+  if constexpr (K >= 0) k.kind = K;
+  if constexpr (LEAN)
+    comp_full_lean(k, x, y, z, phi, g, H, dp);
+  else
     comp_full(k, x, y, z, phi, g, H, dp);
-  }
 }
 
 // Number of components and kind of component C of a known host signature.
@@ -440,21 +554,30 @@ constexpr int hs_kind(int HS, int C) {
 }
 
 // host_full as straight-line code for a known signature (same accumulation order).
-template <int HS>
+template <int HS, bool LEAN = false>
 __device__ __forceinline__ void host_full_sig(const ssc_host_potential& hp, double x, double y,
                                               double z, double& phi, double* g, double* H) {
-  if constexpr (HS == HS_GENERIC) {
+  if constexpr (HS == HS_GENERIC && !LEAN) {
     host_full(hp, x, y, z, phi, g, H);
+  } else if constexpr (HS == HS_GENERIC) {
+    phi = 0.0;
+    g[0] = g[1] = g[2] = 0.0;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) H[i] = 0.0;
+    for (int c = 0; c < hp.n_components; ++c) comp_full_lean(hp.c[c], x, y, z, phi, g, H);
+    H[3] = H[1];
+    H[6] = H[2];
+    H[7] = H[5];
   } else {
     phi = 0.0;
     g[0] = g[1] = g[2] = 0.0;
 #pragma unroll
     for (int i = 0; i < 9; ++i) H[i] = 0.0;
-    comp_full_as<hs_kind(HS, 0)>(hp.c[0], x, y, z, phi, g, H);
-    comp_full_as<hs_kind(HS, 1)>(hp.c[1], x, y, z, phi, g, H);
-    comp_full_as<hs_kind(HS, 2)>(hp.c[2], x, y, z, phi, g, H);
+    comp_full_as<hs_kind(HS, 0), LEAN>(hp.c[0], x, y, z, phi, g, H);
+    comp_full_as<hs_kind(HS, 1), LEAN>(hp.c[1], x, y, z, phi, g, H);
+    comp_full_as<hs_kind(HS, 2), LEAN>(hp.c[2], x, y, z, phi, g, H);
     if constexpr (hs_components(HS) > 3)
-      comp_full_as<hs_kind(HS, 3)>(hp.c[3], x, y, z, phi, g, H);
+      comp_full_as<hs_kind(HS, 3), LEAN>(hp.c[3], x, y, z, phi, g, H);
     H[3] = H[1];
     H[6] = H[2];
     H[7] = H[5];

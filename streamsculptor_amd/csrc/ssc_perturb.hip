@@ -36,7 +36,7 @@ template <int HS>
 __device__ __forceinline__ void host_grad_hess(const ssc_host_potential& hp, double x, double y,
                                                double z, double* g, double* H) {
   double phi;
-  host_full_sig<HS>(hp, x, y, z, phi, g, H);
+  host_full_sig<HS, true>(hp, x, y, z, phi, g, H);
 }
 
 template <int SK, int G, int RAD, int HS>
