@@ -81,6 +81,22 @@ class LeapfrogAdjoint:
         return dict(lam0=lam0, g_host=gh[:nc], g_gm=gg, g_pos=gp)
 
 
+def _release_vjp_into(pot, prog_w, rel_idx, g_msat, seed, kval_arr, g_pc, g_pf, g_vc, g_vf,
+                      g_prog, g_host, scratch):
+    """ssc_release_vjp into caller-provided outputs (contiguous device tensors; scratch of
+    at least (nrel, 2 n_components) doubles)."""
+    h, _ = pot._descriptor()
+    kv = np.asarray(kval_arr, dtype=np.float64)
+    kvp = None
+    if kv.ndim > 0:
+        kv = np.ascontiguousarray(np.ones(8) * kv)
+        kvp = kv.ctypes.data_as(_lib.ctypes.c_void_p)
+    _lib.call("ssc_release_vjp", _lib.ctypes.byref(h), _lib.ptr(prog_w), _lib.ptr(rel_idx),
+              _lib.ptr(g_msat), prog_w.shape[0], int(seed), kvp, _lib.ptr(g_pc), _lib.ptr(g_pf),
+              _lib.ptr(g_vc), _lib.ptr(g_vf), _lib.ptr(g_prog), _lib.ptr(g_host),
+              _lib.ptr(scratch))
+
+
 def release_vjp(pot, prog_w, rel_idx, g_msat, seed, g_pc, g_pf, g_vc, g_vf, kval_arr=1.0):
     """dL/d(progenitor rows) (nrel, 6) and dL/d(host G m, p1) (n_comp, 2) of the release model
     in the analytic potential ``pot`` (the release of the perturbed streams, reference
@@ -165,14 +181,29 @@ class StreamModel:
         from .workloads import spray_layout
         even, odd = spray_layout(self.per_step)
         T = self.nsteps
-        sel_s = np.empty((T, self.per_step), dtype=np.int64)
-        sel_a = np.empty((T, self.per_step), dtype=np.int64)
-        for i in range(T):
-            lay = even if i % 2 == 0 else odd
-            sel_s[i] = [p[0] for p in lay]
-            sel_a[i] = [p[1] for p in lay]
+        lay = np.array([even, odd], dtype=np.int64)      # (parity, per_step, [seed, arm])
+        par = np.arange(T) % 2
         rel = np.repeat(np.arange(T), self.per_step)
-        return sel_s.reshape(-1), sel_a.reshape(-1), rel
+        return lay[par, :, 0].reshape(-1), lay[par, :, 1].reshape(-1), rel
+
+    def _device_layout(self, dev):
+        """The spray layout's index tensors and the release masses on the device, made once
+        (every loss evaluation uses the same)."""
+        key = str(dev)
+        if getattr(self, "_dl_key", None) != key:
+            T = self.nsteps
+            sel_s, sel_a, rel = self.layout()
+            si, ai, ri = (torch.as_tensor(a, device=dev) for a in (sel_s, sel_a, rel))
+            # rows of (seed, arm, release) in the backward pass's adjoint table
+            # [seed][arm][x|v][row][3] viewed as rows of 3: positions, and velocities T + 1 on
+            flat = (si * 4 + ai * 2) * (T + 1) + ri
+            self._dl = dict(si=si, ai=ai, ri=ri, flat_x=flat, flat_v=flat + (T + 1),
+                            start=_lib.i32(rel.astype(np.int32)),
+                            idx=torch.arange(T + 1, dtype=torch.int32, device=dev),
+                            gms=torch.full((T + 1,), G * self.msat, dtype=torch.float64,
+                                           device=dev))
+            self._dl_key = key
+        return self._dl
 
     def forward(self, m_halo, rs_halo, sub_masses, prog_w0):
         """Final particle rows and everything the backward pass needs."""
@@ -185,16 +216,16 @@ class StreamModel:
         prog_rows = pys[:, 0, :].contiguous()
         nseed = (self.per_step + 1) // 2
         dev = prog_rows.device
+        dl = self._device_layout(dev)
         table = torch.empty(nseed, 2, T + 1, 6, dtype=torch.float64, device=dev)
         for s in range(nseed):
-            pc, pf, vc, vf = host._release_along(prog_rows, self.t0 + self.dt * np.arange(T + 1),
-                                                 self.msat, self.seed + s, self.kval_arr)
+            pc, pf, vc, vf = host._release(prog_rows, dl["idx"], dl["gms"], self.seed + s,
+                                           self.kval_arr)
             table[s, 0, :, :3], table[s, 0, :, 3:] = pc, vc
             table[s, 1, :, :3], table[s, 1, :, 3:] = pf, vf
-        sel_s, sel_a, rel = self.layout()
-        si, ai, ri = (torch.as_tensor(a, device=dev) for a in (sel_s, sel_a, rel))
+        si, ai, ri = dl["si"], dl["ai"], dl["ri"]
         w0 = table[si, ai, ri].contiguous()
-        start = rel.astype(np.int32)
+        start = dl["start"]
         padj = LeapfrogAdjoint(total, w0, self.t0, self.dt, T, start=start,
                                checkpoint_every=self.E, src=src)
         wT = padj.forward()
@@ -231,19 +262,17 @@ class StreamModel:
         dev = lam_T.device
         M = self.sub_w0.shape[0]
         nc = max(host._descriptor()[0].n_components, 1)
-        si, ai, ri = st["idx"]
-        flat = (si * 2 + ai) * (T + 1) + ri
+        dl = self._device_layout(dev)
         ps = self.per_step
         # accumulators (current stream)
         g_host = torch.zeros(nc, 2, dtype=torch.float64, device=dev)
         g_gm = torch.zeros(M, dtype=torch.float64, device=dev)
-        g_table = torch.zeros(st["nseed"], 2, T + 1, 6, dtype=torch.float64, device=dev)
+        g_table = torch.zeros(st["nseed"], 2, 2, T + 1, 3, dtype=torch.float64, device=dev)
         g_prog_rows = torch.zeros(T + 1, 1, 6, dtype=torch.float64, device=dev)
         F = torch.zeros(T + 1, M, 6, dtype=torch.float64, device=dev)
         side_host = torch.zeros_like(g_host)
         side_gm = torch.zeros_like(g_gm)
-        gms = torch.full((T + 1,), G * self.msat, dtype=torch.float64, device=dev)
-        rel_all = torch.arange(T + 1, dtype=torch.int32, device=dev)
+        gms, rel_all = dl["gms"], dl["idx"]
         z1 = torch.zeros(1, 6, dtype=torch.float64, device=dev)
         zM = torch.zeros(M, 6, dtype=torch.float64, device=dev)
         main = torch.cuda.current_stream(dev)
@@ -261,16 +290,26 @@ class StreamModel:
             top = hi_ + 1 if hi_ == T else hi_
             with torch.cuda.stream(side):
                 side.wait_event(done)
-                # releases at rows [lo, hi): the particles in that slice (sorted by release row)
+                # releases at rows [lo, hi): the particles in that slice (sorted by release row);
+                # the adjoint table keeps positions and velocities apart, [seed][arm][x|v][row],
+                # so each seed's four VJP inputs are contiguous slices
                 a, b = lo * ps, min(hi_, T) * ps
-                g_table.view(-1, 6).index_add_(0, flat[a:b], A["lam0"][a:b])
+                lam0 = A["lam0"][a:b]
+                g_table.view(-1, 3).index_add_(0, dl["flat_x"][a:b], lam0[:, :3])
+                g_table.view(-1, 3).index_add_(0, dl["flat_v"][a:b], lam0[:, 3:])
+                nr = hi_ - lo
+                gp_all = torch.empty(st["nseed"], nr, 6, dtype=torch.float64, device=dev)
+                gh_all = torch.empty(st["nseed"], nc, 2, dtype=torch.float64, device=dev)
+                scratch = torch.empty(nr, 2 * nc, dtype=torch.float64, device=dev)
+                pw = st["prog_rows"][lo:hi_]
                 for s in range(st["nseed"]):
-                    gp, gh = release_vjp(host, st["prog_rows"][lo:hi_], rel_all[lo:hi_],
-                                         gms[lo:hi_], self.seed + s, g_table[s, 0, lo:hi_, :3],
-                                         g_table[s, 1, lo:hi_, :3], g_table[s, 0, lo:hi_, 3:],
-                                         g_table[s, 1, lo:hi_, 3:], self.kval_arr)
-                    g_prog_rows[lo:hi_, 0] += gp
-                    side_host += gh
+                    _release_vjp_into(host, pw, rel_all[lo:hi_], gms[lo:hi_], self.seed + s,
+                                      self.kval_arr, g_table[s, 0, 0, lo:hi_],
+                                      g_table[s, 1, 0, lo:hi_], g_table[s, 0, 1, lo:hi_],
+                                      g_table[s, 1, 1, lo:hi_], gp_all[s], gh_all[s], scratch)
+                g_prog_rows[lo:hi_, 0] += gp_all.sum(0)
+                side_host += gh_all.sum(0)
+                keep.append((gp_all, gh_all, scratch))
                 # progenitor orbit (in host + subhalos) over the same rows
                 C = st["pa"].backward(z1, forcing=g_prog_rows, rows=(lo, hi_))
                 side_host += C["g_host"]
