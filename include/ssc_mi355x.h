@@ -354,6 +354,14 @@ int ssc_mlp_loss_backward(const float* params, const float* x, const float* targ
                           float* grads, float* loss, void* workspace, int64_t workspace_bytes,
                           void* stream);
 
+/* ssc_mlp_loss_backward with the weight sum read from device memory (weight_sum: one float on
+ * the device), so a training loop need not bring each minibatch's sum to the host -- a copy
+ * that waits for the stream and leaves the launches of every iteration exposed. */
+int ssc_mlp_loss_backward_dev(const float* params, const float* x, const float* target,
+                              const float* weight, const float* weight_sum, int64_t batch,
+                              float* y, float* grads, float* loss, void* workspace,
+                              int64_t workspace_bytes, void* stream);
+
 /* Gradient of sum_b dy_b . y_b with respect to the parameters (vector-Jacobian product). */
 int ssc_mlp_backward(const float* params, const float* x, const float* dy, int64_t batch,
                      float* grads, void* workspace, int64_t workspace_bytes, void* stream);

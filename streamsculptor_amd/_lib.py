@@ -62,6 +62,7 @@ SIGNATURES = {
                            _vp, _vp, _vp],
     "ssc_mlp_forward": [_vp, _vp, _i64, _vp, _vp, _i64],
     "ssc_mlp_loss_backward": [_vp, _vp, _vp, _vp, ctypes.c_float, _i64, _vp, _vp, _vp, _vp, _i64],
+    "ssc_mlp_loss_backward_dev": [_vp, _vp, _vp, _vp, _vp, _i64, _vp, _vp, _vp, _vp, _i64],
     "ssc_mlp_backward": [_vp, _vp, _vp, _i64, _vp, _vp, _i64],
     "ssc_mlp_adam": [_vp, _vp, _vp, _vp, _i64, ctypes.c_float, ctypes.c_float, ctypes.c_float,
                      ctypes.c_float, _i32, ctypes.c_float],

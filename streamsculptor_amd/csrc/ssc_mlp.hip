@@ -125,14 +125,17 @@ __global__ void act_bwd_kernel(float* __restrict__ dA, const float* __restrict__
 }
 
 // Weighted squared error of the 4 outputs: loss = sum_b w_b |Y_b - T_b|^2 / sum_b w_b;
-// dY = 2 w_b (Y_b - T_b) / sum_b w_b (fp32 and BF16); per-block partial losses.
+// dY = 2 w_b (Y_b - T_b) / sum_b w_b (fp32 and BF16); per-block partial losses.  The weight sum
+// comes as 1/sum (inv_wsum) or, when wsum_dev is set, as the sum itself in device memory.
 __global__ __launch_bounds__(256) void mse_kernel(const float* __restrict__ Y,
                                                   const float* __restrict__ T,
                                                   const float* __restrict__ wt, float inv_wsum,
+                                                  const float* __restrict__ wsum_dev,
                                                   int64_t nb, float* __restrict__ dY,
                                                   uint16_t* __restrict__ dYb,
                                                   float* __restrict__ partial) {
   __shared__ float red[256];
+  if (wsum_dev) inv_wsum = 1.0f / *wsum_dev;
   const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   float acc = 0.0f;
   if (b < nb) {
@@ -492,24 +495,44 @@ int ssc_mlp_forward(const float* params, const float* x, int64_t batch, float* y
   return check_launch("ssc_mlp_forward");
 }
 
-int ssc_mlp_loss_backward(const float* params, const float* x, const float* target,
-                          const float* weight, float weight_sum, int64_t batch, float* y,
-                          float* grads, float* loss, void* workspace, int64_t workspace_bytes,
-                          void* stream) {
+static int loss_backward(const char* name, const float* params, const float* x,
+                         const float* target, const float* weight, float weight_sum,
+                         const float* weight_sum_dev, int64_t batch, float* y, float* grads,
+                         float* loss, void* workspace, int64_t workspace_bytes, void* stream) {
   if (!params || !x || !target || !y || !grads || !loss || !workspace || batch <= 0)
-    return fail(SSC_E_ARG, "ssc_mlp_loss_backward: NULL pointer or empty batch");
-  if (batch > (1 << 24)) return fail(SSC_E_LIMIT, "ssc_mlp_loss_backward: batch above 16M");
-  if (workspace_bytes < ws_bytes(batch)) return fail(SSC_E_ARG, "ssc_mlp_loss_backward: workspace too small");
-  if (!(weight_sum > 0.0f)) return fail(SSC_E_ARG, "ssc_mlp_loss_backward: weight_sum must be > 0");
+    return fail(SSC_E_ARG, std::string(name) + ": NULL pointer or empty batch");
+  if (batch > (1 << 24)) return fail(SSC_E_LIMIT, std::string(name) + ": batch above 16M");
+  if (workspace_bytes < ws_bytes(batch))
+    return fail(SSC_E_ARG, std::string(name) + ": workspace too small");
+  if (!weight_sum_dev && !(weight_sum > 0.0f))
+    return fail(SSC_E_ARG, std::string(name) + ": weight_sum must be > 0");
   hipStream_t st = (hipStream_t)stream;
   Workspace w = carve(workspace, batch);
   forward(params, x, batch, y, w, st);
   const unsigned nbk = nblk(batch);
-  hipLaunchKernelGGL(mse_kernel, dim3(nbk), dim3(256), 0, st, y, target, weight, 1.0f / weight_sum,
-                     batch, w.dY, w.dYb, w.red);
+  hipLaunchKernelGGL(mse_kernel, dim3(nbk), dim3(256), 0, st, y, target, weight,
+                     weight_sum_dev ? 0.0f : 1.0f / weight_sum, weight_sum_dev, batch, w.dY,
+                     w.dYb, w.red);
   hipLaunchKernelGGL(sum_kernel, dim3(1), dim3(256), 0, st, w.red, (int)nbk, loss);
   backward(x, batch, grads, w, st);
-  return check_launch("ssc_mlp_loss_backward");
+  return check_launch(name);
+}
+
+int ssc_mlp_loss_backward(const float* params, const float* x, const float* target,
+                          const float* weight, float weight_sum, int64_t batch, float* y,
+                          float* grads, float* loss, void* workspace, int64_t workspace_bytes,
+                          void* stream) {
+  return loss_backward("ssc_mlp_loss_backward", params, x, target, weight, weight_sum, nullptr,
+                       batch, y, grads, loss, workspace, workspace_bytes, stream);
+}
+
+int ssc_mlp_loss_backward_dev(const float* params, const float* x, const float* target,
+                              const float* weight, const float* weight_sum, int64_t batch,
+                              float* y, float* grads, float* loss, void* workspace,
+                              int64_t workspace_bytes, void* stream) {
+  if (!weight_sum) return fail(SSC_E_ARG, "ssc_mlp_loss_backward_dev: NULL weight_sum");
+  return loss_backward("ssc_mlp_loss_backward_dev", params, x, target, weight, 0.0f,
+                       weight_sum, batch, y, grads, loss, workspace, workspace_bytes, stream);
 }
 
 int ssc_mlp_backward(const float* params, const float* x, const float* dy, int64_t batch,

@@ -68,18 +68,24 @@ class MLP:
         return y
 
     def loss_backward(self, x, target, weight=None):
-        """Weighted squared error and its gradient (into self.grads).  Returns (loss, y)."""
+        """Weighted squared error and its gradient (into self.grads).  Returns (loss, y).  The
+        minibatch's weight sum stays on the device (no host round trip per iteration)."""
         x = _f32(x).reshape(-1, IN)
         t = _f32(target).reshape(-1, OUT)
         n = x.shape[0]
-        w = None if weight is None else _f32(weight).reshape(-1)
-        wsum = float(n if w is None else w.sum().item())
         y = torch.empty(n, OUT, dtype=torch.float32, device=x.device)
         loss = torch.empty(1, dtype=torch.float32, device=x.device)
         ws = self._workspace(n)
-        _lib.call("ssc_mlp_loss_backward", _lib.ptr(self.params), _lib.ptr(x), _lib.ptr(t),
-                  _lib.ptr(w), ctypes.c_float(wsum), n, _lib.ptr(y), _lib.ptr(self.grads),
-                  _lib.ptr(loss), _lib.ptr(ws), ws.numel())
+        if weight is None:
+            _lib.call("ssc_mlp_loss_backward", _lib.ptr(self.params), _lib.ptr(x), _lib.ptr(t),
+                      None, ctypes.c_float(float(n)), n, _lib.ptr(y), _lib.ptr(self.grads),
+                      _lib.ptr(loss), _lib.ptr(ws), ws.numel())
+        else:
+            w = _f32(weight).reshape(-1)
+            wsum = w.sum().reshape(1)
+            _lib.call("ssc_mlp_loss_backward_dev", _lib.ptr(self.params), _lib.ptr(x),
+                      _lib.ptr(t), _lib.ptr(w), _lib.ptr(wsum), n, _lib.ptr(y),
+                      _lib.ptr(self.grads), _lib.ptr(loss), _lib.ptr(ws), ws.numel())
         return loss, y
 
     def vjp(self, x, dy):

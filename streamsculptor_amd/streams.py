@@ -194,19 +194,22 @@ def spray_particles(prog_pot, release_pot, ts, prog_w0, Msat, seed_num, per_step
     even, odd = spray_layout(per_step)
     nseed = (per_step + 1) // 2
     dev = ws.device
-    # table[s, arm, i, 6]
+    from .main import _as_float
+    # table[s, arm, i, 6]; the release masses and indices go to the device once for all seeds
+    # (a copy from pageable host memory waits for the stream)
     table = torch.empty(nseed, 2, T + 1, 6, dtype=torch.float64, device=dev)
+    msat = np.broadcast_to(np.asarray(_as_float(Msat), dtype=np.float64), (T + 1,))
+    gms = release_pot._G * _lib.f64(np.ascontiguousarray(msat))
+    idx = torch.arange(T + 1, dtype=torch.int32, device=dev)
     for s in range(nseed):
-        pc, pf, vc, vf = release_pot._release_along(ws, ts_np, Msat, seed_num + s, kval_arr)
+        pc, pf, vc, vf = release_pot._release(ws, idx, gms, seed_num + s, kval_arr, t_rel=ts_np)
         table[s, 0, :, :3], table[s, 0, :, 3:] = pc, vc
         table[s, 1, :, :3], table[s, 1, :, 3:] = pf, vf
     lay = np.array([even, odd], dtype=np.int64)            # (parity, per_step, [seed, arm])
     par = np.arange(T) % 2
     sel_s, sel_a = lay[par, :, 0], lay[par, :, 1]
     rel = np.repeat(np.arange(T), per_step)
-    s_idx = torch.as_tensor(sel_s.reshape(-1), device=dev)
-    a_idx = torch.as_tensor(sel_a.reshape(-1), device=dev)
-    r_idx = torch.as_tensor(rel, device=dev)
-    w0 = table[s_idx, a_idx, r_idx].contiguous()
+    sar = torch.as_tensor(np.stack([sel_s.reshape(-1), sel_a.reshape(-1), rel]), device=dev)
+    w0 = table[sar[0], sar[1], sar[2]].contiguous()
     start = kidx[rel].astype(np.int32)
     return w0, start, sel_a.reshape(-1), (t0, dt, nsteps, kidx)

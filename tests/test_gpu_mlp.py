@@ -65,6 +65,28 @@ def test_loss_and_gradients_match_oracle(ssc):
     assert np.linalg.norm(gv - go) / np.linalg.norm(go) < 3e-3
 
 
+def test_device_weight_sum_equals_host_weight_sum(ssc):
+    """ssc_mlp_loss_backward_dev (the minibatch weight sum read from device memory) and
+    ssc_mlp_loss_backward (the same sum passed by value) give the same loss and gradients,
+    bit for bit: a training loop can drop the per-iteration host round trip."""
+    import ctypes
+    from streamsculptor_amd import _lib
+    from streamsculptor_amd.mlp import MLP
+    p, x, t, w = _case(n=4096, seed=2)
+    net = MLP(params=p)
+    loss_d, y_d = net.loss_backward(x, t, w)
+    g_d = net.grads.clone()
+    xt, tt, wt = (torch.as_tensor(a, device="cuda") for a in (x, t, w))
+    wsum = float(wt.sum().item())
+    y = torch.empty_like(y_d)
+    loss = torch.empty_like(loss_d)
+    ws = net._workspace(4096)
+    _lib.call("ssc_mlp_loss_backward", _lib.ptr(net.params), _lib.ptr(xt), _lib.ptr(tt),
+              _lib.ptr(wt), ctypes.c_float(wsum), 4096, _lib.ptr(y), _lib.ptr(net.grads),
+              _lib.ptr(loss), _lib.ptr(ws), ws.numel())
+    assert torch.equal(loss, loss_d) and torch.equal(y, y_d) and torch.equal(net.grads, g_d)
+
+
 def test_adam_matches_oracle(ssc):
     from streamsculptor_amd.mlp import MLP
     rng = np.random.default_rng(5)
