@@ -137,6 +137,127 @@ __device__ __forceinline__ void mlp_tile_forward(Weights& wsrc, const float* __r
   for (int j = 0; j < OUT; ++j) y[j] = out[j];
 }
 
+// Layer-1 table for mlp_tiles_forward: slot (4 j + g) 8 + e holds (W1[n][0], W1[n][1], W1[n][2],
+// b1[n]) of neuron n = kperm(32 j + 8 g + e), i.e. lane group g reads its 64 neurons as 64
+// consecutive float4 (one broadcast LDS read per neuron instead of four scattered ones).
+// Filled cooperatively by the threads of a block (tid, nthreads).
+__device__ __forceinline__ void build_l1_table(const float* __restrict__ pk32, float4* tab, int tid,
+                                               int nthreads) {
+  using namespace mlp;
+  for (int s = tid; s < H; s += nthreads) {
+    const int n = kperm(s);  // s = 32 j + 8 g + e
+    tab[s] = make_float4(pk32[PK_W1 + 3 * n], pk32[PK_W1 + 3 * n + 1], pk32[PK_W1 + 3 * n + 2],
+                         pk32[PK_B1 + n]);
+  }
+}
+
+// NT tiles of 16 points per wave (point t of lane l is point l % 16 of tile t), every weight
+// fragment read once from the weight source and applied to all NT tiles: the matrix-core
+// operand traffic per point drops NT-fold while each point's arithmetic -- the order of every
+// FMA and MFMA accumulation -- is that of mlp_tile_forward, so the outputs are bit for bit the
+// same.  Layer 1 reads the table of build_l1_table.  xin[t]: inputs of point t (every lane of
+// the column); y[t][0..3]: outputs of point t, valid in lane group 0.
+template <int NT, class Weights>
+__device__ __forceinline__ void mlp_tiles_forward(Weights& wsrc, const float* __restrict__ pk32,
+                                                  const float4* __restrict__ l1tab,
+                                                  const float (&xin)[NT][3], float (&y)[NT][4]) {
+  using namespace mlp;
+  constexpr int KB = H / 32;
+  int l = threadIdx.x & 63;
+  asm volatile("" : "+v"(l));
+  const int g = l >> 4;
+  const float4* tab = l1tab + 8 * g;
+  bf16x8 bin[NT][KB];
+#pragma unroll
+  for (int j = 0; j < KB; ++j) {
+    float v[NT][8];
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      const float4 w = tab[32 * j + e];
+#pragma unroll
+      for (int t = 0; t < NT; ++t) {
+        float z = w.w;
+        z = fmaf(xin[t][0], w.x, z);
+        z = fmaf(xin[t][1], w.y, z);
+        z = fmaf(xin[t][2], w.z, z);
+        v[t][e] = silu(z);
+      }
+    }
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+      bin[t][j] = pack8(v[t]);
+      pin(bin[t][j]);
+    }
+  }
+#pragma unroll
+  for (int layer = 0; layer < 2; ++layer) {
+    const float* bias = pk32 + (layer == 0 ? PK_B2 : PK_B3);
+    bf16x8 bout[NT][KB];
+#pragma unroll
+    for (int qc = 0; qc < Weights::NQ; ++qc) {
+      const bf16x8* W = wsrc.chunk(Weights::NQ * layer + qc);
+#pragma unroll
+      for (int jj = 0; jj < KB / Weights::NQ; ++jj) {
+        const int j = (KB / Weights::NQ) * qc + jj;
+        f32x4 a0[NT], a1[NT];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const float b0 = bias[32 * j + 4 * g + r], b1 = bias[32 * j + 16 + 4 * g + r];
+#pragma unroll
+          for (int t = 0; t < NT; ++t) {
+            a0[t][r] = b0;
+            a1[t][r] = b1;
+          }
+        }
+#pragma unroll
+        for (int q = 0; q < KB; ++q) {
+          const bf16x8 A0 = W[((2 * jj) * KB + q) * 64 + l];
+          const bf16x8 A1 = W[((2 * jj + 1) * KB + q) * 64 + l];
+#pragma unroll
+          for (int t = 0; t < NT; ++t) {
+            a0[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A0, bin[t][q], a0[t], 0, 0, 0);
+            a1[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A1, bin[t][q], a1[t], 0, 0, 0);
+          }
+        }
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+          float v[8];
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            v[r] = silu(a0[t][r]);
+            v[4 + r] = silu(a1[t][r]);
+          }
+          bout[t][j] = pack8(v);
+          pin(bout[t][j]);
+        }
+      }
+    }
+#pragma unroll
+    for (int t = 0; t < NT; ++t)
+#pragma unroll
+      for (int q = 0; q < KB; ++q) bin[t][q] = bout[t][q];
+  }
+  const bf16x8* W4 = wsrc.out();
+  f32x4 out[NT];
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const float b = pk32[PK_B4 + 4 * g + r];
+#pragma unroll
+    for (int t = 0; t < NT; ++t) out[t][r] = b;
+  }
+#pragma unroll
+  for (int q = 0; q < KB; ++q) {
+    const bf16x8 A = W4[q * 64 + l];
+#pragma unroll
+    for (int t = 0; t < NT; ++t)
+      out[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, bin[t][q], out[t], 0, 0, 0);
+  }
+#pragma unroll
+  for (int t = 0; t < NT; ++t)
+#pragma unroll
+    for (int j = 0; j < OUT; ++j) y[t][j] = out[t][j];
+}
+
 // Input vector-Jacobian product dx = d(dy . MLP(x)) / dx for the 16 points of a wave, together
 // with the forward output y (same lane layout as mlp_tile_forward: dy[0..3] of point l % 16 in
 // every lane of its column; y valid in lane group 0, dx in every lane of the column).  The

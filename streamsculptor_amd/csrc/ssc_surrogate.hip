@@ -1,13 +1,14 @@
 // Leapfrog with the learned clump-force surrogate (gfx950).
 //
-// A block of 8 waves (two per SIMD, so one wave's MFMAs overlap the other's activations)
-// integrates 128 particles for the whole grid.  The FP64 state of particle p lives in the four
-// lanes p + 16 g of its wave (all four compute the same update, so the MLP's 64-lane MFMA
-// operands need no exchange).  Every step:
+// A block of 12 waves (three per SIMD, so one wave's MFMAs overlap another's activations)
+// integrates 384 particles for the whole grid.  A wave holds 32 particles as two MLP point
+// tiles of 16: the FP64 state of particle (tile t, column c) lives in lanes c + 32 t and
+// c + 32 t + 16, and every lane of column c takes part in both tiles' MFMAs, so each weight
+// fragment read from LDS feeds two MFMAs.  Every step:
 //   kick + drift (FP64) -> MLP(x) on the matrix cores: the two 256x256 layers and the output
 //   layer run as BF16 v_mfma_f32_16x16x32 with fp32 accumulation, the activations chained in
-//   VGPRs and each layer's weight fragments staged once per block in LDS -> clump force =
-//   g_scale * MLP
+//   VGPRs and each layer's weight fragments streamed through LDS in chunks of 32 KB -> clump
+//   force = g_scale * MLP
 //   -> fallback: particles in flagged cells (or outside the training box) are compacted into an
 //   LDS list and get the exact clump sum, one wave per particle with the clumps split over
 //   lanes -> host force (FP64, analytic) -> kick.
@@ -23,9 +24,18 @@
 
 namespace ssc {
 
-// Block shape: 12 waves (three per SIMD, 168-register budget each), 16 particles per wave.
-constexpr int SG_BLOCK_THREADS = 768;
+// Block shape: 12 waves (three per SIMD, 168-register budget each), 32 particles per wave.
+constexpr int SG_BLOCK_THREADS = 512;
+// Chunks per chained 256x256 layer in the LDS weight stream.
+constexpr int SG_NQ = 2;
 
+// Dynamic LDS of the integrator: two weight-chunk buffers, the output-layer fragments, the fp32
+// section of the packed network (W1, biases) and the layer-1 table.
+template <int SG_BLOCK>
+constexpr size_t surrogate_lds_bytes() {
+  return (size_t)2 * LdsWeights<SG_BLOCK, false, SG_NQ>::CH * 16 + (size_t)mlp::FRAG_4 * 2 +
+         (size_t)mlp::PK_F32_END * 4 + (size_t)mlp::H * 16;
+}
 
 // CK: also write checkpoint rows ckpt[r][n][9] = (x, v, a) at grid rows min(r E, nsteps),
 // r = 0..ceil(nsteps / E) (a particle that has not started holds its initial row and the
@@ -38,21 +48,24 @@ __global__ __launch_bounds__(SG_BLOCK) void surrogate_leapfrog_kernel(
     double dt, int nsteps, double* __restrict__ w_out, unsigned long long* __restrict__ counters,
     int E, double* __restrict__ ckpt) {
   using namespace mlp;
-  constexpr int SG_PARTS = SG_BLOCK / 4;
+  using LW = LdsWeights<SG_BLOCK, false, SG_NQ>;
+  constexpr int SG_PARTS = SG_BLOCK / 2;
   extern __shared__ __align__(16) char lds_raw[];
   __shared__ int kmin_sh;
   __shared__ int nfb;
   __shared__ double fb_buf[SG_PARTS][3];  // fallback positions, overwritten by their forces
 
   bf16x8* wbuf = reinterpret_cast<bf16x8*>(lds_raw);
-  bf16x8* w4 = reinterpret_cast<bf16x8*>(lds_raw + FRAG_L * 2);
-  float* pk32 = reinterpret_cast<float*>(lds_raw + FRAG_L * 2 + FRAG_4 * 2);
+  bf16x8* w4 = reinterpret_cast<bf16x8*>(lds_raw + 2 * LW::CH * 16);
+  float* pk32 = reinterpret_cast<float*>(lds_raw + 2 * LW::CH * 16 + FRAG_4 * 2);
+  float4* l1tab = reinterpret_cast<float4*>(pk32 + PK_F32_END);  // 16-byte aligned offset
 
   const int l = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int pl = wv * 16 + (l & 15);
+  const int tl = l >> 5;  // MLP point tile of this lane's particle
+  const int pl = wv * 32 + tl * 16 + (l & 15);
   const int64_t i = (int64_t)blockIdx.x * SG_PARTS + pl;
   const bool valid = i < n;
-  const bool owner = (l < 16) && valid;
+  const bool owner = ((l & 16) == 0) && valid;  // lanes l and l + 16 hold the same particle
 
   double x = 10.0, y = 0.0, z = 0.0, vx = 0.0, vy = 0.0, vz = 0.0;
   int s = nsteps;
@@ -66,19 +79,19 @@ __global__ __launch_bounds__(SG_BLOCK) void surrogate_leapfrog_kernel(
     kmin_sh = INT_MAX;
     nfb = 0;
   }
-  // resident pieces: output-layer fragments, W1 and biases
+  // resident pieces: output-layer fragments, W1 and biases, the layer-1 table
   {
     const bf16x8* g4 = reinterpret_cast<const bf16x8*>(pk16 + PK_W4);
     for (int e = threadIdx.x; e < (int)(FRAG_4 / 8); e += SG_BLOCK) w4[e] = g4[e];
     for (int e = threadIdx.x; e < (int)PK_F32_END; e += SG_BLOCK) pk32[e] = pk32g[e];
+    build_l1_table(pk32g, l1tab, threadIdx.x, SG_BLOCK);
   }
   __syncthreads();
   atomicMin(&kmin_sh, s);
   __syncthreads();
   const int kmin = kmin_sh;
 
-  LdsWeights<SG_BLOCK, false> lw{wbuf, w4, nullptr, reinterpret_cast<const bf16x8*>(pk16 + PK_W2),
-                                 nullptr, {}};
+  LW lw{wbuf, w4, nullptr, reinterpret_cast<const bf16x8*>(pk16 + PK_W2), nullptr, {}};
   lw.prefetch(0);
   unsigned long long nfall = 0;
 
@@ -87,24 +100,32 @@ __global__ __launch_bounds__(SG_BLOCK) void surrogate_leapfrog_kernel(
     const float u0 = (float)((x - sd.center[0]) * sd.inv_half[0]);
     const float u1 = (float)((y - sd.center[1]) * sd.inv_half[1]);
     const float u2 = (float)((z - sd.center[2]) * sd.inv_half[2]);
-    float o[4];
-    mlp_tile_forward(lw, pk32, u0, u1, u2, o);
-    // outputs live in lane group 0; copy them to the particle's other three lanes
-    const float o1 = __shfl(o[1], l & 15, 64), o2 = __shfl(o[2], l & 15, 64),
-                o3 = __shfl(o[3], l & 15, 64);
-    double gx = sd.g_scale * (double)o1, gy = sd.g_scale * (double)o2, gz = sd.g_scale * (double)o3;
+    // the other tile's particle of this column sits 32 lanes away
+    const float p0 = __shfl_xor(u0, 32, 64), p1 = __shfl_xor(u1, 32, 64),
+                p2 = __shfl_xor(u2, 32, 64);
+    const float xin[2][3] = {{tl ? p0 : u0, tl ? p1 : u1, tl ? p2 : u2},
+                             {tl ? u0 : p0, tl ? u1 : p1, tl ? u2 : p2}};
+    float o[2][4];
+    mlp_tiles_forward<2>(lw, pk32, l1tab, xin, o);
+    // both tiles' outputs live in lane group 0: take this lane's tile from lane l % 16
+    const int src = l & 15;
+    const float a1 = __shfl(o[0][1], src, 64), b1 = __shfl(o[1][1], src, 64);
+    const float a2 = __shfl(o[0][2], src, 64), b2 = __shfl(o[1][2], src, 64);
+    const float a3 = __shfl(o[0][3], src, 64), b3 = __shfl(o[1][3], src, 64);
+    double gx = sd.g_scale * (double)(tl ? b1 : a1), gy = sd.g_scale * (double)(tl ? b2 : a2),
+           gz = sd.g_scale * (double)(tl ? b3 : a3);
     // fallback test: outside the training box or a flagged cell
     const bool fb = valid && surrogate_fallback(sd, flags, x, y, z);
     if (__syncthreads_or(fb)) {
       int slot = -1;
-      if (fb && l < 16) {
+      if (fb && (l & 16) == 0) {
         slot = atomicAdd(&nfb, 1);
         fb_buf[slot][0] = x;
         fb_buf[slot][1] = y;
         fb_buf[slot][2] = z;
         ++nfall;
       }
-      slot = __shfl(slot, l & 15, 64);
+      slot = __shfl(slot, l & ~16, 64);
       __syncthreads();
       const int cnt = nfb;
       for (int f = wv; f < cnt; f += SG_BLOCK / 64) {
@@ -198,16 +219,16 @@ int launch_surrogate_leapfrog(const ssc_host_potential* host, const ssc_sources*
   SurrogateDesc sd;
   rc = parse_scaling(scaling, sd, who);
   if (rc) return rc;
-  const size_t lds = (size_t)(mlp::FRAG_L + mlp::FRAG_4) * 2 + (size_t)mlp::PK_F32_END * 4;
+  constexpr int B = SG_BLOCK_THREADS;
+  const size_t lds = surrogate_lds_bytes<B>();
   const uint16_t* pk16 = (const uint16_t*)packed;
   const float* pk32 = (const float*)((const char*)packed + mlp::PK_F32_BASE_BYTES);
-  constexpr int B = SG_BLOCK_THREADS;
   const void* kern = ckpt ? (const void*)surrogate_leapfrog_kernel<B, true>
                           : (const void*)surrogate_leapfrog_kernel<B, false>;
   hipError_t e = hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (e != hipSuccess)
     return fail(SSC_E_HIP, std::string("hipFuncSetAttribute: ") + hipGetErrorString(e));
-  const int64_t parts = B / 4;
+  const int64_t parts = B / 2;
   const int64_t blocks = (n + parts - 1) / parts;
   if (blocks > (int64_t)INT_MAX) return fail(SSC_E_LIMIT, std::string(who) + ": too many particles");
   if (ckpt)
