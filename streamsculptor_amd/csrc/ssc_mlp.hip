@@ -9,8 +9,8 @@
 //
 // Training path: batch-major activations, generic strided GEMM C = A . B (64x64 block tile,
 // four 32x32 wave tiles, split-K for the batch reductions of the weight gradients).
-// Inference path: weights pre-packed in MFMA fragment order; a wave evaluates 32 points with
-// the activations kept in registers between layers (particles on the MFMA N axis).
+// Inference path: weights pre-packed in MFMA fragment order; a wave evaluates 16 points per
+// tile with the activations kept in registers between layers (points on the MFMA N axis).
 #include <hip/hip_runtime.h>
 
 #include <cstring>
@@ -31,6 +31,63 @@ constexpr int GT = 64;   // block tile
 constexpr int GK = 32;   // k per LDS stage
 constexpr int GPAD = 40; // LDS row pitch (elements): 80 B keeps 16-byte fragment reads aligned
 
+// Operand layouts of the tile loader.  GL_K: k contiguous (stride 1 in k, row stride a multiple
+// of 8, 16-byte aligned base): a thread moves 8 k's of one row with one 16-byte load and one
+// 16-byte LDS store.  GL_R: rows contiguous (stride 1 in m or n, k stride a multiple of 8,
+// aligned base): a thread loads 8 rows of one k with one 16-byte load and stores them across
+// the 8 LDS rows.  GL_ANY: element loads at any strides.  Every layout leaves the same values
+// (zeros past the edges) in LDS, so the products do not depend on the choice.
+enum { GL_ANY = 0, GL_K = 1, GL_R = 2 };
+
+// One operand's 64 x 32 stage of the tile in registers: 8 BF16 values per thread.
+template <int L>
+struct TileLoad {
+  uint4 v;
+  // X(r, k) = X[r * sr + k * sk]; rows r0 .. r0 + 63 of R, k's k0 .. k0 + 31 of [.., ke)
+  __device__ void load(const uint16_t* __restrict__ X, int64_t sr, int64_t sk, int r0, int R,
+                       int k0, int ke, int t) {
+    uint16_t e8[8];
+    if (L == GL_K) {
+      const int r = r0 + (t >> 2), k = k0 + (t & 3) * 8;
+      if (r < R && k + 8 <= ke) {
+        v = *reinterpret_cast<const uint4*>(X + (int64_t)r * sr + k);
+        return;
+      }
+#pragma unroll
+      for (int e = 0; e < 8; ++e)
+        e8[e] = (r < R && k + e < ke) ? X[(int64_t)r * sr + (k + e)] : (uint16_t)0;
+    } else if (L == GL_R) {
+      const int k = k0 + (t >> 3), r = r0 + (t & 7) * 8;
+      if (k < ke && r + 8 <= R) {
+        v = *reinterpret_cast<const uint4*>(X + (int64_t)k * sk + r);
+        return;
+      }
+#pragma unroll
+      for (int e = 0; e < 8; ++e)
+        e8[e] = (k < ke && r + e < R) ? X[(int64_t)k * sk + (r + e)] : (uint16_t)0;
+    } else {
+      const int r = r0 + (t >> 2), k = k0 + (t & 3) * 8;
+#pragma unroll
+      for (int e = 0; e < 8; ++e)
+        e8[e] = (r < R && k + e < ke) ? X[(int64_t)r * sr + (int64_t)(k + e) * sk] : (uint16_t)0;
+    }
+    memcpy(&v, e8, 16);
+  }
+  __device__ void store(uint16_t (*Xs)[GPAD], int t) const {
+    if (L == GL_R) {
+      const int kk = t >> 3, rr = (t & 7) * 8;
+      uint16_t e8[8];
+      memcpy(e8, &v, 16);
+#pragma unroll
+      for (int e = 0; e < 8; ++e) Xs[rr + e][kk] = e8[e];
+    } else {
+      *reinterpret_cast<uint4*>(&Xs[t >> 2][(t & 3) * 8]) = v;
+    }
+  }
+};
+
+// The next stage's operands are loaded into registers while the current stage's MFMAs run.
+template <int LA, int LB>
 __global__ __launch_bounds__(256) void gemm_kernel(const uint16_t* __restrict__ A, int64_t sAm,
                                                    int64_t sAk, const uint16_t* __restrict__ B,
                                                    int64_t sBk, int64_t sBn, float* __restrict__ C,
@@ -43,23 +100,20 @@ __global__ __launch_bounds__(256) void gemm_kernel(const uint16_t* __restrict__ 
   const int t = threadIdx.x, l = t & 63, w = t >> 6;
   const int wm = w & 1, wn = w >> 1;
   f32x16 acc = {};
-  const int lr = t >> 2, lk = (t & 3) * 8;  // loader: row/col of the tile, 8 k's each
+  TileLoad<LA> la;
+  TileLoad<LB> lb;
+  if (kb < ke) {
+    la.load(A, sAm, sAk, m0, M, kb, ke, t);
+    lb.load(B, sBn, sBk, n0, N, kb, ke, t);
+  }
   for (int k0 = kb; k0 < ke; k0 += GK) {
-    {
-      const int m = m0 + lr;
-#pragma unroll
-      for (int e = 0; e < 8; ++e) {
-        const int k = k0 + lk + e;
-        As[lr][lk + e] = (m < M && k < ke) ? A[(int64_t)m * sAm + (int64_t)k * sAk] : (uint16_t)0;
-      }
-      const int n = n0 + lr;
-#pragma unroll
-      for (int e = 0; e < 8; ++e) {
-        const int k = k0 + lk + e;
-        Bs[lr][lk + e] = (n < N && k < ke) ? B[(int64_t)k * sBk + (int64_t)n * sBn] : (uint16_t)0;
-      }
-    }
+    la.store(As, t);
+    lb.store(Bs, t);
     __syncthreads();
+    if (k0 + GK < ke) {
+      la.load(A, sAm, sAk, m0, M, k0 + GK, ke, t);
+      lb.load(B, sBn, sBk, n0, N, k0 + GK, ke, t);
+    }
 #pragma unroll
     for (int ks = 0; ks < GK; ks += 16) {
       const bf16x8 a = *reinterpret_cast<const bf16x8*>(&As[32 * wm + (l & 31)][ks + 8 * (l >> 5)]);
@@ -184,7 +238,15 @@ __global__ __launch_bounds__(256) void colsum_partial_kernel(const float* __rest
   const int64_t b0 = (int64_t)blockIdx.y * CS_ROWS, b1 = min(nb, b0 + CS_ROWS);
   float s = 0.0f;
   if (c < ncol)
-    for (int64_t b = b0 + ph; b < b1; b += 4) s += D[b * ld + c];
+    // rows b0 + ph, b0 + ph + 4, ... added in order; the loads of 8 of them are issued first
+    for (int64_t b = b0 + ph; b < b1; b += 32) {
+      float v[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) v[u] = b + 4 * u < b1 ? D[(b + 4 * u) * ld + c] : 0.0f;
+#pragma unroll
+      for (int u = 0; u < 8; ++u)
+        if (b + 4 * u < b1) s += v[u];
+    }
   red[ph][threadIdx.x & 63] = s;
   __syncthreads();
   if (ph == 0 && c < ncol)
@@ -202,11 +264,23 @@ __global__ __launch_bounds__(256) void layer1_grad_partial_kernel(const float* _
   const int o = blockIdx.x * 64 + (threadIdx.x & 63), ph = threadIdx.x >> 6;
   const int64_t b0 = (int64_t)blockIdx.y * CS_ROWS, b1 = min(nb, b0 + CS_ROWS);
   float s[IN + 1] = {0.0f, 0.0f, 0.0f, 0.0f};
-  for (int64_t b = b0 + ph; b < b1; b += 4) {
-    const float d = dZ[b * H + o];
+  // rows b0 + ph, b0 + ph + 4, ... accumulated in order; the loads of 4 of them are issued first
+  for (int64_t b = b0 + ph; b < b1; b += 16) {
+    float d[4], xv[4][IN];
 #pragma unroll
-    for (int i = 0; i < IN; ++i) s[i] = fmaf(d, x[b * IN + i], s[i]);
-    s[IN] += d;
+    for (int u = 0; u < 4; ++u) {
+      const bool in = b + 4 * u < b1;
+      d[u] = in ? dZ[(b + 4 * u) * H + o] : 0.0f;
+#pragma unroll
+      for (int i = 0; i < IN; ++i) xv[u][i] = in ? x[(b + 4 * u) * IN + i] : 0.0f;
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      if (b + 4 * u >= b1) break;
+#pragma unroll
+      for (int i = 0; i < IN; ++i) s[i] = fmaf(d[u], xv[u][i], s[i]);
+      s[IN] += d[u];
+    }
   }
 #pragma unroll
   for (int i = 0; i <= IN; ++i) red[ph][threadIdx.x & 63][i] = s[i];
@@ -410,19 +484,60 @@ void colsum(const float* D, int64_t ld, int64_t nb, int ncol, float* out, float*
   hipLaunchKernelGGL(chunk_sum_kernel, dim3(nblk(ncol)), dim3(256), 0, st, scratch, nz, ncol, out);
 }
 
+// Loader layout of an operand X(r, k) = X[r * sr + k * sk] (see TileLoad).
+int gemm_layout(const uint16_t* X, int64_t sr, int64_t sk) {
+  if ((reinterpret_cast<uintptr_t>(X) & 15) != 0) return GL_ANY;
+  if (sk == 1 && sr % 8 == 0) return GL_K;
+  if (sr == 1 && sk % 8 == 0) return GL_R;
+  return GL_ANY;
+}
+
+template <int LA, int LB>
+void gemm_launch(dim3 grid, hipStream_t st, const uint16_t* A, int64_t sAm, int64_t sAk,
+                 const uint16_t* B, int64_t sBk, int64_t sBn, float* C, int64_t ldc, int64_t cz,
+                 int M, int N, int K, int kchunk, const float* bias) {
+  hipLaunchKernelGGL((gemm_kernel<LA, LB>), grid, dim3(256), 0, st, A, sAm, sAk, B, sBk, sBn, C,
+                     ldc, cz, M, N, K, kchunk, bias);
+}
+
+template <int LA>
+void gemm_launch_b(int lb, dim3 grid, hipStream_t st, const uint16_t* A, int64_t sAm, int64_t sAk,
+                   const uint16_t* B, int64_t sBk, int64_t sBn, float* C, int64_t ldc, int64_t cz,
+                   int M, int N, int K, int kchunk, const float* bias) {
+  if (lb == GL_K)
+    gemm_launch<LA, GL_K>(grid, st, A, sAm, sAk, B, sBk, sBn, C, ldc, cz, M, N, K, kchunk, bias);
+  else if (lb == GL_R)
+    gemm_launch<LA, GL_R>(grid, st, A, sAm, sAk, B, sBk, sBn, C, ldc, cz, M, N, K, kchunk, bias);
+  else
+    gemm_launch<LA, GL_ANY>(grid, st, A, sAm, sAk, B, sBk, sBn, C, ldc, cz, M, N, K, kchunk, bias);
+}
+
+void gemm_dispatch(dim3 grid, hipStream_t st, const uint16_t* A, int64_t sAm, int64_t sAk,
+                   const uint16_t* B, int64_t sBk, int64_t sBn, float* C, int64_t ldc, int64_t cz,
+                   int M, int N, int K, int kchunk, const float* bias) {
+  const int la = gemm_layout(A, sAm, sAk), lb = gemm_layout(B, sBn, sBk);
+  if (la == GL_K)
+    gemm_launch_b<GL_K>(lb, grid, st, A, sAm, sAk, B, sBk, sBn, C, ldc, cz, M, N, K, kchunk, bias);
+  else if (la == GL_R)
+    gemm_launch_b<GL_R>(lb, grid, st, A, sAm, sAk, B, sBk, sBn, C, ldc, cz, M, N, K, kchunk, bias);
+  else
+    gemm_launch_b<GL_ANY>(lb, grid, st, A, sAm, sAk, B, sBk, sBn, C, ldc, cz, M, N, K, kchunk,
+                          bias);
+}
+
 void gemm(const uint16_t* A, int64_t sAm, int64_t sAk, const uint16_t* B, int64_t sBk,
           int64_t sBn, float* C, int M, int N, int K, const float* bias, hipStream_t st,
           float* part = nullptr, int nz = 1) {
   if (nz <= 1 || !part) {
-    hipLaunchKernelGGL(gemm_kernel, dim3(nblk(M, GT), nblk(N, GT), 1), dim3(256), 0, st, A, sAm,
-                       sAk, B, sBk, sBn, C, (int64_t)N, (int64_t)0, M, N, K, K, bias);
+    gemm_dispatch(dim3(nblk(M, GT), nblk(N, GT), 1), st, A, sAm, sAk, B, sBk, sBn, C, (int64_t)N,
+                  (int64_t)0, M, N, K, K, bias);
     return;
   }
   int kchunk = (K + nz - 1) / nz;
   kchunk = (kchunk + GK - 1) / GK * GK;
   nz = (K + kchunk - 1) / kchunk;
-  hipLaunchKernelGGL(gemm_kernel, dim3(nblk(M, GT), nblk(N, GT), nz), dim3(256), 0, st, A, sAm, sAk,
-                     B, sBk, sBn, part, (int64_t)N, (int64_t)M * N, M, N, K, kchunk, nullptr);
+  gemm_dispatch(dim3(nblk(M, GT), nblk(N, GT), nz), st, A, sAm, sAk, B, sBk, sBn, part, (int64_t)N,
+                (int64_t)M * N, M, N, K, kchunk, nullptr);
   hipLaunchKernelGGL(splitk_reduce_kernel, dim3(nblk((int64_t)M * N)), dim3(256), 0, st, part, nz,
                      (int64_t)M * N, C);
 }
