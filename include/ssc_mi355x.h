@@ -204,6 +204,18 @@ int ssc_release(const ssc_host_potential* host, const ssc_sources* src, const do
                 double* pos_close, double* pos_far, double* vel_close, double* vel_far,
                 void* stream);
 
+/* ssc_release for the seeds seed0 .. seed0 + nseed - 1 of the same releases, in one launch: the
+ * many-particle spray draws ceil(per_step / 2) seeds per release row, and the reference calls
+ * release_model once per seed (gen_stream_ics with seed_num, streamsculptor/main.py:209-306).
+ * The tidal radius and frame of a release (the Hessian of the whole potential) are formed once
+ * and shared by its seeds.  rows[nseed][2][nrel][6] (device): row i of arm 0 = (pos_close,
+ * vel_close), of arm 1 = (pos_far, vel_far) of seed seed0 + s -- the values ssc_release writes
+ * for that seed, bit for bit.  Other arguments as in ssc_release; nrel * nseed <= 2^31. */
+int ssc_release_seeds(const ssc_host_potential* host, const ssc_sources* src, const double* prog_w,
+                      const int32_t* rel_idx, const double* t_rel, const int32_t* step_rel,
+                      const double* g_msat, int32_t nrel, int64_t seed0, int32_t nseed,
+                      const double* kvals, double* rows, void* stream);
+
 /* Base orbits in the host potential plus first-order responses to each perturbing Plummer
  * source (reference fields.integrate_field with MassRadiusPerturbation_OTF,
  * streamsculptor/fields.py:34-99,159-206, driven per particle as in

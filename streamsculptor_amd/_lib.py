@@ -71,6 +71,8 @@ SIGNATURES = {
     "ssc_mlp_pack_vjp": [_vp, _vp],
     "ssc_mlp_input_vjp": [_vp, _vp, _vp, _vp, _i64, _vp],
     "ssc_surrogate_leapfrog": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _i64, _d, _d, _i32, _vp, _vp],
+    "ssc_release_seeds": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _i32, ctypes.c_int64, _i32, _vp, _vp,
+                          _vp],
     "ssc_release_vjp": [_vp, _vp, _vp, _vp, _i32, ctypes.c_int64, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
                         _vp],
     "ssc_release_vjp_noalloc": [_vp, _vp, _vp, _vp, _i32, ctypes.c_int64, _vp, _vp, _vp, _vp, _vp,

@@ -217,12 +217,8 @@ class StreamModel:
         nseed = (self.per_step + 1) // 2
         dev = prog_rows.device
         dl = self._device_layout(dev)
-        table = torch.empty(nseed, 2, T + 1, 6, dtype=torch.float64, device=dev)
-        for s in range(nseed):
-            pc, pf, vc, vf = host._release(prog_rows, dl["idx"], dl["gms"], self.seed + s,
-                                           self.kval_arr)
-            table[s, 0, :, :3], table[s, 0, :, 3:] = pc, vc
-            table[s, 1, :, :3], table[s, 1, :, 3:] = pf, vf
+        table = host._release_seeds(prog_rows, dl["idx"], dl["gms"], self.seed, nseed,
+                                    self.kval_arr)
         si, ai, ri = dl["si"], dl["ai"], dl["ri"]
         w0 = table[si, ai, ri].contiguous()
         start = dl["start"]

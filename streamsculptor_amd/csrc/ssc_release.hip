@@ -92,22 +92,19 @@ struct KVals {
   double k[8];
 };
 
+// Geometry of one release: the progenitor row, its frame (r-hat, z-hat along L, phi-hat along
+// the tangential velocity), the tidal radius and the release speed scale Omega r_t.
+struct ReleaseFrame {
+  double x[3], v[3], rh[3], zh[3], ph[3], rt, relv;
+};
+
 // Release i happens at time t_rel[i] (LINE sources) and source-table row step_rel[i] (TRACK
 // sources); the point sources add their Hessian to the host's, since the reference takes
 // d2phidr2 of the whole potential.  A TRACK row outside the table gives NaN rows.
-__global__ __launch_bounds__(256) void release_kernel(ssc_host_potential hp, ssc_sources src,
-                                                      const double* __restrict__ prog_w,
-                                                      const int32_t* __restrict__ rel_idx,
-                                                      const double* __restrict__ t_rel,
-                                                      const int32_t* __restrict__ step_rel,
-                                                      const double* __restrict__ g_msat, int nrel,
-                                                      int64_t seed, KVals kv,
-                                                      double* __restrict__ pos_close,
-                                                      double* __restrict__ pos_far,
-                                                      double* __restrict__ vel_close,
-                                                      double* __restrict__ vel_far) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nrel) return;
+__device__ void release_frame(const ssc_host_potential& hp, const ssc_sources& src,
+                              const double* __restrict__ prog_w, const double* __restrict__ t_rel,
+                              const int32_t* __restrict__ step_rel,
+                              const double* __restrict__ g_msat, int i, ReleaseFrame& f) {
   const double* w = prog_w + 6 * (int64_t)i;
   const double x[3] = {w[0], w[1], w[2]}, v[3] = {w[3], w[4], w[5]};
   double phi, g[3], H[9];
@@ -144,37 +141,92 @@ __global__ __launch_bounds__(256) void release_kernel(ssc_host_potential hp, ssc
                        x[0] * v[1] - x[1] * v[0]};
   const double r2 = rad * rad;
   const double om = sqrt(L[0] * L[0] + L[1] * L[1] + L[2] * L[2]) / r2;
-  const double rt = cbrt(g_msat[i] / (om * om - d2));
-  const double relv = om * rt;
+  f.rt = cbrt(g_msat[i] / (om * om - d2));
+  f.relv = om * f.rt;
   const double Ln = sqrt(L[0] * L[0] + L[1] * L[1] + L[2] * L[2]);
-  const double zh[3] = {L[0] / Ln, L[1] / Ln, L[2] / Ln};
   const double vr = v[0] * rh[0] + v[1] * rh[1] + v[2] * rh[2];
   const double pv[3] = {v[0] - vr * rh[0], v[1] - vr * rh[1], v[2] - vr * rh[2]};
   const double pn = sqrt(pv[0] * pv[0] + pv[1] * pv[1] + pv[2] * pv[2]);
-  const double ph[3] = {pv[0] / pn, pv[1] / pn, pv[2] / pn};
-  // Gaussian multipliers with keys PRNGKey(i * r_k) (main.py:220-228,263-266)
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    f.x[a] = x[a];
+    f.v[a] = v[a];
+    f.rh[a] = rh[a];
+    f.zh[a] = L[a] / Ln;
+    f.ph[a] = pv[a] / pn;
+  }
+}
+
+// The two released rows of frame f with release index ridx and seed: Gaussian multipliers
+// with keys PRNGKey(ridx * r_k) (main.py:220-228,263-266); trailing arm = far, leading arm =
+// close (main.py:269-278).  pc/pf/vc/vf may alias the strided slots of one output row.
+__device__ void release_rows(const ReleaseFrame& f, int64_t ridx, int64_t seed, const KVals& kv,
+                             double* pc, double* pf, double* vc, double* vf) {
   double zr[4];
-  const int64_t ridx = rel_idx ? (int64_t)rel_idx[i] : (int64_t)i;
 #pragma unroll
   for (int k = 0; k < 4; ++k) zr[k] = prng_normal(ridx * randint1000(seed, k));
   const double kr = kv.k[0] + zr[0] * kv.k[4];
   const double kvphi = kr * (kv.k[1] + zr[1] * kv.k[5]);
   const double kz = kv.k[2] + zr[2] * kv.k[6];
   const double kvz = kv.k[3] + zr[3] * kv.k[7];
-  // trailing arm = far, leading arm = close (main.py:269-278)
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
-    const double ptr = x[a] + kr * rh[a] * rt + zh[a] * kz * rt;
-    const double vtr = v[a] + kvphi * relv * ph[a] + kvz * relv * zh[a];
-    const double pld = x[a] - kr * rh[a] * rt - zh[a] * kz * rt;
-    const double vld = v[a] - kvphi * relv * ph[a] - kvz * relv * zh[a];
-    pos_close[3 * (int64_t)i + a] = pld;
-    pos_far[3 * (int64_t)i + a] = ptr;
-    vel_close[3 * (int64_t)i + a] = vld;
-    vel_far[3 * (int64_t)i + a] = vtr;
+    const double ptr = f.x[a] + kr * f.rh[a] * f.rt + f.zh[a] * kz * f.rt;
+    const double vtr = f.v[a] + kvphi * f.relv * f.ph[a] + kvz * f.relv * f.zh[a];
+    const double pld = f.x[a] - kr * f.rh[a] * f.rt - f.zh[a] * kz * f.rt;
+    const double vld = f.v[a] - kvphi * f.relv * f.ph[a] - kvz * f.relv * f.zh[a];
+    pc[a] = pld;
+    pf[a] = ptr;
+    vc[a] = vld;
+    vf[a] = vtr;
   }
 }
 
+__global__ __launch_bounds__(256) void release_kernel(ssc_host_potential hp, ssc_sources src,
+                                                      const double* __restrict__ prog_w,
+                                                      const int32_t* __restrict__ rel_idx,
+                                                      const double* __restrict__ t_rel,
+                                                      const int32_t* __restrict__ step_rel,
+                                                      const double* __restrict__ g_msat, int nrel,
+                                                      int64_t seed, KVals kv,
+                                                      double* __restrict__ pos_close,
+                                                      double* __restrict__ pos_far,
+                                                      double* __restrict__ vel_close,
+                                                      double* __restrict__ vel_far) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nrel) return;
+  ReleaseFrame f;
+  release_frame(hp, src, prog_w, t_rel, step_rel, g_msat, i, f);
+  const int64_t ridx = rel_idx ? (int64_t)rel_idx[i] : (int64_t)i;
+  const int64_t o = 3 * (int64_t)i;
+  release_rows(f, ridx, seed, kv, pos_close + o, pos_far + o, vel_close + o, vel_far + o);
+}
+
+// nseed seeds of the same releases: a block forms the frames of RS_REL releases once (the
+// tidal radius needs the Hessian of the whole potential, a sum over every point source), then
+// its threads draw the rows of every (release, seed) pair from them.
+// rows[s][arm][i][6] = (position, velocity), arm 0 = close (leading), 1 = far (trailing).
+constexpr int RS_REL = 32;
+__global__ __launch_bounds__(256) void release_seeds_kernel(
+    ssc_host_potential hp, ssc_sources src, const double* __restrict__ prog_w,
+    const int32_t* __restrict__ rel_idx, const double* __restrict__ t_rel,
+    const int32_t* __restrict__ step_rel, const double* __restrict__ g_msat, int nrel,
+    int64_t seed0, int nseed, KVals kv, double* __restrict__ rows) {
+  __shared__ ReleaseFrame fr[RS_REL];
+  const int i0 = blockIdx.x * RS_REL;
+  const int nb = min(RS_REL, nrel - i0);
+  if ((int)threadIdx.x < nb)
+    release_frame(hp, src, prog_w, t_rel, step_rel, g_msat, i0 + threadIdx.x, fr[threadIdx.x]);
+  __syncthreads();
+  const int64_t arm = (int64_t)nrel * 6, seed_stride = 2 * arm;
+  for (int it = threadIdx.x; it < nb * nseed; it += blockDim.x) {
+    const int r = it % nb, s = it / nb, i = i0 + r;
+    const int64_t ridx = rel_idx ? (int64_t)rel_idx[i] : (int64_t)i;
+    double* close = rows + s * seed_stride + 6 * (int64_t)i;
+    double* far = close + arm;
+    release_rows(fr[r], ridx, seed0 + s, kv, close, far, close + 3, far + 3);
+  }
+}
 
 // ------------------------------------------------------------------------------------------
 // Vector-Jacobian product of the release model.  The Jacobian is formed column by column in
@@ -373,6 +425,36 @@ extern "C" int ssc_release(const ssc_host_potential* host, const ssc_sources* sr
                      *host, with_src ? *src : none, prog_w, rel_idx, t_rel, step_rel, g_msat,
                      nrel, seed, kv, pos_close, pos_far, vel_close, vel_far);
   return check_launch("ssc_release");
+}
+
+extern "C" int ssc_release_seeds(const ssc_host_potential* host, const ssc_sources* src,
+                                 const double* prog_w, const int32_t* rel_idx,
+                                 const double* t_rel, const int32_t* step_rel,
+                                 const double* g_msat, int32_t nrel, int64_t seed0,
+                                 int32_t nseed, const double* kvals, double* rows, void* stream) {
+  int rc = validate_host(host);
+  if (rc) return rc;
+  rc = validate_sources(src, 0);
+  if (rc) return rc;
+  if (nrel < 0 || nseed < 0) return fail(SSC_E_ARG, "ssc_release_seeds: negative size");
+  if (nrel == 0 || nseed == 0) return SSC_OK;
+  if ((int64_t)nrel * nseed > (int64_t)1 << 31)
+    return fail(SSC_E_LIMIT, "ssc_release_seeds: nrel * nseed above 2^31");
+  if (!prog_w || !g_msat || !rows) return fail(SSC_E_ARG, "ssc_release_seeds: NULL array argument");
+  const ssc_sources none = {};
+  const bool with_src = src && src->kind != SSC_SRC_NONE && src->n > 0;
+  if (with_src && (src->kind == SSC_SRC_TRACK || src->kind == SSC_SRC_TRACK_HERNQUIST) &&
+      !step_rel)
+    return fail(SSC_E_ARG, "ssc_release_seeds: TRACK sources need step_rel");
+  if (with_src && (src->kind == SSC_SRC_LINE || src->kind == SSC_SRC_LINE_HERNQUIST) && !t_rel)
+    return fail(SSC_E_ARG, "ssc_release_seeds: LINE sources need t_rel");
+  KVals kv;
+  const double def[8] = {2.0, 0.3, 0.0, 0.0, 0.4, 0.4, 0.5, 0.5};
+  for (int k = 0; k < 8; ++k) kv.k[k] = kvals ? kvals[k] : def[k];
+  hipLaunchKernelGGL(release_seeds_kernel, dim3((nrel + RS_REL - 1) / RS_REL), dim3(256), 0,
+                     (hipStream_t)stream, *host, with_src ? *src : none, prog_w, rel_idx, t_rel,
+                     step_rel, g_msat, nrel, seed0, nseed, kv, rows);
+  return check_launch("ssc_release_seeds");
 }
 
 // The release VJP proper: per-release partials of the host parameters go to `scratch`

@@ -508,6 +508,36 @@ class Potential:
                   *[_lib.ptr(o) for o in outs])
         return outs  # pos_close (lead), pos_far (trail), vel_close, vel_far
 
+    def _release_seeds(self, prog_w, rel_idx, g_msat, seed0, nseed, kval_arr, t_rel=None):
+        """``_release`` for seeds seed0 .. seed0 + nseed - 1 in one launch (ssc_release_seeds):
+        returns rows (nseed, 2, n, 6), arm 0 = (pos_close, vel_close) (lead), arm 1 =
+        (pos_far, vel_far) (trail), equal to the per-seed calls bit for bit."""
+        h, src = self._descriptor()
+        n = prog_w.shape[0]
+        kv = np.asarray(_as_float(kval_arr), dtype=np.float64)
+        kvp = None
+        if kv.ndim > 0:
+            kv = np.ascontiguousarray(np.ones(8) * kv)
+            kvp = kv.ctypes.data_as(_lib.ctypes.c_void_p)
+        dev = prog_w.device
+        sd, t_dev, step_dev = None, None, None
+        if src is not None:
+            if t_rel is None:
+                raise ValueError("the release in a potential with point sources needs the "
+                                 "release times")
+            tr = np.asarray(_as_float(t_rel), dtype=np.float64).reshape(-1)
+            src, steps = src.rows_for(tr)
+            if steps is not None:
+                step_dev = _lib.i32(steps)
+            t_dev = _lib.f64(tr)
+            sd = src.desc()
+        rows = torch.empty(int(nseed), 2, n, 6, dtype=torch.float64, device=dev)
+        _lib.call("ssc_release_seeds", _lib.ctypes.byref(h),
+                  None if sd is None else _lib.ctypes.byref(sd), _lib.ptr(prog_w.contiguous()),
+                  _lib.ptr(rel_idx), _lib.ptr(t_dev), _lib.ptr(step_dev),
+                  _lib.ptr(g_msat.contiguous()), n, int(seed0), int(nseed), kvp, _lib.ptr(rows))
+        return rows
+
     def gen_stream_ics(self, ts=None, prog_w0=None, Msat=None, seed_num=None, solver=None,
                        kval_arr=1.0, rtol=1e-7, atol=1e-7, dtmin=0.3, dtmax=None,
                        max_steps=10_000, dt0=None):

@@ -195,16 +195,11 @@ def spray_particles(prog_pot, release_pot, ts, prog_w0, Msat, seed_num, per_step
     nseed = (per_step + 1) // 2
     dev = ws.device
     from .main import _as_float
-    # table[s, arm, i, 6]; the release masses and indices go to the device once for all seeds
-    # (a copy from pageable host memory waits for the stream)
-    table = torch.empty(nseed, 2, T + 1, 6, dtype=torch.float64, device=dev)
+    # table[s, arm, i, 6] of every seed in one launch
     msat = np.broadcast_to(np.asarray(_as_float(Msat), dtype=np.float64), (T + 1,))
     gms = release_pot._G * _lib.f64(np.ascontiguousarray(msat))
     idx = torch.arange(T + 1, dtype=torch.int32, device=dev)
-    for s in range(nseed):
-        pc, pf, vc, vf = release_pot._release(ws, idx, gms, seed_num + s, kval_arr, t_rel=ts_np)
-        table[s, 0, :, :3], table[s, 0, :, 3:] = pc, vc
-        table[s, 1, :, :3], table[s, 1, :, 3:] = pf, vf
+    table = release_pot._release_seeds(ws, idx, gms, seed_num, nseed, kval_arr, t_rel=ts_np)
     lay = np.array([even, odd], dtype=np.int64)            # (parity, per_step, [seed, arm])
     par = np.arange(T) % 2
     sel_s, sel_a = lay[par, :, 0], lay[par, :, 1]
