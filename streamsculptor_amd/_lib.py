@@ -8,6 +8,7 @@ from __future__ import annotations
 import ctypes
 import os
 
+import numpy as np
 import torch
 
 HERE = os.path.dirname(os.path.abspath(__file__))
@@ -176,6 +177,8 @@ def f64(x, dev=None):
     dev = dev or device()
     if isinstance(x, torch.Tensor):
         return x.detach().to(device=dev, dtype=torch.float64).contiguous()
+    if isinstance(x, np.ndarray) and not x.flags.writeable:
+        x = np.array(x, dtype=np.float64)  # read-only views (broadcasts): copy on the host first
     return torch.as_tensor(x, dtype=torch.float64, device=dev).contiguous()
 
 
