@@ -37,6 +37,11 @@ constexpr size_t surrogate_lds_bytes() {
          (size_t)mlp::PK_F32_END * 4 + (size_t)mlp::H * 16;
 }
 
+// One clump of the exact-sum fallback as the pair loop consumes it.
+struct FbClump {
+  double x, y, z, rs2, gm;
+};
+
 // CK: also write checkpoint rows ckpt[r][n][9] = (x, v, a) at grid rows min(r E, nsteps),
 // r = 0..ceil(nsteps / E) (a particle that has not started holds its initial row and the
 // acceleration there) for the discrete adjoint.
@@ -95,6 +100,16 @@ __global__ __launch_bounds__(SG_BLOCK) void surrogate_leapfrog_kernel(
   lw.prefetch(0);
   unsigned long long nfall = 0;
 
+  // Exact-sum fallback from LDS: when some particle of the block falls back, thread t loads
+  // clump t (at most one clump per thread) while the particles are compacted and writes it into
+  // the weight-chunk buffer that the network's last chunk left free (buffer 0: chunk 2 of 4;
+  // the next call's first chunk overwrites it only after the fallback's closing barrier).
+  // Larger tables are read from global memory in the pair loop.
+  static_assert(LW::NCH % 2 == 0, "a network call must end on chunk buffer 1");
+  static_assert(SG_BLOCK * sizeof(FbClump) <= LW::CH * 16, "fallback table exceeds a buffer");
+  const bool fb_lds = clumps.n <= SG_BLOCK;
+  FbClump* fb_tab = reinterpret_cast<FbClump*>(wbuf);
+
   // total acceleration at (x, y, z); block-uniform call (barriers inside)
   auto force = [&](double& ax, double& ay, double& az) {
     const float u0 = (float)((x - sd.center[0]) * sd.inv_half[0]);
@@ -117,6 +132,13 @@ __global__ __launch_bounds__(SG_BLOCK) void surrogate_leapfrog_kernel(
     // fallback test: outside the training box or a flagged cell
     const bool fb = valid && surrogate_fallback(sd, flags, x, y, z);
     if (__syncthreads_or(fb)) {
+      FbClump mine{};
+      if (fb_lds && (int)threadIdx.x < clumps.n) {
+        const int j = threadIdx.x;
+        const double rs = clumps.rs[j];
+        mine = FbClump{clumps.pos[3 * j], clumps.pos[3 * j + 1], clumps.pos[3 * j + 2], rs * rs,
+                       clumps.gm[j]};
+      }
       int slot = -1;
       if (fb && (l & 16) == 0) {
         slot = atomicAdd(&nfb, 1);
@@ -126,15 +148,23 @@ __global__ __launch_bounds__(SG_BLOCK) void surrogate_leapfrog_kernel(
         ++nfall;
       }
       slot = __shfl(slot, l & ~16, 64);
+      if (fb_lds && (int)threadIdx.x < clumps.n) fb_tab[threadIdx.x] = mine;
       __syncthreads();
       const int cnt = nfb;
       for (int f = wv; f < cnt; f += SG_BLOCK / 64) {
         const double px = fb_buf[f][0], py = fb_buf[f][1], pz = fb_buf[f][2];
         double sx = 0.0, sy = 0.0, sz = 0.0;
-        for (int j = l; j < clumps.n; j += 64) {
-          const double rs = clumps.rs[j];
-          plummer_grad(px - clumps.pos[3 * j], py - clumps.pos[3 * j + 1], pz - clumps.pos[3 * j + 2],
-                       clumps.gm[j], rs * rs, sx, sy, sz);
+        if (fb_lds) {
+          for (int j = l; j < clumps.n; j += 64) {
+            const FbClump c = fb_tab[j];
+            plummer_grad(px - c.x, py - c.y, pz - c.z, c.gm, c.rs2, sx, sy, sz);
+          }
+        } else {
+          for (int j = l; j < clumps.n; j += 64) {
+            const double rs = clumps.rs[j];
+            plummer_grad(px - clumps.pos[3 * j], py - clumps.pos[3 * j + 1],
+                         pz - clumps.pos[3 * j + 2], clumps.gm[j], rs * rs, sx, sy, sz);
+          }
         }
         sx = group_sum<64>(sx);
         sy = group_sum<64>(sy);
