@@ -95,9 +95,9 @@ def spray(wl):
 
 
 def plain_stream_rows(wl):
-    """Initial rows and start rows of the configs[0] stream in the order gen_stream integrates
-    them (per release: lead, then trail), computed once outside the timed region for the
-    sample check."""
+    """Initial rows and start rows of the plain 8000-particle stream in the order gen_stream
+    integrates them (per release: lead, then trail), computed once outside the timed region for
+    the sample check."""
     import numpy as np
     import torch
     from streamsculptor_amd import workloads as W
@@ -116,7 +116,7 @@ def plain_stream_rows(wl):
 
 def run_plain(wl, ev=None):
     if wl["name"] == "plain":
-        # the configs[0] stream through gen_stream: progenitor orbit, releases and particles,
+        # the plain 8000-particle stream through gen_stream: progenitor orbit, releases and particles,
         # pipelined over ranges of release rows
         import torch
         from streamsculptor_amd import workloads as W

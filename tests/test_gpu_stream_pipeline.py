@@ -1,6 +1,6 @@
 """The fixed-step spray stream pipelined over windows of grid rows (the progenitor orbit
 advancing window by window while, on a second stream, every particle released so far advances
-through the window behind it) against the one-pass path: the configs[0] stream (host only,
+through the window behind it) against the one-pass path: the plain spray stream (host only,
 4000 steps, two particles per step) bit for bit in 4, 7 and the default 8 windows; a clumpy
 halo with a mass-loss history (Msat per release) to 1e-11 (a restart forms the acceleration at
 a window's first row in the kernel's prologue, whose FMA contraction can differ in the last bit
