@@ -11,7 +11,9 @@
 // dPhi_j/da_j the structural potential of SubhaloLinePotential_dRadius (potential.py:852-904)
 // or SubhaloLinePotentialCustom_dRadius_fromFunc (potential.py:1215-1268).
 //
-// Layout: L = ceil(m / G) lanes per particle, each owning G responses.  With a workspace the
+// Layout: L = ceil(m / G) lanes per particle; lane g of a particle owns the G responses g,
+// g + L, g + 2L, ... (strided: at every response slot the L lanes read consecutive source
+// records, which keeps their LDS reads free of bank conflicts).  With a workspace the
 // grid is cut into segments of rows: perturb_base_kernel (one particle per lane) advances the
 // base orbits through a segment and records, at every row a particle is kicked at, its position
 // and the host Hessian; perturb_rows_kernel then advances the responses through the segment
@@ -52,7 +54,6 @@ __global__ __launch_bounds__(PT_BLOCK) void perturb_kernel(
   const int lp = threadIdx.x / L, grp = threadIdx.x - lp * L;
   const int64_t i = (int64_t)blockIdx.x * ppb + lp;
   const bool valid = lp < ppb && i < n;
-  const int j0 = grp * G;
 
   double x = 1.0, y = 0.0, z = 0.0, vx = 0.0, vy = 0.0, vz = 0.0;
   double dx[G][3], dv[G][3], rx[RAD ? G : 1][3], rv[RAD ? G : 1][3];
@@ -65,7 +66,7 @@ __global__ __launch_bounds__(PT_BLOCK) void perturb_kernel(
   }
 #pragma unroll
   for (int q = 0; q < G; ++q) {
-    const int j = j0 + q;
+    const int j = grp + L * q;
     const bool ok = valid && j < m && dw0;
     const double* d = ok ? dw0 + ((int64_t)i * m + j) * DD : nullptr;
 #pragma unroll
@@ -98,7 +99,7 @@ __global__ __launch_bounds__(PT_BLOCK) void perturb_kernel(
     }
 #pragma unroll
     for (int q = 0; q < G; ++q) {
-      const int j = j0 + q;
+      const int j = grp + L * q;
       double sx, sy, sz, gm, q2;
       const bool act = (j < m) && lds.get(k, t, j, sx, sy, sz, gm, q2);
       double fx = 0.0, fy = 0.0, fz = 0.0, ux = 0.0, uy = 0.0, uz = 0.0;
@@ -175,7 +176,7 @@ __global__ __launch_bounds__(PT_BLOCK) void perturb_kernel(
     }
 #pragma unroll
     for (int q = 0; q < G; ++q) {
-      const int j = j0 + q;
+      const int j = grp + L * q;
       if (j < m) {
         double* o = dw_out + ((int64_t)i * m + j) * DD;
 #pragma unroll
@@ -243,11 +244,11 @@ __global__ __launch_bounds__(PT_BLOCK) void perturb_rows_kernel(
   __shared__ int kmin_sh;
   constexpr int DD = RAD ? 12 : 6;
   constexpr bool HQ = SK == SSC_SRC_LINE_HERNQUIST;
+  constexpr bool FOLD = (SK == SSC_SRC_STATIC || SK == SSC_SRC_TRACK) && !RAD;
   const int ppb = PT_BLOCK / L;
   const int lp = threadIdx.x / L, grp = threadIdx.x - lp * L;
   const int64_t i = (int64_t)blockIdx.x * ppb + lp;
   const bool valid = lp < ppb && i < n;
-  const int j0 = grp * G;
   int s = nsteps;
   if (valid) {
     s = start ? start[i] : 0;
@@ -257,7 +258,7 @@ __global__ __launch_bounds__(PT_BLOCK) void perturb_rows_kernel(
   double dx[G][3], dv[G][3], rx[RAD ? G : 1][3], rv[RAD ? G : 1][3];
 #pragma unroll
   for (int q = 0; q < G; ++q) {
-    const int j = j0 + q;
+    const int j = grp + L * q;
     const bool ok = moves && j < m;
     const double* d = ok ? dw + ((int64_t)i * m + j) * DD : nullptr;
 #pragma unroll
@@ -295,9 +296,33 @@ __global__ __launch_bounds__(PT_BLOCK) void perturb_rows_kernel(
     const double H[9] = {r[3], r[4], r[5], r[4], r[6], r[7], r[5], r[7], r[8]};
     const double t = t0 + k * dt;
     const double c = (k == s || k == nsteps) ? h : dt;
+    if constexpr (FOLD) {
+      // Plummer sources that are always active: no branch per response, the folded
+      // reciprocal-square-root correction of plummer_grad_k, and the source force added into
+      // the Hessian product by one FMA per component
+#pragma unroll
+      for (int q = 0; q < G; ++q) {
+        const int j = grp + L * q;
+        double sx, sy, sz, q2, c1, c2;
+        lds.get_k(k, j < m ? j : m - 1, sx, sy, sz, q2, c1, c2);
+        const double ex = x - sx, ey = y - sy, ez = z - sz;
+        const double s2 = fma(ex, ex, fma(ey, ey, fma(ez, ez, q2)));
+        const double yr = __builtin_amdgcn_rsq(s2);
+        const double y2 = yr * yr;
+        double f = yr * y2 * fma(s2 * y2, c1, c2);
+        f = j < m ? f : 0.0;
+        const double hx = fma(H[2], dx[q][2], fma(H[1], dx[q][1], H[0] * dx[q][0]));
+        const double hy = fma(H[5], dx[q][2], fma(H[4], dx[q][1], H[3] * dx[q][0]));
+        const double hz = fma(H[8], dx[q][2], fma(H[7], dx[q][1], H[6] * dx[q][0]));
+        dv[q][0] = fma(-c, fma(f, ex, hx), dv[q][0]);
+        dv[q][1] = fma(-c, fma(f, ey, hy), dv[q][1]);
+        dv[q][2] = fma(-c, fma(f, ez, hz), dv[q][2]);
+      }
+      continue;
+    }
 #pragma unroll
     for (int q = 0; q < G; ++q) {
-      const int j = j0 + q;
+      const int j = grp + L * q;
       double sx, sy, sz, gm, q2;
       const bool act = (j < m) && lds.get(k, t, j, sx, sy, sz, gm, q2);
       double fx = 0.0, fy = 0.0, fz = 0.0, ux = 0.0, uy = 0.0, uz = 0.0;
@@ -343,7 +368,7 @@ __global__ __launch_bounds__(PT_BLOCK) void perturb_rows_kernel(
   if (moves) {
 #pragma unroll
     for (int q = 0; q < G; ++q) {
-      const int j = j0 + q;
+      const int j = grp + L * q;
       if (j < m) {
         double* o = dw + ((int64_t)i * m + j) * DD;
 #pragma unroll

@@ -87,6 +87,13 @@ struct SourceLDS<SSC_SRC_STATIC> {
     sx = g.x; sy = g.y; sz = g.z; gm = gmv[j]; q = g.rs2;
     return true;
   }
+  // Source j in the folded form of plummer_grad_k: position, r_s^2, (-1.5 G m, 2.5 G m).
+  __device__ void get_k(int, int j, double& sx, double& sy, double& sz, double& rs2, double& c1,
+                        double& c2) const {
+    const SrcGeo g = geo[j];
+    const SrcK c = kc[j];
+    sx = g.x; sy = g.y; sz = g.z; rs2 = g.rs2; c1 = c.c1; c2 = c.c2;
+  }
 };
 
 // Grid steps staged per LDS fill for TRACK sources: (chunk + 1) rows of n records stay under
@@ -166,6 +173,13 @@ struct TrackLDS {
     const SrcGeo g = rec[(k - kb) * n + j];
     sx = g.x; sy = g.y; sz = g.z; gm = gmv[j]; q = g.rs2;
     return true;
+  }
+  // Source j at grid row k in the folded form of plummer_grad_k (Plummer tables).
+  __device__ void get_k(int k, int j, double& sx, double& sy, double& sz, double& rs2, double& c1,
+                        double& c2) const {
+    const SrcGeo g = rec[(k - kb) * n + j];
+    const SrcK c = kc[j];
+    sx = g.x; sy = g.y; sz = g.z; rs2 = g.rs2; c1 = c.c1; c2 = c.c2;
   }
 };
 
