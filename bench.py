@@ -94,6 +94,15 @@ def spray(wl):
                            wl["per_step"], W.DT)
 
 
+def spray_rows(wl):
+    """Initial rows and start rows of the workload's spray in integration order, computed once
+    outside the timed region for the sample check."""
+    if "spray_rows" not in wl:
+        w0, start, _, _ = spray(wl)
+        wl["spray_rows"] = (w0, start)
+    return wl["spray_rows"]
+
+
 def plain_stream_rows(wl):
     """Initial rows and start rows of the plain 8000-particle stream in the order gen_stream
     integrates them (per release: lead, then trail), computed once outside the timed region for
@@ -130,6 +139,19 @@ def run_plain(wl, ev=None):
         out = torch.empty(2 * lead.shape[0], 6, dtype=torch.float64, device=lead.device)
         out[0::2], out[1::2] = lead, trail
         w0, start = plain_stream_rows(wl)
+        return w0, start, out
+    if wl["name"] == "subhalos":
+        # the live-subhalo stream through spray_stream: the progenitor orbit in the live
+        # subhalos pipelined with the particles over windows of grid rows
+        from streamsculptor_amd import workloads as W
+        from streamsculptor_amd.streams import spray_stream
+        if ev:
+            ev[0].record()
+        out = spray_stream(wl["pot"], wl["host"], wl["ts"], wl["prog_w0"], W.MSAT, 0,
+                           wl["per_step"], W.DT)
+        if ev:
+            ev[1].record()
+        w0, start = spray_rows(wl)
         return w0, start, out
     w0, start, arm, (t0, dt, nsteps, _) = spray(wl)
     if ev:
@@ -715,8 +737,8 @@ def main():
                           "kernel": ("perturb_base_kernel + perturb_rows_kernel (one launch chain)"
                                      if args.workload == "responses" else
                                      "the whole stream: progenitor orbit, releases and particle "
-                                     "leapfrog, pipelined" if args.workload == "plain" else
-                                     "leapfrog_kernel"),
+                                     "leapfrog, pipelined" if args.workload in ("plain", "subhalos")
+                                     else "leapfrog_kernel"),
                           "flop_per_particle_step": fl_p}}
     result = {
         "metric": "particle-steps/s, surrogate vs plain FP64 force, at <=10 pc median stream error",

@@ -42,6 +42,17 @@ def _static_in_time(pot):
     return src is None or src.kind == _lib.SRC_STATIC
 
 
+def _restartable(pot):
+    """True when a fixed-step orbit in the potential can restart at a grid row with the sources
+    it would meet in one pass: no sources, STATIC sources, or TRACK sources given as a table on
+    the grid (a window reads the table's own rows).  Straight-line sources and tracks sampled
+    from a function form their positions from the window's times."""
+    from .main import SampledTrack
+    _, src = pot._descriptor()
+    return (src is None or src.kind == _lib.SRC_STATIC or
+            (src.kind in _lib.TRACK_KINDS and not isinstance(src, SampledTrack)))
+
+
 _SIDE = {}
 
 
@@ -177,6 +188,80 @@ def gen_stream_dense(prog_pot, release_pot, ts, prog_w0, Msat, seed_num, solver=
     sd = src.desc() if src is not None else None
     return adaptive.run_dense(h, sd, w0, float(ts_np[-1]), t0=float(ts_np[0]), t_start=t_start,
                               batched=True, **adaptive_kw)
+
+
+def spray_stream(prog_pot, release_pot, ts, prog_w0, Msat, seed_num, per_step, dt0,
+                 kval_arr=1.0, particle_pot=None, integrate=None, chunks=8):
+    """Final rows of the many-particle spray stream -- the particles of ``spray_particles``, in
+    its order, advanced with the fixed-step leapfrog to the last grid row -- with the progenitor
+    orbit pipelined with the particles over ``chunks`` windows of grid rows on two streams, as
+    ``gen_stream`` does for two particles per step: the current stream advances the progenitor
+    through a window and releases that window's particles (every seed in one launch); the
+    second stream then advances every particle released so far through the window while the
+    progenitor moves on.  ``integrate(w, t0, dt, nsteps, start)`` advances rows (default: the
+    leapfrog in ``particle_pot``, else ``prog_pot``); TRACK tables are cut to each window's
+    rows.  The rows equal the one-pass order (``spray_particles``, then one integration) up to
+    the last bits of the accelerations re-formed at the windows' first rows.  Fewer than
+    4 * chunks release steps, or sources that form their positions from the time (straight
+    lines, sampled tracks): one pass."""
+    from .main import _as_float
+    from .workloads import spray_layout
+    particle_pot = particle_pot or prog_pot
+    if integrate is None:
+        def integrate(w, t_start, step, n, start):
+            return particle_pot._leapfrog(w, t_start, step, n, start=start)[0]
+    ts_np = np.asarray(ts, dtype=np.float64).reshape(-1)
+    T = len(ts_np) - 1
+    if (chunks <= 1 or T < 4 * chunks or not _restartable(prog_pot)
+            or not _restartable(particle_pot)):
+        w0, start, _, (t0, dt, nsteps, _) = spray_particles(
+            prog_pot, release_pot, ts_np, prog_w0, Msat, seed_num, per_step, dt0, kval_arr)
+        return integrate(w0, t0, dt, nsteps, start)
+    t0, dt, nsteps, kidx = release_grid(ts_np, dt0)
+    dev = _lib.device()
+    nseed = (per_step + 1) // 2
+    lay = np.array(spray_layout(per_step), dtype=np.int64)  # (parity, per_step, [seed, arm])
+    msat = np.broadcast_to(np.asarray(_as_float(Msat), dtype=np.float64), (T + 1,))
+    gms_all = release_pot._G * _lib.f64(np.ascontiguousarray(msat[:T]))
+    idx_all = torch.arange(T, dtype=torch.int32, device=dev)
+    cuts = sorted({0, T} | {T * j // chunks for j in range(1, chunks)})
+    # every window's index arrays in one host-to-device copy: the progenitor rows it saves,
+    # (seed, arm, release) of its new particles, and the start rows of all particles so far
+    parts, spans = [], []
+    for c in range(len(cuts) - 1):
+        a, b = cuts[c], cuts[c + 1]
+        g0 = int(kidx[a])
+        sel = lay[np.arange(a, b) % 2]                            # (b - a, per_step, 2)
+        rel_all = np.repeat(np.arange(b), per_step)
+        for arr in (kidx[a:b] - g0, sel[..., 0].reshape(-1), sel[..., 1].reshape(-1),
+                    np.repeat(np.arange(b - a), per_step), np.maximum(kidx[rel_all] - g0, 0)):
+            spans.append((sum(p.size for p in parts), arr.size))
+            parts.append(np.asarray(arr).astype(np.int32))
+    flat = _lib.i32(np.concatenate(parts))
+    views = [flat[o:o + k] for o, k in spans]
+    main = torch.cuda.current_stream(dev)
+    side = _side_stream(dev)
+    w = _lib.f64(prog_w0).reshape(1, 6)
+    state, keep = None, []
+    for c in range(len(cuts) - 1):
+        a, b = cuts[c], cuts[c + 1]
+        g0 = int(kidx[a])
+        g1 = int(kidx[b]) if b < T else nsteps
+        save, s_i, a_i, r_i, start = views[5 * c:5 * c + 5]
+        pn = (g1 if b < T else int(kidx[T - 1])) - g0
+        w_next, ys = prog_pot._leapfrog(w, t0 + g0 * dt, dt, pn, save_idx=save)
+        table = release_pot._release_seeds(ys[:, 0, :].contiguous(), idx_all[a:b].contiguous(),
+                                           gms_all[a:b].contiguous(), seed_num, nseed,
+                                           kval_arr, t_rel=ts_np[a:b])
+        new = table[s_i.long(), a_i.long(), r_i.long()]
+        side.wait_stream(main)
+        with torch.cuda.stream(side):
+            w_in = new if state is None else torch.cat([state, new], 0)
+            state = integrate(w_in, t0 + g0 * dt, dt, g1 - g0, start)
+        keep.append((new, w_in))
+        w = w_next
+    main.wait_stream(side)
+    return state
 
 
 def spray_particles(prog_pot, release_pot, ts, prog_w0, Msat, seed_num, per_step, dt0,

@@ -76,3 +76,51 @@ def test_time_dependent_potential_takes_one_pass(ssc):
     back = np.array([11.0, 1.0, 0.5, 0.02, 0.17, 0.04])
     (l4, t4), (l1, t1) = _both(pot, ts, back, 1e4, 1)
     assert torch.equal(l4, l1) and torch.equal(t4, t1)
+
+
+def _spray_both(pot, rel_pot, ts, prog_w0, per_step, seed=0):
+    from streamsculptor_amd import workloads as W
+    from streamsculptor_amd.streams import spray_particles, spray_stream
+    w0, start, _, (t0, dt, nsteps, _) = spray_particles(pot, rel_pot, ts, prog_w0, W.MSAT, seed,
+                                                        per_step, 1.0)
+    one, _ = pot._leapfrog(w0, t0, dt, nsteps, start=start)
+    piped = spray_stream(pot, rel_pot, ts, prog_w0, W.MSAT, seed, per_step, 1.0)
+    e = _rel(piped, one)
+    print(f"[spray pipeline] {w0.shape[0]} particles, windows vs one pass: {e:.2e}", flush=True)
+    return piped, one, e
+
+
+def test_spray_stream_equals_one_pass(ssc):
+    """The many-particle spray with the progenitor pipelined over windows (spray_stream)
+    against spray_particles + one integration: bit for bit in the host alone; to 1e-11 with
+    STATIC clumps and with live subhalos on a TRACK table (the acceleration at a window's first
+    row comes from the kernel's prologue); a potential with straight-line subhalos takes the
+    one-pass path."""
+    from streamsculptor_amd import workloads as W
+    from streamsculptor_amd.main import usys
+    from streamsculptor_amd.potential import (LiveSubhaloPotential, PlummerClumps,
+                                              Potential_Combine, SubhaloLinePotential)
+    from streamsculptor_amd.streams import _restartable
+    host = W.host_potential()
+    ts = np.linspace(-400.0, 0.0, 401)
+    back = host.integrate_orbit(w0=W.PROG_TODAY, ts=[0.0, ts[0]], t0=0.0, t1=float(ts[0]),
+                                solver="leapfrog", dt0=1.0).ys[-1]
+    piped, one, _ = _spray_both(host, host, ts, back, 25)
+    assert tuple(piped.shape) == (25 * 400, 6) and torch.equal(piped, one)
+    pos, m, rs = W.clump_table(96, seed=3)
+    clumpy = Potential_Combine([host, PlummerClumps(m, rs, pos, units=usys)], units=usys)
+    assert _spray_both(clumpy, clumpy, ts, back, 24, seed=2)[2] <= 1e-11
+    sw0, sm, srs = W.subhalo_ics(n=16, seed=2)
+    live = LiveSubhaloPotential.from_orbits(host, sw0, sm, srs, float(ts[0]), 1.0, 400)
+    pot = Potential_Combine([host, live], units=usys)
+    assert _restartable(pot)
+    assert _spray_both(pot, host, ts, back, 25)[2] <= 1e-11
+    rng = np.random.default_rng(5)
+    line = SubhaloLinePotential(10 ** rng.uniform(6, 8, 8), rng.uniform(0.1, 0.5, 8),
+                                rng.normal(scale=15.0, size=(8, 3)),
+                                rng.normal(scale=0.2, size=(8, 3)), rng.uniform(-400, 0, 8),
+                                rng.uniform(50, 200, 8), units=usys)
+    pot_l = Potential_Combine([host, line], units=usys)
+    assert not _restartable(pot_l)
+    piped, one, _ = _spray_both(pot_l, host, ts, back, 6)
+    assert torch.equal(piped, one)
