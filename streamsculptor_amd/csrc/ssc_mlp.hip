@@ -238,13 +238,13 @@ __global__ __launch_bounds__(256) void colsum_partial_kernel(const float* __rest
   const int64_t b0 = (int64_t)blockIdx.y * CS_ROWS, b1 = min(nb, b0 + CS_ROWS);
   float s = 0.0f;
   if (c < ncol)
-    // rows b0 + ph, b0 + ph + 4, ... added in order; the loads of 8 of them are issued first
-    for (int64_t b = b0 + ph; b < b1; b += 32) {
-      float v[8];
+    // rows b0 + ph, b0 + ph + 4, ... added in order; the loads of 32 of them are issued first
+    for (int64_t b = b0 + ph; b < b1; b += 128) {
+      float v[32];
 #pragma unroll
-      for (int u = 0; u < 8; ++u) v[u] = b + 4 * u < b1 ? D[(b + 4 * u) * ld + c] : 0.0f;
+      for (int u = 0; u < 32; ++u) v[u] = b + 4 * u < b1 ? D[(b + 4 * u) * ld + c] : 0.0f;
 #pragma unroll
-      for (int u = 0; u < 8; ++u)
+      for (int u = 0; u < 32; ++u)
         if (b + 4 * u < b1) s += v[u];
     }
   red[ph][threadIdx.x & 63] = s;
@@ -264,18 +264,18 @@ __global__ __launch_bounds__(256) void layer1_grad_partial_kernel(const float* _
   const int o = blockIdx.x * 64 + (threadIdx.x & 63), ph = threadIdx.x >> 6;
   const int64_t b0 = (int64_t)blockIdx.y * CS_ROWS, b1 = min(nb, b0 + CS_ROWS);
   float s[IN + 1] = {0.0f, 0.0f, 0.0f, 0.0f};
-  // rows b0 + ph, b0 + ph + 4, ... accumulated in order; the loads of 4 of them are issued first
-  for (int64_t b = b0 + ph; b < b1; b += 16) {
-    float d[4], xv[4][IN];
+  // rows b0 + ph, b0 + ph + 4, ... accumulated in order; the loads of 16 of them are issued first
+  for (int64_t b = b0 + ph; b < b1; b += 64) {
+    float d[16], xv[16][IN];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
+    for (int u = 0; u < 16; ++u) {
       const bool in = b + 4 * u < b1;
       d[u] = in ? dZ[(b + 4 * u) * H + o] : 0.0f;
 #pragma unroll
       for (int i = 0; i < IN; ++i) xv[u][i] = in ? x[(b + 4 * u) * IN + i] : 0.0f;
     }
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
+    for (int u = 0; u < 16; ++u) {
       if (b + 4 * u >= b1) break;
 #pragma unroll
       for (int i = 0; i < IN; ++i) s[i] = fmaf(d[u], xv[u][i], s[i]);
