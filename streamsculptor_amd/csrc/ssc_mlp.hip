@@ -149,6 +149,19 @@ __global__ void round_kernel(const float* __restrict__ src, uint16_t* __restrict
   if (i < n) dst[i] = f2bf(src[i]);
 }
 
+// The three matrix-core weights W2, W3, W4 to BF16 in one launch.
+__global__ void round_weights_kernel(const float* __restrict__ p, uint16_t* __restrict__ w2,
+                                     uint16_t* __restrict__ w3, uint16_t* __restrict__ w4) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < H * H) {
+    w2[i] = f2bf(p[OW2 + i]);
+  } else if (i < 2 * H * H) {
+    w3[i - H * H] = f2bf(p[OW3 + i - H * H]);
+  } else if (i < 2 * H * H + OUT * H) {
+    w4[i - 2 * H * H] = f2bf(p[OW4 + i - 2 * H * H]);
+  }
+}
+
 // Z1[b][o] = sum_i x[b][i] bf16(W1[o][i]) + b1[o];  A1 = bf16(silu(Z1))
 __global__ void layer1_fwd_kernel(const float* __restrict__ p, const float* __restrict__ x,
                                   int64_t nb, float* __restrict__ Z, uint16_t* __restrict__ Aout) {
@@ -543,9 +556,8 @@ void gemm(const uint16_t* A, int64_t sAm, int64_t sAk, const uint16_t* B, int64_
 }
 
 void round_weights(const float* p, Workspace& w, hipStream_t st) {
-  hipLaunchKernelGGL(round_kernel, dim3(nblk(H * H)), dim3(256), 0, st, p + OW2, w.W2b, (int64_t)H * H);
-  hipLaunchKernelGGL(round_kernel, dim3(nblk(H * H)), dim3(256), 0, st, p + OW3, w.W3b, (int64_t)H * H);
-  hipLaunchKernelGGL(round_kernel, dim3(nblk(OUT * H)), dim3(256), 0, st, p + OW4, w.W4b, (int64_t)OUT * H);
+  hipLaunchKernelGGL(round_weights_kernel, dim3(nblk(2 * H * H + OUT * H)), dim3(256), 0, st, p,
+                     w.W2b, w.W3b, w.W4b);
 }
 
 void forward(const float* p, const float* x, int64_t nb, float* y, Workspace& w, hipStream_t st) {
