@@ -360,11 +360,11 @@ __global__ __launch_bounds__(WAVE) void leapfrog_wave_kernel(
   const double h = 0.5 * dt;
   double ax = 0, ay = 0, az = 0;
   if (s < nsteps) {
-    lds.stage(src, s, nsteps + 1);
+    lds.template stage<8>(src, s, nsteps + 1);
     accel_wave<SK, HS>(hp, lds, s, t0 + s * dt, x, y, z, ax, ay, az);
   }
   for (int k = s; k < nsteps; ++k) {
-    lds.stage(src, k, nsteps + 1);
+    lds.template stage<8>(src, k, nsteps + 1);
     vx = vx + h * ax; vy = vy + h * ay; vz = vz + h * az;
     x = x + dt * vx; y = y + dt * vy; z = z + dt * vz;
     accel_wave<SK, HS>(hp, lds, k + 1, t0 + (k + 1) * dt, x, y, z, ax, ay, az);

@@ -113,14 +113,36 @@ __device__ __forceinline__ double dpp_move(double v) {
   return __hiloint2double(hi, lo);
 }
 
+// v[lane] + v[lane ^ 16] (W = 16) or v[lane] + v[lane ^ 32] (W = 32) with the gfx950 permlane
+// swaps, VALU instructions in place of two ds_bpermute round trips through the LDS pipe.  With
+// both operands v, a swap exchanges the matching lanes of paired 16-lane rows (W = 16) or of the
+// two 32-lane halves (W = 32), so every lane ends with {v[lane], v[lane ^ W]} in some order;
+// IEEE addition commutes, so the sum equals v + __shfl_xor(v, W) bit for bit.  Every lane of
+// the wave must be active.
+template <int W>
+__device__ __forceinline__ double swap_sum(double v) {
+  const unsigned lo = (unsigned)__double2loint(v), hi = (unsigned)__double2hiint(v);
+  if constexpr (W == 16) {
+    const auto l = __builtin_amdgcn_permlane16_swap(lo, lo, false, false);
+    const auto h = __builtin_amdgcn_permlane16_swap(hi, hi, false, false);
+    return __hiloint2double((int)h[0], (int)l[0]) + __hiloint2double((int)h[1], (int)l[1]);
+  } else {
+    static_assert(W == 32, "swap_sum exchanges lane ^ 16 or lane ^ 32");
+    const auto l = __builtin_amdgcn_permlane32_swap(lo, lo, false, false);
+    const auto h = __builtin_amdgcn_permlane32_swap(hi, hi, false, false);
+    return __hiloint2double((int)h[0], (int)l[0]) + __hiloint2double((int)h[1], (int)l[1]);
+  }
+}
+
+// Sum over aligned groups of G lanes; G = 32 and 64 need every lane of the wave active.
 template <int G>
 __device__ __forceinline__ double group_sum(double v) {
   if (G >= 2) v += dpp_move<0xB1>(v);   // quad_perm [1,0,3,2]: lane ^ 1
   if (G >= 4) v += dpp_move<0x4E>(v);   // quad_perm [2,3,0,1]: lane ^ 2
   if (G >= 8) v += dpp_move<0x141>(v);  // row_half_mirror: the other quad of the 8-lane half
   if (G >= 16) v += dpp_move<0x140>(v); // row_mirror: the other half of the 16-lane row
-  if (G >= 32) v += __shfl_xor(v, 16, 64);
-  if (G >= 64) v += __shfl_xor(v, 32, 64);
+  if (G >= 32) v = swap_sum<16>(v);
+  if (G >= 64) v = swap_sum<32>(v);
   return v;
 }
 

@@ -26,6 +26,7 @@ template <>
 struct SourceLDS<SSC_SRC_NONE> {
   static constexpr int n = 0;
   __device__ void init(const ssc_sources&, char*) {}
+  template <int SB = 1>
   __device__ void stage(const ssc_sources&, int, int) {}
   __device__ void stage_row(const ssc_sources&, int, int) {}
   template <int P>
@@ -68,6 +69,7 @@ struct SourceLDS<SSC_SRC_STATIC> {
     }
     __syncthreads();
   }
+  template <int SB = 1>
   __device__ void stage(const ssc_sources&, int, int) {}
   __device__ void stage_row(const ssc_sources&, int, int) {}
   template <int P>
@@ -131,18 +133,34 @@ struct TrackLDS {
     }
     kb = INT_MIN;
   }
-  // Make rows k and k+1 resident (block-uniform k; contains barriers).
+  // Make rows k and k+1 resident (block-uniform k; contains barriers).  Each thread issues the
+  // loads of SB records before its first LDS store: a block of one wave (one orbit; 17 records
+  // per lane at 64 sources) then waits for one memory round trip per SB records instead of one
+  // per record.  The SB records take 8 SB VGPRs, so blocks of many orbits keep SB = 1.
+  template <int SB = 1>
   __device__ void stage(const ssc_sources& s, int k, int nrows) {
     if (kb != INT_MIN && k >= kb && k + 1 <= kb + chunk) return;
     __syncthreads();
     kb = k;
     const int rows = min(chunk + 1, nrows - k);
     const int tot = rows * n;
-    for (int e = threadIdx.x; e < tot; e += blockDim.x) {
-      const int r = e / n, j = e - r * n;
-      const double* p = s.pos + ((int64_t)(k + r) * n + j) * 3;
-      const double rs = s.rs[j];
-      rec[r * n + j] = SrcGeo{p[0], p[1], p[2], HQ ? rs : rs * rs};
+    for (int e0 = threadIdx.x; e0 < tot; e0 += SB * blockDim.x) {
+      SrcGeo v[SB];
+#pragma unroll
+      for (int b = 0; b < SB; ++b) {
+        const int e = e0 + b * (int)blockDim.x;
+        if (e < tot) {
+          const int r = e / n, j = e - r * n;
+          const double* p = s.pos + ((int64_t)(k + r) * n + j) * 3;
+          const double rs = s.rs[j];
+          v[b] = SrcGeo{p[0], p[1], p[2], HQ ? rs : rs * rs};
+        }
+      }
+#pragma unroll
+      for (int b = 0; b < SB; ++b) {
+        const int e = e0 + b * (int)blockDim.x;
+        if (e < tot) rec[e] = v[b];  // e = r * n + j
+      }
     }
     __syncthreads();
   }
@@ -208,6 +226,7 @@ struct LineLDS {
                        HQ ? s.rs[j] : s.rs[j] * s.rs[j], s.t0[j], s.t_window[j]};
     __syncthreads();
   }
+  template <int SB = 1>
   __device__ void stage(const ssc_sources&, int, int) {}
   __device__ void stage_row(const ssc_sources&, int, int) {}
   template <int P>
